@@ -1,0 +1,498 @@
+// mnc_capi.cpp -- the C-ABI (include/mnclosure.h): context, device memory,
+// kernel sequencing and timing.  Host code compiled with hipcc.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mnc_internal.h"
+
+struct mnc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    mnc::Tables tab;
+    int num_cu = 256;
+    // tables on device
+    double *d_w = nullptr, *d_p = nullptr, *d_xi = nullptr, *d_eta = nullptr, *d_gv = nullptr;
+    int *d_gptr = nullptr, *d_gk = nullptr;
+    // queue / stats
+    int *d_queue = nullptr;          // [0] next cell, [1] error bits
+    long long *d_stats = nullptr;    // [8]
+    double *d_scratch = nullptr;
+    size_t scratch_elems = 0;
+    // grid
+    bool has_grid = false;
+    mnc_grid_desc grid{};
+    double *d_mom = nullptr, *d_alpha = nullptr, *d_old = nullptr, *d_st = nullptr, *d_ss = nullptr,
+           *d_flux = nullptr, *d_ansatz = nullptr;
+    int ansatz_chunk = 0;
+    // batch staging for host entry points
+    double *d_bu = nullptr, *d_ba = nullptr;
+    mnc_outs *d_bo = nullptr;
+    size_t batch_cap = 0;
+    // timing
+    bool timing = false;
+    struct Ev { int kc; hipEvent_t a, b; };
+    std::vector<Ev> events;
+    long long t_launch[MNC_NUM_KCLASS] = {0, 0, 0, 0};
+    double t_ms[MNC_NUM_KCLASS] = {0, 0, 0, 0};
+    std::string last_error;
+};
+
+namespace {
+
+int ctx_fail(mnc_ctx *ctx, int code, const char *what, hipError_t e = hipSuccess) {
+    if (ctx) {
+        char buf[512];
+        if (e != hipSuccess)
+            snprintf(buf, sizeof buf, "%s failed: %s", what, hipGetErrorString(e));
+        else
+            snprintf(buf, sizeof buf, "%s", what);
+        ctx->last_error = buf;
+    }
+    return code;
+}
+
+template <class T>
+int dalloc(mnc_ctx *ctx, T **p, size_t n) {
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+    if (n == 0) return MNC_OK;
+    MNC_HIP_CHECK(hipMalloc((void **)p, n * sizeof(T)));
+    return MNC_OK;
+}
+
+// Records an event pair around one launch when timing is on.
+struct Timed {
+    mnc_ctx *ctx;
+    int kc;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(mnc_ctx *c, int k) : ctx(c), kc(k) {
+        if (ctx->timing) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, ctx->stream);
+        }
+    }
+    ~Timed() {
+        if (ctx->timing) {
+            (void)hipEventRecord(b, ctx->stream);
+            ctx->events.push_back({kc, a, b});
+        }
+    }
+};
+
+int ensure_scratch(mnc_ctx *ctx, size_t elems) {
+    if (elems <= ctx->scratch_elems) return MNC_OK;
+    int st = dalloc(ctx, &ctx->d_scratch, elems);
+    if (st) return st;
+    ctx->scratch_elems = elems;
+    return MNC_OK;
+}
+
+int check_opts(mnc_ctx *ctx, const mnc_options *o) {
+    if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
+    if (!o) return ctx_fail(ctx, MNC_ERR_ARG, "options pointer is NULL");
+    if (o->max_iter < 1) return ctx_fail(ctx, MNC_ERR_ARG, "max_iter must be >= 1");
+    if (!(o->tol_rel >= 0.0)) return ctx_fail(ctx, MNC_ERR_ARG, "tol_rel must be >= 0");
+    if (o->precision != MNC_PREC_FP64 && o->precision != MNC_PREC_MIXED)
+        return ctx_fail(ctx, MNC_ERR_ARG, "unknown precision");
+    if (o->warm_start < MNC_WARM_PREVIOUS || o->warm_start > MNC_WARM_LEARNED)
+        return ctx_fail(ctx, MNC_ERR_ARG, "unknown warm_start");
+    return MNC_OK;
+}
+
+// Runs the batched Newton on device arrays; accumulates stats into d_stats.
+int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_options *o, mnc_outs *outs,
+               const double *alpha_guess) {
+    if (ncells <= 0) return MNC_OK;
+    const auto &t = ctx->tab;
+    if (o->precision != MNC_PREC_FP64)
+        return ctx_fail(ctx, MNC_ERR_ARG, "mixed precision requested but not available for this order");
+    const int spb = mnc::newton_slots_per_block();
+    // persistent grid: enough waves to fill the chip, no more than the cells need
+    const int max_blocks = ctx->num_cu * 4;
+    int nblocks = std::min(max_blocks, (ncells + spb - 1) / spb);
+    nblocks = std::max(nblocks, 1);
+    const int nslots = nblocks * spb;
+    int st = ensure_scratch(ctx, size_t(mnc::newton_scratch_fields(t.nm, t.np2)) * nslots);
+    if (st) return st;
+    MNC_HIP_CHECK(hipMemsetAsync(ctx->d_queue, 0, 2 * sizeof(int), ctx->stream));
+    mnc::NewtonArgs a{};
+    a.ncells = ncells;
+    a.u = u;
+    a.alpha = alpha;
+    a.outs = outs;
+    a.w = ctx->d_w;
+    a.p = ctx->d_p;
+    a.nq = t.nq;
+    a.gptr = ctx->d_gptr;
+    a.gk = ctx->d_gk;
+    a.gv = ctx->d_gv;
+    a.max_iter = o->max_iter;
+    a.tol = o->tol_rel;
+    a.cond_h_max = o->cond_h_max;
+    a.warm = o->warm_start;
+    a.queue = ctx->d_queue;
+    a.stats = ctx->d_stats;
+    a.scratch = ctx->d_scratch;
+    a.nslots = nslots;
+    a.alpha_guess = alpha_guess;
+    {
+        Timed tm(ctx, MNC_KCLASS_NEWTON);
+        if (!mnc::launch_newton_fp64(a, t.nm, t.np2, t.use_gaunt != 0, nblocks, ctx->stream))
+            return ctx_fail(ctx, MNC_ERR_ARG, "no FP64 Newton kernel instance for this moment order");
+    }
+    MNC_HIP_CHECK(hipGetLastError());
+    int qh[2] = {0, 0};
+    MNC_HIP_CHECK(hipMemcpyAsync(qh, ctx->d_queue, sizeof qh, hipMemcpyDeviceToHost, ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (qh[1] & (1 << MNC_ERR_U0)) return ctx_fail(ctx, MNC_ERR_U0, "u[0] <= 0 in at least one cell");
+    if (qh[1] & (1 << MNC_ERR_MAXREG)) return ctx_fail(ctx, MNC_ERR_MAXREG, "Max regularization achieved");
+    return MNC_OK;
+}
+
+mnc::GridArgs grid_args(mnc_ctx *ctx) {
+    mnc::GridArgs g{};
+    const auto &t = ctx->tab;
+    g.nx = ctx->grid.num_cells_x;
+    g.ny = ctx->grid.num_cells_y;
+    g.NX = g.nx + 4;
+    g.NY = g.ny + 4;
+    g.nm = t.nm;
+    g.np2 = t.np2;
+    g.nq = t.nq;
+    g.dx = ctx->grid.dx;
+    g.dy = ctx->grid.dy;
+    g.theta = ctx->grid.theta;
+    g.init_x = ctx->grid.init_x;
+    g.init_y = ctx->grid.init_y;
+    g.init_cond = ctx->grid.init_cond;
+    g.w = ctx->d_w;
+    g.p = ctx->d_p;
+    g.xi = ctx->d_xi;
+    g.eta = ctx->d_eta;
+    g.moments = ctx->d_mom;
+    g.alpha = ctx->d_alpha;
+    g.moments_old = ctx->d_old;
+    g.sigma_t = ctx->d_st;
+    g.sigma_s = ctx->d_ss;
+    g.flux = ctx->d_flux;
+    g.ansatz = ctx->d_ansatz;
+    return g;
+}
+
+// solveFlux (+ Euler stage when stage > 0) over all quadrature chunks.
+int flux_stage(mnc_ctx *ctx, int stage, double dt) {
+    mnc::GridArgs g = grid_args(ctx);
+    const int nq = ctx->tab.nq, ch = ctx->ansatz_chunk;
+    for (int q0 = 0; q0 < nq; q0 += ch) {
+        const int nqc = std::min(ch, nq - q0);
+        {
+            Timed tm(ctx, MNC_KCLASS_FLUX);
+            mnc::launch_ansatz(g, q0, nqc, ctx->stream);
+        }
+        {
+            Timed tm(ctx, MNC_KCLASS_FLUX);
+            if (!mnc::launch_flux_euler(g, q0, nqc, stage, dt, q0 == 0, q0 + nqc >= nq, ctx->stream))
+                return ctx_fail(ctx, MNC_ERR_ARG, "no flux kernel instance for this moment order");
+        }
+    }
+    MNC_HIP_CHECK(hipGetLastError());
+    return MNC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
+    if (!out) return MNC_ERR_ARG;
+    *out = nullptr;
+    if (!setup) return MNC_ERR_ARG;
+    if (setup->moment_type != 0) return MNC_ERR_ARG;   // only Mn is built
+    mnc_ctx *ctx = new mnc_ctx();
+    ctx->device = setup->device;
+    int st = mnc::build_tables(setup->moment_order, setup->quad_order, setup->use_gaunt, ctx->tab);
+    if (st != MNC_OK) { delete ctx; return st; }
+    if (ctx->device < 0) {   // host-only context: tables, no device state
+        *out = ctx;
+        return MNC_OK;
+    }
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) { delete ctx; return MNC_ERR_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return MNC_ERR_HIP; }
+    const auto &t = ctx->tab;
+    auto up = [&](auto **dst, const auto &vec) -> int {
+        using T = typename std::remove_reference<decltype(vec)>::type::value_type;
+        size_t n = vec.size() ? vec.size() : 1;
+        if (hipMalloc((void **)dst, n * sizeof(T)) != hipSuccess) return MNC_ERR_HIP;
+        if (!vec.empty() && hipMemcpy(*dst, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+            return MNC_ERR_HIP;
+        return MNC_OK;
+    };
+    if (up(&ctx->d_w, t.w) || up(&ctx->d_p, t.p) || up(&ctx->d_xi, t.xi) || up(&ctx->d_eta, t.eta) ||
+        up(&ctx->d_gptr, t.gptr) || up(&ctx->d_gk, t.gk) || up(&ctx->d_gv, t.gv) ||
+        hipMalloc((void **)&ctx->d_queue, 2 * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&ctx->d_stats, 8 * sizeof(long long)) != hipSuccess ||
+        hipMemset(ctx->d_stats, 0, 8 * sizeof(long long)) != hipSuccess) {
+        mnc_destroy(ctx);
+        return MNC_ERR_HIP;
+    }
+    *out = ctx;
+    return MNC_OK;
+}
+
+void mnc_destroy(mnc_ctx *ctx) {
+    if (!ctx) return;
+    if (ctx->device < 0) { delete ctx; return; }
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &ev : ctx->events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
+    void *ptrs[] = {ctx->d_w, ctx->d_p, ctx->d_xi, ctx->d_eta, ctx->d_gv, ctx->d_gptr, ctx->d_gk,
+                    ctx->d_queue, ctx->d_stats, ctx->d_scratch, ctx->d_mom, ctx->d_alpha, ctx->d_old,
+                    ctx->d_st, ctx->d_ss, ctx->d_flux, ctx->d_ansatz, ctx->d_bu, ctx->d_ba, ctx->d_bo};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int mnc_num_moments(const mnc_ctx *ctx) { return ctx ? ctx->tab.nm : 0; }
+int mnc_num_quad(const mnc_ctx *ctx) { return ctx ? ctx->tab.nq : 0; }
+int mnc_table_stride(const mnc_ctx *ctx) { return ctx ? ctx->tab.np2 : 0; }
+int mnc_gaunt_nnz(const mnc_ctx *ctx) { return ctx ? int(ctx->tab.gv.size()) : 0; }
+
+int mnc_get_tables(const mnc_ctx *ctx, double *w, double *p, double *xi, double *eta) {
+    if (!ctx) return MNC_ERR_ARG;
+    const auto &t = ctx->tab;
+    if (w) std::memcpy(w, t.w.data(), t.w.size() * sizeof(double));
+    if (p) std::memcpy(p, t.p.data(), t.p.size() * sizeof(double));
+    if (xi) std::memcpy(xi, t.xi.data(), t.xi.size() * sizeof(double));
+    if (eta) std::memcpy(eta, t.eta.data(), t.eta.size() * sizeof(double));
+    return MNC_OK;
+}
+
+int mnc_get_gaunt(const mnc_ctx *ctx, int *ptr, int *k, double *v) {
+    if (!ctx) return MNC_ERR_ARG;
+    const auto &t = ctx->tab;
+    if (!t.use_gaunt) return MNC_ERR_STATE;
+    if (ptr) std::memcpy(ptr, t.gptr.data(), t.gptr.size() * sizeof(int));
+    if (k) std::memcpy(k, t.gk.data(), t.gk.size() * sizeof(int));
+    if (v) std::memcpy(v, t.gv.data(), t.gv.size() * sizeof(double));
+    return MNC_OK;
+}
+
+int mnc_opt_batch(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_options *opt, mnc_outs *outs) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (ncells < 0 || (ncells > 0 && (!u || !alpha))) return ctx_fail(ctx, MNC_ERR_ARG, "bad batch arguments");
+    int st = check_opts(ctx, opt);
+    if (st) return st;
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    return run_newton(ctx, ncells, u, alpha, opt, outs, nullptr);
+}
+
+int mnc_opt_batch_host(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_options *opt, mnc_outs *outs) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (ncells < 0 || (ncells > 0 && (!u || !alpha))) return ctx_fail(ctx, MNC_ERR_ARG, "bad batch arguments");
+    int st = check_opts(ctx, opt);
+    if (st) return st;
+    if (ncells == 0) return MNC_OK;
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = size_t(ncells) * ctx->tab.nm;
+    if (size_t(ncells) > ctx->batch_cap) {
+        if ((st = dalloc(ctx, &ctx->d_bu, n)) || (st = dalloc(ctx, &ctx->d_ba, n)) ||
+            (st = dalloc(ctx, &ctx->d_bo, size_t(ncells))))
+            return st;
+        ctx->batch_cap = ncells;
+    }
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_bu, u, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_ba, alpha, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    st = run_newton(ctx, ncells, ctx->d_bu, ctx->d_ba, opt, outs ? ctx->d_bo : nullptr, nullptr);
+    // copy back whatever the kernel produced, also on MNC_ERR_U0 / MNC_ERR_MAXREG
+    if (st != MNC_OK && st != MNC_ERR_U0 && st != MNC_ERR_MAXREG) return st;
+    MNC_HIP_CHECK(hipMemcpyAsync(u, ctx->d_bu, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    MNC_HIP_CHECK(hipMemcpyAsync(alpha, ctx->d_ba, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (outs)
+        MNC_HIP_CHECK(hipMemcpyAsync(outs, ctx->d_bo, size_t(ncells) * sizeof(mnc_outs), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return st;
+}
+
+int mnc_grid_create(mnc_ctx *ctx, const mnc_grid_desc *grid, const double *moments, const double *alpha,
+                    const double *sigma_t, const double *sigma_s) {
+    if (!ctx || !grid || !moments || !alpha || !sigma_t || !sigma_s) return MNC_ERR_ARG;
+    if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
+    if (grid->num_cells_x < 2 || grid->num_cells_y < 2)
+        return ctx_fail(ctx, MNC_ERR_ARG, "grid needs at least 2x2 interior cells");
+    if (!(grid->dx > 0.0) || !(grid->dy > 0.0)) return ctx_fail(ctx, MNC_ERR_ARG, "dx, dy must be > 0");
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    ctx->grid = *grid;
+    const size_t nc = size_t(grid->num_cells_x + 4) * (grid->num_cells_y + 4);
+    const size_t n = nc * ctx->tab.nm;
+    int st;
+    if ((st = dalloc(ctx, &ctx->d_mom, n)) || (st = dalloc(ctx, &ctx->d_alpha, n)) ||
+        (st = dalloc(ctx, &ctx->d_old, n)) || (st = dalloc(ctx, &ctx->d_flux, n)) ||
+        (st = dalloc(ctx, &ctx->d_st, nc)) || (st = dalloc(ctx, &ctx->d_ss, nc)))
+        return st;
+    // ansatz buffer: all q at once unless that exceeds 8 GiB
+    const size_t max_elems = (size_t(8) << 30) / sizeof(double);
+    int ch = ctx->tab.nq;
+    while (ch > 1 && size_t(ch) * nc > max_elems) ch = (ch + 1) / 2;
+    ctx->ansatz_chunk = ch;
+    if ((st = dalloc(ctx, &ctx->d_ansatz, size_t(ch) * nc))) return st;
+    MNC_HIP_CHECK(hipMemcpy(ctx->d_mom, moments, n * sizeof(double), hipMemcpyHostToDevice));
+    MNC_HIP_CHECK(hipMemcpy(ctx->d_alpha, alpha, n * sizeof(double), hipMemcpyHostToDevice));
+    MNC_HIP_CHECK(hipMemcpy(ctx->d_st, sigma_t, nc * sizeof(double), hipMemcpyHostToDevice));
+    MNC_HIP_CHECK(hipMemcpy(ctx->d_ss, sigma_s, nc * sizeof(double), hipMemcpyHostToDevice));
+    MNC_HIP_CHECK(hipMemset(ctx->d_flux, 0, n * sizeof(double)));
+    MNC_HIP_CHECK(hipMemset(ctx->d_old, 0, n * sizeof(double)));
+    ctx->has_grid = true;
+    return MNC_OK;
+}
+
+int mnc_grid_get(mnc_ctx *ctx, double *moments, double *alpha) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (!ctx->has_grid) return ctx_fail(ctx, MNC_ERR_STATE, "no grid");
+    const size_t n = size_t(ctx->grid.num_cells_x + 4) * (ctx->grid.num_cells_y + 4) * ctx->tab.nm;
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    if (moments) MNC_HIP_CHECK(hipMemcpyAsync(moments, ctx->d_mom, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (alpha) MNC_HIP_CHECK(hipMemcpyAsync(alpha, ctx->d_alpha, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MNC_OK;
+}
+
+int mnc_grid_set(mnc_ctx *ctx, const double *moments, const double *alpha) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (!ctx->has_grid) return ctx_fail(ctx, MNC_ERR_STATE, "no grid");
+    const size_t n = size_t(ctx->grid.num_cells_x + 4) * (ctx->grid.num_cells_y + 4) * ctx->tab.nm;
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    if (moments) MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_mom, moments, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (alpha) MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_alpha, alpha, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MNC_OK;
+}
+
+int mnc_ghost_fill(mnc_ctx *ctx) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (!ctx->has_grid) return ctx_fail(ctx, MNC_ERR_STATE, "no grid");
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    {
+        Timed tm(ctx, MNC_KCLASS_OTHER);
+        mnc::launch_ghost_fill(grid_args(ctx), ctx->stream);
+    }
+    MNC_HIP_CHECK(hipGetLastError());
+    return MNC_OK;
+}
+
+int mnc_solve_grid(mnc_ctx *ctx, const mnc_options *opt) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (!ctx->has_grid) return ctx_fail(ctx, MNC_ERR_STATE, "no grid");
+    int st = check_opts(ctx, opt);
+    if (st) return st;
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    const int nc = (ctx->grid.num_cells_x + 4) * (ctx->grid.num_cells_y + 4);
+    return run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr);
+}
+
+int mnc_flux(mnc_ctx *ctx, double *flux_host) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (!ctx->has_grid) return ctx_fail(ctx, MNC_ERR_STATE, "no grid");
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = size_t(ctx->grid.num_cells_x + 4) * (ctx->grid.num_cells_y + 4) * ctx->tab.nm;
+    MNC_HIP_CHECK(hipMemsetAsync(ctx->d_flux, 0, n * sizeof(double), ctx->stream));
+    int st = flux_stage(ctx, 0, 0.0);
+    if (st) return st;
+    if (flux_host)
+        MNC_HIP_CHECK(hipMemcpyAsync(flux_host, ctx->d_flux, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MNC_OK;
+}
+
+int mnc_update(mnc_ctx *ctx, double dt, const mnc_options *opt) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (!ctx->has_grid) return ctx_fail(ctx, MNC_ERR_STATE, "no grid");
+    int st = check_opts(ctx, opt);
+    if (st) return st;
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    const int nc = (ctx->grid.num_cells_x + 4) * (ctx->grid.num_cells_y + 4);
+    const size_t n = size_t(nc) * ctx->tab.nm;
+    // momentsOld <- moments before the first solve (momopt_update.cpp:299-308)
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_old, ctx->d_mom, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    for (int stage = 1; stage <= 2; ++stage) {
+        if ((st = mnc_ghost_fill(ctx))) return st;
+        if ((st = run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr))) return st;
+        if ((st = flux_stage(ctx, stage, dt))) return st;
+    }
+    return MNC_OK;
+}
+
+int mnc_get_stats(mnc_ctx *ctx, mnc_stats *st) {
+    if (!ctx || !st) return MNC_ERR_ARG;
+    if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
+    long long h[8];
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    MNC_HIP_CHECK(hipMemcpyAsync(h, ctx->d_stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    st->max_iter = int(h[0]);
+    st->iter_sum = h[1];
+    st->num_dual_solves = h[2];
+    for (int r = 0; r < 5; ++r) st->hist_reg[r] = h[3 + r];
+    return MNC_OK;
+}
+
+int mnc_reset_stats(mnc_ctx *ctx) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    MNC_HIP_CHECK(hipMemsetAsync(ctx->d_stats, 0, 8 * sizeof(long long), ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MNC_OK;
+}
+
+int mnc_sync(mnc_ctx *ctx) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MNC_OK;
+}
+
+int mnc_timing_enable(mnc_ctx *ctx, int on) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (auto &ev : ctx->events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
+    ctx->events.clear();
+    for (int k = 0; k < MNC_NUM_KCLASS; ++k) { ctx->t_launch[k] = 0; ctx->t_ms[k] = 0.0; }
+    ctx->timing = on != 0;
+    return MNC_OK;
+}
+
+int mnc_timing_get(mnc_ctx *ctx, int kclass, long long *launches, double *ms) {
+    if (!ctx || kclass < 0 || kclass >= MNC_NUM_KCLASS) return MNC_ERR_ARG;
+    if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    for (auto &ev : ctx->events) {
+        float el = 0.f;
+        MNC_HIP_CHECK(hipEventElapsedTime(&el, ev.a, ev.b));
+        ctx->t_launch[ev.kc] += 1;
+        ctx->t_ms[ev.kc] += el;
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
+    }
+    ctx->events.clear();
+    if (launches) *launches = ctx->t_launch[kclass];
+    if (ms) *ms = ctx->t_ms[kclass];
+    return MNC_OK;
+}
+
+const char *mnc_last_error(const mnc_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
+
+}  // extern "C"

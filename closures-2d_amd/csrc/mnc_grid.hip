@@ -1,0 +1,207 @@
+// mnc_grid.hip -- ansatz, kinetic flux, Euler/Heun stage and ghost fill of the
+// Mn solver on gfx950 (src/momopt/momopt_update.cpp:38-202,283-382 and
+// src/momopt/momopt_boundaries.cpp:190-223).
+//
+// Data layout in HBM: grid arrays are the reference's I3D layout
+// [(i*(ny+4)+j)][k] (x-major, 2 ghost cells per side); the ansatz is stored
+// [q][cell] so that, for one quadrature direction, neighbouring threads (cells)
+// read neighbouring words.
+#include "mnc_internal.h"
+
+namespace mnc {
+namespace {
+
+constexpr int kBlock = 256;
+
+// momopt_update.cpp:77-90 (SIGN/MIN/MAX of utils.h:28-30)
+__device__ __forceinline__ double sgn1(double x) { return (x >= 0.0) ? 1.0 : -1.0; }
+__device__ __forceinline__ double minmod(double x, double y) {
+    const double a = fabs(x), b = y * sgn1(x);
+    const double mn = (a < b) ? a : b;
+    return sgn1(x) * ((0.0 > mn) ? 0.0 : mn);
+}
+__device__ __forceinline__ double slopefit(double l, double c, double r, double theta) {
+    return minmod(theta * (r - c), minmod(0.5 * (r - l), theta * (c - l)));
+}
+
+// E[q][cell] = exp(alpha_cell . m_q) for q in [q0, q0+nqc)  (computeAnsatz, :38-53)
+template <int NM>
+__global__ __launch_bounds__(kBlock) void ansatz_kernel(GridArgs g, int q0, int nqc) {
+    const int ncell = g.NX * g.NY;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= ncell) return;
+    double a[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) a[k] = g.alpha[size_t(c) * NM + k];
+    for (int qq = 0; qq < nqc; ++qq) {
+        const double *row = g.p + size_t(q0 + qq) * g.np2;
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) s = fma(a[k], row[k], s);
+        g.ansatz[size_t(qq) * ncell + c] = exp(s);
+    }
+}
+
+// Interior flux (solveFlux :128-196) accumulated over q in [q0, q0+nqc) in
+// ascending q, then (last chunk) the Euler stage (:316-336 / :344-364) and, on
+// the second stage, the Heun average (:368-377).
+template <int NM>
+__global__ __launch_bounds__(kBlock) void flux_euler_kernel(GridArgs g, int q0, int nqc, int stage, double dt,
+                                                             int first_chunk, int last_chunk) {
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= g.nx * g.ny) return;
+    const int i = 2 + idx / g.ny;
+    const int j = 2 + idx % g.ny;
+    const int NY = g.NY;
+    const int ncell = g.NX * g.NY;
+    const size_t c = size_t(i) * NY + j;
+    double F[NM];
+    if (first_chunk) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) F[k] = 0.0;
+    } else {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) F[k] = g.flux[c * NM + k];
+    }
+    const double theta = g.theta, dx = g.dx, dy = g.dy;
+    for (int qq = 0; qq < nqc; ++qq) {
+        const int q = q0 + qq;
+        const double *E = g.ansatz + size_t(qq) * ncell;
+        const double xi = g.xi[q], eta = g.eta[q], w = g.w[q];
+        double f1 = 0.0, f2 = 0.0, f3 = 0.0, f4 = 0.0;
+        if (xi > 0) {
+            const double k1 = E[(i - 2) * NY + j], k2 = E[(i - 1) * NY + j];
+            const double k3 = E[i * NY + j], k4 = E[(i + 1) * NY + j];
+            f1 = k3 + 0.5 * slopefit(k2, k3, k4, theta) - k2 - 0.5 * slopefit(k1, k2, k3, theta);
+            f1 = f1 * xi * w;
+        }
+        if (xi < 0) {
+            const double k1 = E[(i - 1) * NY + j], k2 = E[i * NY + j];
+            const double k3 = E[(i + 1) * NY + j], k4 = E[(i + 2) * NY + j];
+            f2 = k3 - 0.5 * slopefit(k2, k3, k4, theta) - k2 + 0.5 * slopefit(k1, k2, k3, theta);
+            f2 = f2 * xi * w;
+        }
+        if (eta > 0) {
+            const double k1 = E[i * NY + (j - 2)], k2 = E[i * NY + (j - 1)];
+            const double k3 = E[i * NY + j], k4 = E[i * NY + (j + 1)];
+            f3 = k3 + 0.5 * slopefit(k2, k3, k4, theta) - k2 - 0.5 * slopefit(k1, k2, k3, theta);
+            f3 = f3 * eta * w;
+        }
+        if (eta < 0) {
+            const double k1 = E[i * NY + (j - 1)], k2 = E[i * NY + j];
+            const double k3 = E[i * NY + (j + 1)], k4 = E[i * NY + (j + 2)];
+            f4 = k3 - 0.5 * slopefit(k2, k3, k4, theta) - k2 + 0.5 * slopefit(k1, k2, k3, theta);
+            f4 = f4 * eta * w;
+        }
+        const double comb = (f1 + f2) / dx + (f3 + f4) / dy;
+        const double *row = g.p + size_t(q) * g.np2;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) F[k] = fma(row[k], comb, F[k]);
+    }
+    if (!last_chunk || stage == 0) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) g.flux[c * NM + k] = F[k];
+        return;
+    }
+    double *u = g.moments + c * NM;
+    const double st = g.sigma_t[c], ss = g.sigma_s[c];
+    double v[NM];
+    v[0] = u[0] * (1.0 + dt * ss - dt * st) - dt * F[0];
+#pragma unroll
+    for (int k = 1; k < NM; ++k) v[k] = u[k] * (1.0 - dt * st) - dt * F[k];
+    if (g.init_cond == 1) {                                   // lattice source (:328-334)
+        const double x_i = g.init_x + (i - 2) * dx;
+        const double y_j = g.init_y + (j - 2) * dy;
+        if (fabs(x_i) < 0.5 && fabs(y_j) < 0.5) v[0] = v[0] + dt * 2.0 * sqrt(M_PI);
+    }
+    if (stage == 2) {
+        const double *uo = g.moments_old + c * NM;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) v[k] = 0.5 * (v[k] + uo[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) u[k] = v[k];
+}
+
+// duplicateBoundaries (momopt_boundaries.cpp:190-223): x strips, then y strips.
+__global__ void ghost_x_kernel(GridArgs g) {
+    const int idx = blockIdx.x * kBlock + threadIdx.x;   // over 2 * ny * nm
+    const int total = 2 * g.ny * g.nm;
+    if (idx >= total) return;
+    const int k = idx % g.nm;
+    const int j = 2 + (idx / g.nm) % g.ny;
+    const int i = idx / (g.nm * g.ny);
+    const int x0 = 0, x1 = 2, x2 = 2 + g.nx - 2, x3 = 2 + g.nx;
+    const int NY = g.NY, nm = g.nm;
+    g.moments[(size_t(x0 + i) * NY + j) * nm + k] = g.moments[(size_t(x2 + i) * NY + j) * nm + k];
+    g.moments[(size_t(x3 + i) * NY + j) * nm + k] = g.moments[(size_t(x1 + i) * NY + j) * nm + k];
+}
+
+__global__ void ghost_y_kernel(GridArgs g) {
+    const int idx = blockIdx.x * kBlock + threadIdx.x;   // over nx * 2 * nm
+    const int total = g.nx * 2 * g.nm;
+    if (idx >= total) return;
+    const int k = idx % g.nm;
+    const int jj = (idx / g.nm) % 2;
+    const int i = 2 + idx / (g.nm * 2);
+    const int y0 = 0, y1 = 2, y2 = 2 + g.ny - 2, y3 = 2 + g.ny;
+    const int NY = g.NY, nm = g.nm;
+    g.moments[(size_t(i) * NY + y0 + jj) * nm + k] = g.moments[(size_t(i) * NY + y2 + jj) * nm + k];
+    g.moments[(size_t(i) * NY + y3 + jj) * nm + k] = g.moments[(size_t(i) * NY + y1 + jj) * nm + k];
+}
+
+template <int NM>
+void launch_flux_t(const GridArgs &g, int q0, int nqc, int stage, double dt, bool fc, bool lc, hipStream_t s) {
+    const int n = g.nx * g.ny;
+    hipLaunchKernelGGL(flux_euler_kernel<NM>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g, q0, nqc,
+                       stage, dt, int(fc), int(lc));
+}
+
+template <int NM>
+void launch_ansatz_t(const GridArgs &g, int q0, int nqc, hipStream_t s) {
+    const int n = g.NX * g.NY;
+    hipLaunchKernelGGL(ansatz_kernel<NM>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g, q0, nqc);
+}
+
+}  // namespace
+
+void launch_ghost_fill(const GridArgs &g, hipStream_t s) {
+    if (g.init_cond != 2) return;
+    const int nxs = 2 * g.ny * g.nm, nys = g.nx * 2 * g.nm;
+    hipLaunchKernelGGL(ghost_x_kernel, dim3((nxs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g);
+    hipLaunchKernelGGL(ghost_y_kernel, dim3((nys + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g);
+}
+
+#define MNC_NM_SWITCH(NMV, CALL) \
+    switch (NMV) {               \
+    case 1: CALL(1); break;      \
+    case 3: CALL(3); break;      \
+    case 6: CALL(6); break;      \
+    case 10: CALL(10); break;    \
+    case 15: CALL(15); break;    \
+    case 21: CALL(21); break;    \
+    case 28: CALL(28); break;    \
+    case 36: CALL(36); break;    \
+    case 45: CALL(45); break;    \
+    default: return false;       \
+    }
+
+void launch_ansatz(const GridArgs &g, int q0, int nqc, hipStream_t s) {
+    auto f = [&]() -> bool {
+#define CALLA(N) launch_ansatz_t<N>(g, q0, nqc, s)
+        MNC_NM_SWITCH(g.nm, CALLA)
+#undef CALLA
+        return true;
+    };
+    f();
+}
+
+bool launch_flux_euler(const GridArgs &g, int q0, int nqc, int stage, double dt, bool first_chunk, bool last_chunk,
+                       hipStream_t s) {
+#define CALLF(N) launch_flux_t<N>(g, q0, nqc, stage, dt, first_chunk, last_chunk, s)
+    MNC_NM_SWITCH(g.nm, CALLF)
+#undef CALLF
+    return true;
+}
+
+}  // namespace mnc

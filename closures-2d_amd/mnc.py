@@ -1,0 +1,231 @@
+"""Python view of the C-ABI in include/mnclosure.h (libmnclosure.so).
+
+The library is the product: hand-written gfx950 HIP kernels behind a plain C
+interface.  This module only marshals numpy arrays through ctypes; it has no
+compute fallback and raises if the library is missing or a call fails.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmnclosure.so")
+HEADER = os.path.join(HERE, os.pardir, "include", "mnclosure.h")
+
+OK, ERR_ARG, ERR_U0, ERR_MAXREG, ERR_HIP, ERR_STATE = 0, 1, 2, 3, 4, 5
+PREC_FP64, PREC_MIXED = 0, 1
+WARM_PREVIOUS, WARM_ISOTROPIC, WARM_LEARNED = 0, 1, 2
+KCLASS_NEWTON, KCLASS_FLUX, KCLASS_OTHER, KCLASS_MLP = 0, 1, 2, 3
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Setup(C.Structure):
+    _fields_ = [("moment_order", C.c_int), ("quad_order", C.c_int), ("use_gaunt", C.c_int),
+                ("moment_type", C.c_int), ("device", C.c_int)]
+
+
+class Options(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("tol_rel", C.c_double), ("cond_h_max", C.c_double),
+                ("precision", C.c_int), ("warm_start", C.c_int)]
+
+
+class Outs(C.Structure):
+    _fields_ = [("r", C.c_double), ("iter", C.c_int), ("norm_g", C.c_double)]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("num_cells_x", C.c_int), ("num_cells_y", C.c_int), ("dx", C.c_double),
+                ("dy", C.c_double), ("init_x", C.c_double), ("init_y", C.c_double),
+                ("init_cond", C.c_int), ("theta", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("num_dual_solves", C.c_longlong),
+                ("iter_sum", C.c_longlong), ("hist_reg", C.c_longlong * 5)]
+
+
+class MncError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (status %d)" % (msg, code))
+        self.code = code
+
+
+def header_symbols():
+    """Function names declared in include/mnclosure.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(mnc_\w+)\s*\(", text, re.M)))
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError("libmnclosure.so is not built (run __graft_entry__.build()); "
+                          "there is no fallback path")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.mnc_create.argtypes = [C.POINTER(Setup), C.POINTER(vp)]
+        L.mnc_destroy.argtypes = [vp]
+        L.mnc_destroy.restype = None
+        for f in ("mnc_num_moments", "mnc_num_quad", "mnc_table_stride", "mnc_gaunt_nnz"):
+            getattr(L, f).argtypes = [vp]
+        L.mnc_get_tables.argtypes = [vp, _dp, _dp, _dp, _dp]
+        L.mnc_get_gaunt.argtypes = [vp, _ip, _ip, _dp]
+        L.mnc_opt_batch.argtypes = [vp, C.c_int, vp, vp, C.POINTER(Options), vp]
+        L.mnc_opt_batch_host.argtypes = [vp, C.c_int, _dp, _dp, C.POINTER(Options), C.POINTER(Outs)]
+        L.mnc_grid_create.argtypes = [vp, C.POINTER(GridDesc), _dp, _dp, _dp, _dp]
+        L.mnc_grid_get.argtypes = [vp, _dp, _dp]
+        L.mnc_grid_set.argtypes = [vp, _dp, _dp]
+        L.mnc_ghost_fill.argtypes = [vp]
+        L.mnc_solve_grid.argtypes = [vp, C.POINTER(Options)]
+        L.mnc_flux.argtypes = [vp, _dp]
+        L.mnc_update.argtypes = [vp, C.c_double, C.POINTER(Options)]
+        L.mnc_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.mnc_reset_stats.argtypes = [vp]
+        L.mnc_sync.argtypes = [vp]
+        L.mnc_timing_enable.argtypes = [vp, C.c_int]
+        L.mnc_timing_get.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
+        L.mnc_last_error.argtypes = [vp]
+        L.mnc_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def options(max_iter=100, tol=1e-8, cond_h_max=1e10, precision=PREC_FP64, warm_start=WARM_PREVIOUS):
+    return Options(max_iter, tol, cond_h_max, precision, warm_start)
+
+
+class Context:
+    """One mnc_ctx: tables, a HIP stream and (optionally) one grid."""
+
+    def __init__(self, moment_order, quad_order, use_gaunt=False, device=0):
+        h = C.c_void_p()
+        st = lib().mnc_create(C.byref(Setup(moment_order, quad_order, int(bool(use_gaunt)), 0, device)),
+                              C.byref(h))
+        if st != OK:
+            raise MncError(st, "mnc_create failed")
+        self.h = h
+        L = lib()
+        self.nm = L.mnc_num_moments(h)
+        self.nq = L.mnc_num_quad(h)
+        self.np2 = L.mnc_table_stride(h)
+        self.use_gaunt = bool(use_gaunt)
+        self.grid = None
+
+    def close(self):
+        if self.h:
+            lib().mnc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st, what):
+        if st != OK:
+            raise MncError(st, "%s: %s" % (what, lib().mnc_last_error(self.h).decode()))
+
+    def tables(self):
+        w = np.zeros(self.nq)
+        p = np.zeros((self.nq, self.np2))
+        xi = np.zeros(self.nq)
+        eta = np.zeros(self.nq)
+        self._check(lib().mnc_get_tables(self.h, _p(w), _p(p), _p(xi), _p(eta)), "mnc_get_tables")
+        return w, p, xi, eta
+
+    def gaunt(self):
+        nnz = lib().mnc_gaunt_nnz(self.h)
+        ptr = np.zeros(self.nm * self.nm + 1, dtype=np.int32)
+        k = np.zeros(max(nnz, 1), dtype=np.int32)
+        v = np.zeros(max(nnz, 1))
+        self._check(lib().mnc_get_gaunt(self.h, ptr.ctypes.data_as(_ip), k.ctypes.data_as(_ip), _p(v)),
+                    "mnc_get_gaunt")
+        return ptr, k[:nnz], v[:nnz]
+
+    def opt_batch_host(self, u, alpha, opt=None, allow_status=(OK,)):
+        """Batched opt() on host arrays; returns (status, u, alpha, iter, r, normG)."""
+        opt = opt or options()
+        u = np.array(u, dtype=np.float64, order="C").reshape(-1, self.nm)
+        alpha = np.array(alpha, dtype=np.float64, order="C").reshape(-1, self.nm)
+        n = u.shape[0]
+        outs = (Outs * max(n, 1))()
+        st = lib().mnc_opt_batch_host(self.h, n, _p(u), _p(alpha), C.byref(opt), outs)
+        if st not in allow_status:
+            self._check(st, "mnc_opt_batch_host")
+        it = np.array([o.iter for o in outs[:n]], dtype=np.int32)
+        r = np.array([o.r for o in outs[:n]])
+        ng = np.array([o.norm_g for o in outs[:n]])
+        return st, u, alpha, it, r, ng
+
+    # ---- grid ----
+    def grid_create(self, nx, ny, dx, dy, init_x, init_y, init_cond, theta, moments, alpha, sigma_t, sigma_s):
+        gd = GridDesc(nx, ny, dx, dy, init_x, init_y, init_cond, theta)
+        m = np.ascontiguousarray(moments, dtype=np.float64).ravel()
+        a = np.ascontiguousarray(alpha, dtype=np.float64).ravel()
+        st_ = np.ascontiguousarray(sigma_t, dtype=np.float64).ravel()
+        ss_ = np.ascontiguousarray(sigma_s, dtype=np.float64).ravel()
+        self._check(lib().mnc_grid_create(self.h, C.byref(gd), _p(m), _p(a), _p(st_), _p(ss_)),
+                    "mnc_grid_create")
+        self.grid = (nx, ny)
+
+    def grid_get(self):
+        nx, ny = self.grid
+        n = (nx + 4) * (ny + 4) * self.nm
+        m = np.zeros(n)
+        a = np.zeros(n)
+        self._check(lib().mnc_grid_get(self.h, _p(m), _p(a)), "mnc_grid_get")
+        shp = (nx + 4, ny + 4, self.nm)
+        return m.reshape(shp), a.reshape(shp)
+
+    def grid_set(self, moments, alpha):
+        m = np.ascontiguousarray(moments, dtype=np.float64).ravel()
+        a = np.ascontiguousarray(alpha, dtype=np.float64).ravel()
+        self._check(lib().mnc_grid_set(self.h, _p(m), _p(a)), "mnc_grid_set")
+
+    def ghost_fill(self):
+        self._check(lib().mnc_ghost_fill(self.h), "mnc_ghost_fill")
+
+    def solve_grid(self, opt):
+        self._check(lib().mnc_solve_grid(self.h, C.byref(opt)), "mnc_solve_grid")
+
+    def flux(self):
+        nx, ny = self.grid
+        out = np.zeros((nx + 4) * (ny + 4) * self.nm)
+        self._check(lib().mnc_flux(self.h, _p(out)), "mnc_flux")
+        return out.reshape(nx + 4, ny + 4, self.nm)
+
+    def update(self, dt, opt):
+        self._check(lib().mnc_update(self.h, dt, C.byref(opt)), "mnc_update")
+
+    def stats(self):
+        s = Stats()
+        self._check(lib().mnc_get_stats(self.h, C.byref(s)), "mnc_get_stats")
+        return s
+
+    def reset_stats(self):
+        self._check(lib().mnc_reset_stats(self.h), "mnc_reset_stats")
+
+    def sync(self):
+        self._check(lib().mnc_sync(self.h), "mnc_sync")
+
+    def timing_enable(self, on=True):
+        self._check(lib().mnc_timing_enable(self.h, int(on)), "mnc_timing_enable")
+
+    def timing_get(self, kclass):
+        n = C.c_longlong(0)
+        ms = C.c_double(0.0)
+        self._check(lib().mnc_timing_get(self.h, kclass, C.byref(n), C.byref(ms)), "mnc_timing_get")
+        return n.value, ms.value

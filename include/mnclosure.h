@@ -1,0 +1,170 @@
+/*
+ * mnclosure.h -- C-ABI of the MI355X Mn entropy-closure hot path.
+ *
+ * The reference (ckrisgarrett/closures-2d) runs the Mn closure as one host
+ * call per cell, opt() (src/momopt/opt/opt.h:43-44), from an OpenMP loop in
+ * MomOptSolver::solveOptimization (src/momopt/momopt_update.cpp:208-277), and
+ * consumes the ansatz in MomOptSolver::solveFlux / update
+ * (src/momopt/momopt_update.cpp:96-202,283-382).  This header is the thin
+ * boundary those calls cross into hand-written gfx950 HIP kernels.
+ *
+ * Conventions
+ *   - Plain C types only; every array is row-major double.
+ *   - A moment/multiplier array is [ncells][n] with n = (N+1)(N+2)/2, i.e.
+ *     the reference's I3D layout (src/solver.h:13) when it covers a grid.
+ *   - Functions return an MNC_* status; nothing in the library aborts the
+ *     process.  The reference's fatal paths (utils_abort at opt.cpp:97-101 and
+ *     opt.cpp:257-260) come back as MNC_ERR_U0 / MNC_ERR_MAXREG and the caller
+ *     decides (the reference driver aborts).
+ *   - One context = one device, one HIP stream (default priority), one grid.
+ *     Calls on one context are stream-ordered and must come from one host
+ *     thread at a time.
+ */
+#ifndef MNCLOSURE_H
+#define MNCLOSURE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define MNC_OK 0
+#define MNC_ERR_ARG 1          /* bad argument or unsupported size            */
+#define MNC_ERR_U0 2           /* some cell had u[0] <= 0   (opt.cpp:97-101)  */
+#define MNC_ERR_MAXREG 3       /* some cell passed the last regularization
+                                  level                       (opt.cpp:257-260) */
+#define MNC_ERR_HIP 4          /* a HIP runtime call failed                    */
+#define MNC_ERR_STATE 5        /* call out of order (e.g. no grid yet)         */
+
+/* ---- option values ---- */
+#define MNC_PREC_FP64 0        /* plain FP64 Newton (reproduces the reference) */
+#define MNC_PREC_MIXED 1       /* BF16/FP32-MFMA Hessian, FP64 g/f/Cholesky/stop */
+
+#define MNC_WARM_PREVIOUS 0    /* start from the alpha passed in (the reference) */
+#define MNC_WARM_ISOTROPIC 1   /* start from isotropicAlpha (opt.cpp:27-35)      */
+#define MNC_WARM_LEARNED 2     /* MLP prediction, judged against the candidates
+                                  above by the FP64 gradient norm               */
+
+#define MNC_HESS_DIRECT 0      /* H = sum_q w e m m^T          (fobj.cpp:85-93)  */
+#define MNC_HESS_GAUNT 1       /* H from order-2N moments + Gaunt map (fobj.cpp:76-82,99-114) */
+
+typedef struct mnc_ctx mnc_ctx;
+
+/* Setup keys of MomOptSolver::init (src/momopt/momopt_init.cpp:56-68). */
+typedef struct {
+    int moment_order;          /* MOMOPT_MOMENT_ORDER  N (n = (N+1)(N+2)/2)    */
+    int quad_order;            /* MOMOPT_QUAD_ORDER    Q (even; nq = Q*Q)      */
+    int use_gaunt;             /* MOMOPT_USE_GAUNT     0/1                     */
+    int moment_type;           /* MOMOPT_MOMENT_TYPE   0 = Mn (only one built)  */
+    int device;                /* HIP device ordinal; < 0 = host-only tables    */
+} mnc_setup;
+
+/* OPTIONS (src/momopt/opt/opt.h:11-22), plus the GPU-only choices. */
+typedef struct {
+    int max_iter;              /* MOMOPT_MAX_ITER                               */
+    double tol_rel;            /* MOMOPT_TOL                                    */
+    double cond_h_max;         /* MOMOPT_COND_H_MAX                             */
+    int precision;             /* MNC_PREC_*                                    */
+    int warm_start;            /* MNC_WARM_*                                    */
+} mnc_options;
+
+/* OUTS (src/momopt/opt/opt.h:25-32). */
+typedef struct {
+    double r;                  /* regularization level used                    */
+    int iter;                  /* total Newton iterations incl. restarts       */
+    double norm_g;             /* ||g||_2 at exit (scaled problem)             */
+} mnc_outs;
+
+/* Grid description: the fields of Solver/MomOptSolver the update reads
+ * (src/solver.h:411-424, src/main.cpp:140-155). */
+typedef struct {
+    int num_cells_x;           /* c_sizeX (interior)                            */
+    int num_cells_y;           /* c_sizeY                                       */
+    double dx, dy;
+    double init_x, init_y;     /* c_initX, c_initY (lower-left domain corner)   */
+    int init_cond;             /* 1 = lattice source term, 2 = periodic ghosts  */
+    double theta;              /* MOMOPT_THETA (minmod slope factor)            */
+} mnc_grid_desc;
+
+/* Per-solve statistics (OPTIMIZATION_STATS, src/momopt/momopt_solver.h:33-41). */
+typedef struct {
+    int max_iter;
+    long long num_dual_solves;
+    long long iter_sum;
+    long long hist_reg[5];
+} mnc_stats;
+
+/* Creates a context: builds the quadrature, the real spherical-harmonic table
+ * and (use_gaunt) the Gaunt map on the host and uploads them once.  With
+ * setup->device < 0 the context is host-only (tables, no HIP calls; every
+ * device entry point then returns MNC_ERR_STATE).
+ * Replaces MomOptSolver::init's table setup (momopt_init.cpp:88-259). */
+int mnc_create(const mnc_setup *setup, mnc_ctx **out);
+void mnc_destroy(mnc_ctx *ctx);
+
+int mnc_num_moments(const mnc_ctx *ctx);       /* n  (c_numMoments)            */
+int mnc_num_quad(const mnc_ctx *ctx);          /* nq (c_numQuadPoints)         */
+int mnc_table_stride(const mnc_ctx *ctx);      /* np2 (c_numManyMoments)       */
+/* Host copies of w[nq], p[nq*np2], xi[nq], eta[nq] (momopt_init.cpp:176-217);
+ * any pointer may be NULL. */
+int mnc_get_tables(const mnc_ctx *ctx, double *w, double *p, double *xi, double *eta);
+/* Gaunt map as CSR over the n*n (i,j) pairs (g_hStruct, fobj.h:12-22). */
+int mnc_gaunt_nnz(const mnc_ctx *ctx);
+int mnc_get_gaunt(const mnc_ctx *ctx, int *ptr, int *k, double *v);
+
+/* Batched closure on DEVICE arrays u[ncells][n], alpha[ncells][n] (in/out,
+ * exactly opt()'s in/out meaning: u becomes the reconstructed ansatz moments,
+ * alpha the multipliers); outs is a device array of ncells or NULL.
+ * Replaces the per-cell opt() loop of solveOptimization
+ * (momopt_update.cpp:214-238 -> opt.cpp:82-148). */
+int mnc_opt_batch(mnc_ctx *ctx, int ncells, double *u, double *alpha,
+                  const mnc_options *opt, mnc_outs *outs);
+/* The same on HOST arrays (copies in and out): the drop-in for the
+ * reference's c_moments / c_alpha host buffers. */
+int mnc_opt_batch_host(mnc_ctx *ctx, int ncells, double *u, double *alpha,
+                       const mnc_options *opt, mnc_outs *outs);
+
+/* Grid state (ghost-inclusive, [(nx+4)*(ny+4)][n] moments and alpha;
+ * [(nx+4)*(ny+4)] sigma_t, sigma_s), uploaded from host arrays prepared as
+ * Solver::initializeGrid + MomOptSolver::init do (main.cpp:257-372,
+ * momopt_init.cpp:120-143). */
+int mnc_grid_create(mnc_ctx *ctx, const mnc_grid_desc *grid, const double *moments,
+                    const double *alpha, const double *sigma_t, const double *sigma_s);
+int mnc_grid_get(mnc_ctx *ctx, double *moments, double *alpha);
+int mnc_grid_set(mnc_ctx *ctx, const double *moments, const double *alpha);
+
+/* Serial communicateBoundaries: periodic ghost copy of moments only
+ * (comm.cpp:148-155 -> momopt_boundaries.cpp:190-223); no-op unless
+ * init_cond == 2. */
+int mnc_ghost_fill(mnc_ctx *ctx);
+/* solveOptimization over every cell including ghosts (momopt_update.cpp:208-277). */
+int mnc_solve_grid(mnc_ctx *ctx, const mnc_options *opt);
+/* solveFlux (momopt_update.cpp:96-202) into a host array [(nx+4)*(ny+4)][n]
+ * (interior rows filled, ghost rows zero); flux may be NULL to keep it on device. */
+int mnc_flux(mnc_ctx *ctx, double *flux_host);
+/* MomOptSolver::update (momopt_update.cpp:283-382): one Heun step of dt. */
+int mnc_update(mnc_ctx *ctx, double dt, const mnc_options *opt);
+/* Accumulated solve statistics since the last reset. */
+int mnc_get_stats(mnc_ctx *ctx, mnc_stats *st);
+int mnc_reset_stats(mnc_ctx *ctx);
+
+/* Synchronises the context's stream; returns the first error seen. */
+int mnc_sync(mnc_ctx *ctx);
+/* Kernel timing with HIP events on the context's stream.
+ * mnc_timing_enable(ctx, 1) starts recording every launch; mnc_timing_get
+ * synchronises and returns, for kernel class k (MNC_KCLASS_*), the number of
+ * launches and their summed duration in milliseconds. */
+#define MNC_KCLASS_NEWTON 0
+#define MNC_KCLASS_FLUX 1
+#define MNC_KCLASS_OTHER 2
+#define MNC_KCLASS_MLP 3
+#define MNC_NUM_KCLASS 4
+int mnc_timing_enable(mnc_ctx *ctx, int on);
+int mnc_timing_get(mnc_ctx *ctx, int kclass, long long *launches, double *ms);
+/* Last error text for the most recent failing call on this context. */
+const char *mnc_last_error(const mnc_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

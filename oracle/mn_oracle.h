@@ -1,0 +1,138 @@
+/*
+ * mn_oracle.h -- CPU FP64 restatement of the closures-2d Mn ("momopt") path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product path (closures-2d_amd/) never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Arithmetic is
+ * plain IEEE double, evaluated in the reference's order, built with
+ * -ffp-contract=off (the reference was built by g++ -O3 for x86-64 without
+ * FMA: SConstruct:41, examples/config.py:11).
+ */
+#ifndef MN_ORACLE_H
+#define MN_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (the reference calls utils_abort() instead: utils.cpp:26-33). */
+enum {
+    ORC_OK = 0,
+    ORC_ERR_ARG = 1,          /* bad argument (odd Q, N<0, ...)                 */
+    ORC_ERR_U0 = 2,           /* u[0] <= 0            (opt.cpp:97-101)          */
+    ORC_ERR_MAXREG = 3,       /* max regularization   (opt.cpp:257-260)         */
+    ORC_ERR_ALLOC = 4
+};
+
+/* Mirrors OPTIONS (opt.h:11-22) for the fields the Mn/opt path reads. */
+typedef struct {
+    int max_iter;
+    double tol_rel;
+    double cond_h_max;
+    int use_gaunt;
+} orc_options;
+
+/* Mirrors OUTS (opt.h:25-32). */
+typedef struct {
+    double r;
+    int iter;
+    double norm_g;
+} orc_outs;
+
+typedef struct orc_tables orc_tables;
+
+/* Gauss-Legendre rule on [-1,1], ascending nodes (utils.cpp:55-60). */
+int orc_gauss_legendre(int n, double *x, double *w);
+
+/* Normalised associated Legendre function with Condon-Shortley phase,
+ * sqrt((2l+1)/(4pi) (l-m)!/(l+m)!) P_l^m(x)  (the role GSL sphPlm plays at
+ * momopt_init.cpp:203-209). */
+double orc_sph_plm(int l, int m, double x);
+
+/* Quadrature, real spherical-harmonic table and Gaunt map
+ * (momopt_init.cpp:88-93,164-259). */
+int orc_tables_create(int moment_order, int quad_order, int use_gaunt, orc_tables **out);
+void orc_tables_destroy(orc_tables *t);
+int orc_tables_nm(const orc_tables *t);
+int orc_tables_np2(const orc_tables *t);
+int orc_tables_nq(const orc_tables *t);
+/* copies out w[nq], p[nq*np2], xi[nq], eta[nq] (any may be NULL) */
+void orc_tables_get(const orc_tables *t, double *w, double *p, double *xi, double *eta);
+/* Gaunt map as CSR over the nm*nm (i,j) pairs: ptr[nm*nm+1], k[nnz], v[nnz] */
+int orc_tables_gaunt_nnz(const orc_tables *t);
+void orc_tables_gaunt_get(const orc_tables *t, int *ptr, int *k, double *v);
+
+/* f, g, H of the Mn dual objective (fobj.cpp:37-129,255-335).
+ * h != NULL: full evaluation through the Hessian (g must be non-NULL).
+ * h == NULL: f (and g if non-NULL) by direct quadrature. */
+double orc_fobj(const orc_tables *t, const double *alpha, const double *u,
+                double *g, double *h, int use_gaunt);
+
+/* Unblocked lower Cholesky (role of LAPACK dpotrf 'L'); returns 0 or j+1. */
+int orc_cholesky(int n, double *a);
+/* Reciprocal 1-norm condition estimate from the Cholesky factor (role of
+ * LAPACK dpocon): Higham's estimator (ACM TOMS 14(4), 1988). */
+double orc_rcond(int n, const double *l, double anorm);
+/* max absolute row sum (utils.cpp:36-52) */
+double orc_norm1(int n, const double *a);
+
+/* One closure solve: opt() for MOMENT_TYPE_MN (opt.cpp:82-112,154-343).
+ * u and alpha are in/out exactly as in the reference. */
+int orc_opt(const orc_tables *t, double *u, double *alpha,
+            const orc_options *opt, orc_outs *outs);
+
+/* Batched opt over ncells cells, [ncells][nm] layout, OpenMP schedule(dynamic,1)
+ * as MomOptSolver::solveOptimization (momopt_update.cpp:208-277).
+ * nthreads <= 0: OpenMP default.  Returns the first error status seen. */
+int orc_opt_batch(const orc_tables *t, int ncells, double *u, double *alpha,
+                  const orc_options *opt, orc_outs *outs, int nthreads);
+
+/* ---------------- grid solver (main.cpp, momopt_*.cpp) ---------------- */
+
+/* Deck values the momopt path reads (main.cpp:69-83, momopt_init.cpp:56-68). */
+typedef struct {
+    int num_cells_x, num_cells_y;
+    double a_x, b_x, a_y, b_y;
+    double gaussian_sigma, floor_value, sigma;
+    int init_cond;
+    int moment_order, quad_order;
+    double cfl_factor, tol, cond_h_max;
+    int max_iter, use_gaunt;
+    double theta;
+} orc_deck;
+
+/* Mirrors OPTIMIZATION_STATS (momopt_solver.h:33-41). */
+typedef struct {
+    int max_iter;
+    long long num_dual_solves;
+    long long iter_sum;
+    long long hist_reg[5];
+} orc_stats;
+
+typedef struct orc_solver orc_solver;
+
+int orc_solver_create(const orc_deck *deck, int nthreads, orc_solver **out);
+void orc_solver_destroy(orc_solver *s);
+double orc_solver_max_dt(const orc_solver *s);
+int orc_solver_nm(const orc_solver *s);
+/* One Heun step (MomOptSolver::update, momopt_update.cpp:283-382). */
+int orc_solver_update(orc_solver *s, double dt);
+/* The time loop of main.cpp:204-230 up to t_final; returns steps taken in *steps. */
+int orc_solver_run(orc_solver *s, double t_final, double out_delta_t, int *steps);
+/* Ghost-inclusive state [(nx+4)*(ny+4)*nm] (I3D layout, solver.h:13). */
+void orc_solver_get_state(const orc_solver *s, double *moments, double *alpha);
+void orc_solver_set_state(orc_solver *s, const double *moments, const double *alpha);
+/* Interior moments in .pn order [nx][ny][nm] (momopt_output.cpp:41-47). */
+void orc_solver_get_moments(const orc_solver *s, double *out);
+void orc_solver_get_stats(const orc_solver *s, orc_stats *st);
+/* Solve stages separately (for stage-level parity tests). */
+int orc_solver_solve_all(orc_solver *s);                /* solveOptimization */
+void orc_solver_flux(orc_solver *s, double *flux);       /* solveFlux -> [cells][nm] */
+void orc_solver_ghost_fill(orc_solver *s);               /* communicateBoundaries (serial) */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

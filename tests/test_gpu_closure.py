@@ -1,0 +1,75 @@
+"""Batched closure (opt) on the GPU against the CPU oracle, per cell.
+
+Inputs are config-2-style moment vectors (smooth exponential ansatz moments plus
+20% near-boundary lobes) at M2/M3/M4, both Hessian forms."""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import config2_batch, isotropic_start, load_pkg, oracle
+
+CASES = [(2, 12, False), (2, 12, True), (3, 30, True), (3, 30, False), (4, 16, False), (4, 16, True)]
+
+
+def _run(order, Q, gaunt, n, tol, seed, nb_frac=0.2):
+    mnc = load_pkg("mnc")
+    o = oracle()
+    print("case order=%d Q=%d gaunt=%d n=%d tol=%g seed=%d" % (order, Q, gaunt, n, tol, seed), flush=True)
+    ctx = mnc.Context(order, Q, use_gaunt=gaunt)
+    tab = o.Tables(order, Q, gaunt)
+    rng = np.random.default_rng(seed)
+    u, astar, lobe = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, n, rng, near_boundary_frac=nb_frac)
+    a0 = isotropic_start(u)
+    opt = mnc.options(max_iter=100, tol=tol, cond_h_max=1e10)
+    st, ug, ag, itg, rg, ngg = ctx.opt_batch_host(u, a0, opt)
+    sto, uo, ao, ito, ro, ngo = tab.opt_batch(u, a0, max_iter=100, tol=tol, cond_h_max=1e10, nthreads=8)
+    ctx.close()
+    return dict(u=u, astar=astar, lobe=lobe, ug=ug, ag=ag, itg=itg, rg=rg, uo=uo, ao=ao, ito=ito, ro=ro,
+                st=st, sto=sto, tab=tab)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,Q,gaunt", CASES)
+def test_opt_batch_matches_oracle(order, Q, gaunt):
+    R = _run(order, Q, gaunt, n=3000, tol=1e-8, seed=1234 + order)
+    assert R["st"] == R["sto"] == 0
+    same = (R["itg"] == R["ito"]) & (R["rg"] == R["ro"])
+    frac = same.mean()
+    print("identical (iter, r): %.4f" % frac)
+    assert frac >= 0.99
+    # where the discrete path is identical, the results agree to rounding
+    ua = np.abs(R["ug"][same] - R["uo"][same]).max(axis=1) / np.abs(R["uo"][same]).max(axis=1)
+    aa = np.abs(R["ag"][same] - R["ao"][same]).max(axis=1) / (np.abs(R["ao"][same]).max(axis=1) + 1.0)
+    print("max rel u diff %.3e, max rel alpha diff %.3e" % (ua.max(), aa.max()))
+    assert ua.max() <= 1e-11
+    assert aa.max() <= 1e-9
+    # u[0] is preserved exactly (opt.cpp:296-303,107-109)
+    assert np.array_equal(R["ug"][:, 0], R["u"][:, 0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,Q,gaunt", [(4, 16, False), (3, 30, True)])
+def test_opt_batch_reconstruction_invariant(order, Q, gaunt):
+    """u_out = <m exp(alpha_out . m)> (the in-place reconstruction, opt.cpp:296-303)
+    and, unregularized, ||u_out - u_in|| / u0 is within the stop tolerance scale."""
+    R = _run(order, Q, gaunt, n=2000, tol=1e-8, seed=99)
+    tab = R["tab"]
+    pm = tab.p[:, :tab.nm]
+    recon = (np.exp(R["ag"] @ pm.T) * tab.w[None, :]) @ pm
+    rel = np.abs(recon - R["ug"]).max(axis=1) / np.abs(R["ug"]).max(axis=1)
+    print("reconstruction max rel %.3e" % rel.max())
+    assert rel.max() <= 1e-10
+    r0 = R["rg"] == 0.0
+    dn = np.linalg.norm(R["ug"][r0] - R["u"][r0], axis=1) / np.linalg.norm(R["u"][r0], axis=1)
+    assert dn.max() <= 3e-8
+
+
+@pytest.mark.gpu
+def test_known_answer_alpha():
+    """Smooth inputs are moments of a known ansatz: the solve recovers alpha*."""
+    R = _run(2, 12, False, n=2000, tol=1e-12, seed=7, nb_frac=0.0)
+    err = np.abs(R["ag"] - R["astar"]).max(axis=1) / (np.abs(R["astar"]).max(axis=1) + 1.0)
+    print("alpha* recovery max %.3e" % err.max())
+    assert np.median(err) <= 1e-9
+    assert (err <= 1e-6).mean() >= 0.99
