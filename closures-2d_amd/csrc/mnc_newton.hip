@@ -197,7 +197,10 @@ enum : int { ST_NEWTON = 0, ST_TRIAL = 1 };
 // (SGPRs) plus the lane offset, so no per-field address registers stay live.
 template <int NM, int NP2>
 struct Scratch {
-    static constexpr int fields = 4 * NM + NP2;
+    // u, aold, d, g, then the accumulators: the Gaunt moments (NP2) or the
+    // packed lower triangle of the direct Hessian (NM(NM+1)/2), whichever is larger
+    static constexpr int nacc = NP2 > NM * (NM + 1) / 2 ? NP2 : NM * (NM + 1) / 2;
+    static constexpr int fields = 4 * NM + nacc;
     double *wb;      // wave base (uniform)
     int lane;
     __device__ double &at(int f) const { return wb[f * 64 + lane]; }
@@ -256,6 +259,7 @@ __global__ __launch_bounds__(kBlock) void newton_fp64_kernel(NewtonArgs A) {
     constexpr int NA = GAUNT ? NP2 : Tri<NM>::at(RS, 0);            // first-sweep accumulators
     constexpr int NB = (GAUNT || RS >= NM) ? 1 : Tri<NM>::size - Tri<NM>::at(RS, 0);
     constexpr bool SPLIT = !GAUNT && RS < NM;
+    static_assert(NA <= Scratch<NM, NP2>::nacc, "accumulators exceed the scratch field set");
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
     const Scratch<NM, NP2> S{A.scratch + size_t(wave) * Scratch<NM, NP2>::fields * 64, lane};
@@ -521,7 +525,10 @@ void launch(const NewtonArgs &a, int nblocks, hipStream_t s) {
 
 int newton_slots_per_block() { return kBlock; }
 
-int newton_scratch_fields(int nm, int np2) { return 4 * nm + np2; }
+int newton_scratch_fields(int nm, int np2) {
+    const int tri = nm * (nm + 1) / 2;
+    return 4 * nm + (np2 > tri ? np2 : tri);
+}
 
 bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int nblocks, hipStream_t s) {
 #define MNC_CASE(NMV, NP2D, NP2G, RSPLIT)                                                    \

@@ -1,7 +1,12 @@
 """Batched closure (opt) on the GPU against the CPU oracle, per cell.
 
 Inputs are config-2-style moment vectors (smooth exponential ansatz moments plus
-20% near-boundary lobes) at M2/M3/M4, both Hessian forms."""
+20% near-boundary lobes) at M2/M3/M4, both Hessian forms.  The Gaunt form runs
+at tol 1e-6: at 1e-8 the reference algorithm itself cannot finish some cells,
+because its Armijo test compares the Hessian-derived f (fobj.cpp:278-280) with
+the line search's direct-sum f (fobj.cpp:297-328), which differ by ~1e-14 --
+more than the decrease left near ||g|| ~ 1e-7 -- so it ends in
+"Max regularization" (opt.cpp:257-260)."""
 import math
 
 import numpy as np
@@ -32,7 +37,7 @@ def _run(order, Q, gaunt, n, tol, seed, nb_frac=0.2):
 @pytest.mark.gpu
 @pytest.mark.parametrize("order,Q,gaunt", CASES)
 def test_opt_batch_matches_oracle(order, Q, gaunt):
-    R = _run(order, Q, gaunt, n=3000, tol=1e-8, seed=1234 + order)
+    R = _run(order, Q, gaunt, n=3000, tol=1e-6 if gaunt else 1e-8, seed=1234 + order)
     assert R["st"] == R["sto"] == 0
     same = (R["itg"] == R["ito"]) & (R["rg"] == R["ro"])
     frac = same.mean()
@@ -53,7 +58,8 @@ def test_opt_batch_matches_oracle(order, Q, gaunt):
 def test_opt_batch_reconstruction_invariant(order, Q, gaunt):
     """u_out = <m exp(alpha_out . m)> (the in-place reconstruction, opt.cpp:296-303)
     and, unregularized, ||u_out - u_in|| / u0 is within the stop tolerance scale."""
-    R = _run(order, Q, gaunt, n=2000, tol=1e-8, seed=99)
+    tol = 1e-6 if gaunt else 1e-8
+    R = _run(order, Q, gaunt, n=2000, tol=tol, seed=99)
     tab = R["tab"]
     pm = tab.p[:, :tab.nm]
     recon = (np.exp(R["ag"] @ pm.T) * tab.w[None, :]) @ pm
@@ -62,7 +68,7 @@ def test_opt_batch_reconstruction_invariant(order, Q, gaunt):
     assert rel.max() <= 1e-10
     r0 = R["rg"] == 0.0
     dn = np.linalg.norm(R["ug"][r0] - R["u"][r0], axis=1) / np.linalg.norm(R["u"][r0], axis=1)
-    assert dn.max() <= 3e-8
+    assert dn.max() <= 3 * tol
 
 
 @pytest.mark.gpu
