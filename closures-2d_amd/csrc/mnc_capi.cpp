@@ -22,7 +22,7 @@ struct mnc_ctx {
     int *d_gptr = nullptr, *d_gk = nullptr;
     // queue / stats
     int *d_queue = nullptr;          // [0] next cell, [1] error bits
-    long long *d_stats = nullptr;    // [8]
+    long long *d_stats = nullptr;    // [9]
     double *d_scratch = nullptr;
     size_t scratch_elems = 0;
     // grid
@@ -240,8 +240,8 @@ int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
     if (up(&ctx->d_w, t.w) || up(&ctx->d_p, t.p) || up(&ctx->d_xi, t.xi) || up(&ctx->d_eta, t.eta) ||
         up(&ctx->d_gptr, t.gptr) || up(&ctx->d_gk, t.gk) || up(&ctx->d_gv, t.gv) ||
         hipMalloc((void **)&ctx->d_queue, 2 * sizeof(int)) != hipSuccess ||
-        hipMalloc((void **)&ctx->d_stats, 8 * sizeof(long long)) != hipSuccess ||
-        hipMemset(ctx->d_stats, 0, 8 * sizeof(long long)) != hipSuccess) {
+        hipMalloc((void **)&ctx->d_stats, 9 * sizeof(long long)) != hipSuccess ||
+        hipMemset(ctx->d_stats, 0, 9 * sizeof(long long)) != hipSuccess) {
         mnc_destroy(ctx);
         return MNC_ERR_HIP;
     }
@@ -437,7 +437,7 @@ int mnc_update(mnc_ctx *ctx, double dt, const mnc_options *opt) {
 int mnc_get_stats(mnc_ctx *ctx, mnc_stats *st) {
     if (!ctx || !st) return MNC_ERR_ARG;
     if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
-    long long h[8];
+    long long h[9];
     MNC_HIP_CHECK(hipSetDevice(ctx->device));
     MNC_HIP_CHECK(hipMemcpyAsync(h, ctx->d_stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
     MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -445,6 +445,7 @@ int mnc_get_stats(mnc_ctx *ctx, mnc_stats *st) {
     st->iter_sum = h[1];
     st->num_dual_solves = h[2];
     for (int r = 0; r < 5; ++r) st->hist_reg[r] = h[3 + r];
+    st->trial_sum = h[8];
     return MNC_OK;
 }
 
@@ -452,7 +453,7 @@ int mnc_reset_stats(mnc_ctx *ctx) {
     if (!ctx) return MNC_ERR_ARG;
     if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
     MNC_HIP_CHECK(hipSetDevice(ctx->device));
-    MNC_HIP_CHECK(hipMemsetAsync(ctx->d_stats, 0, 8 * sizeof(long long), ctx->stream));
+    MNC_HIP_CHECK(hipMemsetAsync(ctx->d_stats, 0, 9 * sizeof(long long), ctx->stream));
     MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return MNC_OK;
 }

@@ -270,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void newton_fp64_kernel(NewtonArgs A) {
     double t = 1.0, rtol = 0.0, u0 = 1.0, ascale = 0.0, fcur = 0.0, gd = 0.0, maxf = 0.0;
     double al[NM];           // evaluation point (Newton iterate or line-search trial)
     int st_max = 0;
-    long long st_sum = 0, st_n = 0, st_hist[5] = {0, 0, 0, 0, 0};
+    long long st_sum = 0, st_n = 0, st_hist[5] = {0, 0, 0, 0, 0}, st_trials = 0;
     int err_bits = 0;
 
     for (;;) {
@@ -341,6 +341,7 @@ __global__ __launch_bounds__(kBlock) void newton_fp64_kernel(NewtonArgs A) {
         bool newton = (state == ST_NEWTON);
         bool do_reg = false;
         if (state == ST_TRIAL) {
+            ++st_trials;
             double fnew = sf;
 #pragma unroll
             for (int k = 0; k < NM; ++k) fnew = fnew - al[k] * S.u(k);
@@ -502,6 +503,7 @@ __global__ __launch_bounds__(kBlock) void newton_fp64_kernel(NewtonArgs A) {
         st_max = max(st_max, __shfl_xor(st_max, off));
         st_sum += __shfl_xor(st_sum, off);
         st_n += __shfl_xor(st_n, off);
+        st_trials += __shfl_xor(st_trials, off);
         err_bits |= __shfl_xor(err_bits, off);
 #pragma unroll
         for (int r = 0; r < 5; ++r) st_hist[r] += __shfl_xor(st_hist[r], off);
@@ -512,6 +514,7 @@ __global__ __launch_bounds__(kBlock) void newton_fp64_kernel(NewtonArgs A) {
         atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)st_n);
 #pragma unroll
         for (int r = 0; r < 5; ++r) atomicAdd((unsigned long long *)&A.stats[3 + r], (unsigned long long)st_hist[r]);
+        atomicAdd((unsigned long long *)&A.stats[8], (unsigned long long)st_trials);
         if (err_bits) atomicOr(&A.queue[1], err_bits);
     }
 }

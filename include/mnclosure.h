@@ -90,8 +90,9 @@ typedef struct {
 typedef struct {
     int max_iter;
     long long num_dual_solves;
-    long long iter_sum;
+    long long iter_sum;          /* Newton iterations (full f, g, H evaluations)  */
     long long hist_reg[5];
+    long long trial_sum;         /* Armijo trial evaluations (f only)             */
 } mnc_stats;
 
 /* Creates a context: builds the quadrature, the real spherical-harmonic table

@@ -127,14 +127,23 @@ def test_fixture_regression(ic):
         assert abs(dmass) < 1e-10
 
 
+def _config1(tol):
+    return dict(num_cells_x=50, num_cells_y=50, a_x=-1.0, b_x=1.0, a_y=-1.0, b_y=1.0, gaussian_sigma=0.1,
+                floor_value=1e-4, sigma=1.0, init_cond=0, moment_order=2, quad_order=12, cfl_factor=0.9,
+                tol=tol, cond_h_max=1e10, max_iter=100, use_gaunt=0, theta=2.0)
+
+
 @pytest.mark.gpu
-def test_config1_m2_matches_oracle():
+@pytest.mark.parametrize("tol,limit", [(1e-6, 1e-10), (1e-8, 1e-5)])
+def test_config1_m2_matches_oracle(tol, limit):
     """M2, 50x50 on [-1,1]^2, Gaussian sigma 0.1 (normalised, floor 1e-4), Q=12,
-    TOL 1e-8, direct Hessian, 10 steps: run state within 1e-10."""
+    direct Hessian, 10 steps.  At TOL 1e-6 the reference's decisions are robust
+    and the run state matches to 1e-10.  At TOL 1e-8 the reference itself is
+    rounding-sensitive: the oracle with an extended-precision exp moves the
+    state by ~2e-6 (tests/test_oracle_sensitivity.py), so the bar there is that
+    envelope."""
     mnc, gi = load_pkg("mnc"), load_pkg("grid_init")
-    d = dict(num_cells_x=50, num_cells_y=50, a_x=-1.0, b_x=1.0, a_y=-1.0, b_y=1.0, gaussian_sigma=0.1,
-             floor_value=1e-4, sigma=1.0, init_cond=0, moment_order=2, quad_order=12, cfl_factor=0.9,
-             tol=1e-8, cond_h_max=1e10, max_iter=100, use_gaunt=0, theta=2.0)
+    d = _config1(tol)
     ctx, g = gpu_grid_from_deck(mnc, gi, d)
     opt = mn_options(mnc, d)
     s = _oracle_solver(d)
@@ -145,8 +154,8 @@ def test_config1_m2_matches_oracle():
     mo = s.state()[0]
     ctx.close()
     rel = np.abs(mg - mo).max() / np.abs(mo).max()
-    print("config1 state max rel %.3e" % rel)
-    assert rel <= 1e-10
+    print("config1 tol %.0e state max rel %.3e" % (tol, rel))
+    assert rel <= limit
 
 
 @pytest.mark.gpu
