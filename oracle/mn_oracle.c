@@ -29,6 +29,16 @@
 #endif
 
 #define ORC_NUM_REG 5
+
+/* ORC_EXP_LONG builds a sensitivity probe of the oracle: every exp() of the
+ * closure and the ansatz is evaluated in long double and rounded once.  It is
+ * not a restatement of the reference; tests use it to measure how far the
+ * reference algorithm's own output moves under a last-bit change of exp. */
+#ifdef ORC_EXP_LONG
+#define ORC_EXP(x) ((double)expl((long double)(x)))
+#else
+#define ORC_EXP(x) exp(x)
+#endif
 static const double ORC_REGULARIZATIONS[ORC_NUM_REG] = {0.0, 1e-3, 1e-2, 1e-1, 5e-1}; /* opt.h:39-40 */
 
 /* ------------------------------------------------------------------ */
@@ -335,7 +345,7 @@ static void solve_h(const orc_tables *t, const double *alpha, double *h, int use
         double temp = 0;
         for (int i = 0; i < nm; i++)
             temp = temp + alpha[i] * p[q * np2 + i];
-        temp = exp(temp);
+        temp = ORC_EXP(temp);
         if (use_gaunt) {
             for (int k = 0; k < np2; k++)
                 gext[k] += w[q] * p[q * np2 + k] * temp;   /* :80 */
@@ -391,7 +401,7 @@ double orc_fobj(const orc_tables *t, const double *alpha, const double *u,
         double temp = 0;
         for (int i = 0; i < nm; i++)
             temp = temp + alpha[i] * p[q * np2 + i];
-        temp = exp(temp);
+        temp = ORC_EXP(temp);
         if (g != NULL)
             for (int i = 0; i < nm; i++)
                 g[i] += w[q] * p[q * np2 + i] * temp;
@@ -1001,7 +1011,7 @@ void orc_solver_flux(orc_solver *s, double *flux_out)
                 double kinetic = 0.0;                           /* computeAnsatz :44-53 */
                 for (int k = 0; k < nm; k++)
                     kinetic += a[k] * p[q * np2 + k];
-                s->ansatz[(size_t)qq * ncell + c] = exp(kinetic);
+                s->ansatz[(size_t)qq * ncell + c] = ORC_EXP(kinetic);
             }
         }
 #ifdef _OPENMP

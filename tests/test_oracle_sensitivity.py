@@ -1,0 +1,52 @@
+"""How far the reference algorithm's own output moves under a last-bit change
+of exp: the oracle against its ORC_EXP_LONG build (exp in long double).
+
+This bounds what any non-bit-identical implementation (the GPU path included)
+can reproduce.  On the M2 Gaussian deck (50x50, sigma 0.1, Q=12) at MOMOPT_TOL
+1e-6 the run is insensitive (rounding level); at 1e-8 regularization decisions
+in near-boundary cells flip and the state moves by ~1e-6 within 10 steps."""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from helpers import ROOT
+
+SCRIPT = r'''
+import json, sys
+import numpy as np
+sys.path.insert(0, %(root)r)
+import oracle as o
+d = dict(num_cells_x=50, num_cells_y=50, a_x=-1.0, b_x=1.0, a_y=-1.0, b_y=1.0, gaussian_sigma=0.1,
+         floor_value=1e-4, sigma=1.0, init_cond=0, moment_order=2, quad_order=12, cfl_factor=0.9,
+         tol=%(tol)r, cond_h_max=1e10, max_iter=100, use_gaunt=0, theta=2.0)
+s = o.Solver(d, nthreads=4)
+for _ in range(%(steps)d):
+    s.update(s.max_dt)
+np.save(%(out)r, s.state()[0])
+print(json.dumps(list(s.stats().hist_reg)))
+'''
+
+
+def _run(tmp_path, lib, tol, steps, tag):
+    out = str(tmp_path / ("%s.npy" % tag))
+    env = dict(os.environ)
+    if lib:
+        env["MN_ORACLE_LIB"] = lib
+    r = subprocess.run([sys.executable, "-c", SCRIPT % dict(root=ROOT, tol=tol, steps=steps, out=out)],
+                       env=env, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    return np.load(out), json.loads(r.stdout.strip().splitlines()[-1])
+
+
+@pytest.mark.parametrize("tol,lo,hi", [(1e-6, 0.0, 1e-13), (1e-8, 1e-8, 1e-4)])
+def test_reference_rounding_sensitivity(tmp_path, tol, lo, hi):
+    import oracle
+    a, ha = _run(tmp_path, None, tol, 10, "base")
+    b, hb = _run(tmp_path, oracle.XP_LIB_PATH, tol, 10, "xp")
+    rel = np.abs(a - b).max() / np.abs(a).max()
+    print("tol %.0e: state moved by %.3e; regularization histograms %s vs %s" % (tol, rel, ha, hb))
+    assert lo <= rel <= hi
