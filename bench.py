@@ -3,7 +3,11 @@
 Workload (the north-star lattice case): M4 closure (15 moments), Q = 16
 (256 hemisphere quadrature points), 280 x 280 cells on [-3.5, 3.5]^2 with the
 7 x 7 block checkerboard of absorbers (sigma_a = 10) and a unit isotropic
-source in the centre block, floor 1e-10, MOMOPT_TOL 1e-8, direct Hessian,
+source in the centre block, floor 1e-10, MOMOPT_TOL 1e-8, direct Hessian, the
+robust line search (at TOL 1e-8 the reference's Armijo test cannot resolve the
+last Newton decrease of near-isotropic floor cells and the run aborts with
+"Max regularization" by step 34; both the GPU and the CPU leg run the same
+robust rule),
 Heun steps of dt = CFL 0.9 * 2/(2+theta) * dx dy / (dx + dy) up to t = 3.2
 (569 steps).  Every step solves the closure twice in every cell including the
 2 ghost layers: 2 * 284^2 = 161,312 closures per step.
@@ -29,7 +33,7 @@ PKG = os.path.join(ROOT, "closures-2d_amd")
 LATTICE = dict(num_cells_x=280, num_cells_y=280, a_x=-3.5, b_x=3.5, a_y=-3.5, b_y=3.5,
                gaussian_sigma=0.0, floor_value=1e-10, sigma=1.0, init_cond=1, moment_order=4,
                quad_order=16, cfl_factor=0.9, tol=1e-8, cond_h_max=1e10, max_iter=100,
-               use_gaunt=0, theta=2.0)
+               use_gaunt=0, theta=2.0, line_search=1)
 T_FINAL = 3.2
 FP64_PEAK_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 
@@ -128,7 +132,8 @@ def main():
     d = dict(LATTICE)
     n = (d["moment_order"] + 1) * (d["moment_order"] + 2) // 2
     nq = d["quad_order"] ** 2
-    plain = run_form(mnc, gi, d, mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"]),
+    plain = run_form(mnc, gi, d, mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"],
+                                             line_search=mnc.LS_ROBUST),
                      args.steps, args.warmup)
     best = plain
     form = "plain-fp64"
@@ -152,7 +157,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (the reference's lattice initial condition, generated in-process)",
         "config": {"workload": "M4 lattice 280x280, Q=16 (nq=256), 7x7 checkerboard on [-3.5,3.5]^2, "
-                               "floor 1e-10, TOL 1e-8, direct Hessian, Heun dt=%.6g" % plain["dts"][0],
+                               "floor 1e-10, TOL 1e-8, direct Hessian, robust line search, Heun dt=%.6g" % plain["dts"][0],
                    "form": form, "closures_per_step": 2 * 284 * 284,
                    "s_per_timestep": best["wall"] / args.steps,
                    "plain_fp64_closures_per_s": plain["closures"] / plain["wall"],
@@ -163,7 +168,7 @@ def main():
                    "kernel_launches": {"newton": kn, "flux": best["flux"][0], "other": best["other"][0]}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "newton_fp64_kernel<15,15,false,10>",
+                     "kernel": "Newton solve (init + sweep/logic rounds + tail), M4 direct FP64",
                      "algorithmic_flops_per_launch": alg_flops / max(kn, 1),
                      "avg_launch_ms": kms / max(kn, 1)},
     }

@@ -20,11 +20,17 @@ struct mnc_ctx {
     // tables on device
     double *d_w = nullptr, *d_p = nullptr, *d_xi = nullptr, *d_eta = nullptr, *d_gv = nullptr;
     int *d_gptr = nullptr, *d_gk = nullptr;
-    // queue / stats
-    int *d_queue = nullptr;          // [0] next cell, [1] error bits
+    // solver state / stats
+    int *d_ctl = nullptr;            // [0],[1] list counts, [2] tail queue, [3] error bits
     long long *d_stats = nullptr;    // [9]
-    double *d_scratch = nullptr;
-    size_t scratch_elems = 0;
+    double *d_std = nullptr;         // per-cell solver state (doubles)
+    int *d_sti = nullptr;            // per-cell solver state (ints)
+    int *d_lists = nullptr;          // two active lists
+    size_t state_cap = 0;
+    int state_nm = 0;
+    // tuning
+    int rounds = 4;
+    int tail_blocks = 0;             // 0: 2 per CU
     // grid
     bool has_grid = false;
     mnc_grid_desc grid{};
@@ -86,11 +92,26 @@ struct Timed {
     }
 };
 
-int ensure_scratch(mnc_ctx *ctx, size_t elems) {
-    if (elems <= ctx->scratch_elems) return MNC_OK;
-    int st = dalloc(ctx, &ctx->d_scratch, elems);
-    if (st) return st;
-    ctx->scratch_elems = elems;
+int ensure_state(mnc_ctx *ctx, size_t ncells) {
+    if (ncells <= ctx->state_cap) return MNC_OK;
+    int df = 0, ifl = 0;
+    mnc::newton_store_fields(ctx->tab.nm, ctx->tab.np2, &df, &ifl);
+    const size_t cap = std::max<size_t>(ncells, 64);
+    int st;
+    if ((st = dalloc(ctx, &ctx->d_std, size_t(df) * cap)) || (st = dalloc(ctx, &ctx->d_sti, size_t(ifl) * cap)) ||
+        (st = dalloc(ctx, &ctx->d_lists, 2 * cap)))
+        return st;
+    ctx->state_cap = cap;
+    return MNC_OK;
+}
+
+// Reads the kernels' error bits (synchronising the stream).
+int check_errors(mnc_ctx *ctx) {
+    int ctl[4] = {0, 0, 0, 0};
+    MNC_HIP_CHECK(hipMemcpyAsync(ctl, ctx->d_ctl, sizeof ctl, hipMemcpyDeviceToHost, ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (ctl[3] & (1 << MNC_ERR_U0)) return ctx_fail(ctx, MNC_ERR_U0, "u[0] <= 0 in at least one cell");
+    if (ctl[3] & (1 << MNC_ERR_MAXREG)) return ctx_fail(ctx, MNC_ERR_MAXREG, "Max regularization achieved");
     return MNC_OK;
 }
 
@@ -103,25 +124,21 @@ int check_opts(mnc_ctx *ctx, const mnc_options *o) {
         return ctx_fail(ctx, MNC_ERR_ARG, "unknown precision");
     if (o->warm_start < MNC_WARM_PREVIOUS || o->warm_start > MNC_WARM_LEARNED)
         return ctx_fail(ctx, MNC_ERR_ARG, "unknown warm_start");
+    if (o->line_search != MNC_LS_REFERENCE && o->line_search != MNC_LS_ROBUST)
+        return ctx_fail(ctx, MNC_ERR_ARG, "unknown line_search");
     return MNC_OK;
 }
 
-// Runs the batched Newton on device arrays; accumulates stats into d_stats.
+// Enqueues the batched Newton on device arrays (stats accumulate in d_stats).
+// With check, synchronises and turns the kernels' error bits into a status.
 int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_options *o, mnc_outs *outs,
-               const double *alpha_guess) {
+               const double *alpha_guess, bool check) {
     if (ncells <= 0) return MNC_OK;
     const auto &t = ctx->tab;
     if (o->precision != MNC_PREC_FP64)
         return ctx_fail(ctx, MNC_ERR_ARG, "mixed precision requested but not available for this order");
-    const int spb = mnc::newton_slots_per_block();
-    // persistent grid: enough waves to fill the chip, no more than the cells need
-    const int max_blocks = ctx->num_cu * 4;
-    int nblocks = std::min(max_blocks, (ncells + spb - 1) / spb);
-    nblocks = std::max(nblocks, 1);
-    const int nslots = nblocks * spb;
-    int st = ensure_scratch(ctx, size_t(mnc::newton_scratch_fields(t.nm, t.np2)) * nslots);
+    int st = ensure_state(ctx, size_t(ncells));
     if (st) return st;
-    MNC_HIP_CHECK(hipMemsetAsync(ctx->d_queue, 0, 2 * sizeof(int), ctx->stream));
     mnc::NewtonArgs a{};
     a.ncells = ncells;
     a.u = u;
@@ -137,23 +154,24 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     a.tol = o->tol_rel;
     a.cond_h_max = o->cond_h_max;
     a.warm = o->warm_start;
-    a.queue = ctx->d_queue;
+    a.ls_mode = o->line_search;
     a.stats = ctx->d_stats;
-    a.scratch = ctx->d_scratch;
-    a.nslots = nslots;
     a.alpha_guess = alpha_guess;
+    a.st_d = ctx->d_std;
+    a.st_i = ctx->d_sti;
+    a.lists = ctx->d_lists;
+    a.ctl = ctx->d_ctl;
+    a.cap = ctx->state_cap;
+    const int tail = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
     {
         Timed tm(ctx, MNC_KCLASS_NEWTON);
-        if (!mnc::launch_newton_fp64(a, t.nm, t.np2, t.use_gaunt != 0, nblocks, ctx->stream))
+        mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
+        if (!mnc::launch_newton_fp64(a, t.nm, t.np2, t.use_gaunt != 0, ctx->rounds, tail, ctx->stream))
             return ctx_fail(ctx, MNC_ERR_ARG, "no FP64 Newton kernel instance for this moment order");
     }
     MNC_HIP_CHECK(hipGetLastError());
-    int qh[2] = {0, 0};
-    MNC_HIP_CHECK(hipMemcpyAsync(qh, ctx->d_queue, sizeof qh, hipMemcpyDeviceToHost, ctx->stream));
-    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    if (qh[1] & (1 << MNC_ERR_U0)) return ctx_fail(ctx, MNC_ERR_U0, "u[0] <= 0 in at least one cell");
-    if (qh[1] & (1 << MNC_ERR_MAXREG)) return ctx_fail(ctx, MNC_ERR_MAXREG, "Max regularization achieved");
-    return MNC_OK;
+    if (!check) return MNC_OK;
+    return check_errors(ctx);
 }
 
 mnc::GridArgs grid_args(mnc_ctx *ctx) {
@@ -239,7 +257,8 @@ int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
     };
     if (up(&ctx->d_w, t.w) || up(&ctx->d_p, t.p) || up(&ctx->d_xi, t.xi) || up(&ctx->d_eta, t.eta) ||
         up(&ctx->d_gptr, t.gptr) || up(&ctx->d_gk, t.gk) || up(&ctx->d_gv, t.gv) ||
-        hipMalloc((void **)&ctx->d_queue, 2 * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&ctx->d_ctl, 4 * sizeof(int)) != hipSuccess ||
+        hipMemset(ctx->d_ctl, 0, 4 * sizeof(int)) != hipSuccess ||
         hipMalloc((void **)&ctx->d_stats, 9 * sizeof(long long)) != hipSuccess ||
         hipMemset(ctx->d_stats, 0, 9 * sizeof(long long)) != hipSuccess) {
         mnc_destroy(ctx);
@@ -256,7 +275,7 @@ void mnc_destroy(mnc_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &ev : ctx->events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
     void *ptrs[] = {ctx->d_w, ctx->d_p, ctx->d_xi, ctx->d_eta, ctx->d_gv, ctx->d_gptr, ctx->d_gk,
-                    ctx->d_queue, ctx->d_stats, ctx->d_scratch, ctx->d_mom, ctx->d_alpha, ctx->d_old,
+                    ctx->d_ctl, ctx->d_stats, ctx->d_std, ctx->d_sti, ctx->d_lists, ctx->d_mom, ctx->d_alpha, ctx->d_old,
                     ctx->d_st, ctx->d_ss, ctx->d_flux, ctx->d_ansatz, ctx->d_bu, ctx->d_ba, ctx->d_bo};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -295,7 +314,7 @@ int mnc_opt_batch(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_
     int st = check_opts(ctx, opt);
     if (st) return st;
     MNC_HIP_CHECK(hipSetDevice(ctx->device));
-    return run_newton(ctx, ncells, u, alpha, opt, outs, nullptr);
+    return run_newton(ctx, ncells, u, alpha, opt, outs, nullptr, true);
 }
 
 int mnc_opt_batch_host(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_options *opt, mnc_outs *outs) {
@@ -314,7 +333,7 @@ int mnc_opt_batch_host(mnc_ctx *ctx, int ncells, double *u, double *alpha, const
     }
     MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_bu, u, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_ba, alpha, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    st = run_newton(ctx, ncells, ctx->d_bu, ctx->d_ba, opt, outs ? ctx->d_bo : nullptr, nullptr);
+    st = run_newton(ctx, ncells, ctx->d_bu, ctx->d_ba, opt, outs ? ctx->d_bo : nullptr, nullptr, true);
     // copy back whatever the kernel produced, also on MNC_ERR_U0 / MNC_ERR_MAXREG
     if (st != MNC_OK && st != MNC_ERR_U0 && st != MNC_ERR_MAXREG) return st;
     MNC_HIP_CHECK(hipMemcpyAsync(u, ctx->d_bu, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -399,7 +418,7 @@ int mnc_solve_grid(mnc_ctx *ctx, const mnc_options *opt) {
     if (st) return st;
     MNC_HIP_CHECK(hipSetDevice(ctx->device));
     const int nc = (ctx->grid.num_cells_x + 4) * (ctx->grid.num_cells_y + 4);
-    return run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr);
+    return run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr, true);
 }
 
 int mnc_flux(mnc_ctx *ctx, double *flux_host) {
@@ -428,10 +447,12 @@ int mnc_update(mnc_ctx *ctx, double dt, const mnc_options *opt) {
     MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_old, ctx->d_mom, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     for (int stage = 1; stage <= 2; ++stage) {
         if ((st = mnc_ghost_fill(ctx))) return st;
-        if ((st = run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr))) return st;
+        if ((st = run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr, false))) return st;
+        // errors of the first stage are seen at the end of the step (the
+        // kernels flag them in device memory; the state is left as computed)
         if ((st = flux_stage(ctx, stage, dt))) return st;
     }
-    return MNC_OK;
+    return check_errors(ctx);
 }
 
 int mnc_get_stats(mnc_ctx *ctx, mnc_stats *st) {
@@ -492,6 +513,31 @@ int mnc_timing_get(mnc_ctx *ctx, int kclass, long long *launches, double *ms) {
     if (launches) *launches = ctx->t_launch[kclass];
     if (ms) *ms = ctx->t_ms[kclass];
     return MNC_OK;
+}
+
+int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
+    if (!ctx || !key) return MNC_ERR_ARG;
+    if (std::strcmp(key, "rounds") == 0) {
+        if (value < 0 || value > 64) return ctx_fail(ctx, MNC_ERR_ARG, "rounds out of range");
+        ctx->rounds = value;
+        return MNC_OK;
+    }
+    if (std::strcmp(key, "tail_blocks") == 0) {
+        if (value < 1 || value > 65536) return ctx_fail(ctx, MNC_ERR_ARG, "tail_blocks out of range");
+        ctx->tail_blocks = value;
+        return MNC_OK;
+    }
+    return ctx_fail(ctx, MNC_ERR_ARG, "unknown tuning key");
+}
+
+int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
+    if (!ctx || !key || !value) return MNC_ERR_ARG;
+    if (std::strcmp(key, "rounds") == 0) { *value = ctx->rounds; return MNC_OK; }
+    if (std::strcmp(key, "tail_blocks") == 0) {
+        *value = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
+        return MNC_OK;
+    }
+    return MNC_ERR_ARG;
 }
 
 const char *mnc_last_error(const mnc_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }

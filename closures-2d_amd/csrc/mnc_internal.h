@@ -44,18 +44,26 @@ struct NewtonArgs {
     double tol;
     double cond_h_max;
     int warm;             // MNC_WARM_*
-    int *queue;           // [0] next cell, [1] error bits
-    long long *stats;     // [0] max iter, [1] iter sum, [2] solves, [3..7] hist
-    double *scratch;      // [(2*nm + np2)][nslots]
-    int nslots;
+    long long *stats;     // [0] max iter, [1] iter sum, [2] solves, [3..7] hist, [8] trials
+    int ls_mode;          // MNC_LS_*
     const double *alpha_guess; // [ncells][nm] scaled-problem guess (learned warm start) or nullptr
+    // per-cell solver state and the active lists
+    double *st_d;         // [dfields][cap]
+    int *st_i;            // [ifields][cap]
+    int *lists;           // [2][cap]
+    int *ctl;             // [0],[1] list counts, [2] tail queue, [3] error bits
+    size_t cap;
 };
 
 // Launchers (mnc_newton.hip).  Return false if (nm, np2, gaunt) has no instance.
-bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int nblocks,
+// Enqueues the whole batched solve: prologue, `rounds` lockstep sweep/logic
+// rounds with compaction, then the persistent tail.  No host synchronisation.
+bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
                         hipStream_t s);
-int newton_slots_per_block();
-int newton_scratch_fields(int nm, int np2);
+int newton_block();
+// Zeroes the four solve-control words (a kernel, so the step can be captured).
+void launch_ctl_reset(int *ctl, hipStream_t s);
+void newton_store_fields(int nm, int np2, int *dfields, int *ifields);
 
 // Arguments of the grid kernels (mnc_grid.hip).
 struct GridArgs {

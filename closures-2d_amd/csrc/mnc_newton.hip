@@ -191,25 +191,7 @@ __device__ __forceinline__ unsigned long long lane_mask_lt() {
 }
 
 enum : int { ST_NEWTON = 0, ST_TRIAL = 1 };
-
-// Per-slot scratch fields.  Layout [wave][field][64 lanes]: a wave's accesses
-// to one field are 512 contiguous bytes, and the address is a wave-uniform base
-// (SGPRs) plus the lane offset, so no per-field address registers stay live.
-template <int NM, int NP2>
-struct Scratch {
-    // u, aold, d, g, then the accumulators: the Gaunt moments (NP2) or the
-    // packed lower triangle of the direct Hessian (NM(NM+1)/2), whichever is larger
-    static constexpr int nacc = NP2 > NM * (NM + 1) / 2 ? NP2 : NM * (NM + 1) / 2;
-    static constexpr int fields = 4 * NM + nacc;
-    double *wb;      // wave base (uniform)
-    int lane;
-    __device__ double &at(int f) const { return wb[f * 64 + lane]; }
-    __device__ double &u(int k) const { return at(k); }
-    __device__ double &aold(int k) const { return at(NM + k); }
-    __device__ double &d(int k) const { return at(2 * NM + k); }
-    __device__ double &g(int k) const { return at(3 * NM + k); }
-    __device__ double &acc(int k) const { return at(4 * NM + k); }
-};
+enum : int { LOGIC_CONTINUE = 0, LOGIC_DONE = 1 };
 
 // The table is read-only for the whole kernel; reading it through the constant
 // address space lets wave-uniform row reads become scalar loads (SGPR operands).
@@ -250,295 +232,450 @@ __device__ __forceinline__ double sweep(const double *Pg, const double *Wg, int 
     return sf;
 }
 
-// ---- the persistent Newton kernel ----------------------------------------
-// RS: for the direct form, rows [0,RS) are accumulated in the first sweep and
-// rows [RS,NM) in a second one (RS == NM: a single sweep).  Splitting keeps the
-// M4 accumulators (120 doubles) out of spill space.
-template <int NM, int NP2, bool GAUNT, int RS>
-__global__ __launch_bounds__(kBlock) void newton_fp64_kernel(NewtonArgs A) {
-    constexpr int NA = GAUNT ? NP2 : Tri<NM>::at(RS, 0);            // first-sweep accumulators
-    constexpr int NB = (GAUNT || RS >= NM) ? 1 : Tri<NM>::size - Tri<NM>::at(RS, 0);
-    constexpr bool SPLIT = !GAUNT && RS < NM;
-    static_assert(NA <= Scratch<NM, NP2>::nacc, "accumulators exceed the scratch field set");
+// ---- per-cell solver state in HBM ------------------------------------------
+// Structure of arrays [field][cap]: threads handling neighbouring list entries
+// (mostly ascending cell indices) touch neighbouring words.
+template <int NM, int NP2>
+struct Store {
+    static constexpr int nacc = NP2 > NM * (NM + 1) / 2 ? NP2 : NM * (NM + 1) / 2;
+    enum : int {
+        F_AL = 0, F_U = NM, F_AOLD = 2 * NM, F_D = 3 * NM, F_T = 4 * NM, F_RTOL, F_U0, F_ASCALE, F_FCUR,
+        F_GD, F_MAXF, F_FSCALE, F_SF, F_ACC
+    };
+    static constexpr int dfields = F_ACC + nacc;
+    enum : int { I_ITER = 0, I_TOTAL, I_REG, I_MAXIT, I_MODE, ifields };
+    double *D;
+    int *I;
+    size_t cap;
+    __device__ double &d(int f, int c) const { return D[size_t(f) * cap + c]; }
+    __device__ int &i(int f, int c) const { return I[size_t(f) * cap + c]; }
+};
+
+// Per-lane statistics, reduced per block and flushed with one atomic each.
+struct LaneStats {
+    int max_iter = 0, err_bits = 0;
+    long long sum = 0, n = 0, trials = 0, hist[5] = {0, 0, 0, 0, 0};
+};
+
+__device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
+    __shared__ long long red[kBlock / 64][10];
+    for (int off = 32; off > 0; off >>= 1) {
+        s.max_iter = max(s.max_iter, __shfl_xor(s.max_iter, off));
+        s.err_bits |= __shfl_xor(s.err_bits, off);
+        s.sum += __shfl_xor(s.sum, off);
+        s.n += __shfl_xor(s.n, off);
+        s.trials += __shfl_xor(s.trials, off);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) s.hist[r] += __shfl_xor(s.hist[r], off);
+    }
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) {
+        red[w][0] = s.max_iter; red[w][1] = s.err_bits; red[w][2] = s.sum; red[w][3] = s.n; red[w][4] = s.trials;
+        for (int r = 0; r < 5; ++r) red[w][5 + r] = s.hist[r];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int ww = 0; ww < kBlock / 64; ++ww) {
+            v[0] = max(v[0], red[ww][0]);
+            v[1] |= red[ww][1];
+            for (int k = 2; k < 10; ++k) v[k] += red[ww][k];
+        }
+        if (v[3] || v[4] || v[1]) {
+            atomicMax((unsigned long long *)&A.stats[0], (unsigned long long)v[0]);
+            atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)v[2]);
+            atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)v[3]);
+            for (int r = 0; r < 5; ++r) atomicAdd((unsigned long long *)&A.stats[3 + r], (unsigned long long)v[5 + r]);
+            atomicAdd((unsigned long long *)&A.stats[8], (unsigned long long)v[4]);
+            if (v[1]) atomicOr(&A.ctl[3], int(v[1]));
+        }
+    }
+}
+
+// Appends the calling lanes with `want` set to list `dst` (wave-aggregated).
+__device__ __forceinline__ void list_append(const NewtonArgs &A, int dst, bool want, int c) {
+    const unsigned long long m = __ballot(want);
+    if (!m) return;
     const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
-    const Scratch<NM, NP2> S{A.scratch + size_t(wave) * Scratch<NM, NP2>::fields * 64, lane};
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&A.ctl[dst], __popcll(m));
+    base = __shfl(base, leader);
+    if (want) A.lists[size_t(dst) * A.cap + base + __popcll(m & lane_mask_lt())] = c;
+}
+
+// opt() prologue (opt.cpp:87-101) and optScaled set-up (:170-205) for cell c.
+// Returns false if u[0] <= 0 (the reference aborts: opt.cpp:97-101).
+template <int NM, int NP2>
+__device__ bool cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, LaneStats &st) {
+    using St = Store<NM, NP2>;
+    const double *uin = A.u + size_t(c) * NM;
+    const double *ain = A.alpha + size_t(c) * NM;
+    const double u0 = uin[0];
+    const double ascale = log(u0) * 2.0 * kSqrtPi;
+    double ss = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        const double uk = uin[k] / u0;
+        S.d(St::F_U + k, c) = uk;
+        ss = fma(uk, uk, ss);
+    }
+    double al[NM];
+    if (A.warm == MNC_WARM_ISOTROPIC) {
+        al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
+#pragma unroll
+        for (int k = 1; k < NM; ++k) al[k] = 0.0;
+    } else if (A.warm == MNC_WARM_LEARNED && A.alpha_guess) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) al[k] = A.alpha_guess[size_t(c) * NM + k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) al[k] = ain[k];
+        al[0] = al[0] - ascale;
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) S.d(St::F_AL + k, c) = al[k];
+    if (!(u0 > 0.0)) {
+        st.err_bits |= 1 << MNC_ERR_U0;
+        if (A.outs) { A.outs[c].iter = -1; A.outs[c].r = 0.0; A.outs[c].norm_g = 0.0; }
+        return false;
+    }
+    S.d(St::F_U0, c) = u0;
+    S.d(St::F_ASCALE, c) = ascale;
+    S.d(St::F_RTOL, c) = sqrt(ss) * A.tol;
+    S.d(St::F_T, c) = 1.0;
+    S.i(St::I_ITER, c) = 0;
+    S.i(St::I_TOTAL, c) = 0;
+    S.i(St::I_REG, c) = 0;
+    S.i(St::I_MAXIT, c) = A.max_iter;
+    S.i(St::I_MODE, c) = ST_NEWTON;
+    return true;
+}
+
+// One step of optScaled's loop for cell c after a sweep at its evaluation
+// point (the sweep's results are in S: F_SF and F_ACC).  Mirrors
+// opt.cpp:209-318 and linesearch.cpp:38-98, with the sweep shared between an
+// accepted Armijo trial and the Newton iteration that follows it.
+template <int NM, int NP2, bool GAUNT>
+__device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, LaneStats &st) {
+    using St = Store<NM, NP2>;
     const double p0 = A.p[0];
+    int mode = S.i(St::I_MODE, c), iter = S.i(St::I_ITER, c), total = S.i(St::I_TOTAL, c);
+    int reg = S.i(St::I_REG, c), maxit = S.i(St::I_MAXIT, c);
+    double t = S.d(St::F_T, c);
+    double al[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
+    bool newton = (mode == ST_NEWTON);
+    bool do_reg = false, converged = false;
+    double err = 0.0;
+    if (mode == ST_TRIAL) {                                        // linesearch.cpp:45-70
+        ++st.trials;
+        double fnew = S.d(St::F_SF, c);
+#pragma unroll
+        for (int k = 0; k < NM; ++k) fnew = fnew - al[k] * S.d(St::F_U + k, c);
+        const double fcur = S.d(St::F_FCUR, c), gd = S.d(St::F_GD, c);
+        bool accept = fnew <= fcur + 1e-3 * t * gd;
+        // Robust mode: a full Newton step whose predicted decrease -g.d lies
+        // below the rounding resolution of f cannot be judged by Armijo; in
+        // that (quadratic) regime the step is taken.
+        if (!accept && A.ls_mode == MNC_LS_ROBUST && t == 1.0 &&
+            -gd <= 1e3 * DBL_EPSILON * S.d(St::F_FSCALE, c))
+            accept = true;
+        if (accept) {
+            newton = true;
+        } else {
+            t = 0.5 * t;
+            if (t * S.d(St::F_MAXF, c) > DBL_EPSILON / 2) {
+#pragma unroll
+                for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AOLD + k, c) + t * S.d(St::F_D + k, c);
+            } else {                                               // exhausted: t = 0 (:74-98)
+#pragma unroll
+                for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AOLD + k, c);
+                t = 0.0;
+                ++iter; ++total;
+                do_reg = true;                                     // opt.cpp:244 (t == 0)
+            }
+        }
+    }
+    double g[NM];
+    if (newton) {
+        ++iter; ++total;                                           // opt.cpp:212-213
+        double h[Tri<NM>::size];
+        if constexpr (GAUNT) {                                     // fobj.cpp:99-113
+#pragma unroll
+            for (int i = 0; i < NM; ++i) {
+#pragma unroll
+                for (int j = i; j < NM; ++j) {
+                    double hij = 0.0;
+                    const int pr = i * NM + j;
+                    for (int cc = A.gptr[pr]; cc < A.gptr[pr + 1]; ++cc)
+                        hij = fma(A.gv[cc], S.d(St::F_ACC + A.gk[cc], c), hij);
+                    h[Tri<NM>::at(j, i)] = hij;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < Tri<NM>::size; ++k) h[k] = S.d(St::F_ACC + k, c);
+        }
+        double fcur = h[0] / (p0 * p0);                            // fobj.cpp:270-286
+        double fscale = fabs(fcur), gg = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            const double ui = S.d(St::F_U + i, c);
+            g[i] = h[Tri<NM>::at(i, 0)] / p0 - ui;
+            gg = fma(g[i], g[i], gg);
+            fcur = fcur - al[i] * ui;
+            fscale += fabs(al[i] * ui);
+        }
+        double anorm = 0.0;                                        // utils.cpp:36-52
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < NM; ++j) s += fabs(h[i >= j ? Tri<NM>::at(i, j) : Tri<NM>::at(j, i)]);
+            anorm = s > anorm ? s : anorm;
+        }
+        const bool chol_ok = cholesky<NM>(h);
+        double condest = 0.0;
+        if (chol_ok) condest = 1.0 / rcond<NM>(h, anorm);
+        if (iter >= maxit || !chol_ok || t == 0.0 || condest > A.cond_h_max) {
+            do_reg = true;                                         // opt.cpp:244
+        } else {
+            double d[NM];
+#pragma unroll
+            for (int i = 0; i < NM; ++i) d[i] = g[i];
+            apply_inv<NM>(h, d);                                   // dpotrs (:278-282)
+            err = sqrt(gg);                                        // dnrm2 (:286)
+            if (err <= S.d(St::F_RTOL, c)) {
+                converged = true;                                  // :293-306
+            } else {                                               // line search set-up (linesearch.cpp:28-43)
+                double maxf = 0.0, gd = 0.0;
+#pragma unroll
+                for (int i = 0; i < NM; ++i) {
+                    const double di = -d[i];
+                    const double tmp = fabs(di) / (fabs(al[i]) + 1e-100);
+                    maxf = tmp > maxf ? tmp : maxf;
+                    gd = fma(g[i], di, gd);
+                    S.d(St::F_AOLD + i, c) = al[i];
+                    S.d(St::F_D + i, c) = di;
+                    d[i] = di;
+                }
+                S.d(St::F_MAXF, c) = maxf;
+                S.d(St::F_GD, c) = gd;
+                S.d(St::F_FCUR, c) = fcur;
+                S.d(St::F_FSCALE, c) = fscale;
+                t = 1.0;
+                if (t * maxf > DBL_EPSILON / 2) {
+#pragma unroll
+                    for (int i = 0; i < NM; ++i) al[i] = al[i] + t * d[i];
+                    mode = ST_TRIAL;
+                } else {
+                    // the Armijo loop cannot start: linesearch returns 0 and the
+                    // next iteration regularizes (opt.cpp:212-213,244)
+                    t = 0.0;
+                    ++iter; ++total;
+                    do_reg = true;
+                }
+            }
+        }
+    }
+    int result = LOGIC_CONTINUE;
+    if (do_reg) {                                                  // opt.cpp:246-273
+        if (reg < 4) {
+            ++reg;
+            const double r = kReg[reg];
+            const double *uin = A.u + size_t(c) * NM;
+            const double u0 = S.d(St::F_U0, c);
+            double ss = 1.0;
+            S.d(St::F_U + 0, c) = 1.0;
+#pragma unroll
+            for (int k = 1; k < NM; ++k) {
+                const double uk = (uin[k] / u0) * (1.0 - r);
+                S.d(St::F_U + k, c) = uk;
+                ss = fma(uk, uk, ss);
+            }
+            al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
+#pragma unroll
+            for (int k = 1; k < NM; ++k) al[k] = 0.0;
+            S.d(St::F_RTOL, c) = sqrt(ss) * A.tol;
+            iter = 0;
+            t = 1.0;
+            if (reg == 4) maxit = 10 * maxit;
+            mode = ST_NEWTON;
+        } else {                                                   // "Max regularization achieved."
+            st.err_bits |= 1 << MNC_ERR_MAXREG;
+            if (A.outs) { A.outs[c].iter = total; A.outs[c].r = -1.0; A.outs[c].norm_g = err; }
+            result = LOGIC_DONE;
+        }
+    } else if (converged) {
+        const double u0 = S.d(St::F_U0, c);
+        const double u0p = S.d(St::F_U + 0, c) + g[0];
+        double *uo = A.u + size_t(c) * NM;
+        double *ao = A.alpha + size_t(c) * NM;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) uo[k] = ((S.d(St::F_U + k, c) + g[k]) / u0p) * u0;   // :298-301, opt.cpp:107-108
+        al[0] -= log(u0p) * 2.0 * kSqrtPi;                                               // :302
+        al[0] = al[0] + S.d(St::F_ASCALE, c);                                            // opt.cpp:109
+#pragma unroll
+        for (int k = 0; k < NM; ++k) ao[k] = al[k];
+        if (A.outs) { A.outs[c].iter = total; A.outs[c].r = kReg[reg]; A.outs[c].norm_g = err; }
+        st.max_iter = total > st.max_iter ? total : st.max_iter;
+        st.sum += total;
+        ++st.n;
+        ++st.hist[reg];
+        result = LOGIC_DONE;
+    }
+    if (result == LOGIC_CONTINUE) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) S.d(St::F_AL + k, c) = al[k];
+        S.d(St::F_T, c) = t;
+        S.i(St::I_ITER, c) = iter;
+        S.i(St::I_TOTAL, c) = total;
+        S.i(St::I_REG, c) = reg;
+        S.i(St::I_MAXIT, c) = maxit;
+        S.i(St::I_MODE, c) = mode;
+    }
+    return result;
+}
 
-    int cell = -1;           // -1 wants work, -2 finished, >= 0 working
-    int state = ST_NEWTON, iter = 0, total = 0, reg = 0, max_iter = A.max_iter;
-    double t = 1.0, rtol = 0.0, u0 = 1.0, ascale = 0.0, fcur = 0.0, gd = 0.0, maxf = 0.0;
-    double al[NM];           // evaluation point (Newton iterate or line-search trial)
-    int st_max = 0;
-    long long st_sum = 0, st_n = 0, st_hist[5] = {0, 0, 0, 0, 0}, st_trials = 0;
-    int err_bits = 0;
+// One evaluation at the cell's current point: S.F_SF and S.F_ACC.
+template <int NM, int NP2, bool GAUNT, int RS>
+__device__ __forceinline__ void cell_sweep(const NewtonArgs &A, const Store<NM, NP2> &S, int c) {
+    using St = Store<NM, NP2>;
+    constexpr int NA = GAUNT ? NP2 : Tri<NM>::at(RS, 0);
+    constexpr int NB = (GAUNT || RS >= NM) ? 1 : Tri<NM>::size - Tri<NM>::at(RS, 0);
+    static_assert(NA <= St::nacc, "accumulators exceed the store");
+    double al[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
+    {
+        double acc[NA];
+        const double sf = sweep<NM, NP2, GAUNT, 0, (GAUNT ? NM : RS), NA>(A.p, A.w, A.nq, al, acc);
+        S.d(St::F_SF, c) = sf;
+#pragma unroll
+        for (int k = 0; k < NA; ++k) S.d(St::F_ACC + k, c) = acc[k];
+    }
+    if constexpr (!GAUNT && RS < NM) {
+        double accb[NB];
+        sweep<NM, NP2, false, RS, NM, NB>(A.p, A.w, A.nq, al, accb);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) S.d(St::F_ACC + NA + k, c) = accb[k];
+    }
+}
 
+// ---- kernels ------------------------------------------------------------------
+
+// Prologue of every cell; the cells go on list 0.
+template <int NM, int NP2>
+__global__ __launch_bounds__(kBlock) void init_kernel(NewtonArgs A) {
+    const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    LaneStats st;
+    bool ok = false;
+    if (c < A.ncells) ok = cell_init<NM, NP2>(A, S, c, st);
+    list_append(A, 0, ok, c);
+    flush_stats(A, st);
+}
+
+// One sweep for every cell on list `lin`; also clears the count of the list
+// the following logic kernel fills.
+template <int NM, int NP2, bool GAUNT, int RS>
+__global__ __launch_bounds__(kBlock) void sweep_kernel(NewtonArgs A, int lin) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl[lin ^ 1] = 0;
+    const int slot = blockIdx.x * kBlock + threadIdx.x;
+    if (slot >= A.ctl[lin]) return;
+    const int c = A.lists[size_t(lin) * A.cap + slot];
+    const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
+    cell_sweep<NM, NP2, GAUNT, RS>(A, S, c);
+}
+
+// Newton / Armijo logic for every cell on list `lin`; cells needing another
+// sweep are compacted onto list lin^1.
+template <int NM, int NP2, bool GAUNT>
+__global__ __launch_bounds__(kBlock) void logic_kernel(NewtonArgs A, int lin) {
+    const int slot = blockIdx.x * kBlock + threadIdx.x;
+    const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
+    LaneStats st;
+    bool more = false;
+    int c = -1;
+    if (slot < A.ctl[lin]) {
+        c = A.lists[size_t(lin) * A.cap + slot];
+        more = cell_logic<NM, NP2, GAUNT>(A, S, c, st) == LOGIC_CONTINUE;
+    }
+    list_append(A, lin ^ 1, more, c);
+    flush_stats(A, st);
+}
+
+// Persistent tail: lanes pull the remaining cells of list `lin` from a queue
+// and iterate each to completion, so a few cells needing tens of iterations
+// near the realizability boundary cost one launch, not one per iteration.
+template <int NM, int NP2, bool GAUNT, int RS>
+__global__ __launch_bounds__(kBlock) void tail_kernel(NewtonArgs A, int lin) {
+    const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
+    const int lane = threadIdx.x & 63;
+    const int count = A.ctl[lin];
+    LaneStats st;
+    int c = -1;          // -1 wants work, -2 finished
     for (;;) {
-        // -------- refill lanes from the work queue --------
-        const bool need = (cell == -1);
+        const bool need = (c == -1);
         const unsigned long long m = __ballot(need);
         if (m) {
             const int leader = __ffsll((long long)m) - 1;
             int base = 0;
-            if (lane == leader) base = atomicAdd(&A.queue[0], __popcll(m));
+            if (lane == leader) base = atomicAdd(&A.ctl[2], __popcll(m));
             base = __shfl(base, leader);
             if (need) {
-                const int c = base + __popcll(m & lane_mask_lt());
-                cell = (c < A.ncells) ? c : -2;
-                if (cell >= 0) {
-                    // opt() prologue (opt.cpp:87-101) + optScaled set-up (:170-205)
-                    const double *uin = A.u + size_t(cell) * NM;
-                    const double *ain = A.alpha + size_t(cell) * NM;
-                    u0 = uin[0];
-                    ascale = log(u0) * 2.0 * kSqrtPi;
-                    double ss = 0.0;
-#pragma unroll
-                    for (int k = 0; k < NM; ++k) {
-                        const double uk = uin[k] / u0;
-                        S.u(k) = uk;
-                        ss = fma(uk, uk, ss);
-                        al[k] = ain[k];
-                    }
-                    if (A.warm == MNC_WARM_ISOTROPIC) {
-                        al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
-#pragma unroll
-                        for (int k = 1; k < NM; ++k) al[k] = 0.0;
-                    } else if (A.warm == MNC_WARM_LEARNED && A.alpha_guess) {
-                        const double *ag = A.alpha_guess + size_t(cell) * NM;
-#pragma unroll
-                        for (int k = 0; k < NM; ++k) al[k] = ag[k];
-                    } else {
-                        al[0] = al[0] - ascale;
-                    }
-                    if (!(u0 > 0.0)) {
-                        err_bits |= 1 << MNC_ERR_U0;
-                        if (A.outs) { A.outs[cell].iter = -1; A.outs[cell].r = 0.0; A.outs[cell].norm_g = 0.0; }
-                        cell = -1;
-                    } else {
-                        rtol = sqrt(ss) * A.tol;
-                        iter = 0; total = 0; reg = 0; max_iter = A.max_iter; t = 1.0;
-                        state = ST_NEWTON;
-                    }
-                }
-            }
-            continue;   // re-ballot: lanes that failed the prologue want work again
-        }
-        if (__ballot(cell >= 0) == 0) break;   // every lane finished
-
-        // -------- first quadrature sweep at each lane's evaluation point --------
-        double sf = 0.0;
-        {
-            double acc[NA];
-            if (cell >= 0) {
-                sf = sweep<NM, NP2, GAUNT, 0, (GAUNT ? NM : RS), NA>(A.p, A.w, A.nq, al, acc);
-#pragma unroll
-                for (int k = 0; k < NA; ++k) S.acc(k) = acc[k];
+                const int s = base + __popcll(m & lane_mask_lt());
+                c = (s < count) ? A.lists[size_t(lin) * A.cap + s] : -2;
             }
         }
-        if (cell < 0) continue;
-
-        // -------- Armijo test for lanes inside the line search (linesearch.cpp:45-70) --------
-        bool newton = (state == ST_NEWTON);
-        bool do_reg = false;
-        if (state == ST_TRIAL) {
-            ++st_trials;
-            double fnew = sf;
-#pragma unroll
-            for (int k = 0; k < NM; ++k) fnew = fnew - al[k] * S.u(k);
-            if (fnew <= fcur + 1e-3 * t * gd) {
-                newton = true;                                     // accepted: al is the new iterate
-            } else {
-                t = 0.5 * t;
-                if (t * maxf > DBL_EPSILON / 2) {
-#pragma unroll
-                    for (int k = 0; k < NM; ++k) al[k] = S.aold(k) + t * S.d(k);
-                } else {                                           // exhausted: t = 0 (:74-98)
-#pragma unroll
-                    for (int k = 0; k < NM; ++k) al[k] = S.aold(k);
-                    t = 0.0;
-                    ++iter; ++total;
-                    do_reg = true;                                 // opt.cpp:244 (t == 0)
-                }
-            }
-        }
-
-        // -------- second sweep (split direct Hessian) where a Newton step follows --------
-        double accb[NB];
-        if constexpr (SPLIT) {
-            if (__ballot(newton) != 0) {
-                if (newton) sweep<NM, NP2, false, RS, NM, NB>(A.p, A.w, A.nq, al, accb);
-            }
-        }
-        double err = 0.0;
-        bool converged = false;
-        if (newton) {
-            ++iter; ++total;                                       // opt.cpp:212-213
-            double h[Tri<NM>::size];
-            if constexpr (GAUNT) {                                 // fobj.cpp:99-113
-#pragma unroll
-                for (int i = 0; i < NM; ++i) {
-#pragma unroll
-                    for (int j = i; j < NM; ++j) {
-                        double hij = 0.0;
-                        const int pr = i * NM + j;
-                        for (int c = A.gptr[pr]; c < A.gptr[pr + 1]; ++c) hij = fma(A.gv[c], S.acc(A.gk[c]), hij);
-                        h[Tri<NM>::at(j, i)] = hij;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < NA; ++k) h[k] = S.acc(k);
-                if constexpr (SPLIT) {
-#pragma unroll
-                    for (int k = 0; k < NB; ++k) h[NA + k] = accb[k];
-                }
-            }
-            // g and f from H (fobj.cpp:270-286)
-            fcur = h[0] / (p0 * p0);
-            double gg = 0.0;
-#pragma unroll
-            for (int i = 0; i < NM; ++i) {
-                const double ui = S.u(i);
-                const double gi = h[Tri<NM>::at(i, 0)] / p0 - ui;
-                S.g(i) = gi;
-                gg = fma(gi, gi, gg);
-                fcur = fcur - al[i] * ui;
-            }
-            // 1-norm = max absolute row sum of the symmetric H (utils.cpp:36-52)
-            double anorm = 0.0;
-#pragma unroll
-            for (int i = 0; i < NM; ++i) {
-                double s = 0.0;
-#pragma unroll
-                for (int j = 0; j < NM; ++j) s += fabs(h[i >= j ? Tri<NM>::at(i, j) : Tri<NM>::at(j, i)]);
-                anorm = s > anorm ? s : anorm;
-            }
-            const bool chol_ok = cholesky<NM>(h);
-            double condest = 0.0;
-            if (chol_ok) condest = 1.0 / rcond<NM>(h, anorm);
-            if (iter >= max_iter || !chol_ok || t == 0.0 || condest > A.cond_h_max) {
-                do_reg = true;                                      // opt.cpp:244
-            } else {
-                double d[NM];
-#pragma unroll
-                for (int i = 0; i < NM; ++i) d[i] = S.g(i);
-                apply_inv<NM>(h, d);                                // dpotrs (:278-282)
-                err = sqrt(gg);                                     // dnrm2 (:286)
-                if (err <= rtol) {
-                    converged = true;                               // :293-306
-                } else {                                            // set up the line search
-                    maxf = 0.0;
-                    gd = 0.0;
-#pragma unroll
-                    for (int i = 0; i < NM; ++i) {
-                        const double di = -d[i];
-                        const double tmp = fabs(di) / (fabs(al[i]) + 1e-100);
-                        maxf = tmp > maxf ? tmp : maxf;
-                        gd = fma(S.g(i), di, gd);
-                        S.aold(i) = al[i];
-                        S.d(i) = di;
-                    }
-                    t = 1.0;
-                    if (t * maxf > DBL_EPSILON / 2) {
-#pragma unroll
-                        for (int i = 0; i < NM; ++i) al[i] = al[i] + t * S.d(i);
-                        state = ST_TRIAL;
-                    } else {
-                        // the Armijo loop cannot start: linesearch returns 0 and
-                        // the next iteration regularizes (opt.cpp:212-213,244)
-                        t = 0.0;
-                        ++iter; ++total;
-                        do_reg = true;
-                    }
-                }
-            }
-        }
-        if (do_reg) {                                               // opt.cpp:246-273
-            if (reg < 4) {
-                ++reg;
-                const double r = kReg[reg];
-                const double *uin = A.u + size_t(cell) * NM;
-                double ss = 1.0;
-                S.u(0) = 1.0;
-#pragma unroll
-                for (int k = 1; k < NM; ++k) {
-                    const double uk = (uin[k] / u0) * (1.0 - r);
-                    S.u(k) = uk;
-                    ss = fma(uk, uk, ss);
-                }
-                al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
-#pragma unroll
-                for (int k = 1; k < NM; ++k) al[k] = 0.0;
-                rtol = sqrt(ss) * A.tol;
-                iter = 0;
-                t = 1.0;
-                if (reg == 4) max_iter = 10 * max_iter;
-                state = ST_NEWTON;
-            } else {                                                // "Max regularization achieved."
-                err_bits |= 1 << MNC_ERR_MAXREG;
-                if (A.outs) { A.outs[cell].iter = total; A.outs[cell].r = -1.0; A.outs[cell].norm_g = err; }
-                cell = -1;
-            }
-        } else if (converged) {
-            const double u0p = S.u(0) + S.g(0);
-            double *uo = A.u + size_t(cell) * NM;
-            double *ao = A.alpha + size_t(cell) * NM;
-#pragma unroll
-            for (int k = 0; k < NM; ++k) uo[k] = ((S.u(k) + S.g(k)) / u0p) * u0;   // :298-301, opt.cpp:107-108
-            al[0] -= log(u0p) * 2.0 * kSqrtPi;                                      // :302
-            al[0] = al[0] + ascale;                                                 // opt.cpp:109
-#pragma unroll
-            for (int k = 0; k < NM; ++k) ao[k] = al[k];
-            if (A.outs) { A.outs[cell].iter = total; A.outs[cell].r = kReg[reg]; A.outs[cell].norm_g = err; }
-            st_max = total > st_max ? total : st_max;
-            st_sum += total;
-            ++st_n;
-            ++st_hist[reg];
-            cell = -1;
+        if (__ballot(c >= 0) == 0) break;
+        if (c >= 0) {
+            cell_sweep<NM, NP2, GAUNT, RS>(A, S, c);
+            if (cell_logic<NM, NP2, GAUNT>(A, S, c, st) == LOGIC_DONE) c = -1;
         }
     }
-
-    // -------- flush statistics: one atomic per wave --------
-    for (int off = 32; off > 0; off >>= 1) {
-        st_max = max(st_max, __shfl_xor(st_max, off));
-        st_sum += __shfl_xor(st_sum, off);
-        st_n += __shfl_xor(st_n, off);
-        st_trials += __shfl_xor(st_trials, off);
-        err_bits |= __shfl_xor(err_bits, off);
-#pragma unroll
-        for (int r = 0; r < 5; ++r) st_hist[r] += __shfl_xor(st_hist[r], off);
-    }
-    if (lane == 0) {
-        atomicMax((unsigned long long *)&A.stats[0], (unsigned long long)st_max);
-        atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)st_sum);
-        atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)st_n);
-#pragma unroll
-        for (int r = 0; r < 5; ++r) atomicAdd((unsigned long long *)&A.stats[3 + r], (unsigned long long)st_hist[r]);
-        atomicAdd((unsigned long long *)&A.stats[8], (unsigned long long)st_trials);
-        if (err_bits) atomicOr(&A.queue[1], err_bits);
-    }
+    flush_stats(A, st);
 }
 
 template <int NM, int NP2, bool G, int RS>
-void launch(const NewtonArgs &a, int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL((newton_fp64_kernel<NM, NP2, G, RS>), dim3(nblocks), dim3(kBlock), 0, s, a);
+void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, hipStream_t s) {
+    const int nb = (a.ncells + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
+    int lin = 0;
+    for (int r = 0; r < rounds; ++r) {
+        hipLaunchKernelGGL((sweep_kernel<NM, NP2, G, RS>), dim3(nb), dim3(kBlock), 0, s, a, lin);
+        hipLaunchKernelGGL((logic_kernel<NM, NP2, G>), dim3(nb), dim3(kBlock), 0, s, a, lin);
+        lin ^= 1;
+    }
+    hipLaunchKernelGGL((tail_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin);
 }
 
 }  // namespace
 
-int newton_slots_per_block() { return kBlock; }
+int newton_block() { return kBlock; }
 
-int newton_scratch_fields(int nm, int np2) {
-    const int tri = nm * (nm + 1) / 2;
-    return 4 * nm + (np2 > tri ? np2 : tri);
+__global__ void ctl_reset_kernel(int *ctl) {
+    if (threadIdx.x < 4) ctl[threadIdx.x] = 0;
 }
 
-bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int nblocks, hipStream_t s) {
-#define MNC_CASE(NMV, NP2D, NP2G, RSPLIT)                                                    \
-    if (nm == NMV) {                                                                          \
-        if (gaunt && np2 == NP2G) { launch<NMV, NP2G, true, NMV>(a, nblocks, s); return true; }   \
-        if (!gaunt && np2 == NP2D) { launch<NMV, NP2D, false, RSPLIT>(a, nblocks, s); return true; } \
-        return false;                                                                         \
+void launch_ctl_reset(int *ctl, hipStream_t s) { hipLaunchKernelGGL(ctl_reset_kernel, dim3(1), dim3(64), 0, s, ctl); }
+
+void newton_store_fields(int nm, int np2, int *dfields, int *ifields) {
+    const int tri = nm * (nm + 1) / 2;
+    *dfields = 4 * nm + 9 + (np2 > tri ? np2 : tri);
+    *ifields = 5;
+}
+
+bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
+                        hipStream_t s) {
+#define MNC_CASE(NMV, NP2D, NP2G, RSPLIT)                                                             \
+    if (nm == NMV) {                                                                                   \
+        static_assert(Store<NMV, NP2G>::dfields == 4 * NMV + 9 + Store<NMV, NP2G>::nacc, "store");     \
+        if (gaunt && np2 == NP2G) { launch_all<NMV, NP2G, true, NMV>(a, rounds, tail_blocks, s); return true; } \
+        if (!gaunt && np2 == NP2D) { launch_all<NMV, NP2D, false, RSPLIT>(a, rounds, tail_blocks, s); return true; } \
+        return false;                                                                                  \
     }
     MNC_CASE(1, 1, 1, 1)      // M0
     MNC_CASE(3, 3, 6, 3)      // M1

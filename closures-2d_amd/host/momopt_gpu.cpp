@@ -6,7 +6,7 @@
 //
 // Usage: momopt_gpu [deck-path]      (default: ./input.deck, like the reference)
 // Extra deck keys (optional): MOMOPT_GPU_PRECISION 0|1, MOMOPT_GPU_WARM_START 0|1|2,
-// MOMOPT_GPU_DEVICE n.
+// MOMOPT_GPU_LINE_SEARCH 0|1, MOMOPT_GPU_DEVICE n.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -160,9 +160,10 @@ int main(int argc, char **argv) {
         std::printf("only MOMOPT_MOMENT_TYPE 0 (Mn) with MOMOPT_OPTIMIZATION_TYPE 0 is built\n");
         return 1;
     }
-    int prec = MNC_PREC_FP64, warm = MNC_WARM_PREVIOUS, device = 0;
+    int prec = MNC_PREC_FP64, warm = MNC_WARM_PREVIOUS, device = 0, ls = MNC_LS_REFERENCE;
     deck.get("MOMOPT_GPU_PRECISION", prec);
     deck.get("MOMOPT_GPU_WARM_START", warm);
+    deck.get("MOMOPT_GPU_LINE_SEARCH", ls);
     deck.get("MOMOPT_GPU_DEVICE", device);
 
     Grid g;
@@ -191,7 +192,7 @@ int main(int argc, char **argv) {
     if (st != MNC_OK) { std::printf("mnc_grid_create: %s\n", mnc_last_error(ctx)); return 1; }
     const double max_dt = cfl * (2.0 / (2.0 + theta)) * (g.dx * g.dy) / (g.dx + g.dy);   // :72
     std::printf("--- Init done ---\n");
-    mnc_options opt{max_iter, tol, cond_h_max, prec, warm};
+    mnc_options opt{max_iter, tol, cond_h_max, prec, warm, ls};
 
     double t = 0, dt = max_dt;                        // main.cpp:204-230
     write_output(g, ctx, nm, t);

@@ -16,6 +16,7 @@ HEADER = os.path.join(HERE, os.pardir, "include", "mnclosure.h")
 
 OK, ERR_ARG, ERR_U0, ERR_MAXREG, ERR_HIP, ERR_STATE = 0, 1, 2, 3, 4, 5
 PREC_FP64, PREC_MIXED = 0, 1
+LS_REFERENCE, LS_ROBUST = 0, 1
 WARM_PREVIOUS, WARM_ISOTROPIC, WARM_LEARNED = 0, 1, 2
 KCLASS_NEWTON, KCLASS_FLUX, KCLASS_OTHER, KCLASS_MLP = 0, 1, 2, 3
 
@@ -30,7 +31,7 @@ class Setup(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("tol_rel", C.c_double), ("cond_h_max", C.c_double),
-                ("precision", C.c_int), ("warm_start", C.c_int)]
+                ("precision", C.c_int), ("warm_start", C.c_int), ("line_search", C.c_int)]
 
 
 class Outs(C.Structure):
@@ -93,6 +94,8 @@ def lib():
         L.mnc_sync.argtypes = [vp]
         L.mnc_timing_enable.argtypes = [vp, C.c_int]
         L.mnc_timing_get.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
+        L.mnc_tune.argtypes = [vp, C.c_char_p, C.c_int]
+        L.mnc_get_tune.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
         L.mnc_last_error.argtypes = [vp]
         L.mnc_last_error.restype = C.c_char_p
         _lib = L
@@ -103,8 +106,9 @@ def _p(a):
     return a.ctypes.data_as(_dp)
 
 
-def options(max_iter=100, tol=1e-8, cond_h_max=1e10, precision=PREC_FP64, warm_start=WARM_PREVIOUS):
-    return Options(max_iter, tol, cond_h_max, precision, warm_start)
+def options(max_iter=100, tol=1e-8, cond_h_max=1e10, precision=PREC_FP64, warm_start=WARM_PREVIOUS,
+            line_search=LS_REFERENCE):
+    return Options(max_iter, tol, cond_h_max, precision, warm_start, line_search)
 
 
 class Context:
@@ -221,6 +225,14 @@ class Context:
 
     def sync(self):
         self._check(lib().mnc_sync(self.h), "mnc_sync")
+
+    def tune(self, key, value):
+        self._check(lib().mnc_tune(self.h, key.encode(), int(value)), "mnc_tune")
+
+    def get_tune(self, key):
+        v = C.c_int(0)
+        self._check(lib().mnc_get_tune(self.h, key.encode(), C.byref(v)), "mnc_get_tune")
+        return v.value
 
     def timing_enable(self, on=True):
         self._check(lib().mnc_timing_enable(self.h, int(on)), "mnc_timing_enable")

@@ -45,6 +45,12 @@ extern "C" {
 #define MNC_WARM_LEARNED 2     /* MLP prediction, judged against the candidates
                                   above by the FP64 gradient norm               */
 
+#define MNC_LS_REFERENCE 0     /* Armijo on f exactly as linesearch.cpp:21-99     */
+#define MNC_LS_ROBUST 1        /* the same, but a full Newton step whose predicted
+                                  decrease is below the rounding resolution of f
+                                  is accepted (needed for MOMOPT_TOL ~1e-8, where
+                                  the reference aborts with "Max regularization") */
+
 #define MNC_HESS_DIRECT 0      /* H = sum_q w e m m^T          (fobj.cpp:85-93)  */
 #define MNC_HESS_GAUNT 1       /* H from order-2N moments + Gaunt map (fobj.cpp:76-82,99-114) */
 
@@ -66,6 +72,7 @@ typedef struct {
     double cond_h_max;         /* MOMOPT_COND_H_MAX                             */
     int precision;             /* MNC_PREC_*                                    */
     int warm_start;            /* MNC_WARM_*                                    */
+    int line_search;           /* MNC_LS_*                                      */
 } mnc_options;
 
 /* OUTS (src/momopt/opt/opt.h:25-32). */
@@ -148,6 +155,13 @@ int mnc_update(mnc_ctx *ctx, double dt, const mnc_options *opt);
 /* Accumulated solve statistics since the last reset. */
 int mnc_get_stats(mnc_ctx *ctx, mnc_stats *st);
 int mnc_reset_stats(mnc_ctx *ctx);
+
+/* Tuning knobs of the batched solve (set by the autotuner or by hand):
+ *   "rounds"       lockstep sweep/logic rounds before the persistent tail (0-64)
+ *   "tail_blocks"  workgroups of the persistent tail kernel (1-65536)
+ * Returns MNC_ERR_ARG for an unknown key or an out-of-range value. */
+int mnc_tune(mnc_ctx *ctx, const char *key, int value);
+int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value);
 
 /* Synchronises the context's stream; returns the first error seen. */
 int mnc_sync(mnc_ctx *ctx);

@@ -24,7 +24,7 @@ _ip = C.POINTER(C.c_int)
 class Options(C.Structure):
     """OPTIONS (opt.h:11-22), the fields the Mn path reads."""
     _fields_ = [("max_iter", C.c_int), ("tol_rel", C.c_double),
-                ("cond_h_max", C.c_double), ("use_gaunt", C.c_int)]
+                ("cond_h_max", C.c_double), ("use_gaunt", C.c_int), ("line_search", C.c_int)]
 
 
 class Outs(C.Structure):
@@ -41,7 +41,7 @@ class Deck(C.Structure):
                 ("moment_order", C.c_int), ("quad_order", C.c_int),
                 ("cfl_factor", C.c_double), ("tol", C.c_double),
                 ("cond_h_max", C.c_double), ("max_iter", C.c_int),
-                ("use_gaunt", C.c_int), ("theta", C.c_double)]
+                ("use_gaunt", C.c_int), ("theta", C.c_double), ("line_search", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -157,21 +157,23 @@ class Tables:
                            _p(h) if h is not None else None, int(ug))
         return f, g, h
 
-    def opt(self, u, alpha, max_iter=100, tol=1e-8, cond_h_max=1e10, use_gaunt=None):
+    def opt(self, u, alpha, max_iter=100, tol=1e-8, cond_h_max=1e10, use_gaunt=None, line_search=0):
         """One opt() call (opt.cpp:82-148); returns (status, u_out, alpha_out, outs)."""
         u = np.array(u, dtype=np.float64)
         alpha = np.array(alpha, dtype=np.float64)
-        o = Options(max_iter, tol, cond_h_max, int(self.use_gaunt if use_gaunt is None else use_gaunt))
+        o = Options(max_iter, tol, cond_h_max, int(self.use_gaunt if use_gaunt is None else use_gaunt),
+                    line_search)
         outs = Outs()
         st = lib().orc_opt(self.h, _p(u), _p(alpha), C.byref(o), C.byref(outs))
         return st, u, alpha, outs
 
     def opt_batch(self, u, alpha, max_iter=100, tol=1e-8, cond_h_max=1e10, use_gaunt=None,
-                  nthreads=0):
+                  nthreads=0, line_search=0):
         u = np.array(u, dtype=np.float64, order="C")
         alpha = np.array(alpha, dtype=np.float64, order="C")
         n = u.shape[0]
-        o = Options(max_iter, tol, cond_h_max, int(self.use_gaunt if use_gaunt is None else use_gaunt))
+        o = Options(max_iter, tol, cond_h_max, int(self.use_gaunt if use_gaunt is None else use_gaunt),
+                    line_search)
         outs = (Outs * n)()
         st = lib().orc_opt_batch(self.h, n, _p(u), _p(alpha), C.byref(o), outs, nthreads)
         it = np.array([o_.iter for o_ in outs], dtype=np.int32)
@@ -264,7 +266,7 @@ class Solver:
 DECK_DEFAULTS = dict(num_cells_x=50, num_cells_y=50, a_x=-1.5, b_x=1.5, a_y=-1.5, b_y=1.5,
                      gaussian_sigma=0.4, floor_value=1e-4, sigma=1.0, init_cond=0,
                      moment_order=3, quad_order=30, cfl_factor=0.9, tol=1e-4,
-                     cond_h_max=1e10, max_iter=100, use_gaunt=1, theta=2.0)
+                     cond_h_max=1e10, max_iter=100, use_gaunt=1, theta=2.0, line_search=0)
 
 
 def deck_from_dict(d):

@@ -603,7 +603,7 @@ static void isotropic_alpha(int n, double *alpha)          /* opt.cpp:27-46 */
 }
 
 static double linesearch(const orc_tables *t, double *alpha, const double *d, double f,
-                         const double *g, const double *u, double *alpha_old)
+                         const double *g, const double *u, double *alpha_old, int robust, double fscale)
 {                                                          /* linesearch.cpp:21-99 */
     int n1 = t->nm;
     double tt = 1.0;
@@ -624,6 +624,8 @@ static double linesearch(const orc_tables *t, double *alpha, const double *d, do
             temp += g[i] * d[i];
         if (fNew <= f + delta * tt * temp)
             return tt;
+        if (robust && tt == 1.0 && -temp <= 1e3 * DBL_EPSILON * fscale)
+            return tt;                 /* extension: decrease below f's resolution */
         tt = beta * tt;
     }
     for (int i = 0; i < n1; i++) alpha[i] = alpha_old[i];
@@ -657,6 +659,8 @@ static int opt_scaled(const orc_tables *t, double *u, double *alpha, const orc_o
         iter = iter + 1;
         totalIter = totalIter + 1;
         f = orc_fobj(t, alpha, u, g, h, o->use_gaunt);
+        double fscale = fabs(h[0] / (t->p[0] * t->p[0]));
+        for (int i = 0; i < nm; i++) fscale += fabs(alpha[i] * u[i]);
         memcpy(chol, h, sizeof(double) * nm * nm);
         int error = orc_cholesky(nm, chol);
         double norm = orc_norm1(nm, h);
@@ -699,7 +703,7 @@ static int opt_scaled(const orc_tables *t, double *u, double *alpha, const orc_o
             alpha[0] -= log(u0) * 2.0 * sqrt(M_PI);
             break;
         }
-        tt = linesearch(t, alpha, d, f, g, u, alphaOld);
+        tt = linesearch(t, alpha, d, f, g, u, alphaOld, o->line_search, fscale);
     }
     if (outs != NULL) {
         outs->iter = totalIter;
@@ -935,6 +939,7 @@ int orc_solver_solve_all(orc_solver *s)
     o.tol_rel = s->deck.tol;
     o.cond_h_max = s->deck.cond_h_max;
     o.use_gaunt = s->deck.use_gaunt;
+    o.line_search = s->deck.line_search;
     int ncell = s->NX * s->NY, nm = s->nm;
     int status = ORC_OK;
     int maxit = s->stats.max_iter;

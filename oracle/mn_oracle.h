@@ -32,6 +32,7 @@ typedef struct {
     double tol_rel;
     double cond_h_max;
     int use_gaunt;
+    int line_search;   /* 0: linesearch.cpp exactly; 1: robust (see orc_opt) */
 } orc_options;
 
 /* Mirrors OUTS (opt.h:25-32). */
@@ -79,7 +80,11 @@ double orc_rcond(int n, const double *l, double anorm);
 double orc_norm1(int n, const double *a);
 
 /* One closure solve: opt() for MOMENT_TYPE_MN (opt.cpp:82-112,154-343).
- * u and alpha are in/out exactly as in the reference. */
+ * u and alpha are in/out exactly as in the reference.  With
+ * opt->line_search == 1 the Armijo loop additionally accepts the full Newton
+ * step when its predicted decrease -g.d is at most 1e3 * DBL_EPSILON * fscale,
+ * fscale = |H00/m0^2| + sum |alpha_i u_i| (the rounding resolution of f): an
+ * extension for tight tolerances, not in the reference. */
 int orc_opt(const orc_tables *t, double *u, double *alpha,
             const orc_options *opt, orc_outs *outs);
 
@@ -101,6 +106,7 @@ typedef struct {
     double cfl_factor, tol, cond_h_max;
     int max_iter, use_gaunt;
     double theta;
+    int line_search;
 } orc_deck;
 
 /* Mirrors OPTIMIZATION_STATS (momopt_solver.h:33-41). */

@@ -108,7 +108,8 @@ def gpu_grid_from_deck(mnc, gi, d, device=0):
 
 
 def mn_options(mnc, d, **over):
-    return mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"], **over)
+    return mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"],
+                       line_search=d.get("line_search", 0), **over)
 
 
 def config2_batch(w, p, xi, eta, nm, n, rng, near_boundary_frac=0.2, kappa=400.0, scale_k=3.0):

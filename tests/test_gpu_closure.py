@@ -102,3 +102,34 @@ def test_known_answer_alpha():
           % (np.median(eg), np.quantile(eg, 0.99), np.median(eo), np.quantile(eo, 0.99)))
     assert np.median(eg) <= 1e-9
     assert np.quantile(eg, 0.99) <= 10 * np.quantile(eo, 0.99) + 1e-12
+
+
+@pytest.mark.gpu
+def test_lattice_floor_cells_reference_and_robust():
+    """Near-isotropic floor cells of the M4 lattice at TOL 1e-8 (captured from
+    a lattice run at step 34, tests/golden/lattice_state): the reference line
+    search cannot resolve their last decrease and climbs the regularization
+    ladder, which the oracle shows too; the robust rule solves them at r = 0,
+    matching the oracle's robust mode."""
+    import os
+    from helpers import GOLDEN
+    mnc = load_pkg("mnc")
+    o = oracle()
+    z = np.load(os.path.join(GOLDEN, "lattice_state", "floor_cells.npz"))
+    u, a = z["u"], z["alpha"]
+    ctx = mnc.Context(4, 16, use_gaunt=False)
+    tab = o.Tables(4, 16, False)
+    sto, uo, ao, ito, ro, _ = tab.opt_batch(u, a, tol=1e-8, nthreads=8)
+    print("reference mode, oracle: r hist", [int((ro == x).sum()) for x in o.REGULARIZATIONS])
+    assert (ro >= 1e-2).sum() >= 2         # the ladder is climbed for the captured cells
+    opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, line_search=mnc.LS_ROBUST)
+    st, ug, ag, itg, rg, _ = ctx.opt_batch_host(u, a, opt)
+    sto, uo, ao, ito, ro, _ = tab.opt_batch(u, a, tol=1e-8, nthreads=8, line_search=1)
+    ctx.close()
+    assert st == 0 and sto == 0
+    assert np.all(rg == 0.0) and np.all(ro == 0.0)
+    same = itg == ito
+    print("robust mode: identical iteration counts %.4f, max iters %d" % (same.mean(), itg.max()))
+    assert same.mean() >= 0.99
+    rel = np.abs(ug - uo).max(axis=1) / np.abs(uo).max(axis=1)
+    assert rel.max() <= 1e-12
