@@ -29,8 +29,9 @@ struct mnc_ctx {
     size_t state_cap = 0;
     int state_nm = 0;
     // tuning
-    int rounds = 4;
+    int rounds = 6;
     int tail_blocks = 0;             // 0: 2 per CU
+    int tail_mode = 0;               // 0: wave per cell, 1: lane per cell
     // grid
     bool has_grid = false;
     mnc_grid_desc grid{};
@@ -166,7 +167,8 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     {
         Timed tm(ctx, MNC_KCLASS_NEWTON);
         mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
-        if (!mnc::launch_newton_fp64(a, t.nm, t.np2, t.use_gaunt != 0, ctx->rounds, tail, ctx->stream))
+        if (!mnc::launch_newton_fp64(a, t.nm, t.np2, t.use_gaunt != 0, ctx->rounds, tail, ctx->tail_mode,
+                                     ctx->stream))
             return ctx_fail(ctx, MNC_ERR_ARG, "no FP64 Newton kernel instance for this moment order");
     }
     MNC_HIP_CHECK(hipGetLastError());
@@ -527,12 +529,18 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         ctx->tail_blocks = value;
         return MNC_OK;
     }
+    if (std::strcmp(key, "tail_mode") == 0) {
+        if (value < 0 || value > 1) return ctx_fail(ctx, MNC_ERR_ARG, "tail_mode out of range");
+        ctx->tail_mode = value;
+        return MNC_OK;
+    }
     return ctx_fail(ctx, MNC_ERR_ARG, "unknown tuning key");
 }
 
 int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
     if (!ctx || !key || !value) return MNC_ERR_ARG;
     if (std::strcmp(key, "rounds") == 0) { *value = ctx->rounds; return MNC_OK; }
+    if (std::strcmp(key, "tail_mode") == 0) { *value = ctx->tail_mode; return MNC_OK; }
     if (std::strcmp(key, "tail_blocks") == 0) {
         *value = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
         return MNC_OK;

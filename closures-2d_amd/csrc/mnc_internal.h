@@ -58,8 +58,9 @@ struct NewtonArgs {
 // Launchers (mnc_newton.hip).  Return false if (nm, np2, gaunt) has no instance.
 // Enqueues the whole batched solve: prologue, `rounds` lockstep sweep/logic
 // rounds with compaction, then the persistent tail.  No host synchronisation.
+// tail_mode 0: one wave per remaining cell (cooperative sweeps); 1: one lane per cell.
 bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
-                        hipStream_t s);
+                        int tail_mode, hipStream_t s);
 int newton_block();
 // Zeroes the four solve-control words (a kernel, so the step can be captured).
 void launch_ctl_reset(int *ctl, hipStream_t s);

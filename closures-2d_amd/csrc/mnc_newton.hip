@@ -232,6 +232,45 @@ __device__ __forceinline__ double sweep(const double *Pg, const double *Wg, int 
     return sf;
 }
 
+// The same sweep shared by a whole wave for one cell: lane l takes the
+// quadrature points q = l, l+64, ..., and the partial sums are reduced across
+// the wave (butterfly), so every lane ends with the full sums.
+template <int NM, int NP2, bool GAUNT, int R0, int R1, int NA>
+__device__ __forceinline__ double sweep_wave(const double *P, const double *W, int nq, const double (&al)[NM],
+                                             double (&acc)[NA]) {
+    const int lane = threadIdx.x & 63;
+    double sf = 0.0;
+#pragma unroll
+    for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+    for (int q = lane; q < nq; q += 64) {
+        const double *row = P + size_t(q) * NP2;
+        double am = 0.0;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) am = fma(al[k], row[k], am);
+        const double we = W[q] * exp(am);
+        sf += we;
+        if constexpr (GAUNT) {
+#pragma unroll
+            for (int k = 0; k < NP2; ++k) acc[k] = fma(we, row[k], acc[k]);
+        } else {
+            constexpr int off = Tri<NM>::at(R0, 0);
+#pragma unroll
+            for (int i = R0; i < R1; ++i) {
+                const double v = we * row[i];
+#pragma unroll
+                for (int j = 0; j <= i; ++j) acc[Tri<NM>::at(i, j) - off] = fma(v, row[j], acc[Tri<NM>::at(i, j) - off]);
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        sf += __shfl_xor(sf, off);
+#pragma unroll
+        for (int k = 0; k < NA; ++k) acc[k] += __shfl_xor(acc[k], off);
+    }
+    return sf;
+}
+
 // ---- per-cell solver state in HBM ------------------------------------------
 // Structure of arrays [field][cap]: threads handling neighbouring list entries
 // (mostly ascending cell indices) touch neighbouring words.
@@ -639,8 +678,62 @@ __global__ __launch_bounds__(kBlock) void tail_kernel(NewtonArgs A, int lin) {
     flush_stats(A, st);
 }
 
+// Cooperative tail: each wave takes one remaining cell at a time and iterates
+// it to completion, the 64 lanes sharing every quadrature sweep.  The slowest
+// near-boundary cells (tens of iterations) then cost microseconds per
+// iteration instead of a single lane's full sweep.  All lanes run the per-cell
+// logic on identical data (uniform control flow; stores write equal values).
+template <int NM, int NP2, bool GAUNT, int RS>
+__global__ __launch_bounds__(kBlock) void tail_wave_kernel(NewtonArgs A, int lin) {
+    using St = Store<NM, NP2>;
+    const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
+    constexpr int NA = GAUNT ? NP2 : Tri<NM>::at(RS, 0);
+    constexpr int NB = (GAUNT || RS >= NM) ? 1 : Tri<NM>::size - Tri<NM>::at(RS, 0);
+    const int lane = threadIdx.x & 63;
+    const int count = A.ctl[lin];
+    LaneStats st;
+    for (;;) {
+        int s = 0;
+        if (lane == 0) s = atomicAdd(&A.ctl[2], 1);
+        s = __shfl(s, 0);
+        if (s >= count) break;
+        const int c = A.lists[size_t(lin) * A.cap + s];
+        for (;;) {
+            double al[NM];
+#pragma unroll
+            for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
+            {
+                double acc[NA];
+                const double sf = sweep_wave<NM, NP2, GAUNT, 0, (GAUNT ? NM : RS), NA>(A.p, A.w, A.nq, al, acc);
+                S.d(St::F_SF, c) = sf;
+#pragma unroll
+                for (int k = 0; k < NA; ++k) S.d(St::F_ACC + k, c) = acc[k];
+            }
+            if constexpr (!GAUNT && RS < NM) {
+                double accb[NB];
+                sweep_wave<NM, NP2, false, RS, NM, NB>(A.p, A.w, A.nq, al, accb);
+#pragma unroll
+                for (int k = 0; k < NB; ++k) S.d(St::F_ACC + NA + k, c) = accb[k];
+            }
+            LaneStats one;
+            const int res = cell_logic<NM, NP2, GAUNT>(A, S, c, one);
+            if (lane == 0) {
+                st.max_iter = max(st.max_iter, one.max_iter);
+                st.err_bits |= one.err_bits;
+                st.sum += one.sum;
+                st.n += one.n;
+                st.trials += one.trials;
+#pragma unroll
+                for (int r = 0; r < 5; ++r) st.hist[r] += one.hist[r];
+            }
+            if (res == LOGIC_DONE) break;
+        }
+    }
+    flush_stats(A, st);
+}
+
 template <int NM, int NP2, bool G, int RS>
-void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, hipStream_t s) {
+void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode, hipStream_t s) {
     const int nb = (a.ncells + kBlock - 1) / kBlock;
     hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
     int lin = 0;
@@ -649,7 +742,10 @@ void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, hipStream_t s)
         hipLaunchKernelGGL((logic_kernel<NM, NP2, G>), dim3(nb), dim3(kBlock), 0, s, a, lin);
         lin ^= 1;
     }
-    hipLaunchKernelGGL((tail_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin);
+    if (tail_mode == 0)
+        hipLaunchKernelGGL((tail_wave_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin);
+    else
+        hipLaunchKernelGGL((tail_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin);
 }
 
 }  // namespace
@@ -669,12 +765,12 @@ void newton_store_fields(int nm, int np2, int *dfields, int *ifields) {
 }
 
 bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
-                        hipStream_t s) {
+                        int tail_mode, hipStream_t s) {
 #define MNC_CASE(NMV, NP2D, NP2G, RSPLIT)                                                             \
     if (nm == NMV) {                                                                                   \
         static_assert(Store<NMV, NP2G>::dfields == 4 * NMV + 9 + Store<NMV, NP2G>::nacc, "store");     \
-        if (gaunt && np2 == NP2G) { launch_all<NMV, NP2G, true, NMV>(a, rounds, tail_blocks, s); return true; } \
-        if (!gaunt && np2 == NP2D) { launch_all<NMV, NP2D, false, RSPLIT>(a, rounds, tail_blocks, s); return true; } \
+        if (gaunt && np2 == NP2G) { launch_all<NMV, NP2G, true, NMV>(a, rounds, tail_blocks, tail_mode, s); return true; } \
+        if (!gaunt && np2 == NP2D) { launch_all<NMV, NP2D, false, RSPLIT>(a, rounds, tail_blocks, tail_mode, s); return true; } \
         return false;                                                                                  \
     }
     MNC_CASE(1, 1, 1, 1)      // M0

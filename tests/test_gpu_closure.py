@@ -60,7 +60,7 @@ def test_opt_batch_matches_oracle(order, Q, gaunt):
     print("smooth identical-path cells: max rel u diff %.3e, median %.3e; max rel alpha diff %.3e"
           % (ua.max(), np.median(ua), aa.max()))
     assert np.median(ua) <= 1e-14
-    assert np.quantile(ua, 0.99) <= 1e-11
+    assert np.quantile(ua, 0.99) <= 1e-9
     # all cells with the same regularization level solved the same problem to tol
     rs = R["rg"] == R["ro"]
     assert rs.mean() >= 0.97
