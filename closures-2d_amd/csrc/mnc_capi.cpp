@@ -530,7 +530,7 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         return MNC_OK;
     }
     if (std::strcmp(key, "tail_mode") == 0) {
-        if (value < 0 || value > 1) return ctx_fail(ctx, MNC_ERR_ARG, "tail_mode out of range");
+        if (value < 0 || value > 2) return ctx_fail(ctx, MNC_ERR_ARG, "tail_mode out of range");
         ctx->tail_mode = value;
         return MNC_OK;
     }

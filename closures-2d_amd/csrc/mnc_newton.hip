@@ -105,7 +105,7 @@ __device__ __forceinline__ int idamax(const double (&x)[N]) {
 // dpocon): Higham's estimate of ||A^{-1}||_1, with a single call site of the
 // A^{-1} product inside a small state machine to keep the unrolled code short.
 template <int N>
-__device__ double rcond(const double (&l)[Tri<N>::size], double anorm) {
+__device__ __forceinline__ double rcond(const double (&l)[Tri<N>::size], double anorm) {
     if (anorm == 0.0) return 0.0;
     double x[N];
     unsigned isgn = 0;   // bit i set <=> isgn_i = -1
@@ -113,6 +113,7 @@ __device__ double rcond(const double (&l)[Tri<N>::size], double anorm) {
     int jump = 1, j = 0, iter = 0;
 #pragma unroll
     for (int i = 0; i < N; ++i) x[i] = 1.0 / N;
+#pragma unroll 1
     for (int guard = 0; guard < 16; ++guard) {
         apply_inv<N>(l, x);
         if (jump == 1) {                       // first product
@@ -437,9 +438,11 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
             }
         }
     }
-    double g[NM];
+    double g0[NM];   // the gradient at convergence (reconstruction)
     if (newton) {
         ++iter; ++total;                                           // opt.cpp:212-213
+#pragma unroll
+        for (int k = 0; k < NM; ++k) S.d(St::F_AL + k, c) = al[k];
         double h[Tri<NM>::size];
         if constexpr (GAUNT) {                                     // fobj.cpp:99-113
 #pragma unroll
@@ -457,17 +460,26 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
 #pragma unroll
             for (int k = 0; k < Tri<NM>::size; ++k) h[k] = S.d(St::F_ACC + k, c);
         }
-        double fcur = h[0] / (p0 * p0);                            // fobj.cpp:270-286
-        double fscale = fabs(fcur), gg = 0.0;
+        // Phase 1: g, f and the 1-norm from H (fobj.cpp:270-286, utils.cpp:36-52).
+        // g goes to the gradient slot of the state (F_D doubles as it until the
+        // direction is formed), so only the factor stays live below.
+        {
+            double fcur = h[0] / (p0 * p0);
+            double fscale = fabs(fcur), gg = 0.0;
 #pragma unroll
-        for (int i = 0; i < NM; ++i) {
-            const double ui = S.d(St::F_U + i, c);
-            g[i] = h[Tri<NM>::at(i, 0)] / p0 - ui;
-            gg = fma(g[i], g[i], gg);
-            fcur = fcur - al[i] * ui;
-            fscale += fabs(al[i] * ui);
+            for (int i = 0; i < NM; ++i) {
+                const double ui = S.d(St::F_U + i, c);
+                const double gi = h[Tri<NM>::at(i, 0)] / p0 - ui;
+                S.d(St::F_D + i, c) = gi;
+                gg = fma(gi, gi, gg);
+                fcur = fcur - al[i] * ui;
+                fscale += fabs(al[i] * ui);
+            }
+            S.d(St::F_FCUR, c) = fcur;
+            S.d(St::F_FSCALE, c) = fscale;
+            err = sqrt(gg);                                        // dnrm2 (:286)
         }
-        double anorm = 0.0;                                        // utils.cpp:36-52
+        double anorm = 0.0;
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
             double s = 0.0;
@@ -475,6 +487,7 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
             for (int j = 0; j < NM; ++j) s += fabs(h[i >= j ? Tri<NM>::at(i, j) : Tri<NM>::at(j, i)]);
             anorm = s > anorm ? s : anorm;
         }
+        // Phase 2: factor, condition estimate, direction (dpotrf, dpocon, dpotrs).
         const bool chol_ok = cholesky<NM>(h);
         double condest = 0.0;
         if (chol_ok) condest = 1.0 / rcond<NM>(h, anorm);
@@ -483,27 +496,30 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
         } else {
             double d[NM];
 #pragma unroll
-            for (int i = 0; i < NM; ++i) d[i] = g[i];
+            for (int i = 0; i < NM; ++i) d[i] = S.d(St::F_D + i, c);
             apply_inv<NM>(h, d);                                   // dpotrs (:278-282)
-            err = sqrt(gg);                                        // dnrm2 (:286)
+            // Phase 3: stop test, or the line search set-up (linesearch.cpp:28-43)
             if (err <= S.d(St::F_RTOL, c)) {
                 converged = true;                                  // :293-306
-            } else {                                               // line search set-up (linesearch.cpp:28-43)
+#pragma unroll
+                for (int i = 0; i < NM; ++i) g0[i] = S.d(St::F_D + i, c);
+            } else {
                 double maxf = 0.0, gd = 0.0;
 #pragma unroll
                 for (int i = 0; i < NM; ++i) {
+                    const double ai = S.d(St::F_AL + i, c);
+                    const double gi = S.d(St::F_D + i, c);
                     const double di = -d[i];
-                    const double tmp = fabs(di) / (fabs(al[i]) + 1e-100);
+                    const double tmp = fabs(di) / (fabs(ai) + 1e-100);
                     maxf = tmp > maxf ? tmp : maxf;
-                    gd = fma(g[i], di, gd);
-                    S.d(St::F_AOLD + i, c) = al[i];
+                    gd = fma(gi, di, gd);
+                    S.d(St::F_AOLD + i, c) = ai;
                     S.d(St::F_D + i, c) = di;
+                    al[i] = ai;
                     d[i] = di;
                 }
                 S.d(St::F_MAXF, c) = maxf;
                 S.d(St::F_GD, c) = gd;
-                S.d(St::F_FCUR, c) = fcur;
-                S.d(St::F_FSCALE, c) = fscale;
                 t = 1.0;
                 if (t * maxf > DBL_EPSILON / 2) {
 #pragma unroll
@@ -517,6 +533,10 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
                     do_reg = true;
                 }
             }
+        }
+        if (!converged && mode != ST_TRIAL) {
+#pragma unroll
+            for (int i = 0; i < NM; ++i) al[i] = S.d(St::F_AL + i, c);
         }
     }
     int result = LOGIC_CONTINUE;
@@ -549,11 +569,13 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
         }
     } else if (converged) {
         const double u0 = S.d(St::F_U0, c);
-        const double u0p = S.d(St::F_U + 0, c) + g[0];
+#pragma unroll
+        for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
+        const double u0p = S.d(St::F_U + 0, c) + g0[0];
         double *uo = A.u + size_t(c) * NM;
         double *ao = A.alpha + size_t(c) * NM;
 #pragma unroll
-        for (int k = 0; k < NM; ++k) uo[k] = ((S.d(St::F_U + k, c) + g[k]) / u0p) * u0;   // :298-301, opt.cpp:107-108
+        for (int k = 0; k < NM; ++k) uo[k] = ((S.d(St::F_U + k, c) + g0[k]) / u0p) * u0;   // :298-301, opt.cpp:107-108
         al[0] -= log(u0p) * 2.0 * kSqrtPi;                                               // :302
         al[0] = al[0] + S.d(St::F_ASCALE, c);                                            // opt.cpp:109
 #pragma unroll
@@ -732,6 +754,384 @@ __global__ __launch_bounds__(kBlock) void tail_wave_kernel(NewtonArgs A, int lin
     flush_stats(A, st);
 }
 
+// ---- cooperative tail: one wave per cell, dense algebra in LDS -------------
+//
+// For the few cells still iterating after the lockstep rounds, a wave owns one
+// cell: lane l sweeps the quadrature points q = l, l+64, ...; the partial sums
+// are reduced through LDS; the Hessian and its factor live in LDS and lane i
+// owns row i of the Cholesky factorisation and of the triangular solves.  The
+// order of every sum is the serial one (the column-oriented back substitution
+// subtracts x_{n-1}, x_{n-2}, ... in the order of the dot-product form), so a
+// cell takes the same path here as in the lockstep kernels up to FMA rounding.
+template <int NM, int NP2, bool GAUNT, int RS>
+struct CoopShared {
+    static constexpr int NA = GAUNT ? NP2 : Tri<NM>::at(RS, 0);
+    static constexpr int NB = (GAUNT || RS >= NM) ? 1 : Tri<NM>::size - Tri<NM>::at(RS, 0);
+    static constexpr int NPART = NA > NB ? NA : NB;
+    static constexpr int NACC = Store<NM, NP2>::nacc;
+    double part[NPART][65];   // padded: lane-contiguous writes, column sums conflict-free
+    double accs[NACC];        // reduced sweep sums
+    double L[NM][NM];         // H, then its Cholesky factor (lower)
+    double x[NM], gvec[NM];
+    double red[64];
+};
+
+// x <- L^{-1} x then L^{-T} x, lane i owning x_i (i < NM); division as dtrsv.
+template <int NM>
+__device__ __forceinline__ void coop_apply_inv(double (*L)[NM], double *x, int lane) {
+    for (int j = 0; j < NM; ++j) {
+        if (lane == j) x[j] = x[j] / L[j][j];
+        __builtin_amdgcn_wave_barrier();
+        const double xj = x[j];
+        if (lane > j && lane < NM) x[lane] = fma(-xj, L[lane][j], x[lane]);
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (int j = NM - 1; j >= 0; --j) {
+        if (lane == j) x[j] = x[j] / L[j][j];
+        __builtin_amdgcn_wave_barrier();
+        const double xj = x[j];
+        if (lane < j) x[lane] = fma(-L[j][lane], xj, x[lane]);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Cholesky in place on the lower triangle (dpotf2 order); uniform result.
+template <int NM>
+__device__ __forceinline__ bool coop_cholesky(double (*L)[NM], int lane) {
+    bool ok = true;
+    for (int j = 0; j < NM; ++j) {
+        double dot = 0.0;
+        for (int k = 0; k < j; ++k) dot = fma(L[j][k], L[j][k], dot);
+        double ajj = L[j][j] - dot;
+        ok = ok && (ajj > 0.0);
+        ajj = sqrt(ajj > 0.0 ? ajj : 1.0);
+        double y = 0.0;
+        if (lane > j && lane < NM) {
+            y = L[lane][j];
+            for (int k = 0; k < j; ++k) y = fma(-L[j][k], L[lane][k], y);
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane == j) L[j][j] = ajj;
+        const double r = 1.0 / ajj;
+        if (lane > j && lane < NM) L[lane][j] = y * r;
+        __builtin_amdgcn_wave_barrier();
+    }
+    return ok;
+}
+
+// Higham's estimate of ||A^{-1}||_1 (dpocon's role); every lane ends with it.
+template <int NM>
+__device__ __forceinline__ double coop_rcond(double (*L)[NM], double *x, double anorm, int lane) {
+    if (anorm == 0.0) return 0.0;
+    unsigned isgn = 0;
+    double est = 0.0;
+    int jump = 1, j = 0, iter = 0;
+    if (lane < NM) x[lane] = 1.0 / NM;
+    __builtin_amdgcn_wave_barrier();
+    for (int guard = 0; guard < 16; ++guard) {
+        coop_apply_inv<NM>(L, x, lane);
+        double s = 0.0, m = -1.0;
+        unsigned sg = 0;
+        int im = 0;
+        for (int i = 0; i < NM; ++i) {
+            const double v = x[i];
+            s += fabs(v);
+            if (fabs(v) > m) { m = fabs(v); im = i; }
+            sg |= !(v >= 0.0) ? (1u << i) : 0u;
+        }
+        const double xj_old = x[j];
+        __builtin_amdgcn_wave_barrier();
+        double nx = 0.0;
+        bool done = false;
+        if (jump == 1) {
+            if (NM == 1) { est = fabs(x[0]); done = true; }
+            else { est = s; isgn = sg; nx = (lane < NM && (sg >> lane & 1u)) ? -1.0 : 1.0; jump = 2; }
+        } else if (jump == 2) {
+            j = im; iter = 2; nx = (lane == j) ? 1.0 : 0.0; jump = 3;
+        } else if (jump == 3) {
+            const double estold = est;
+            est = s;
+            if (sg == isgn || est <= estold) {
+                nx = ((lane & 1) ? -1.0 : 1.0) * (1.0 + double(lane) / double(NM - 1));
+                jump = 5;
+            } else {
+                nx = (lane < NM && (sg >> lane & 1u)) ? -1.0 : 1.0;
+                isgn = sg;
+                jump = 4;
+            }
+        } else if (jump == 4) {
+            const int jlast = j;
+            j = im;
+            (void)jlast;   // xj_old is x[jlast], read above before j moved
+            if (xj_old != m && iter < 5) {
+                ++iter; nx = (lane == j) ? 1.0 : 0.0; jump = 3;
+            } else {
+                nx = ((lane & 1) ? -1.0 : 1.0) * (1.0 + double(lane) / double(NM - 1));
+                jump = 5;
+            }
+        } else {
+            const double temp = 2.0 * (s / double(3 * NM));
+            if (temp > est) est = temp;
+            done = true;
+        }
+        if (done) break;
+        if (lane < NM) x[lane] = nx;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (!isfinite(est) || est == 0.0) return 0.0;
+    return (1.0 / est) / anorm;
+}
+
+template <int NM, int NP2, bool GAUNT, int RS>
+__global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
+    using St = Store<NM, NP2>;
+    using CS = CoopShared<NM, NP2, GAUNT, RS>;
+    constexpr int NA = CS::NA, NB = CS::NB;
+    __shared__ CS sh;
+    const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
+    const int lane = threadIdx.x & 63;
+    const int count = A.ctl[lin];
+    const double p0 = A.p[0];
+    LaneStats st;
+    for (;;) {
+        int s = 0;
+        if (lane == 0) s = atomicAdd(&A.ctl[2], 1);
+        s = __shfl(s, 0);
+        if (s >= count) break;
+        const int c = A.lists[size_t(lin) * A.cap + s];
+        for (;;) {
+            double al[NM];
+#pragma unroll
+            for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
+            // ---- sweep(s): partial sums per lane, reduced through LDS ----
+            double sf = 0.0;
+#pragma unroll
+            for (int pass = 0; pass < ((!GAUNT && RS < NM) ? 2 : 1); ++pass) {
+                const int R0 = pass == 0 ? 0 : RS, R1 = pass == 0 ? (GAUNT ? NM : RS) : NM;
+                const int nacc = pass == 0 ? NA : NB;
+                const int off = Tri<NM>::at(R0, 0);
+                double acc[CS::NPART];
+#pragma unroll
+                for (int k = 0; k < CS::NPART; ++k) acc[k] = 0.0;
+                for (int q = lane; q < A.nq; q += 64) {
+                    const double *row = A.p + size_t(q) * NP2;
+                    double am = 0.0;
+#pragma unroll
+                    for (int k = 0; k < NM; ++k) am = fma(al[k], row[k], am);
+                    const double we = A.w[q] * exp(am);
+                    if (pass == 0) sf += we;
+                    if constexpr (GAUNT) {
+#pragma unroll
+                        for (int k = 0; k < NP2; ++k) acc[k] = fma(we, row[k], acc[k]);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < NM; ++i) {
+                            if (i < R0 || i >= R1) continue;
+                            const double v = we * row[i];
+#pragma unroll
+                            for (int jj = 0; jj <= i; ++jj) {
+                                const int k = Tri<NM>::at(i, jj) - off;
+                                if (k >= 0 && k < CS::NPART) acc[k] = fma(v, row[jj], acc[k]);
+                            }
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < CS::NPART; ++k)
+                    if (k < nacc) sh.part[k][lane] = acc[k];
+                __builtin_amdgcn_wave_barrier();
+                for (int k = lane; k < nacc; k += 64) {
+                    double tot = 0.0;
+                    for (int l2 = 0; l2 < 64; ++l2) tot += sh.part[k][l2];
+                    sh.accs[off + k] = tot;
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) sf += __shfl_xor(sf, o);
+            __builtin_amdgcn_wave_barrier();
+            // ---- per-cell logic (cell_logic's steps, cooperative) ----
+            int mode = S.i(St::I_MODE, c), iter = S.i(St::I_ITER, c), total = S.i(St::I_TOTAL, c);
+            int reg = S.i(St::I_REG, c), maxit = S.i(St::I_MAXIT, c);
+            double t = S.d(St::F_T, c);
+            bool newton = (mode == ST_NEWTON), do_reg = false, converged = false;
+            double err = 0.0;
+            if (mode == ST_TRIAL) {
+                if (lane == 0) ++st.trials;
+                double fnew = sf;
+#pragma unroll
+                for (int k = 0; k < NM; ++k) fnew = fnew - al[k] * S.d(St::F_U + k, c);
+                const double fcur = S.d(St::F_FCUR, c), gd = S.d(St::F_GD, c);
+                bool accept = fnew <= fcur + 1e-3 * t * gd;
+                if (!accept && A.ls_mode == MNC_LS_ROBUST && t == 1.0 &&
+                    -gd <= 1e3 * DBL_EPSILON * S.d(St::F_FSCALE, c))
+                    accept = true;
+                if (accept) {
+                    newton = true;
+                } else {
+                    t = 0.5 * t;
+                    if (t * S.d(St::F_MAXF, c) > DBL_EPSILON / 2) {
+#pragma unroll
+                        for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AOLD + k, c) + t * S.d(St::F_D + k, c);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AOLD + k, c);
+                        t = 0.0;
+                        ++iter; ++total;
+                        do_reg = true;
+                    }
+                }
+            }
+            if (newton) {
+                ++iter; ++total;
+                // H into LDS (full symmetric)
+                if (lane < NM) {
+                    for (int jj = 0; jj < NM; ++jj) {
+                        const int i0 = lane > jj ? lane : jj, j0 = lane > jj ? jj : lane;
+                        double hij;
+                        if constexpr (GAUNT) {
+                            const int pr = j0 * NM + i0;
+                            hij = 0.0;
+                            for (int cc = A.gptr[pr]; cc < A.gptr[pr + 1]; ++cc)
+                                hij = fma(A.gv[cc], sh.accs[A.gk[cc]], hij);
+                        } else {
+                            hij = sh.accs[Tri<NM>::at(i0, j0)];
+                        }
+                        sh.L[lane][jj] = hij;
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                double fcur = sh.L[0][0] / (p0 * p0), fscale = fabs(fcur), gg = 0.0, anorm = 0.0;
+                for (int i = 0; i < NM; ++i) {
+                    const double ui = S.d(St::F_U + i, c);
+                    const double gi = sh.L[i][0] / p0 - ui;
+                    gg = fma(gi, gi, gg);
+                    fcur = fcur - al[i] * ui;
+                    fscale += fabs(al[i] * ui);
+                    if (lane == i) sh.gvec[i] = gi;
+                    double rs = 0.0;
+                    for (int jj = 0; jj < NM; ++jj) rs += fabs(sh.L[i][jj]);
+                    anorm = rs > anorm ? rs : anorm;
+                }
+                err = sqrt(gg);
+                __builtin_amdgcn_wave_barrier();
+                const bool chol_ok = coop_cholesky<NM>(sh.L, lane);
+                double condest = 0.0;
+                if (chol_ok) condest = 1.0 / coop_rcond<NM>(sh.L, sh.x, anorm, lane);
+                if (iter >= maxit || !chol_ok || t == 0.0 || condest > A.cond_h_max) {
+                    do_reg = true;
+                } else {
+                    if (lane < NM) sh.x[lane] = sh.gvec[lane];
+                    __builtin_amdgcn_wave_barrier();
+                    coop_apply_inv<NM>(sh.L, sh.x, lane);
+                    if (err <= S.d(St::F_RTOL, c)) {
+                        converged = true;
+                    } else {
+                        double maxf = 0.0, gd = 0.0;
+                        for (int i = 0; i < NM; ++i) {
+                            const double di = -sh.x[i];
+                            const double tmp = fabs(di) / (fabs(al[i]) + 1e-100);
+                            maxf = tmp > maxf ? tmp : maxf;
+                            gd = fma(sh.gvec[i], di, gd);
+                        }
+                        for (int i = 0; i < NM; ++i) {
+                            S.d(St::F_AOLD + i, c) = al[i];
+                            S.d(St::F_D + i, c) = -sh.x[i];
+                        }
+                        S.d(St::F_MAXF, c) = maxf;
+                        S.d(St::F_GD, c) = gd;
+                        S.d(St::F_FCUR, c) = fcur;
+                        S.d(St::F_FSCALE, c) = fscale;
+                        t = 1.0;
+                        if (t * maxf > DBL_EPSILON / 2) {
+#pragma unroll
+                            for (int i = 0; i < NM; ++i) al[i] = al[i] + t * (-sh.x[i]);
+                            mode = ST_TRIAL;
+                        } else {
+                            t = 0.0;
+                            ++iter; ++total;
+                            do_reg = true;
+                        }
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            bool done = false;
+            if (do_reg) {
+                if (reg < 4) {
+                    ++reg;
+                    const double r = kReg[reg];
+                    const double *uin = A.u + size_t(c) * NM;
+                    const double u0 = S.d(St::F_U0, c);
+                    double ss = 1.0;
+                    for (int k = 1; k < NM; ++k) {
+                        const double uk = (uin[k] / u0) * (1.0 - r);
+                        ss = fma(uk, uk, ss);
+                        S.d(St::F_U + k, c) = uk;
+                    }
+                    S.d(St::F_U, c) = 1.0;
+                    S.d(St::F_RTOL, c) = sqrt(ss) * A.tol;
+                    al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
+#pragma unroll
+                    for (int k = 1; k < NM; ++k) al[k] = 0.0;
+                    iter = 0;
+                    t = 1.0;
+                    if (reg == 4) maxit = 10 * maxit;
+                    mode = ST_NEWTON;
+                } else {
+                    if (lane == 0) {
+                        st.err_bits |= 1 << MNC_ERR_MAXREG;
+                        if (A.outs) { A.outs[c].iter = total; A.outs[c].r = -1.0; A.outs[c].norm_g = err; }
+                    }
+                    done = true;
+                }
+            } else if (converged) {
+                const double u0 = S.d(St::F_U0, c);
+                const double u0p = S.d(St::F_U, c) + sh.gvec[0];
+                double a0 = al[0];
+                a0 -= log(u0p) * 2.0 * kSqrtPi;
+                a0 = a0 + S.d(St::F_ASCALE, c);
+                if (lane < NM) {
+                    A.u[size_t(c) * NM + lane] = ((S.d(St::F_U + lane, c) + sh.gvec[lane]) / u0p) * u0;
+                    double ak = 0.0;
+#pragma unroll
+                    for (int kk = 0; kk < NM; ++kk) ak = (kk == lane) ? al[kk] : ak;
+                    A.alpha[size_t(c) * NM + lane] = lane == 0 ? a0 : ak;
+                }
+                if (lane == 0) {
+                    if (A.outs) { A.outs[c].iter = total; A.outs[c].r = kReg[reg]; A.outs[c].norm_g = err; }
+                    st.max_iter = total > st.max_iter ? total : st.max_iter;
+                    st.sum += total;
+                    ++st.n;
+                    ++st.hist[reg];
+                }
+                done = true;
+            }
+            if (!done) {
+#pragma unroll
+                for (int k = 0; k < NM; ++k) S.d(St::F_AL + k, c) = al[k];
+                S.d(St::F_T, c) = t;
+                S.i(St::I_ITER, c) = iter;
+                S.i(St::I_TOTAL, c) = total;
+                S.i(St::I_REG, c) = reg;
+                S.i(St::I_MAXIT, c) = maxit;
+                S.i(St::I_MODE, c) = mode;
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (done) break;
+        }
+    }
+    // statistics: only lane 0 counted; one wave per block
+    if (lane == 0 && (st.n || st.trials || st.err_bits)) {
+        atomicMax((unsigned long long *)&A.stats[0], (unsigned long long)st.max_iter);
+        atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)st.sum);
+        atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)st.n);
+        for (int r = 0; r < 5; ++r) atomicAdd((unsigned long long *)&A.stats[3 + r], (unsigned long long)st.hist[r]);
+        atomicAdd((unsigned long long *)&A.stats[8], (unsigned long long)st.trials);
+        if (st.err_bits) atomicOr(&A.ctl[3], st.err_bits);
+    }
+}
+
 template <int NM, int NP2, bool G, int RS>
 void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode, hipStream_t s) {
     const int nb = (a.ncells + kBlock - 1) / kBlock;
@@ -743,6 +1143,8 @@ void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode,
         lin ^= 1;
     }
     if (tail_mode == 0)
+        hipLaunchKernelGGL((tail_coop_kernel<NM, NP2, G, RS>), dim3(4 * tail_blocks), dim3(64), 0, s, a, lin);
+    else if (tail_mode == 2)
         hipLaunchKernelGGL((tail_wave_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin);
     else
         hipLaunchKernelGGL((tail_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin);

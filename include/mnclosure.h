@@ -159,7 +159,8 @@ int mnc_reset_stats(mnc_ctx *ctx);
 /* Tuning knobs of the batched solve (set by the autotuner or by hand):
  *   "rounds"       lockstep sweep/logic rounds before the persistent tail (0-64)
  *   "tail_blocks"  workgroups of the persistent tail kernel (1-65536)
- *   "tail_mode"    0: one wave per remaining cell (cooperative sweeps), 1: one lane per cell
+ *   "tail_mode"    0: one wave per remaining cell, dense algebra shared in LDS;
+ *                  1: one lane per cell; 2: one wave per cell, algebra per lane
  * Returns MNC_ERR_ARG for an unknown key or an out-of-range value. */
 int mnc_tune(mnc_ctx *ctx, const char *key, int value);
 int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value);
