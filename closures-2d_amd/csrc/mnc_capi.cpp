@@ -21,17 +21,25 @@ struct mnc_ctx {
     double *d_w = nullptr, *d_p = nullptr, *d_xi = nullptr, *d_eta = nullptr, *d_gv = nullptr;
     int *d_gptr = nullptr, *d_gk = nullptr;
     // solver state / stats
-    int *d_ctl = nullptr;            // [0],[1] list counts, [2] tail queue, [3] error bits
-    long long *d_stats = nullptr;    // [9]
+    int *d_ctl = nullptr;            // [0],[1] list counts, [2] tail queue, [3] error bits, [4] deferred
+    long long *d_stats = nullptr;    // [kStatWords]
     double *d_std = nullptr;         // per-cell solver state (doubles)
     int *d_sti = nullptr;            // per-cell solver state (ints)
-    int *d_lists = nullptr;          // two active lists
+    int *d_lists = nullptr;          // two active lists and the deferred list
+    // mixed precision operand tables (absent when the order/quadrature has none)
+    mnc::MixTables mix;
+    bool has_mix = false;
+    double *d_mz = nullptr, *d_mg = nullptr, *d_mgv = nullptr;
+    unsigned short *d_mh = nullptr;
+    int *d_mgptr = nullptr, *d_mgk = nullptr;
     size_t state_cap = 0;
     int state_nm = 0;
     // tuning
     int rounds = 6;
     int tail_blocks = 0;             // 0: 2 per CU
-    int tail_mode = 0;               // 0: wave per cell, 1: lane per cell
+    int tail_mode = 0;               // 0: wave per cell, 1: lane per cell, 2: wave sweep + serial logic
+    int mix_cond_log10 = 4;          // mixed steps need cond(H) <= 10^this, else FP64
+    int mix_blocks = 0;              // 0: one per CU
     // grid
     bool has_grid = false;
     mnc_grid_desc grid{};
@@ -42,6 +50,16 @@ struct mnc_ctx {
     double *d_bu = nullptr, *d_ba = nullptr;
     mnc_outs *d_bo = nullptr;
     size_t batch_cap = 0;
+    // learned warm start
+    bool mlp_ready = false, mlp_stats_set = false;
+    int mlp_step = 0;
+    float *d_mlp = nullptr;          // parameters + standardisation, grad, m, v (4 blocks)
+    float *d_mlp_scratch = nullptr, *d_mlp_loss = nullptr;
+    size_t mlp_scratch_cap = 0, mlp_loss_cap = 0;
+    double *d_mlp_io = nullptr;      // host-call staging (inputs, targets/outputs)
+    size_t mlp_io_cap = 0;
+    double *d_us = nullptr, *d_guess = nullptr;   // Newton prologue: u/u0 and the MLP guesses
+    size_t guess_cap = 0;
     // timing
     bool timing = false;
     struct Ev { int kc; hipEvent_t a, b; };
@@ -100,7 +118,7 @@ int ensure_state(mnc_ctx *ctx, size_t ncells) {
     const size_t cap = std::max<size_t>(ncells, 64);
     int st;
     if ((st = dalloc(ctx, &ctx->d_std, size_t(df) * cap)) || (st = dalloc(ctx, &ctx->d_sti, size_t(ifl) * cap)) ||
-        (st = dalloc(ctx, &ctx->d_lists, 2 * cap)))
+        (st = dalloc(ctx, &ctx->d_lists, 3 * cap)))
         return st;
     ctx->state_cap = cap;
     return MNC_OK;
@@ -130,16 +148,86 @@ int check_opts(mnc_ctx *ctx, const mnc_options *o) {
     return MNC_OK;
 }
 
+// ---- learned warm start -------------------------------------------------------
+constexpr int kMlpChunk = 65536;   // rows per forward pass of the Newton prologue / predict
+
+mnc::MlpBuffers mlp_buffers(mnc_ctx *ctx) {
+    const size_t np = size_t(mnc::mlp_param_count()) + mnc::kMlpStats;
+    return {ctx->d_mlp, ctx->d_mlp + np, ctx->d_mlp + 2 * np, ctx->d_mlp + 3 * np, ctx->d_mlp_scratch,
+            ctx->d_mlp_loss};
+}
+
+int mlp_alloc(mnc_ctx *ctx, int rows, size_t losses) {
+    int st;
+    if (!ctx->d_mlp &&
+        (st = dalloc(ctx, &ctx->d_mlp, 4 * (size_t(mnc::mlp_param_count()) + mnc::kMlpStats))))
+        return st;
+    const size_t need = mnc::mlp_scratch_floats(rows);
+    if (need > ctx->mlp_scratch_cap) {
+        if ((st = dalloc(ctx, &ctx->d_mlp_scratch, need))) return st;
+        ctx->mlp_scratch_cap = need;
+    }
+    if (losses > ctx->mlp_loss_cap) {
+        if ((st = dalloc(ctx, &ctx->d_mlp_loss, losses))) return st;
+        ctx->mlp_loss_cap = losses;
+    }
+    return MNC_OK;
+}
+
+int mlp_io(mnc_ctx *ctx, size_t doubles) {
+    if (doubles <= ctx->mlp_io_cap) return MNC_OK;
+    int st = dalloc(ctx, &ctx->d_mlp_io, doubles);
+    if (st) return st;
+    ctx->mlp_io_cap = doubles;
+    return MNC_OK;
+}
+
+int mlp_check(mnc_ctx *ctx, bool need_ready) {
+    if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
+    if (ctx->tab.nm != 15) return ctx_fail(ctx, MNC_ERR_ARG, "the learned warm start exists for moment order 4 only");
+    if (need_ready && !ctx->mlp_ready)
+        return ctx_fail(ctx, MNC_ERR_STATE, "no MLP parameters (mnc_mlp_init/mnc_mlp_set_params first)");
+    return MNC_OK;
+}
+
+// Scaled multipliers predicted for n rows of scaled moments (device arrays).
+int mlp_predict_dev(mnc_ctx *ctx, int n, const double *us, double *alpha) {
+    int st = mlp_alloc(ctx, std::min(n, kMlpChunk), 1);
+    if (st) return st;
+    const auto b = mlp_buffers(ctx);
+    for (int i0 = 0; i0 < n; i0 += kMlpChunk) {
+        const int rows = std::min(kMlpChunk, n - i0);
+        mnc::mlp_gather(b, us, nullptr, n, rows, 0, i0, ctx->stream);
+        mnc::mlp_forward(b, rows, ctx->stream);
+        mnc::mlp_unscale(b, rows, alpha + size_t(i0) * 15, ctx->stream);
+    }
+    MNC_HIP_CHECK(hipGetLastError());
+    return MNC_OK;
+}
+
 // Enqueues the batched Newton on device arrays (stats accumulate in d_stats).
 // With check, synchronises and turns the kernels' error bits into a status.
 int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_options *o, mnc_outs *outs,
                const double *alpha_guess, bool check) {
     if (ncells <= 0) return MNC_OK;
     const auto &t = ctx->tab;
-    if (o->precision != MNC_PREC_FP64)
-        return ctx_fail(ctx, MNC_ERR_ARG, "mixed precision requested but not available for this order");
+    const bool mixed = o->precision == MNC_PREC_MIXED;
+    if (mixed && !ctx->has_mix)
+        return ctx_fail(ctx, MNC_ERR_ARG, "mixed precision needs moment order <= 4 and quad order <= 16");
     int st = ensure_state(ctx, size_t(ncells));
     if (st) return st;
+    if (o->warm_start == MNC_WARM_LEARNED) {    // the network's guesses for every cell
+        if ((st = mlp_check(ctx, true))) return st;
+        if (size_t(ncells) > ctx->guess_cap) {
+            if ((st = dalloc(ctx, &ctx->d_us, size_t(ncells) * 15)) || (st = dalloc(ctx, &ctx->d_guess, size_t(ncells) * 15)))
+                return st;
+            ctx->guess_cap = size_t(ncells);
+        }
+        Timed tm(ctx, MNC_KCLASS_MLP);
+        mnc::mlp_scale_moments(u, ncells, ctx->d_us, ctx->stream);
+        if ((st = mlp_predict_dev(ctx, ncells, ctx->d_us, ctx->d_guess))) return st;
+        alpha_guess = ctx->d_guess;
+    }
     mnc::NewtonArgs a{};
     a.ncells = ncells;
     a.u = u;
@@ -163,13 +251,27 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     a.lists = ctx->d_lists;
     a.ctl = ctx->d_ctl;
     a.cap = ctx->state_cap;
+    a.mz = ctx->d_mz;
+    a.mg = ctx->d_mg;
+    a.mh = ctx->d_mh;
+    a.mnt = ctx->mix.nt;
+    a.mgptr = ctx->d_mgptr;
+    a.mgk = ctx->d_mgk;
+    a.mgv = ctx->d_mgv;
+    a.cmix = std::pow(10.0, ctx->mix_cond_log10);
     const int tail = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
     {
         Timed tm(ctx, MNC_KCLASS_NEWTON);
         mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
-        if (!mnc::launch_newton_fp64(a, t.nm, t.np2, t.use_gaunt != 0, ctx->rounds, tail, ctx->tail_mode,
-                                     ctx->stream))
+        if (mixed) {
+            const int mb = ctx->mix_blocks > 0 ? ctx->mix_blocks : ctx->num_cu;
+            if (!mnc::launch_newton_mixed(a, t.nm, t.np2, t.use_gaunt != 0, ctx->rounds, tail, ctx->tail_mode, mb,
+                                          ctx->stream))
+                return ctx_fail(ctx, MNC_ERR_ARG, "no mixed-precision Newton kernel instance for this order");
+        } else if (!mnc::launch_newton_fp64(a, t.nm, t.np2, t.use_gaunt != 0, ctx->rounds, tail, ctx->tail_mode,
+                                            ctx->stream)) {
             return ctx_fail(ctx, MNC_ERR_ARG, "no FP64 Newton kernel instance for this moment order");
+        }
     }
     MNC_HIP_CHECK(hipGetLastError());
     if (!check) return MNC_OK;
@@ -259,12 +361,21 @@ int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
     };
     if (up(&ctx->d_w, t.w) || up(&ctx->d_p, t.p) || up(&ctx->d_xi, t.xi) || up(&ctx->d_eta, t.eta) ||
         up(&ctx->d_gptr, t.gptr) || up(&ctx->d_gk, t.gk) || up(&ctx->d_gv, t.gv) ||
-        hipMalloc((void **)&ctx->d_ctl, 4 * sizeof(int)) != hipSuccess ||
-        hipMemset(ctx->d_ctl, 0, 4 * sizeof(int)) != hipSuccess ||
-        hipMalloc((void **)&ctx->d_stats, 9 * sizeof(long long)) != hipSuccess ||
-        hipMemset(ctx->d_stats, 0, 9 * sizeof(long long)) != hipSuccess) {
+        hipMalloc((void **)&ctx->d_ctl, mnc::kCtlWords * sizeof(int)) != hipSuccess ||
+        hipMemset(ctx->d_ctl, 0, mnc::kCtlWords * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&ctx->d_stats, mnc::kStatWords * sizeof(long long)) != hipSuccess ||
+        hipMemset(ctx->d_stats, 0, mnc::kStatWords * sizeof(long long)) != hipSuccess) {
         mnc_destroy(ctx);
         return MNC_ERR_HIP;
+    }
+    if (mnc::build_mix_tables(setup->moment_order, setup->quad_order, ctx->mix) == MNC_OK &&
+        ctx->mix.nnt == mnc::mix_tiles(t.nm)) {
+        if (up(&ctx->d_mz, ctx->mix.z) || up(&ctx->d_mg, ctx->mix.g) || up(&ctx->d_mh, ctx->mix.h) ||
+            up(&ctx->d_mgptr, ctx->mix.gptr) || up(&ctx->d_mgk, ctx->mix.gk) || up(&ctx->d_mgv, ctx->mix.gv)) {
+            mnc_destroy(ctx);
+            return MNC_ERR_HIP;
+        }
+        ctx->has_mix = true;
     }
     *out = ctx;
     return MNC_OK;
@@ -277,6 +388,8 @@ void mnc_destroy(mnc_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &ev : ctx->events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
     void *ptrs[] = {ctx->d_w, ctx->d_p, ctx->d_xi, ctx->d_eta, ctx->d_gv, ctx->d_gptr, ctx->d_gk,
+                    ctx->d_mz, ctx->d_mg, ctx->d_mh, ctx->d_mgptr, ctx->d_mgk, ctx->d_mgv,
+                    ctx->d_mlp, ctx->d_mlp_scratch, ctx->d_mlp_loss, ctx->d_mlp_io, ctx->d_us, ctx->d_guess,
                     ctx->d_ctl, ctx->d_stats, ctx->d_std, ctx->d_sti, ctx->d_lists, ctx->d_mom, ctx->d_alpha, ctx->d_old,
                     ctx->d_st, ctx->d_ss, ctx->d_flux, ctx->d_ansatz, ctx->d_bu, ctx->d_ba, ctx->d_bo};
     for (void *p : ptrs)
@@ -460,7 +573,7 @@ int mnc_update(mnc_ctx *ctx, double dt, const mnc_options *opt) {
 int mnc_get_stats(mnc_ctx *ctx, mnc_stats *st) {
     if (!ctx || !st) return MNC_ERR_ARG;
     if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
-    long long h[9];
+    long long h[mnc::kStatWords];
     MNC_HIP_CHECK(hipSetDevice(ctx->device));
     MNC_HIP_CHECK(hipMemcpyAsync(h, ctx->d_stats, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
     MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -469,6 +582,7 @@ int mnc_get_stats(mnc_ctx *ctx, mnc_stats *st) {
     st->num_dual_solves = h[2];
     for (int r = 0; r < 5; ++r) st->hist_reg[r] = h[3 + r];
     st->trial_sum = h[8];
+    for (int r = 0; r < 3; ++r) st->warm_pick[r] = h[9 + r];
     return MNC_OK;
 }
 
@@ -476,7 +590,7 @@ int mnc_reset_stats(mnc_ctx *ctx) {
     if (!ctx) return MNC_ERR_ARG;
     if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
     MNC_HIP_CHECK(hipSetDevice(ctx->device));
-    MNC_HIP_CHECK(hipMemsetAsync(ctx->d_stats, 0, 9 * sizeof(long long), ctx->stream));
+    MNC_HIP_CHECK(hipMemsetAsync(ctx->d_stats, 0, mnc::kStatWords * sizeof(long long), ctx->stream));
     MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return MNC_OK;
 }
@@ -534,6 +648,16 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         ctx->tail_mode = value;
         return MNC_OK;
     }
+    if (std::strcmp(key, "mix_cond_log10") == 0) {
+        if (value < 0 || value > 10) return ctx_fail(ctx, MNC_ERR_ARG, "mix_cond_log10 out of range");
+        ctx->mix_cond_log10 = value;
+        return MNC_OK;
+    }
+    if (std::strcmp(key, "mix_blocks") == 0) {
+        if (value < 1 || value > 65536) return ctx_fail(ctx, MNC_ERR_ARG, "mix_blocks out of range");
+        ctx->mix_blocks = value;
+        return MNC_OK;
+    }
     return ctx_fail(ctx, MNC_ERR_ARG, "unknown tuning key");
 }
 
@@ -541,6 +665,12 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
     if (!ctx || !key || !value) return MNC_ERR_ARG;
     if (std::strcmp(key, "rounds") == 0) { *value = ctx->rounds; return MNC_OK; }
     if (std::strcmp(key, "tail_mode") == 0) { *value = ctx->tail_mode; return MNC_OK; }
+    if (std::strcmp(key, "mix_cond_log10") == 0) { *value = ctx->mix_cond_log10; return MNC_OK; }
+    if (std::strcmp(key, "mix_blocks") == 0) {
+        *value = ctx->mix_blocks > 0 ? ctx->mix_blocks : ctx->num_cu;
+        return MNC_OK;
+    }
+    if (std::strcmp(key, "has_mixed") == 0) { *value = ctx->has_mix ? 1 : 0; return MNC_OK; }
     if (std::strcmp(key, "tail_blocks") == 0) {
         *value = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
         return MNC_OK;
@@ -549,5 +679,170 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
 }
 
 const char *mnc_last_error(const mnc_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
+
+int mnc_mlp_num_params(void) { return mnc::mlp_param_count() + mnc::kMlpStats; }
+
+int mnc_mlp_init(mnc_ctx *ctx, unsigned long long seed) {
+    if (!ctx) return MNC_ERR_ARG;
+    int st = mlp_check(ctx, false);
+    if (st) return st;
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    if ((st = mlp_alloc(ctx, 1, 1))) return st;
+    const int np = mnc::mlp_param_count();
+    std::vector<float> h(size_t(np) + mnc::kMlpStats, 0.0f);
+    // SplitMix64 stream -> U(-1, 1); weights U(-b, b), b = sqrt(6 / (fan_in + fan_out)); biases 0
+    unsigned long long x = seed;
+    auto uni = [&x]() {
+        unsigned long long z = (x += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        return 2.0 * (double(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+    };
+    const int dims[5] = {15, 128, 128, 128, 15};
+    size_t off = 0;
+    for (int l = 0; l < 4; ++l) {
+        const double bnd = std::sqrt(6.0 / double(dims[l] + dims[l + 1]));
+        for (int i = 0; i < dims[l] * dims[l + 1]; ++i) h[off++] = float(bnd * uni());
+        off += dims[l + 1];   // bias
+    }
+    for (int k = 0; k < 15; ++k) { h[np + 15 + k] = 1.0f; h[np + 45 + k] = 1.0f; }   // identity standardisation
+    const size_t nall = h.size();
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_mlp, h.data(), nall * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    MNC_HIP_CHECK(hipMemsetAsync(ctx->d_mlp + 2 * nall, 0, 2 * nall * sizeof(float), ctx->stream));   // Adam m, v
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->mlp_ready = true;
+    ctx->mlp_stats_set = false;
+    ctx->mlp_step = 0;
+    return MNC_OK;
+}
+
+int mnc_mlp_train(mnc_ctx *ctx, long long npairs, const double *u_scaled, const double *alpha_scaled, int steps,
+                  int batch, double lr, unsigned long long seed, float *loss_out) {
+    if (!ctx) return MNC_ERR_ARG;
+    int st = mlp_check(ctx, true);
+    if (st) return st;
+    if (npairs < 1 || !u_scaled || !alpha_scaled || steps < 0 || batch < 1 || batch > (1 << 20) || !(lr > 0.0))
+        return ctx_fail(ctx, MNC_ERR_ARG, "bad training arguments");
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    if ((st = mlp_alloc(ctx, batch, size_t(std::max(steps, 1)))) || (st = mlp_io(ctx, size_t(npairs) * 30))) return st;
+    const size_t n15 = size_t(npairs) * 15;
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_mlp_io, u_scaled, n15 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_mlp_io + n15, alpha_scaled, n15 * sizeof(double), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    const int np = mnc::mlp_param_count();
+    if (!ctx->mlp_stats_set) {   // standardisation from this data
+        float stats[mnc::kMlpStats];
+        for (int part = 0; part < 2; ++part) {
+            const double *src = part == 0 ? u_scaled : alpha_scaled;
+            for (int k = 0; k < 15; ++k) {
+                double m = 0.0, q = 0.0;
+                for (long long i = 0; i < npairs; ++i) m += src[i * 15 + k];
+                m /= double(npairs);
+                for (long long i = 0; i < npairs; ++i) q += (src[i * 15 + k] - m) * (src[i * 15 + k] - m);
+                const double sd = std::sqrt(q / double(npairs));
+                stats[30 * part + k] = float(m);
+                stats[30 * part + 15 + k] = float(sd > 1e-6 ? sd : 1.0);
+            }
+        }
+        MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_mlp + np, stats, sizeof stats, hipMemcpyHostToDevice, ctx->stream));
+        MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->mlp_stats_set = true;
+    }
+    const auto b0 = mlp_buffers(ctx);
+    MNC_HIP_CHECK(hipMemsetAsync(ctx->d_mlp_loss, 0, size_t(std::max(steps, 1)) * sizeof(float), ctx->stream));
+    {
+        Timed tm(ctx, MNC_KCLASS_MLP);
+        for (int k = 0; k < steps; ++k) {
+            auto b = b0;
+            b.loss = ctx->d_mlp_loss + k;
+            const int step = ++ctx->mlp_step;
+            const unsigned long long sk = (seed * 0x9e3779b97f4a7c15ull) ^ (unsigned long long)(step) ^ 0x5bd1e995ull;
+            mnc::mlp_gather(b, ctx->d_mlp_io, ctx->d_mlp_io + n15, npairs, batch, sk | 1ull, 0, ctx->stream);
+            mnc::mlp_forward(b, batch, ctx->stream);
+            mnc::mlp_backward(b, batch, ctx->stream);
+            mnc::mlp_adam(b, float(lr), step, ctx->stream);
+        }
+    }
+    MNC_HIP_CHECK(hipGetLastError());
+    if (loss_out && steps > 0)
+        MNC_HIP_CHECK(hipMemcpyAsync(loss_out, ctx->d_mlp_loss, size_t(steps) * sizeof(float), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MNC_OK;
+}
+
+int mnc_mlp_predict(mnc_ctx *ctx, int n, const double *u_scaled, double *alpha_scaled) {
+    if (!ctx) return MNC_ERR_ARG;
+    int st = mlp_check(ctx, true);
+    if (st) return st;
+    if (n < 0 || (n > 0 && (!u_scaled || !alpha_scaled))) return ctx_fail(ctx, MNC_ERR_ARG, "bad predict arguments");
+    if (n == 0) return MNC_OK;
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    if ((st = mlp_io(ctx, size_t(n) * 30))) return st;
+    const size_t n15 = size_t(n) * 15;
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_mlp_io, u_scaled, n15 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    {
+        Timed tm(ctx, MNC_KCLASS_MLP);
+        if ((st = mlp_predict_dev(ctx, n, ctx->d_mlp_io, ctx->d_mlp_io + n15))) return st;
+    }
+    MNC_HIP_CHECK(hipMemcpyAsync(alpha_scaled, ctx->d_mlp_io + n15, n15 * sizeof(double), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MNC_OK;
+}
+
+int mnc_mlp_get_params(mnc_ctx *ctx, float *params) {
+    if (!ctx || !params) return MNC_ERR_ARG;
+    int st = mlp_check(ctx, true);
+    if (st) return st;
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    MNC_HIP_CHECK(hipMemcpyAsync(params, ctx->d_mlp, size_t(mnc_mlp_num_params()) * sizeof(float),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MNC_OK;
+}
+
+int mnc_mlp_set_params(mnc_ctx *ctx, const float *params) {
+    if (!ctx || !params) return MNC_ERR_ARG;
+    int st = mlp_check(ctx, false);
+    if (st) return st;
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    if ((st = mlp_alloc(ctx, 1, 1))) return st;
+    const size_t nall = size_t(mnc_mlp_num_params());
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_mlp, params, nall * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    MNC_HIP_CHECK(hipMemsetAsync(ctx->d_mlp + 2 * nall, 0, 2 * nall * sizeof(float), ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->mlp_ready = true;
+    ctx->mlp_stats_set = true;
+    ctx->mlp_step = 0;
+    return MNC_OK;
+}
+
+int mnc_mlp_gradient(mnc_ctx *ctx, int rows, const double *u_scaled, const double *alpha_scaled, float *grad,
+                     float *loss) {
+    if (!ctx) return MNC_ERR_ARG;
+    int st = mlp_check(ctx, true);
+    if (st) return st;
+    if (rows < 1 || rows > (1 << 20) || !u_scaled || !alpha_scaled || !grad || !loss)
+        return ctx_fail(ctx, MNC_ERR_ARG, "bad gradient arguments");
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    if ((st = mlp_alloc(ctx, rows, 1)) || (st = mlp_io(ctx, size_t(rows) * 30))) return st;
+    const size_t n15 = size_t(rows) * 15;
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_mlp_io, u_scaled, n15 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_mlp_io + n15, alpha_scaled, n15 * sizeof(double), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    MNC_HIP_CHECK(hipMemsetAsync(ctx->d_mlp_loss, 0, sizeof(float), ctx->stream));
+    const auto b = mlp_buffers(ctx);
+    mnc::mlp_gather(b, ctx->d_mlp_io, ctx->d_mlp_io + n15, rows, rows, 0, 0, ctx->stream);
+    mnc::mlp_forward(b, rows, ctx->stream);
+    mnc::mlp_backward(b, rows, ctx->stream);
+    MNC_HIP_CHECK(hipGetLastError());
+    MNC_HIP_CHECK(hipMemcpyAsync(grad, b.grad, size_t(mnc::mlp_param_count()) * sizeof(float), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    MNC_HIP_CHECK(hipMemcpyAsync(loss, ctx->d_mlp_loss, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MNC_OK;
+}
 
 }  // extern "C"

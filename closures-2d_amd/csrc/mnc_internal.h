@@ -34,8 +34,6 @@ struct NewtonArgs {
     mnc_outs *outs;       // [ncells] or nullptr
     const double *w;      // [nq]
     const double *p;      // [nq][np2]
-    const float *pf;      // [nq][np2] fp32 copy (mixed precision)
-    const unsigned short *pbf; // bf16 MFMA operand table (mixed precision)
     int nq;
     const int *gptr;      // Gaunt CSR (nm*nm+1)
     const int *gk;
@@ -44,16 +42,42 @@ struct NewtonArgs {
     double tol;
     double cond_h_max;
     int warm;             // MNC_WARM_*
-    long long *stats;     // [0] max iter, [1] iter sum, [2] solves, [3..7] hist, [8] trials
+    long long *stats;     // [0] max iter, [1] iter sum, [2] solves, [3..7] hist, [8] trials,
+                          // [9..11] warm starts taken: learned, isotropic, previous
     int ls_mode;          // MNC_LS_*
     const double *alpha_guess; // [ncells][nm] scaled-problem guess (learned warm start) or nullptr
     // per-cell solver state and the active lists
     double *st_d;         // [dfields][cap]
     int *st_i;            // [ifields][cap]
-    int *lists;           // [2][cap]
-    int *ctl;             // [0],[1] list counts, [2] tail queue, [3] error bits
+    int *lists;           // [3][cap]: two ping-pong active lists, then the deferred list
+    int *ctl;             // [0],[1] list counts, [2] tail queue, [3] error bits, [4] deferred count
     size_t cap;
+    // mixed precision (MixTables, device copies)
+    const double *mz;     // [nt][4][64]  alpha.m MFMA operand
+    const double *mg;     // [nt][4][64]  w m MFMA operand (column 15: w)
+    const unsigned short *mh; // [nt/2][nnt][2][64][8] bf16 hi/lo of w m2N
+    int mnt;              // quadrature tiles of 16 points (even)
+    const int *mgptr, *mgk;   // Gaunt map of the order-2N table
+    const double *mgv;
+    double cmix;          // condition bound above which a mixed-precision step is redone in FP64
 };
+
+// Number of control words of a batched solve.
+constexpr int kCtlWords = 8;
+constexpr int kStatWords = 12;
+
+// Operand tables of the mixed-precision sweep (built on the host from Tables):
+// the quadrature is padded to a multiple of 32 points (zero weight) and the
+// order-2N harmonics to a multiple of 16 columns.
+struct MixTables {
+    int nt = 0, nnt = 0, nmm = 0;
+    std::vector<double> z, g;
+    std::vector<unsigned short> h;
+    std::vector<int> gptr, gk;
+    std::vector<double> gv;
+};
+// Returns MNC_OK, or MNC_ERR_ARG when the order/quadrature has no mixed kernel.
+int build_mix_tables(int moment_order, int quad_order, MixTables &m);
 
 // Launchers (mnc_newton.hip).  Return false if (nm, np2, gaunt) has no instance.
 // Enqueues the whole batched solve: prologue, `rounds` lockstep sweep/logic
@@ -61,10 +85,37 @@ struct NewtonArgs {
 // tail_mode 0: one wave per remaining cell (cooperative sweeps); 1: one lane per cell.
 bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
                         int tail_mode, hipStream_t s);
+// The same with the lockstep rounds in mixed precision (MFMA sweeps); the tail
+// is FP64.  mix_blocks: persistent blocks of the mixed sweep.
+bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
+                         int tail_mode, int mix_blocks, hipStream_t s);
+constexpr int kMixMaxTiles = 16;   // quadrature tiles of 16 points held in LDS (nq <= 256)
+int mix_tiles(int nm);             // 16-column tiles of the order-2N moments the mixed kernel expects
 int newton_block();
-// Zeroes the four solve-control words (a kernel, so the step can be captured).
+// Zeroes the solve-control words (a kernel, so the step can be captured).
 void launch_ctl_reset(int *ctl, hipStream_t s);
 void newton_store_fields(int nm, int np2, int *dfields, int *ifields);
+
+// The learned warm start (mnc_mlp.hip).  w: the 37,007 parameters followed by
+// in_mean, in_std, out_mean, out_std (15 each); grad, m, v: like the
+// parameters; scratch: mlp_scratch_floats(rows); loss: one float (accumulated).
+struct MlpBuffers {
+    float *w, *grad, *m, *v, *scratch, *loss;
+};
+constexpr int kMlpStats = 60;
+int mlp_param_count();
+size_t mlp_scratch_floats(int rows);
+// Minibatch (seed != 0: rows drawn with replacement; seed == 0: rows i0.. in
+// order; as == nullptr: inputs only) into the scratch's input/target slots.
+void mlp_gather(const MlpBuffers &b, const double *us, const double *as, long long npairs, int rows,
+                unsigned long long seed, long long i0, hipStream_t s);
+void mlp_forward(const MlpBuffers &b, int rows, hipStream_t s);
+void mlp_backward(const MlpBuffers &b, int rows, hipStream_t s);
+void mlp_adam(const MlpBuffers &b, float lr, int step, hipStream_t s);
+// Scaled multipliers of the last forward pass into alpha[rows][15] (FP64).
+void mlp_unscale(const MlpBuffers &b, int rows, double *alpha, hipStream_t s);
+void mlp_scale_moments(const double *u, int ncells, double *us, hipStream_t s);
+float *mlp_output(const MlpBuffers &b, int rows);
 
 // Arguments of the grid kernels (mnc_grid.hip).
 struct GridArgs {

@@ -1,24 +1,24 @@
-// mnc_newton.hip -- batched FP64 Newton solve of the Mn dual problem on gfx950.
+// mnc_newton.hip -- batched Newton solve of the Mn dual problem on gfx950.
 //
-// One lane owns one cell at a time (thread-per-cell).  Each wave is a
-// persistent worker: lanes whose cell has converged pull the next cell index
-// from an on-device work queue (one atomic per wave, wave-aggregated), so cells
-// needing 30+ iterations near the realizability boundary do not hold back the
-// ~2-iteration interior cells that share their wave.  All lanes of a wave sweep
-// the quadrature together (uniform q loop: the table rows are wave-uniform loads),
-// each evaluating its own cell at its own point -- the Newton iterate for lanes
-// about to take a Newton step, or the line-search trial point for lanes inside
-// the Armijo loop.  One sweep yields both the trial's f and the full Hessian at
-// the trial point, so an accepted trial goes straight to its Newton step with
-// no second sweep (the reference re-evaluates at the same alpha, giving the same
-// numbers).
+// Lockstep rounds with compaction: every round is one quadrature sweep for all
+// active cells (sweep_kernel: one lane per cell, wave-uniform table rows as
+// scalar loads; or sweep_mix_kernel: sixteen cells per wave on the matrix
+// cores) followed by one step of the per-cell Newton/Armijo state machine
+// (logic_kernel), which compacts the cells still iterating onto the next list.
+// One sweep yields both a trial's f and the full Hessian at the trial point, so
+// an accepted trial goes straight to its Newton step with no second sweep (the
+// reference re-evaluates at the same alpha, giving the same numbers).  After the
+// rounds, a persistent tail (tail_coop_kernel: one wave per remaining cell)
+// takes the few slow cells near the realizability boundary from a queue and
+// iterates each to completion.
 //
 // The per-cell algorithm restates src/momopt/opt/opt.cpp:82-343 (opt,
 // optScaled), src/momopt/opt/linesearch.cpp:21-99 and fobj.cpp:37-129,255-335,
-// with the LAPACK roles (dpotrf 'L', dpocon, dpotrs, dnrm2) done in registers:
-// unblocked column Cholesky, Higham's 1-norm estimator (ACM TOMS 14(4), 1988),
-// forward/back substitution.
+// with the LAPACK roles (dpotrf 'L', dpocon, dpotrs, dnrm2) done in registers
+// or LDS: unblocked column Cholesky, Higham's 1-norm estimator (ACM TOMS 14(4),
+// 1988), forward/back substitution.
 #include <cfloat>
+#include <type_traits>
 
 #include "mnc_internal.h"
 
@@ -36,27 +36,31 @@ template <int N> struct Tri {
 
 // ---- dense kernels on a packed lower triangle in registers ----------------
 
+// The dense kernels are generic in the element type T: FP64 everywhere except
+// the mixed-precision logic, whose Hessian is only ~1e-5 accurate and is
+// factored in FP32 (the gradient, f and the stop test stay FP64).
+
 // dpotf2-order Cholesky in place; returns false on a non-positive (or NaN) pivot.
-template <int N>
-__device__ __forceinline__ bool cholesky(double (&a)[Tri<N>::size]) {
+template <int N, class T = double>
+__device__ __forceinline__ bool cholesky(T (&a)[Tri<N>::size]) {
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < N; ++j) {
-        double dot = 0.0;
+        T dot = T(0);
 #pragma unroll
         for (int k = 0; k < j; ++k) dot = fma(a[Tri<N>::at(j, k)], a[Tri<N>::at(j, k)], dot);
-        double ajj = a[Tri<N>::at(j, j)] - dot;
-        ok = ok && (ajj > 0.0);
-        ajj = sqrt(ajj > 0.0 ? ajj : 1.0);
+        T ajj = a[Tri<N>::at(j, j)] - dot;
+        ok = ok && (ajj > T(0));
+        ajj = sqrt(ajj > T(0) ? ajj : T(1));
         a[Tri<N>::at(j, j)] = ajj;
 #pragma unroll
         for (int i = j + 1; i < N; ++i) {
-            double y = a[Tri<N>::at(i, j)];
+            T y = a[Tri<N>::at(i, j)];
 #pragma unroll
             for (int k = 0; k < j; ++k) y = fma(-a[Tri<N>::at(j, k)], a[Tri<N>::at(i, k)], y);
             a[Tri<N>::at(i, j)] = y;
         }
-        const double r = 1.0 / ajj;
+        const T r = T(1) / ajj;
 #pragma unroll
         for (int i = j + 1; i < N; ++i) a[Tri<N>::at(i, j)] *= r;
     }
@@ -64,8 +68,8 @@ __device__ __forceinline__ bool cholesky(double (&a)[Tri<N>::size]) {
 }
 
 // x <- (L L^T)^{-1} x
-template <int N>
-__device__ __forceinline__ void apply_inv(const double (&l)[Tri<N>::size], double (&x)[N]) {
+template <int N, class T = double>
+__device__ __forceinline__ void apply_inv(const T (&l)[Tri<N>::size], T (&x)[N]) {
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         x[j] = x[j] / l[Tri<N>::at(j, j)];
@@ -74,28 +78,28 @@ __device__ __forceinline__ void apply_inv(const double (&l)[Tri<N>::size], doubl
     }
 #pragma unroll
     for (int j = N - 1; j >= 0; --j) {
-        double t = x[j];
+        T t = x[j];
 #pragma unroll
         for (int i = N - 1; i > j; --i) t = fma(-l[Tri<N>::at(i, j)], x[i], t);
         x[j] = t / l[Tri<N>::at(j, j)];
     }
 }
 
-template <int N>
-__device__ __forceinline__ double dasum(const double (&x)[N]) {
-    double s = 0.0;
+template <int N, class T>
+__device__ __forceinline__ T dasum(const T (&x)[N]) {
+    T s = T(0);
 #pragma unroll
     for (int i = 0; i < N; ++i) s += fabs(x[i]);
     return s;
 }
 
-template <int N>
-__device__ __forceinline__ int idamax(const double (&x)[N]) {
+template <int N, class T>
+__device__ __forceinline__ int idamax(const T (&x)[N]) {
     int idx = 0;
-    double m = fabs(x[0]);
+    T m = fabs(x[0]);
 #pragma unroll
     for (int i = 1; i < N; ++i) {
-        const double v = fabs(x[i]);
+        const T v = fabs(x[i]);
         if (v > m) { m = v; idx = i; }
     }
     return idx;
@@ -104,78 +108,78 @@ __device__ __forceinline__ int idamax(const double (&x)[N]) {
 // Reciprocal condition estimate of A = L L^T in the 1-norm (the role of
 // dpocon): Higham's estimate of ||A^{-1}||_1, with a single call site of the
 // A^{-1} product inside a small state machine to keep the unrolled code short.
-template <int N>
-__device__ __forceinline__ double rcond(const double (&l)[Tri<N>::size], double anorm) {
+template <int N, class T = double>
+__device__ __forceinline__ double rcond(const T (&l)[Tri<N>::size], double anorm) {
     if (anorm == 0.0) return 0.0;
-    double x[N];
+    T x[N];
     unsigned isgn = 0;   // bit i set <=> isgn_i = -1
-    double est = 0.0;
+    T est = T(0);
     int jump = 1, j = 0, iter = 0;
 #pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = 1.0 / N;
+    for (int i = 0; i < N; ++i) x[i] = T(1) / T(N);
 #pragma unroll 1
     for (int guard = 0; guard < 16; ++guard) {
-        apply_inv<N>(l, x);
+        apply_inv<N, T>(l, x);
         if (jump == 1) {                       // first product
             if (N == 1) { est = fabs(x[0]); break; }
-            est = dasum<N>(x);
+            est = dasum<N, T>(x);
             isgn = 0;
 #pragma unroll
             for (int i = 0; i < N; ++i) {
-                const bool neg = !(x[i] >= 0.0);
-                x[i] = neg ? -1.0 : 1.0;
+                const bool neg = !(x[i] >= T(0));
+                x[i] = neg ? T(-1) : T(1);
                 isgn |= neg ? (1u << i) : 0u;
             }
             jump = 2;
         } else if (jump == 2) {                // after the transposed product
-            j = idamax<N>(x);
+            j = idamax<N, T>(x);
             iter = 2;
 #pragma unroll
-            for (int i = 0; i < N; ++i) x[i] = (i == j) ? 1.0 : 0.0;
+            for (int i = 0; i < N; ++i) x[i] = (i == j) ? T(1) : T(0);
             jump = 3;
         } else if (jump == 3) {                // x = A^{-1} e_j
-            const double estold = est;
-            est = dasum<N>(x);
+            const T estold = est;
+            est = dasum<N, T>(x);
             unsigned s = 0;
 #pragma unroll
-            for (int i = 0; i < N; ++i) s |= !(x[i] >= 0.0) ? (1u << i) : 0u;
+            for (int i = 0; i < N; ++i) s |= !(x[i] >= T(0)) ? (1u << i) : 0u;
             bool done = (s == isgn) || (est <= estold);
             if (!done) {
 #pragma unroll
-                for (int i = 0; i < N; ++i) x[i] = (s >> i & 1u) ? -1.0 : 1.0;
+                for (int i = 0; i < N; ++i) x[i] = (s >> i & 1u) ? T(-1) : T(1);
                 isgn = s;
                 jump = 4;
             } else {
-                double alt = 1.0;
+                T alt = T(1);
 #pragma unroll
-                for (int i = 0; i < N; ++i) { x[i] = alt * (1.0 + double(i) / double(N - 1)); alt = -alt; }
+                for (int i = 0; i < N; ++i) { x[i] = alt * (T(1) + T(i) / T(N - 1)); alt = -alt; }
                 jump = 5;
             }
         } else if (jump == 4) {                // after the transposed product
             const int jlast = j;
-            j = idamax<N>(x);
-            double xjl = 0.0, xj = 0.0;
+            j = idamax<N, T>(x);
+            T xjl = T(0), xj = T(0);
 #pragma unroll
             for (int i = 0; i < N; ++i) { xjl = (i == jlast) ? x[i] : xjl; xj = (i == j) ? x[i] : xj; }
             if (xjl != fabs(xj) && iter < 5) {
                 ++iter;
 #pragma unroll
-                for (int i = 0; i < N; ++i) x[i] = (i == j) ? 1.0 : 0.0;
+                for (int i = 0; i < N; ++i) x[i] = (i == j) ? T(1) : T(0);
                 jump = 3;
             } else {
-                double alt = 1.0;
+                T alt = T(1);
 #pragma unroll
-                for (int i = 0; i < N; ++i) { x[i] = alt * (1.0 + double(i) / double(N - 1)); alt = -alt; }
+                for (int i = 0; i < N; ++i) { x[i] = alt * (T(1) + T(i) / T(N - 1)); alt = -alt; }
                 jump = 5;
             }
         } else {                               // jump == 5: final alternating vector
-            const double temp = 2.0 * (dasum<N>(x) / double(3 * N));
+            const T temp = T(2) * (dasum<N, T>(x) / T(3 * N));
             if (temp > est) est = temp;
             break;
         }
     }
-    if (!isfinite(est) || est == 0.0) return 0.0;
-    return (1.0 / est) / anorm;
+    if (!isfinite(est) || est == T(0)) return 0.0;
+    return (1.0 / double(est)) / anorm;
 }
 
 template <int N>
@@ -192,7 +196,7 @@ __device__ __forceinline__ unsigned long long lane_mask_lt() {
 }
 
 enum : int { ST_NEWTON = 0, ST_TRIAL = 1 };
-enum : int { LOGIC_CONTINUE = 0, LOGIC_DONE = 1 };
+enum : int { LOGIC_CONTINUE = 0, LOGIC_DONE = 1, LOGIC_DEFER = 2 };
 
 // The table is read-only for the whole kernel; reading it through the constant
 // address space lets wave-uniform row reads become scalar loads (SGPR operands).
@@ -275,12 +279,18 @@ __device__ __forceinline__ double sweep_wave(const double *P, const double *W, i
 // ---- per-cell solver state in HBM ------------------------------------------
 // Structure of arrays [field][cap]: threads handling neighbouring list entries
 // (mostly ascending cell indices) touch neighbouring words.
+// Moment order N of n = (N+1)(N+2)/2 moments, and the order-2N count padded to
+// the 16 columns of an MFMA tile (the mixed sweep's Hessian moments).
+constexpr int order_of(int nm) { int N = 0; while ((N + 1) * (N + 2) / 2 < nm) ++N; return N; }
+constexpr int mix_cols(int nm) { return ((2 * order_of(nm) + 1) * (2 * order_of(nm) + 2) / 2 + 15) / 16 * 16; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
 template <int NM, int NP2>
 struct Store {
-    static constexpr int nacc = NP2 > NM * (NM + 1) / 2 ? NP2 : NM * (NM + 1) / 2;
+    static constexpr int nacc = cmax(cmax(NP2, NM * (NM + 1) / 2), mix_cols(NM));
     enum : int {
         F_AL = 0, F_U = NM, F_AOLD = 2 * NM, F_D = 3 * NM, F_T = 4 * NM, F_RTOL, F_U0, F_ASCALE, F_FCUR,
-        F_GD, F_MAXF, F_FSCALE, F_SF, F_ACC
+        F_GD, F_MAXF, F_FSCALE, F_SF, F_GS, F_ACC = F_GS + NM
     };
     static constexpr int dfields = F_ACC + nacc;
     enum : int { I_ITER = 0, I_TOTAL, I_REG, I_MAXIT, I_MODE, ifields };
@@ -294,11 +304,11 @@ struct Store {
 // Per-lane statistics, reduced per block and flushed with one atomic each.
 struct LaneStats {
     int max_iter = 0, err_bits = 0;
-    long long sum = 0, n = 0, trials = 0, hist[5] = {0, 0, 0, 0, 0};
+    long long sum = 0, n = 0, trials = 0, hist[5] = {0, 0, 0, 0, 0}, warm[3] = {0, 0, 0};
 };
 
 __device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
-    __shared__ long long red[kBlock / 64][10];
+    __shared__ long long red[kBlock / 64][13];
     for (int off = 32; off > 0; off >>= 1) {
         s.max_iter = max(s.max_iter, __shfl_xor(s.max_iter, off));
         s.err_bits |= __shfl_xor(s.err_bits, off);
@@ -307,39 +317,72 @@ __device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
         s.trials += __shfl_xor(s.trials, off);
 #pragma unroll
         for (int r = 0; r < 5; ++r) s.hist[r] += __shfl_xor(s.hist[r], off);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) s.warm[r] += __shfl_xor(s.warm[r], off);
     }
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) {
         red[w][0] = s.max_iter; red[w][1] = s.err_bits; red[w][2] = s.sum; red[w][3] = s.n; red[w][4] = s.trials;
         for (int r = 0; r < 5; ++r) red[w][5 + r] = s.hist[r];
+        for (int r = 0; r < 3; ++r) red[w][10 + r] = s.warm[r];
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        long long v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        long long v[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int ww = 0; ww < kBlock / 64; ++ww) {
             v[0] = max(v[0], red[ww][0]);
             v[1] |= red[ww][1];
-            for (int k = 2; k < 10; ++k) v[k] += red[ww][k];
+            for (int k = 2; k < 13; ++k) v[k] += red[ww][k];
         }
-        if (v[3] || v[4] || v[1]) {
+        if (v[3] || v[4] || v[1] || v[10] || v[11] || v[12]) {
             atomicMax((unsigned long long *)&A.stats[0], (unsigned long long)v[0]);
             atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)v[2]);
             atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)v[3]);
             for (int r = 0; r < 5; ++r) atomicAdd((unsigned long long *)&A.stats[3 + r], (unsigned long long)v[5 + r]);
             atomicAdd((unsigned long long *)&A.stats[8], (unsigned long long)v[4]);
+            for (int r = 0; r < 3; ++r) atomicAdd((unsigned long long *)&A.stats[9 + r], (unsigned long long)v[10 + r]);
             if (v[1]) atomicOr(&A.ctl[3], int(v[1]));
         }
     }
 }
 
-// Appends the calling lanes with `want` set to list `dst` (wave-aggregated).
-__device__ __forceinline__ void list_append(const NewtonArgs &A, int dst, bool want, int c) {
+// Squared FP64 gradient norm of the scaled dual problem at al:
+// || sum_q w_q e^{al.m_q} m_q - us ||^2 (fobj.cpp:291-334 without f).
+template <int NM, int NP2>
+__device__ double grad_norm2(const NewtonArgs &A, const double (&al)[NM], const double (&us)[NM]) {
+    cdouble *P = (cdouble *)A.p;
+    cdouble *W = (cdouble *)A.w;
+    double g[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) g[k] = 0.0;
+#pragma unroll 2
+    for (int q = 0; q < A.nq; ++q) {
+        cdouble *row = P + size_t(q) * NP2;
+        double am = 0.0;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) am = fma(al[k], row[k], am);
+        const double we = W[q] * exp(am);
+#pragma unroll
+        for (int k = 0; k < NM; ++k) g[k] = fma(we, row[k], g[k]);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        const double d = g[k] - us[k];
+        s = fma(d, d, s);
+    }
+    return s;
+}
+
+// Appends the calling lanes with `want` set to list `dst`, whose count is
+// control word `cw` (wave-aggregated).
+__device__ __forceinline__ void list_append(const NewtonArgs &A, int dst, int cw, bool want, int c) {
     const unsigned long long m = __ballot(want);
     if (!m) return;
     const int lane = threadIdx.x & 63;
     const int leader = __ffsll((long long)m) - 1;
     int base = 0;
-    if (lane == leader) base = atomicAdd(&A.ctl[dst], __popcll(m));
+    if (lane == leader) base = atomicAdd(&A.ctl[cw], __popcll(m));
     base = __shfl(base, leader);
     if (want) A.lists[size_t(dst) * A.cap + base + __popcll(m & lane_mask_lt())] = c;
 }
@@ -366,8 +409,53 @@ __device__ bool cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
 #pragma unroll
         for (int k = 1; k < NM; ++k) al[k] = 0.0;
     } else if (A.warm == MNC_WARM_LEARNED && A.alpha_guess) {
+        // Judge: the candidate with the smallest FP64 gradient norm.  At
+        // isotropicAlpha the integrals are (1, 0, ...) up to rounding (the
+        // basis is orthonormal and <m0 e> = 1 there), so its gradient norm is
+        // sum_{k>0} us_k^2 without a sweep.
+        double us[NM];
+        double best = 0.0;
 #pragma unroll
-        for (int k = 0; k < NM; ++k) al[k] = A.alpha_guess[size_t(c) * NM + k];
+        for (int k = 0; k < NM; ++k) us[k] = uin[k] / u0;
+#pragma unroll
+        for (int k = 1; k < NM; ++k) best = fma(us[k], us[k], best);
+        al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
+#pragma unroll
+        for (int k = 1; k < NM; ++k) al[k] = 0.0;
+        int pick = 1;
+        double cand[NM];
+        bool finite = true;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            cand[k] = ain[k];
+            finite = finite && isfinite(cand[k]);
+        }
+        cand[0] = cand[0] - ascale;
+        if (finite && u0 > 0.0) {
+            const double n2 = grad_norm2<NM, NP2>(A, cand, us);
+            if (n2 < best) {
+                best = n2;
+                pick = 2;
+#pragma unroll
+                for (int k = 0; k < NM; ++k) al[k] = cand[k];
+            }
+        }
+        finite = true;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            cand[k] = A.alpha_guess[size_t(c) * NM + k];
+            finite = finite && isfinite(cand[k]);
+        }
+        if (finite && u0 > 0.0) {
+            const double n2 = grad_norm2<NM, NP2>(A, cand, us);
+            if (n2 < best) {
+                best = n2;
+                pick = 0;
+#pragma unroll
+                for (int k = 0; k < NM; ++k) al[k] = cand[k];
+            }
+        }
+        if (u0 > 0.0) ++st.warm[pick];
     } else {
 #pragma unroll
         for (int k = 0; k < NM; ++k) al[k] = ain[k];
@@ -396,7 +484,17 @@ __device__ bool cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
 // point (the sweep's results are in S: F_SF and F_ACC).  Mirrors
 // opt.cpp:209-318 and linesearch.cpp:38-98, with the sweep shared between an
 // accepted Armijo trial and the Newton iteration that follows it.
-template <int NM, int NP2, bool GAUNT>
+//
+// MIX: the sweep was the mixed-precision one: f from the FP64 sum of w e, g from
+// the FP64 sums of w e m (F_GS), and H from order-2N moments accumulated on the
+// matrix cores in bf16 hi/lo pairs with FP32 sums (relative error ~1e-5).  The
+// stop test, the Armijo test and the reconstruction are FP64 as in the
+// reference; only the Newton direction uses the approximate H.  Whenever the
+// reference would act on H itself (a failed factorisation, a condition estimate
+// above min(cmix, cond_h_max), an exhausted line search, the iteration limit),
+// the iteration is not taken: the cell returns LOGIC_DEFER at the point where
+// that Newton iteration starts, to be redone by an FP64 sweep.
+template <int NM, int NP2, bool GAUNT, bool MIX = false>
 __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, LaneStats &st) {
     using St = Store<NM, NP2>;
     const double p0 = A.p[0];
@@ -407,7 +505,7 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
 #pragma unroll
     for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
     bool newton = (mode == ST_NEWTON);
-    bool do_reg = false, converged = false;
+    bool do_reg = false, converged = false, defer = false;
     double err = 0.0;
     if (mode == ST_TRIAL) {                                        // linesearch.cpp:45-70
         ++st.trials;
@@ -432,9 +530,15 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
             } else {                                               // exhausted: t = 0 (:74-98)
 #pragma unroll
                 for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AOLD + k, c);
-                t = 0.0;
-                ++iter; ++total;
-                do_reg = true;                                     // opt.cpp:244 (t == 0)
+                if constexpr (MIX) {        // redo the iteration that produced d in FP64
+                    t = 1.0;
+                    --iter; --total;
+                    defer = true;
+                } else {
+                    t = 0.0;
+                    ++iter; ++total;
+                    do_reg = true;                                 // opt.cpp:244 (t == 0)
+                }
             }
         }
     }
@@ -443,8 +547,21 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
         ++iter; ++total;                                           // opt.cpp:212-213
 #pragma unroll
         for (int k = 0; k < NM; ++k) S.d(St::F_AL + k, c) = al[k];
-        double h[Tri<NM>::size];
-        if constexpr (GAUNT) {                                     // fobj.cpp:99-113
+        using HT = typename std::conditional<MIX, float, double>::type;
+        HT h[Tri<NM>::size];
+        if constexpr (MIX) {                                       // order-2N moments -> H
+#pragma unroll
+            for (int i = 0; i < NM; ++i) {
+#pragma unroll
+                for (int j = i; j < NM; ++j) {
+                    double hij = 0.0;
+                    const int pr = i * NM + j;
+                    for (int cc = A.mgptr[pr]; cc < A.mgptr[pr + 1]; ++cc)
+                        hij = fma(A.mgv[cc], S.d(St::F_ACC + A.mgk[cc], c), hij);
+                    h[Tri<NM>::at(j, i)] = HT(hij);
+                }
+            }
+        } else if constexpr (GAUNT) {                              // fobj.cpp:99-113
 #pragma unroll
             for (int i = 0; i < NM; ++i) {
 #pragma unroll
@@ -464,12 +581,12 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
         // g goes to the gradient slot of the state (F_D doubles as it until the
         // direction is formed), so only the factor stays live below.
         {
-            double fcur = h[0] / (p0 * p0);
+            double fcur = MIX ? S.d(St::F_SF, c) : h[0] / (p0 * p0);
             double fscale = fabs(fcur), gg = 0.0;
 #pragma unroll
             for (int i = 0; i < NM; ++i) {
                 const double ui = S.d(St::F_U + i, c);
-                const double gi = h[Tri<NM>::at(i, 0)] / p0 - ui;
+                const double gi = (MIX ? S.d(St::F_GS + i, c) : h[Tri<NM>::at(i, 0)] / p0) - ui;
                 S.d(St::F_D + i, c) = gi;
                 gg = fma(gi, gi, gg);
                 fcur = fcur - al[i] * ui;
@@ -488,16 +605,27 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
             anorm = s > anorm ? s : anorm;
         }
         // Phase 2: factor, condition estimate, direction (dpotrf, dpocon, dpotrs).
-        const bool chol_ok = cholesky<NM>(h);
+        const bool chol_ok = cholesky<NM, HT>(h);
         double condest = 0.0;
-        if (chol_ok) condest = 1.0 / rcond<NM>(h, anorm);
-        if (iter >= maxit || !chol_ok || t == 0.0 || condest > A.cond_h_max) {
-            do_reg = true;                                         // opt.cpp:244
+        if (chol_ok) condest = 1.0 / rcond<NM, HT>(h, anorm);
+        const double climit = MIX ? fmin(A.cmix, A.cond_h_max) : A.cond_h_max;
+        if (iter >= maxit || !chol_ok || t == 0.0 || condest > climit) {
+            if constexpr (MIX) {
+                --iter; --total;
+                defer = true;
+            } else {
+                do_reg = true;                                     // opt.cpp:244
+            }
         } else {
             double d[NM];
+            {
+                HT dh[NM];
 #pragma unroll
-            for (int i = 0; i < NM; ++i) d[i] = S.d(St::F_D + i, c);
-            apply_inv<NM>(h, d);                                   // dpotrs (:278-282)
+                for (int i = 0; i < NM; ++i) dh[i] = HT(S.d(St::F_D + i, c));
+                apply_inv<NM, HT>(h, dh);                          // dpotrs (:278-282)
+#pragma unroll
+                for (int i = 0; i < NM; ++i) d[i] = double(dh[i]);
+            }
             // Phase 3: stop test, or the line search set-up (linesearch.cpp:28-43)
             if (err <= S.d(St::F_RTOL, c)) {
                 converged = true;                                  // :293-306
@@ -525,6 +653,9 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
 #pragma unroll
                     for (int i = 0; i < NM; ++i) al[i] = al[i] + t * d[i];
                     mode = ST_TRIAL;
+                } else if constexpr (MIX) {
+                    --iter; --total;
+                    defer = true;
                 } else {
                     // the Armijo loop cannot start: linesearch returns 0 and the
                     // next iteration regularizes (opt.cpp:212-213,244)
@@ -540,7 +671,10 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
         }
     }
     int result = LOGIC_CONTINUE;
-    if (do_reg) {                                                  // opt.cpp:246-273
+    if (defer) {
+        mode = ST_NEWTON;
+        result = LOGIC_DEFER;
+    } else if (do_reg) {                                           // opt.cpp:246-273
         if (reg < 4) {
             ++reg;
             const double r = kReg[reg];
@@ -587,7 +721,7 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
         ++st.hist[reg];
         result = LOGIC_DONE;
     }
-    if (result == LOGIC_CONTINUE) {
+    if (result != LOGIC_DONE) {
 #pragma unroll
         for (int k = 0; k < NM; ++k) S.d(St::F_AL + k, c) = al[k];
         S.d(St::F_T, c) = t;
@@ -635,7 +769,7 @@ __global__ __launch_bounds__(kBlock) void init_kernel(NewtonArgs A) {
     LaneStats st;
     bool ok = false;
     if (c < A.ncells) ok = cell_init<NM, NP2>(A, S, c, st);
-    list_append(A, 0, ok, c);
+    list_append(A, 0, 0, ok, c);
     flush_stats(A, st);
 }
 
@@ -653,19 +787,30 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(NewtonArgs A, int lin) {
 
 // Newton / Armijo logic for every cell on list `lin`; cells needing another
 // sweep are compacted onto list lin^1.
-template <int NM, int NP2, bool GAUNT>
+// MIX: after a mixed-precision sweep; deferred cells go to list 2 (count in
+// control word 4) for the FP64 tail.
+template <int NM, int NP2, bool GAUNT, bool MIX = false>
 __global__ __launch_bounds__(kBlock) void logic_kernel(NewtonArgs A, int lin) {
     const int slot = blockIdx.x * kBlock + threadIdx.x;
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
     LaneStats st;
-    bool more = false;
+    int res = LOGIC_DONE;
     int c = -1;
     if (slot < A.ctl[lin]) {
         c = A.lists[size_t(lin) * A.cap + slot];
-        more = cell_logic<NM, NP2, GAUNT>(A, S, c, st) == LOGIC_CONTINUE;
+        res = cell_logic<NM, NP2, GAUNT, MIX>(A, S, c, st);
     }
-    list_append(A, lin ^ 1, more, c);
+    list_append(A, lin ^ 1, lin ^ 1, res == LOGIC_CONTINUE, c);
+    if constexpr (MIX) list_append(A, 2, 4, res == LOGIC_DEFER, c);
     flush_stats(A, st);
+}
+
+// Entry s of the tail's work: the cells left on list `lin`, then the cells a
+// mixed-precision round deferred to FP64.  -1 past the end.
+__device__ __forceinline__ int tail_cell(const NewtonArgs &A, int lin, int count, int dcount, int s) {
+    if (s < count) return A.lists[size_t(lin) * A.cap + s];
+    if (s < count + dcount) return A.lists[2 * A.cap + (s - count)];
+    return -1;
 }
 
 // Persistent tail: lanes pull the remaining cells of list `lin` from a queue
@@ -675,7 +820,7 @@ template <int NM, int NP2, bool GAUNT, int RS>
 __global__ __launch_bounds__(kBlock) void tail_kernel(NewtonArgs A, int lin) {
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
     const int lane = threadIdx.x & 63;
-    const int count = A.ctl[lin];
+    const int count = A.ctl[lin], dcount = A.ctl[4];
     LaneStats st;
     int c = -1;          // -1 wants work, -2 finished
     for (;;) {
@@ -688,7 +833,8 @@ __global__ __launch_bounds__(kBlock) void tail_kernel(NewtonArgs A, int lin) {
             base = __shfl(base, leader);
             if (need) {
                 const int s = base + __popcll(m & lane_mask_lt());
-                c = (s < count) ? A.lists[size_t(lin) * A.cap + s] : -2;
+                c = tail_cell(A, lin, count, dcount, s);
+                if (c < 0) c = -2;
             }
         }
         if (__ballot(c >= 0) == 0) break;
@@ -712,14 +858,14 @@ __global__ __launch_bounds__(kBlock) void tail_wave_kernel(NewtonArgs A, int lin
     constexpr int NA = GAUNT ? NP2 : Tri<NM>::at(RS, 0);
     constexpr int NB = (GAUNT || RS >= NM) ? 1 : Tri<NM>::size - Tri<NM>::at(RS, 0);
     const int lane = threadIdx.x & 63;
-    const int count = A.ctl[lin];
+    const int count = A.ctl[lin], dcount = A.ctl[4];
     LaneStats st;
     for (;;) {
         int s = 0;
         if (lane == 0) s = atomicAdd(&A.ctl[2], 1);
         s = __shfl(s, 0);
-        if (s >= count) break;
-        const int c = A.lists[size_t(lin) * A.cap + s];
+        const int c = tail_cell(A, lin, count, dcount, s);
+        if (c < 0) break;
         for (;;) {
             double al[NM];
 #pragma unroll
@@ -890,15 +1036,15 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
     __shared__ CS sh;
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
     const int lane = threadIdx.x & 63;
-    const int count = A.ctl[lin];
+    const int count = A.ctl[lin], dcount = A.ctl[4];
     const double p0 = A.p[0];
     LaneStats st;
     for (;;) {
         int s = 0;
         if (lane == 0) s = atomicAdd(&A.ctl[2], 1);
         s = __shfl(s, 0);
-        if (s >= count) break;
-        const int c = A.lists[size_t(lin) * A.cap + s];
+        const int c = tail_cell(A, lin, count, dcount, s);
+        if (c < 0) break;
         for (;;) {
             double al[NM];
 #pragma unroll
@@ -1132,6 +1278,136 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
     }
 }
 
+// ---- mixed-precision sweep on the matrix cores ------------------------------
+//
+// A wave evaluates sixteen cells of the active list at once.  Lane l works for
+// cell l%16 on the quadrature points q = 16T + 4(l/16) + r (r = 0..3) of every
+// 16-point tile T, which is the row layout of the MFMA results below, so no
+// value crosses lanes:
+//   z = alpha.m       v_mfma_f64_16x16x4f64, (16 points x 16 moments) x (16 moments x 16 cells)
+//   e = exp(z)        FP64, per lane
+//   G += e (w m, w)   v_mfma_f64_16x16x4f64: FP64 sums of w e m_i and w e (columns 0..14, 15)
+//   Hm += e (w m2N)   v_mfma_f32_16x16x32_bf16 on hi/lo bf16 pairs of e and of the
+//                     table (three products, FP32 sums): the order-2N moments from
+//                     which the logic forms H through the Gaunt map.
+// The operand tables (built by build_mix_tables) sit in LDS for the whole
+// launch; blocks loop over tiles.
+typedef double mf64x4 __attribute__((ext_vector_type(4)));
+typedef float mf32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 mbf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short mu16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kMixWaves = 16;
+
+template <int NNT>
+struct MixShared {
+    double z[kMixMaxTiles][4][64];
+    double g[kMixMaxTiles][4][64];
+    mu16x8 h[kMixMaxTiles / 2][NNT][2][64];
+};
+
+// x = hi + lo up to 2^-17 |x|, both rounded to nearest even (non-finite x
+// gives a non-finite pair, which fails the factorisation downstream).
+__device__ __forceinline__ void split_bf16(float x, unsigned short &hi, unsigned short &lo) {
+    const unsigned u = __float_as_uint(x);
+    const unsigned r = u + 0x7fffu + ((u >> 16) & 1u);
+    hi = (unsigned short)(r >> 16);
+    const unsigned v = __float_as_uint(x - __uint_as_float(r & 0xffff0000u));
+    lo = (unsigned short)((v + 0x7fffu + ((v >> 16) & 1u)) >> 16);
+}
+
+template <int NM, int NP2, int NNT>
+__global__ __launch_bounds__(64 * kMixWaves) void sweep_mix_kernel(NewtonArgs A, int lin) {
+    using St = Store<NM, NP2>;
+    static_assert(NM <= 15, "moment 15 of the sum tile holds sum w e");
+    static_assert(16 * NNT <= St::nacc, "Hessian moments exceed the store");
+    __shared__ MixShared<NNT> sh;
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl[lin ^ 1] = 0;
+    const int count = A.ctl[lin];
+    const int ntiles = (count + 15) >> 4;
+    if (blockIdx.x * kMixWaves >= ntiles) return;          // uniform per block
+    const int nt = A.mnt, nkb = nt >> 1;
+    {
+        double *sz = &sh.z[0][0][0], *sg = &sh.g[0][0][0];
+        for (int i = threadIdx.x; i < nt * 256; i += blockDim.x) {
+            sz[i] = A.mz[i];
+            sg[i] = A.mg[i];
+        }
+        const mu16x8 *gh = reinterpret_cast<const mu16x8 *>(A.mh);
+        mu16x8 *hh = &sh.h[0][0][0][0];
+        for (int i = threadIdx.x; i < nkb * NNT * 128; i += blockDim.x) hh[i] = gh[i];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ci = lane & 15, g = lane >> 4;
+    const St S{A.st_d, A.st_i, A.cap};
+    for (int tile = blockIdx.x * kMixWaves + wave; tile < ntiles; tile += gridDim.x * kMixWaves) {
+        const int slot = tile * 16 + ci;
+        const int c = slot < count ? A.lists[size_t(lin) * A.cap + slot] : -1;
+        double ab[4];
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int k = 4 * kk + g;
+            ab[kk] = (c >= 0 && k < NM) ? S.d(St::F_AL + k, c) : 0.0;
+        }
+        mf64x4 gacc = {0.0, 0.0, 0.0, 0.0};
+        mf32x4 hacc[NNT];
+#pragma unroll
+        for (int n = 0; n < NNT; ++n) hacc[n] = mf32x4{0.f, 0.f, 0.f, 0.f};
+        for (int kb = 0; kb < nkb; ++kb) {
+            mu16x8 ahi, alo;
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const int T = 2 * kb + hh;
+                mf64x4 z = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk)
+                    z = __builtin_amdgcn_mfma_f64_16x16x4f64(sh.z[T][kk][lane], ab[kk], z, 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const double e = exp(z[r]);
+                    gacc = __builtin_amdgcn_mfma_f64_16x16x4f64(e, sh.g[T][r][lane], gacc, 0, 0, 0);
+                    unsigned short hi, lo;
+                    split_bf16((float)e, hi, lo);
+                    ahi[4 * hh + r] = hi;
+                    alo[4 * hh + r] = lo;
+                }
+            }
+            const mbf16x8 Ahi = __builtin_bit_cast(mbf16x8, ahi), Alo = __builtin_bit_cast(mbf16x8, alo);
+#pragma unroll
+            for (int n = 0; n < NNT; ++n) {
+                const mbf16x8 Bhi = __builtin_bit_cast(mbf16x8, sh.h[kb][n][0][lane]);
+                const mbf16x8 Blo = __builtin_bit_cast(mbf16x8, sh.h[kb][n][1][lane]);
+                hacc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi, Bhi, hacc[n], 0, 0, 0);
+                hacc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi, Blo, hacc[n], 0, 0, 0);
+                hacc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Alo, Bhi, hacc[n], 0, 0, 0);
+            }
+        }
+        // Result row 4(l/16) + r belongs to the cell of slot 16 tile + 4(l/16) + r,
+        // held by lane 4(l/16) + r; column l%16.
+        const int j = lane & 15;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int cr = __shfl(c, 4 * g + r);
+            if (cr < 0) continue;
+            if (j < NM) S.d(St::F_GS + j, cr) = gacc[r];
+            if (j == 15) S.d(St::F_SF, cr) = gacc[r];
+#pragma unroll
+            for (int n = 0; n < NNT; ++n) S.d(St::F_ACC + 16 * n + j, cr) = (double)hacc[n][r];
+        }
+    }
+}
+
+template <int NM, int NP2, bool G, int RS>
+void launch_tail(const NewtonArgs &a, int lin, int tail_blocks, int tail_mode, hipStream_t s) {
+    if (tail_mode == 0)
+        hipLaunchKernelGGL((tail_coop_kernel<NM, NP2, G, RS>), dim3(4 * tail_blocks), dim3(64), 0, s, a, lin);
+    else if (tail_mode == 2)
+        hipLaunchKernelGGL((tail_wave_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin);
+    else
+        hipLaunchKernelGGL((tail_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin);
+}
+
 template <int NM, int NP2, bool G, int RS>
 void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode, hipStream_t s) {
     const int nb = (a.ncells + kBlock - 1) / kBlock;
@@ -1142,12 +1418,22 @@ void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode,
         hipLaunchKernelGGL((logic_kernel<NM, NP2, G>), dim3(nb), dim3(kBlock), 0, s, a, lin);
         lin ^= 1;
     }
-    if (tail_mode == 0)
-        hipLaunchKernelGGL((tail_coop_kernel<NM, NP2, G, RS>), dim3(4 * tail_blocks), dim3(64), 0, s, a, lin);
-    else if (tail_mode == 2)
-        hipLaunchKernelGGL((tail_wave_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin);
-    else
-        hipLaunchKernelGGL((tail_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin);
+    launch_tail<NM, NP2, G, RS>(a, lin, tail_blocks, tail_mode, s);
+}
+
+template <int NM, int NP2, bool G, int RS>
+void launch_mixed_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode, int mix_blocks,
+                      hipStream_t s) {
+    constexpr int NNT = mix_cols(NM) / 16;
+    const int nb = (a.ncells + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
+    int lin = 0;
+    for (int r = 0; r < rounds; ++r) {
+        hipLaunchKernelGGL((sweep_mix_kernel<NM, NP2, NNT>), dim3(mix_blocks), dim3(64 * kMixWaves), 0, s, a, lin);
+        hipLaunchKernelGGL((logic_kernel<NM, NP2, G, true>), dim3(nb), dim3(kBlock), 0, s, a, lin);
+        lin ^= 1;
+    }
+    launch_tail<NM, NP2, G, RS>(a, lin, tail_blocks, tail_mode, s);
 }
 
 }  // namespace
@@ -1155,14 +1441,14 @@ void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode,
 int newton_block() { return kBlock; }
 
 __global__ void ctl_reset_kernel(int *ctl) {
-    if (threadIdx.x < 4) ctl[threadIdx.x] = 0;
+    if (threadIdx.x < kCtlWords) ctl[threadIdx.x] = 0;
 }
 
 void launch_ctl_reset(int *ctl, hipStream_t s) { hipLaunchKernelGGL(ctl_reset_kernel, dim3(1), dim3(64), 0, s, ctl); }
 
 void newton_store_fields(int nm, int np2, int *dfields, int *ifields) {
     const int tri = nm * (nm + 1) / 2;
-    *dfields = 4 * nm + 9 + (np2 > tri ? np2 : tri);
+    *dfields = 5 * nm + 9 + cmax(cmax(np2, tri), mix_cols(nm));
     *ifields = 5;
 }
 
@@ -1170,7 +1456,7 @@ bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int ro
                         int tail_mode, hipStream_t s) {
 #define MNC_CASE(NMV, NP2D, NP2G, RSPLIT)                                                             \
     if (nm == NMV) {                                                                                   \
-        static_assert(Store<NMV, NP2G>::dfields == 4 * NMV + 9 + Store<NMV, NP2G>::nacc, "store");     \
+        static_assert(Store<NMV, NP2G>::dfields == 5 * NMV + 9 + Store<NMV, NP2G>::nacc, "store");     \
         if (gaunt && np2 == NP2G) { launch_all<NMV, NP2G, true, NMV>(a, rounds, tail_blocks, tail_mode, s); return true; } \
         if (!gaunt && np2 == NP2D) { launch_all<NMV, NP2D, false, RSPLIT>(a, rounds, tail_blocks, tail_mode, s); return true; } \
         return false;                                                                                  \
@@ -1183,5 +1469,25 @@ bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int ro
 #undef MNC_CASE
     return false;
 }
+
+bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
+                         int tail_mode, int mix_blocks, hipStream_t s) {
+#define MNC_CASE(NMV, NP2D, NP2G, RSPLIT)                                                                \
+    if (nm == NMV) {                                                                                      \
+        if (a.mnt < 2 || a.mnt > kMixMaxTiles || (a.mnt & 1)) return false;                              \
+        if (gaunt && np2 == NP2G) { launch_mixed_all<NMV, NP2G, true, NMV>(a, rounds, tail_blocks, tail_mode, mix_blocks, s); return true; } \
+        if (!gaunt && np2 == NP2D) { launch_mixed_all<NMV, NP2D, false, RSPLIT>(a, rounds, tail_blocks, tail_mode, mix_blocks, s); return true; } \
+        return false;                                                                                     \
+    }
+    MNC_CASE(1, 1, 1, 1)
+    MNC_CASE(3, 3, 6, 3)
+    MNC_CASE(6, 6, 15, 6)
+    MNC_CASE(10, 10, 28, 10)
+    MNC_CASE(15, 15, 45, 10)
+#undef MNC_CASE
+    return false;
+}
+
+int mix_tiles(int nm) { return mix_cols(nm) / 16; }
 
 }  // namespace mnc

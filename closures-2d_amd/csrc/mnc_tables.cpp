@@ -188,4 +188,74 @@ int build_tables(int moment_order, int quad_order, int use_gaunt, Tables &t) {
     return MNC_OK;
 }
 
+namespace {
+
+// Round-to-nearest-even bf16 of a float (finite inputs).
+unsigned short bf16_rne(float f) {
+    unsigned u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+float bf16_val(unsigned short b) {
+    const unsigned u = unsigned(b) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+}  // namespace
+
+// Operand tables of the mixed-precision sweep.  Quadrature point q of a 16-point
+// tile T sits at row q - 16T of the MFMA operands; the lane layouts are those of
+// v_mfma_f64_16x16x4f64 (lane l: A[l%16][l/16], B[l/16][l%16]) and
+// v_mfma_f32_16x16x32_bf16 (lane l: A[l%16][8(l/16)+t], B[8(l/16)+t][l%16]).
+//   z[T][kk][l] = p[16T + l%16][4kk + l/16]                (alpha.m, A operand)
+//   g[T][r][l]  = w_q p[q][l%16], q = 16T + 4(l/16) + r    (sums, B operand;
+//                 column 15 holds w_q: the sum of w e)
+//   h[kb][nt][hl][l][t] = hi/lo bf16 of w_q p2N[q][16nt + l%16],
+//                 q = 16(2kb + t/4) + 4(l/16) + t%4          (Hessian moments, B)
+int build_mix_tables(int moment_order, int quad_order, MixTables &m) {
+    Tables t;
+    int st = build_tables(moment_order, quad_order, 1, t);   // order-2N table + Gaunt map
+    if (st != MNC_OK) return st;
+    const int nm = t.nm, nq = t.nq, np2 = t.np2;
+    if (nm > 15) return MNC_ERR_ARG;
+    const int nqp = (nq + 31) / 32 * 32;
+    if (nqp / 16 > kMixMaxTiles) return MNC_ERR_ARG;
+    m.nt = nqp / 16;
+    m.nmm = t.nmm;
+    m.nnt = (t.nmm + 15) / 16;
+    auto P = [&](int q, int k) { return (q < nq && k < np2) ? t.p[size_t(q) * np2 + k] : 0.0; };
+    auto W = [&](int q) { return q < nq ? t.w[q] : 0.0; };
+    m.z.assign(size_t(m.nt) * 4 * 64, 0.0);
+    m.g.assign(size_t(m.nt) * 4 * 64, 0.0);
+    for (int T = 0; T < m.nt; ++T)
+        for (int k4 = 0; k4 < 4; ++k4)
+            for (int l = 0; l < 64; ++l) {
+                const int kz = 4 * k4 + l / 16;
+                m.z[(size_t(T) * 4 + k4) * 64 + l] = kz < nm ? P(16 * T + l % 16, kz) : 0.0;
+                const int q = 16 * T + 4 * (l / 16) + k4, j = l % 16;
+                m.g[(size_t(T) * 4 + k4) * 64 + l] = j < nm ? W(q) * P(q, j) : (j == 15 ? W(q) : 0.0);
+            }
+    const int nkb = m.nt / 2;
+    m.h.assign(size_t(nkb) * m.nnt * 2 * 64 * 8, 0);
+    for (int kb = 0; kb < nkb; ++kb)
+        for (int nt = 0; nt < m.nnt; ++nt)
+            for (int l = 0; l < 64; ++l)
+                for (int tt = 0; tt < 8; ++tt) {
+                    const int q = 16 * (2 * kb + tt / 4) + 4 * (l / 16) + tt % 4, col = 16 * nt + l % 16;
+                    const float v = (float)(col < t.nmm ? W(q) * P(q, col) : 0.0);
+                    const unsigned short hi = bf16_rne(v);
+                    const unsigned short lo = bf16_rne(v - bf16_val(hi));
+                    const size_t base = ((size_t(kb) * m.nnt + nt) * 2) * 64 * 8;
+                    m.h[base + size_t(l) * 8 + tt] = hi;
+                    m.h[base + 64 * 8 + size_t(l) * 8 + tt] = lo;
+                }
+    m.gptr = t.gptr;
+    m.gk = t.gk;
+    m.gv = t.gv;
+    return MNC_OK;
+}
+
 }  // namespace mnc

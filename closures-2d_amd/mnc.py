@@ -47,7 +47,7 @@ class GridDesc(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("num_dual_solves", C.c_longlong),
                 ("iter_sum", C.c_longlong), ("hist_reg", C.c_longlong * 5),
-                ("trial_sum", C.c_longlong)]
+                ("trial_sum", C.c_longlong), ("warm_pick", C.c_longlong * 3)]
 
 
 class MncError(RuntimeError):
@@ -98,6 +98,14 @@ def lib():
         L.mnc_get_tune.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
         L.mnc_last_error.argtypes = [vp]
         L.mnc_last_error.restype = C.c_char_p
+        _fp = C.POINTER(C.c_float)
+        L.mnc_mlp_num_params.argtypes = []
+        L.mnc_mlp_init.argtypes = [vp, C.c_ulonglong]
+        L.mnc_mlp_train.argtypes = [vp, C.c_longlong, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_ulonglong, _fp]
+        L.mnc_mlp_predict.argtypes = [vp, C.c_int, _dp, _dp]
+        L.mnc_mlp_get_params.argtypes = [vp, _fp]
+        L.mnc_mlp_set_params.argtypes = [vp, _fp]
+        L.mnc_mlp_gradient.argtypes = [vp, C.c_int, _dp, _dp, _fp, _fp]
         _lib = L
     return _lib
 
@@ -237,8 +245,56 @@ class Context:
     def timing_enable(self, on=True):
         self._check(lib().mnc_timing_enable(self.h, int(on)), "mnc_timing_enable")
 
+    # ---- learned warm start ----
+    def mlp_init(self, seed=1):
+        self._check(lib().mnc_mlp_init(self.h, seed), "mnc_mlp_init")
+
+    def mlp_train(self, u_scaled, alpha_scaled, steps, batch=4096, lr=1e-3, seed=1):
+        """Adam steps on (u/u0, scaled alpha) pairs; returns the per-step losses."""
+        us = np.ascontiguousarray(u_scaled, dtype=np.float64).reshape(-1, 15)
+        a = np.ascontiguousarray(alpha_scaled, dtype=np.float64).reshape(-1, 15)
+        loss = np.zeros(max(steps, 1), dtype=np.float32)
+        self._check(lib().mnc_mlp_train(self.h, us.shape[0], _p(us), _p(a), steps, batch, lr, seed,
+                                        loss.ctypes.data_as(C.POINTER(C.c_float))), "mnc_mlp_train")
+        return loss[:steps]
+
+    def mlp_predict(self, u_scaled):
+        us = np.ascontiguousarray(u_scaled, dtype=np.float64).reshape(-1, 15)
+        out = np.zeros_like(us)
+        self._check(lib().mnc_mlp_predict(self.h, us.shape[0], _p(us), _p(out)), "mnc_mlp_predict")
+        return out
+
+    def mlp_params(self):
+        out = np.zeros(lib().mnc_mlp_num_params(), dtype=np.float32)
+        self._check(lib().mnc_mlp_get_params(self.h, out.ctypes.data_as(C.POINTER(C.c_float))), "mnc_mlp_get_params")
+        return out
+
+    def mlp_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == lib().mnc_mlp_num_params()
+        self._check(lib().mnc_mlp_set_params(self.h, p.ctypes.data_as(C.POINTER(C.c_float))), "mnc_mlp_set_params")
+
+    def mlp_gradient(self, u_scaled, alpha_scaled):
+        us = np.ascontiguousarray(u_scaled, dtype=np.float64).reshape(-1, 15)
+        a = np.ascontiguousarray(alpha_scaled, dtype=np.float64).reshape(-1, 15)
+        g = np.zeros(37007, dtype=np.float32)
+        loss = C.c_float(0.0)
+        self._check(lib().mnc_mlp_gradient(self.h, us.shape[0], _p(us), _p(a), g.ctypes.data_as(C.POINTER(C.c_float)),
+                                           C.byref(loss)), "mnc_mlp_gradient")
+        return g, float(loss.value)
+
     def timing_get(self, kclass):
         n = C.c_longlong(0)
         ms = C.c_double(0.0)
         self._check(lib().mnc_timing_get(self.h, kclass, C.byref(n), C.byref(ms)), "mnc_timing_get")
         return n.value, ms.value
+
+
+def scaled_pairs(u, alpha):
+    """(u/u0, alpha of the u0 = 1 problem) from unscaled moments and multipliers:
+    the shift of opt.cpp:91-92 (alpha_0 - 2 sqrt(pi) ln u0)."""
+    u = np.asarray(u, dtype=np.float64).reshape(-1, u.shape[-1])
+    a = np.array(alpha, dtype=np.float64).reshape(-1, u.shape[-1])
+    us = u / u[:, :1]
+    a[:, 0] -= 2.0 * np.sqrt(np.pi) * np.log(u[:, 0])
+    return us, a

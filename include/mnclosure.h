@@ -100,6 +100,8 @@ typedef struct {
     long long iter_sum;          /* Newton iterations (full f, g, H evaluations)  */
     long long hist_reg[5];
     long long trial_sum;         /* Armijo trial evaluations (f only)             */
+    long long warm_pick[3];      /* MNC_WARM_LEARNED starts taken from: [0] the
+                                    MLP, [1] isotropicAlpha, [2] the alpha passed in */
 } mnc_stats;
 
 /* Creates a context: builds the quadrature, the real spherical-harmonic table
@@ -161,6 +163,12 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *   "tail_blocks"  workgroups of the persistent tail kernel (1-65536)
  *   "tail_mode"    0: one wave per remaining cell, dense algebra shared in LDS;
  *                  1: one lane per cell; 2: one wave per cell, algebra per lane
+ *   "mix_cond_log10"  MNC_PREC_MIXED: a Newton step uses the mixed-precision
+ *                  Hessian only while its condition estimate is <= 10^value
+ *                  (0-10, default 4); otherwise the step is redone in FP64
+ *   "mix_blocks"   workgroups of the mixed-precision sweep (1-65536)
+ * mnc_get_tune also reads "has_mixed" (1 if this context's order and
+ * quadrature have the mixed-precision kernels: N <= 4, Q <= 16).
  * Returns MNC_ERR_ARG for an unknown key or an out-of-range value. */
 int mnc_tune(mnc_ctx *ctx, const char *key, int value);
 int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value);
@@ -180,6 +188,37 @@ int mnc_timing_enable(mnc_ctx *ctx, int on);
 int mnc_timing_get(mnc_ctx *ctx, int kclass, long long *launches, double *ms);
 /* Last error text for the most recent failing call on this context. */
 const char *mnc_last_error(const mnc_ctx *ctx);
+
+/* ---- learned warm start (no counterpart in the reference, which starts every
+ * solve from the previous step's alpha: momopt_update.cpp:236-238) ----
+ *
+ * An MLP 15 -> 128 -> 128 -> 128 -> 15 (GELU) maps the scaled moments u/u0 of
+ * an M4 closure to the multipliers of the scaled (u0 = 1) problem.  It is
+ * trained and evaluated on the context's device (bf16 matrix-core products,
+ * FP32 sums and Adam).  With MNC_WARM_LEARNED the Newton prologue evaluates
+ * the network for every cell and starts from whichever of {MLP guess,
+ * isotropicAlpha, the alpha passed in} has the smallest FP64 gradient norm.
+ * The parameter block is mnc_mlp_num_params() floats: the 37,007 weights and
+ * biases (W1[15][128] b1 W2[128][128] b2 W3[128][128] b3 W4[128][15] b4)
+ * followed by the input/target standardisation (in_mean, in_std, out_mean,
+ * out_std; 15 each).  Only moment order 4 (n = 15) has a network. */
+int mnc_mlp_num_params(void);
+/* Random initial weights (uniform, fan-in scaled; SplitMix64 stream `seed`). */
+int mnc_mlp_init(mnc_ctx *ctx, unsigned long long seed);
+/* Adam steps on minibatches drawn with replacement from npairs host pairs
+ * (u_scaled[npairs][15], alpha_scaled[npairs][15]).  The first call after
+ * mnc_mlp_init fixes the standardisation from these pairs.  loss_out[steps]
+ * (may be NULL) receives each step's mean squared error in standardised units. */
+int mnc_mlp_train(mnc_ctx *ctx, long long npairs, const double *u_scaled, const double *alpha_scaled, int steps,
+                  int batch, double lr, unsigned long long seed, float *loss_out);
+/* Network output for n host rows of scaled moments -> scaled multipliers. */
+int mnc_mlp_predict(mnc_ctx *ctx, int n, const double *u_scaled, double *alpha_scaled);
+int mnc_mlp_get_params(mnc_ctx *ctx, float *params);
+int mnc_mlp_set_params(mnc_ctx *ctx, const float *params);
+/* One training step's gradient (no update) on the given rows, for checking:
+ * grad[37007], loss = mean squared error in standardised units. */
+int mnc_mlp_gradient(mnc_ctx *ctx, int rows, const double *u_scaled, const double *alpha_scaled, float *grad,
+                     float *loss);
 
 #ifdef __cplusplus
 }

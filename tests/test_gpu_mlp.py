@@ -1,0 +1,150 @@
+"""The learned warm start on the GPU: the network's forward pass and gradient
+against the numpy restatement (tests/mlp_ref.py, itself pinned by finite
+differences), training, and Newton solves started from its guesses."""
+import numpy as np
+import pytest
+
+import mlp_ref
+from helpers import config2_batch, isotropic_start, load_pkg, oracle
+
+
+def _ctx():
+    mnc = load_pkg("mnc")
+    return mnc, mnc.Context(4, 16, use_gaunt=False)
+
+
+def _params_with_stats(ctx, rng):
+    ctx.mlp_init(seed=11)
+    p = ctx.mlp_params().astype(np.float64)
+    # non-trivial standardisation
+    p[mlp_ref.NPARAM:mlp_ref.NPARAM + 15] = rng.normal(0, 0.1, 15)
+    p[mlp_ref.NPARAM + 15:mlp_ref.NPARAM + 30] = rng.uniform(0.2, 2.0, 15)
+    p[mlp_ref.NPARAM + 30:mlp_ref.NPARAM + 45] = rng.normal(0, 1.0, 15)
+    p[mlp_ref.NPARAM + 45:mlp_ref.NPARAM + 60] = rng.uniform(0.5, 3.0, 15)
+    # non-zero biases
+    layers_off = [15 * 128, 15 * 128 + 128 + 128 * 128, 15 * 128 + 2 * (128 + 128 * 128)]
+    for o in layers_off:
+        p[o:o + 128] = rng.normal(0, 0.1, 128)
+    p[mlp_ref.NPARAM - 15:mlp_ref.NPARAM] = rng.normal(0, 0.1, 15)
+    p = p.astype(np.float32)
+    ctx.mlp_set_params(p)
+    return p.astype(np.float64)
+
+
+@pytest.mark.gpu
+def test_mlp_forward_matches_reference():
+    print("case mlp forward", flush=True)
+    mnc, ctx = _ctx()
+    rng = np.random.default_rng(1)
+    p = _params_with_stats(ctx, rng)
+    us = rng.normal(0, 0.3, (3000, 15))
+    us[:, 0] = 1.0
+    got = ctx.mlp_predict(us)
+    x, _ = mlp_ref.standardise(p, us)
+    _, st = mlp_ref.unpack(p)
+    y = mlp_ref.forward(p, x, bf=True)[1][-1] * st[3] + st[2]
+    y64 = mlp_ref.forward(p, x, bf=False)[1][-1] * st[3] + st[2]
+    err = np.abs(got - y) / (np.abs(y).max() + 1e-30)
+    err64 = np.abs(got - y64) / (np.abs(y64).max() + 1e-30)
+    print("forward vs bf16 reference: max %.3e median %.3e; vs exact fp64: max %.3e" %
+          (err.max(), np.median(err), err64.max()))
+    assert np.median(err) <= 1e-5
+    assert err.max() <= 1e-2
+    assert err64.max() <= 5e-2
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_mlp_gradient_matches_reference():
+    print("case mlp gradient", flush=True)
+    mnc, ctx = _ctx()
+    rng = np.random.default_rng(2)
+    p = _params_with_stats(ctx, rng)
+    us = rng.normal(0, 0.3, (512, 15))
+    us[:, 0] = 1.0
+    al = rng.normal(0, 2.0, (512, 15))
+    g, loss = ctx.mlp_gradient(us, al)
+    x, t = mlp_ref.standardise(p, us, al)
+    lref, gref = mlp_ref.loss_and_grad(p, x, t, bf=True)
+    print("loss gpu %.8e ref %.8e" % (loss, lref))
+    assert abs(loss - lref) <= 1e-4 * lref
+    off = 0
+    for i in range(4):
+        a, b = mlp_ref.DIMS[i], mlp_ref.DIMS[i + 1]
+        for name, n in (("W%d" % (i + 1), a * b), ("b%d" % (i + 1), b)):
+            gg, gr = g[off:off + n], gref[off:off + n]
+            rel = np.linalg.norm(gg - gr) / (np.linalg.norm(gr) + 1e-30)
+            print("%s: rel err %.3e" % (name, rel))
+            assert rel <= 1e-2, name
+            off += n
+    ctx.close()
+
+
+def _smooth_pairs(mnc, tab, n, seed):
+    rng = np.random.default_rng(seed)
+    u, astar, lobe = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, n, rng, near_boundary_frac=0.0)
+    return mnc.scaled_pairs(u, astar)
+
+
+@pytest.mark.gpu
+def test_mlp_training_reduces_loss():
+    print("case mlp training", flush=True)
+    mnc, ctx = _ctx()
+    tab = oracle().Tables(4, 16, False)
+    us, al = _smooth_pairs(mnc, tab, 100000, 3)
+    ctx.mlp_init(seed=5)
+    loss = ctx.mlp_train(us, al, steps=600, batch=4096, lr=1e-3, seed=9)
+    first, last = loss[:10].mean(), loss[-20:].mean()
+    print("training loss: first %.4e last %.4e" % (first, last))
+    assert np.all(np.isfinite(loss))
+    assert last <= 0.1 * first
+    # held-out pairs: prediction error well below the spread of the targets
+    ut, at = _smooth_pairs(mnc, tab, 5000, 4)
+    pred = ctx.mlp_predict(ut)
+    rel = np.linalg.norm(pred - at) / np.linalg.norm(at - at.mean(axis=0))
+    print("held-out relative error %.3e" % rel)
+    assert rel <= 0.3
+    ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_learned_warm_start_solves_like_oracle(precision):
+    """Train on exact smooth pairs, then solve fresh inputs (20% near-boundary
+    lobes) from the learned start: every answer is a solution to the same
+    tolerance as the oracle's from the isotropic start, and the learned start
+    needs fewer Newton iterations than the isotropic one."""
+    print("case learned warm start %s" % precision, flush=True)
+    mnc, ctx = _ctx()
+    o = oracle()
+    tab = o.Tables(4, 16, False)
+    us, al = _smooth_pairs(mnc, tab, 200000, 6)
+    ctx.mlp_init(seed=7)
+    ctx.mlp_train(us, al, steps=1500, batch=4096, lr=1e-3, seed=8)
+    rng = np.random.default_rng(10)
+    u, astar, lobe = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, 3000, rng, near_boundary_frac=0.2)
+    a0 = isotropic_start(u)
+    prec = mnc.PREC_MIXED if precision == "mixed" else mnc.PREC_FP64
+    tol = 1e-8
+    res = {}
+    for warm in (mnc.WARM_ISOTROPIC, mnc.WARM_LEARNED):
+        ctx.reset_stats()
+        opt = mnc.options(max_iter=100, tol=tol, cond_h_max=1e10, precision=prec, warm_start=warm)
+        res[warm] = ctx.opt_batch_host(u, a0, opt) + (ctx.stats(),)
+    sto, uo, ao, ito, ro, _ = tab.opt_batch(u, a0, max_iter=100, tol=tol, cond_h_max=1e10, nthreads=8)
+    ctx.close()
+    st, ug, ag, itg, rg, ngg, stats = res[mnc.WARM_LEARNED]
+    iso_iters = res[mnc.WARM_ISOTROPIC][3]
+    picks = list(stats.warm_pick)
+    print("iterations: isotropic %.3f learned %.3f (oracle %.3f); picks mlp/iso/prev %s"
+          % (iso_iters.mean(), itg.mean(), ito.mean(), picks))
+    assert st == 0 and sto == 0
+    assert sum(picks) == len(u)
+    assert picks[0] >= 0.5 * len(u)
+    assert itg.mean() < iso_iters.mean()
+    assert np.array_equal(ug[:, 0], u[:, 0])
+    rs = rg == ro
+    assert rs.mean() >= 0.97
+    un = np.linalg.norm(ug[rs] - uo[rs], axis=1) / np.linalg.norm(uo[rs], axis=1)
+    print("same-r cells: max ||u - u_oracle|| / ||u|| = %.3e" % un.max())
+    assert un.max() <= 4 * tol
