@@ -16,6 +16,17 @@ A "step" is one Heun time step of the whole lattice.  W warm-up steps run first
 (then the state is reset to t = 0); the timed region is the first K steps from
 t = 0 (K = 569 is the whole run), with all state resident in HBM.
 
+Three forms run on the same steps:
+  plain-fp64    FP64 batched Newton from the previous step's alpha (the reference's method)
+  mixed         mixed-precision lockstep rounds (MFMA Hessian), FP64 tail
+  learned+mixed the same, started from the best of {MLP guess, isotropic alpha,
+                previous alpha} by FP64 gradient norm; the MLP is first trained on
+                GPU with the library's own kernels on (u, alpha) pairs harvested from
+                plain-form lattice steps and an M4 Gaussian-pulse run (not timed,
+                reported in config.mlp)
+`value` is the learned+mixed form's rate (the metric's headline form); the
+other forms are reported beside it.
+
 Prints one JSON line.  Single GPU only.
 """
 import argparse
@@ -66,15 +77,67 @@ def make_grid(mnc, gi, d):
     return ctx, g
 
 
-def run_form(mnc, gi, d, opt, steps, warmup):
+def harvest(mnc, gi, d, steps, every):
+    """(u/u0, scaled alpha) pairs of every cell after every `every`-th plain step."""
     ctx, g = make_grid(mnc, gi, d)
+    dts = gi.time_steps(g.max_dt, T_FINAL)
+    opt = mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"], line_search=mnc.LS_ROBUST)
+    us, al = [], []
+    for k in range(steps):
+        ctx.update(dts[k % len(dts)], opt)
+        if (k + 1) % every == 0:
+            m, a = ctx.grid_get()
+            u1, a1 = mnc.scaled_pairs(m.reshape(-1, ctx.nm), a.reshape(-1, ctx.nm))
+            us.append(u1)
+            al.append(a1)
+    ctx.close()
+    return np.concatenate(us), np.concatenate(al)
+
+
+def train_mlp(mnc, gi, d, pairs_target=2_000_000, train_steps=2000, seed=1):
+    """Trains the warm-start network on pairs from plain runs: the lattice's
+    first steps and an M4 Gaussian pulse (50 x 50 on [-1,1]^2, sigma 0.1, floor
+    1e-4).  Returns (params, info)."""
+    t0 = time.perf_counter()
+    u1, a1 = harvest(mnc, gi, d, 48, 2)
+    pulse = dict(d, num_cells_x=50, num_cells_y=50, a_x=-1.0, b_x=1.0, a_y=-1.0, b_y=1.0, gaussian_sigma=0.1,
+                 floor_value=1e-4, init_cond=0)
+    u2, a2 = harvest(mnc, gi, pulse, 100, 1)
+    us, al = np.concatenate([u1, u2]), np.concatenate([a1, a2])
+    if len(us) > pairs_target:
+        idx = np.random.default_rng(seed).choice(len(us), pairs_target, replace=False)
+        us, al = us[idx], al[idx]
+    t1 = time.perf_counter()
+    ctx = mnc.Context(d["moment_order"], d["quad_order"], use_gaunt=d["use_gaunt"])
+    ctx.mlp_init(seed)
+    ctx.timing_enable(True)
+    loss = ctx.mlp_train(us, al, steps=train_steps, batch=4096, lr=1e-3, seed=seed)
+    ctx.sync()
+    n_mlp, ms_mlp = ctx.timing_get(mnc.KCLASS_MLP)
+    params = ctx.mlp_params()
+    ctx.close()
+    info = {"pairs": int(len(us)), "harvest_s": t1 - t0, "train_steps": train_steps, "batch": 4096, "lr": 1e-3,
+            "train_gpu_ms": ms_mlp, "loss_first": float(loss[:10].mean()), "loss_last": float(loss[-50:].mean())}
+    return params, info
+
+
+def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=60, autotune=True):
+    """W warm-up steps, the library's autotuner on the state they reach, then
+    (from t = 0 again) K steps timed by wall clock with the tuned launch shape
+    (hipGraph replay if the tuner picked it; per-launch timing off), then the
+    first min(K, kernel_steps) steps again, eagerly, with HIP-event timing of
+    every launch for the per-kernel-class split and the roofline."""
+    ctx, g = make_grid(mnc, gi, d)
+    if mlp_params is not None:
+        ctx.mlp_set_params(mlp_params)
     dts = gi.time_steps(g.max_dt, T_FINAL)
     for k in range(min(warmup, len(dts))):
         ctx.update(dts[k], opt)
     ctx.sync()
+    report = ctx.autotune(dts[0], opt, steps=3) if autotune else []
+    tuned = {k: ctx.get_tune(k) for k in ("graph", "rounds", "tail_mode", "tail_blocks", "mix_blocks")}
     ctx.grid_set(g.moments, g.alpha)        # back to t = 0
     ctx.reset_stats()
-    ctx.timing_enable(True)
     ctx.sync()
     t0 = time.perf_counter()
     for k in range(steps):
@@ -82,16 +145,102 @@ def run_form(mnc, gi, d, opt, steps, warmup):
     ctx.sync()
     wall = time.perf_counter() - t0
     st = ctx.stats()
+    m, a = ctx.grid_get()
+    # eager pass with per-launch timing
+    ks = min(steps, kernel_steps)
+    ctx.grid_set(g.moments, g.alpha)
+    ctx.reset_stats()
+    ctx.timing_enable(True)
+    for k in range(ks):
+        ctx.update(dts[k % len(dts)], opt)
+    ctx.sync()
+    kst = ctx.stats()
     kn, kms = ctx.timing_get(mnc.KCLASS_NEWTON)
     fn, fms = ctx.timing_get(mnc.KCLASS_FLUX)
     on, oms = ctx.timing_get(mnc.KCLASS_OTHER)
+    mn, mms = ctx.timing_get(mnc.KCLASS_MLP)
     ctx.timing_enable(False)
-    m, a = ctx.grid_get()
     ctx.close()
     closures = 2 * (g.nx + 4) * (g.ny + 4) * steps
     assert st.num_dual_solves == closures, (st.num_dual_solves, closures)
-    return dict(wall=wall, stats=st, newton=(kn, kms), flux=(fn, fms), other=(on, oms), closures=closures,
-                moments=m, dts=dts[:steps])
+    return dict(wall=wall, stats=st, kstats=kst, kernel_steps=ks, newton=(kn, kms), flux=(fn, fms), other=(on, oms),
+                mlp=(mn, mms), closures=closures, moments=m, dts=dts[:steps], tuned=tuned, tune_report=report)
+
+
+def form_summary(r, f_iter, f_trial):
+    st, kst = r["stats"], r["kstats"]
+    kn, kms = r["newton"]
+    return {"closures_per_s": r["closures"] / r["wall"], "s_per_timestep": r["wall"] / len(r["dts"]),
+            "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
+            "trials_per_closure": st.trial_sum / st.num_dual_solves, "max_iter": st.max_iter,
+            "reg_hist": list(st.hist_reg), "warm_pick_mlp_iso_prev": list(st.warm_pick),
+            "tuned": r["tuned"],
+            "kernel_timed_steps": r["kernel_steps"],
+            "kernel_ms": {"newton": kms, "flux": r["flux"][1], "other": r["other"][1], "mlp": r["mlp"][1]},
+            "newton_tflops_fp64_equiv": (kst.iter_sum * f_iter + kst.trial_sum * f_trial) / (kms * 1e-3) / 1e12
+            if kms > 0 else 0.0}
+
+
+def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
+    """The isolated closure batch (n M4 vectors, Q = 16, 20% near-boundary
+    lobes, isotropic start, TOL 1e-8, reference line search): converged
+    closures/s of each form with u and alpha resident in HBM (torch tensors as
+    device buffers), and each form's error against the FP64 CPU oracle on the
+    first `oracle_sample` vectors."""
+    import torch
+    sys.path.insert(0, ROOT)
+    import oracle
+    wl = _load("workloads")
+    ctx = mnc.Context(4, 16, use_gaunt=False)
+    w, p, xi, eta = ctx.tables()
+    t0 = time.perf_counter()
+    u, astar, lobe = wl.isolated_batch(w, p, xi, eta, 15, n, seed)
+    a0 = wl.isotropic_start(u)
+    gen_s = time.perf_counter() - t0
+    tab = oracle.Tables(4, 16, False)
+    m = oracle_sample
+    sto, uo, ao, ito, ro, _ = tab.opt_batch(u[:m], a0[:m], max_iter=100, tol=1e-8, cond_h_max=1e10,
+                                            nthreads=len(os.sched_getaffinity(0)))
+    # learned form: network trained on separate exact (u, alpha*) pairs of the same distribution
+    ut, at, lt = wl.isolated_batch(w, p, xi, eta, 15, 2_000_000, seed + 1, near_boundary_frac=0.0)
+    ust, ast_ = mnc.scaled_pairs(ut, at)
+    ctx.mlp_init(seed)
+    loss = ctx.mlp_train(ust, ast_, steps=train_steps, batch=4096, lr=1e-3, seed=seed)
+    dev = torch.device("cuda:0")
+    u_src = torch.from_numpy(u).to(dev)
+    a_src = torch.from_numpy(a0).to(dev)
+    u_d = torch.empty_like(u_src)
+    a_d = torch.empty_like(a_src)
+    res = {}
+    for name, kw in (("plain-fp64", {}), ("mixed", {"precision": mnc.PREC_MIXED}),
+                     ("learned+mixed", {"precision": mnc.PREC_MIXED, "warm_start": mnc.WARM_LEARNED})):
+        opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, **kw)
+        best = None
+        for rep_ in range(3):                      # first run allocates; best of the rest
+            u_d.copy_(u_src)
+            a_d.copy_(a_src)
+            torch.cuda.synchronize()
+            ctx.reset_stats()
+            t1 = time.perf_counter()
+            ctx.opt_batch_device(n, u_d.data_ptr(), a_d.data_ptr(), opt)
+            dt = time.perf_counter() - t1
+            if rep_ > 0 and (best is None or dt < best):
+                best = dt
+        st = ctx.stats()
+        ug = u_d[:m].cpu().numpy()
+        same_r = None
+        err = np.linalg.norm(ug - uo, axis=1) / np.linalg.norm(uo, axis=1)
+        res[name] = {"closures_per_s": n / best, "s": best,
+                     "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
+                     "trials_per_closure": st.trial_sum / st.num_dual_solves,
+                     "reg_hist": list(st.hist_reg), "warm_pick_mlp_iso_prev": list(st.warm_pick),
+                     "oracle_max_rel_u_err": float(err.max()), "oracle_median_rel_u_err": float(np.median(err))}
+    ctx.close()
+    return {"workload": "isolated M4 batch, %d vectors (20%% near-boundary lobes), Q=16, isotropic start, "
+                        "TOL 1e-8, direct Hessian, reference line search" % n,
+            "generate_s": gen_s, "oracle_sample": m, "mlp_loss_first_last": [float(loss[:10].mean()),
+                                                                           float(loss[-50:].mean())],
+            "forms": res}
 
 
 def cpu_baseline(d, seconds):
@@ -123,6 +272,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-only", action="store_true", help="only the plain FP64 form")
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--isolated", type=int, default=1_000_000,
+                    help="vectors of the isolated closure batch (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's hot path runs on a single MI355X "
@@ -132,19 +285,33 @@ def main():
     d = dict(LATTICE)
     n = (d["moment_order"] + 1) * (d["moment_order"] + 2) // 2
     nq = d["quad_order"] ** 2
-    plain = run_form(mnc, gi, d, mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"],
-                                             line_search=mnc.LS_ROBUST),
-                     args.steps, args.warmup)
-    best = plain
-    form = "plain-fp64"
-    st = best["stats"]
-    value = best["closures"] / best["wall"]
     f_iter, f_trial = flops_per_iter(n, nq)
+
+    def opts(**kw):
+        return mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"],
+                           line_search=mnc.LS_ROBUST, **kw)
+
+    plain = run_form(mnc, gi, d, opts(), args.steps, args.warmup)
+    forms = {"plain-fp64": plain}
+    mlp_info = None
+    if not args.plain_only:
+        forms["mixed"] = run_form(mnc, gi, d, opts(precision=mnc.PREC_MIXED), args.steps, args.warmup)
+        params, mlp_info = train_mlp(mnc, gi, d, train_steps=args.train_steps)
+        forms["learned+mixed"] = run_form(mnc, gi, d, opts(precision=mnc.PREC_MIXED, warm_start=mnc.WARM_LEARNED),
+                                          args.steps, args.warmup, mlp_params=params)
+    form = "learned+mixed" if "learned+mixed" in forms else "plain-fp64"
+    best = forms[form]
+    value = best["closures"] / best["wall"]
     kn, kms = best["newton"]
-    alg_flops = st.iter_sum * f_iter + st.trial_sum * f_trial
+    kst = best["kstats"]
+    alg_flops = kst.iter_sum * f_iter + kst.trial_sum * f_trial
     achieved = alg_flops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    # drift of each form's final state from the plain form's (density rel-L2)
+    dens = {k: v["moments"][2:-2, 2:-2, 0] for k, v in forms.items()}
+    drift = {k: float(np.sqrt(((dens[k] - dens["plain-fp64"]) ** 2).sum() / (dens["plain-fp64"] ** 2).sum()))
+             for k in forms}
     out = {
-        "metric": "converged Mn closures/s (M4 lattice 280^2, t=0..3.2, TOL 1e-8)",
+        "metric": "converged Mn closures/s (M4 lattice 280^2, t=0..3.2, TOL 1e-8), learned+mixed vs plain",
         "value": value,
         "unit": "closures/s",
         "n_gpus": 1,
@@ -154,24 +321,27 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if form == "plain-fp64" else "f64+bf16",
         "data": "synthetic (the reference's lattice initial condition, generated in-process)",
         "config": {"workload": "M4 lattice 280x280, Q=16 (nq=256), 7x7 checkerboard on [-3.5,3.5]^2, "
                                "floor 1e-10, TOL 1e-8, direct Hessian, robust line search, Heun dt=%.6g" % plain["dts"][0],
                    "form": form, "closures_per_step": 2 * 284 * 284,
                    "s_per_timestep": best["wall"] / args.steps,
                    "plain_fp64_closures_per_s": plain["closures"] / plain["wall"],
-                   "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
-                   "trials_per_closure": st.trial_sum / st.num_dual_solves,
-                   "max_iter": st.max_iter, "reg_hist": list(st.hist_reg),
-                   "kernel_ms": {"newton": kms, "flux": best["flux"][1], "other": best["other"][1]},
-                   "kernel_launches": {"newton": kn, "flux": best["flux"][0], "other": best["other"][0]}},
+                   "speedup_vs_plain_fp64": value / (plain["closures"] / plain["wall"]),
+                   "forms": {k: form_summary(v, f_iter, f_trial) for k, v in forms.items()},
+                   "final_density_rel_l2_vs_plain": drift,
+                   "mlp": mlp_info},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "Newton solve (init + sweep/logic rounds + tail), M4 direct FP64",
+                     "kernel": "Newton solve per stage (init + lockstep rounds + tail), %s form, HIP events "
+                               "over the first %d steps; flops are the reference algorithm's FP64 count "
+                               "(iterations x 73.0 kflop + trials x 8.2 kflop)" % (form, best["kernel_steps"]),
                      "algorithmic_flops_per_launch": alg_flops / max(kn, 1),
                      "avg_launch_ms": kms / max(kn, 1)},
     }
+    if args.isolated > 0 and not args.plain_only:
+        out["config"]["isolated"] = bench_isolated(mnc, args.isolated, 12345, 20000, args.train_steps)
     if not args.no_cpu:
         cb = cpu_baseline(d, args.cpu_seconds)
         out["cpu_baseline"] = {"value": cb["value"], "unit": "closures/s", "cores": cb["cores"], "kind": "port",

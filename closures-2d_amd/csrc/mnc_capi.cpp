@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -64,6 +65,18 @@ struct mnc_ctx {
     bool timing = false;
     struct Ev { int kc; hipEvent_t a, b; };
     std::vector<Ev> events;
+    // hipGraphs of whole mnc_update steps, one per distinct (dt, options)
+    int use_graph = 0;
+    unsigned long long epoch = 1;    // bumped by every device (re)allocation and tuning change
+    struct StepGraph {
+        unsigned long long epoch = 0;
+        double dt = 0.0;
+        mnc_options o{};
+        int eager_runs = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+    StepGraph graphs[4];
+    int graph_next = 0;
     long long t_launch[MNC_NUM_KCLASS] = {0, 0, 0, 0};
     double t_ms[MNC_NUM_KCLASS] = {0, 0, 0, 0};
     std::string last_error;
@@ -85,10 +98,16 @@ int ctx_fail(mnc_ctx *ctx, int code, const char *what, hipError_t e = hipSuccess
 
 template <class T>
 int dalloc(mnc_ctx *ctx, T **p, size_t n) {
+    ++ctx->epoch;    // captured steps hold device addresses
     if (*p) { (void)hipFree(*p); *p = nullptr; }
     if (n == 0) return MNC_OK;
     MNC_HIP_CHECK(hipMalloc((void **)p, n * sizeof(T)));
     return MNC_OK;
+}
+
+void drop_graph(mnc_ctx::StepGraph &g) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = mnc_ctx::StepGraph{};
 }
 
 // Records an event pair around one launch when timing is on.
@@ -386,6 +405,7 @@ void mnc_destroy(mnc_ctx *ctx) {
     if (ctx->device < 0) { delete ctx; return; }
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &g : ctx->graphs) drop_graph(g);
     for (auto &ev : ctx->events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
     void *ptrs[] = {ctx->d_w, ctx->d_p, ctx->d_xi, ctx->d_eta, ctx->d_gv, ctx->d_gptr, ctx->d_gk,
                     ctx->d_mz, ctx->d_mg, ctx->d_mh, ctx->d_mgptr, ctx->d_mgk, ctx->d_mgv,
@@ -550,14 +570,17 @@ int mnc_flux(mnc_ctx *ctx, double *flux_host) {
     return MNC_OK;
 }
 
-int mnc_update(mnc_ctx *ctx, double dt, const mnc_options *opt) {
-    if (!ctx) return MNC_ERR_ARG;
-    if (!ctx->has_grid) return ctx_fail(ctx, MNC_ERR_STATE, "no grid");
-    int st = check_opts(ctx, opt);
-    if (st) return st;
-    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+}  // extern "C"
+
+namespace {
+
+// One Heun step (MomOptSolver::update, momopt_update.cpp:283-382) enqueued on
+// the context's stream with no host synchronisation and no allocation once the
+// buffers exist, so that it can be captured.
+int enqueue_step(mnc_ctx *ctx, double dt, const mnc_options *opt) {
     const int nc = (ctx->grid.num_cells_x + 4) * (ctx->grid.num_cells_y + 4);
     const size_t n = size_t(nc) * ctx->tab.nm;
+    int st;
     // momentsOld <- moments before the first solve (momopt_update.cpp:299-308)
     MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_old, ctx->d_mom, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     for (int stage = 1; stage <= 2; ++stage) {
@@ -567,6 +590,66 @@ int mnc_update(mnc_ctx *ctx, double dt, const mnc_options *opt) {
         // kernels flag them in device memory; the state is left as computed)
         if ((st = flux_stage(ctx, stage, dt))) return st;
     }
+    return MNC_OK;
+}
+
+bool same_opts(const mnc_options &a, const mnc_options &b) {
+    return a.max_iter == b.max_iter && a.tol_rel == b.tol_rel && a.cond_h_max == b.cond_h_max &&
+           a.precision == b.precision && a.warm_start == b.warm_start && a.line_search == b.line_search;
+}
+
+// Captures enqueue_step into g.exec.  On any capture error the context falls
+// back to eager launches of the same kernels for the rest of its life.
+int capture_step(mnc_ctx *ctx, mnc_ctx::StepGraph &g, double dt, const mnc_options *opt) {
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    int st = (e == hipSuccess) ? enqueue_step(ctx, dt, opt) : MNC_ERR_HIP;
+    hipError_t e2 = (e == hipSuccess) ? hipStreamEndCapture(ctx->stream, &graph) : e;
+    if (st == MNC_OK && e2 == hipSuccess && graph)
+        e2 = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    if (st != MNC_OK || e2 != hipSuccess || !g.exec) {
+        drop_graph(g);
+        ctx->use_graph = 0;
+        return MNC_ERR_HIP;
+    }
+    return MNC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mnc_update(mnc_ctx *ctx, double dt, const mnc_options *opt) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (!ctx->has_grid) return ctx_fail(ctx, MNC_ERR_STATE, "no grid");
+    int st = check_opts(ctx, opt);
+    if (st) return st;
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    if (ctx->use_graph && !ctx->timing) {   // per-launch timing needs the eager path
+        mnc_ctx::StepGraph *g = nullptr;
+        for (auto &c : ctx->graphs)
+            if (c.epoch == ctx->epoch && c.dt == dt && same_opts(c.o, *opt)) g = &c;
+        if (g && !g->exec && g->eager_runs >= 1 && capture_step(ctx, *g, dt, opt) != MNC_OK) g = nullptr;
+        if (g && g->exec) {
+            MNC_HIP_CHECK(hipGraphLaunch(g->exec, ctx->stream));
+            return check_errors(ctx);
+        }
+        if (ctx->use_graph && !g) {   // first step with this key: eager, then captured from the next one
+            g = &ctx->graphs[ctx->graph_next];
+            ctx->graph_next = (ctx->graph_next + 1) % 4;
+            drop_graph(*g);
+            if ((st = enqueue_step(ctx, dt, opt))) return st;
+            st = check_errors(ctx);
+            g->epoch = ctx->epoch;     // after the step's own allocations
+            g->dt = dt;
+            g->o = *opt;
+            g->eager_runs = 1;
+            return st;
+        }
+    }
+    if ((st = enqueue_step(ctx, dt, opt))) return st;
     return check_errors(ctx);
 }
 
@@ -633,6 +716,12 @@ int mnc_timing_get(mnc_ctx *ctx, int kclass, long long *launches, double *ms) {
 
 int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
     if (!ctx || !key) return MNC_ERR_ARG;
+    ++ctx->epoch;
+    if (std::strcmp(key, "graph") == 0) {
+        if (value < 0 || value > 1) return ctx_fail(ctx, MNC_ERR_ARG, "graph must be 0 or 1");
+        ctx->use_graph = value;
+        return MNC_OK;
+    }
     if (std::strcmp(key, "rounds") == 0) {
         if (value < 0 || value > 64) return ctx_fail(ctx, MNC_ERR_ARG, "rounds out of range");
         ctx->rounds = value;
@@ -671,6 +760,7 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
         return MNC_OK;
     }
     if (std::strcmp(key, "has_mixed") == 0) { *value = ctx->has_mix ? 1 : 0; return MNC_OK; }
+    if (std::strcmp(key, "graph") == 0) { *value = ctx->use_graph; return MNC_OK; }
     if (std::strcmp(key, "tail_blocks") == 0) {
         *value = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
         return MNC_OK;
@@ -679,6 +769,88 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
 }
 
 const char *mnc_last_error(const mnc_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
+
+int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, char *report, int report_len) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (!ctx->has_grid) return ctx_fail(ctx, MNC_ERR_STATE, "no grid");
+    int st = check_opts(ctx, opt);
+    if (st) return st;
+    if (steps < 1 || steps > 1000 || !(dt > 0.0)) return ctx_fail(ctx, MNC_ERR_ARG, "bad autotune arguments");
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = size_t(ctx->grid.num_cells_x + 4) * (ctx->grid.num_cells_y + 4) * ctx->tab.nm;
+    double *bk = nullptr;
+    MNC_HIP_CHECK(hipMalloc((void **)&bk, 2 * n * sizeof(double)));
+    long long stats[mnc::kStatWords];
+    std::string rep;
+    auto save = [&]() -> int {
+        MNC_HIP_CHECK(hipMemcpyAsync(bk, ctx->d_mom, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        MNC_HIP_CHECK(hipMemcpyAsync(bk + n, ctx->d_alpha, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        MNC_HIP_CHECK(hipMemcpyAsync(stats, ctx->d_stats, sizeof stats, hipMemcpyDeviceToHost, ctx->stream));
+        MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return MNC_OK;
+    };
+    auto restore = [&]() -> int {
+        MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_mom, bk, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_alpha, bk + n, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_stats, stats, sizeof stats, hipMemcpyHostToDevice, ctx->stream));
+        MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return MNC_OK;
+    };
+    // One trial: a warm-up step (allocations, graph capture), then `steps` timed steps.
+    auto trial = [&](double *ms) -> int {
+        int s2;
+        if ((s2 = restore()) || (s2 = mnc_update(ctx, dt, opt)) || (s2 = restore())) return s2;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int k = 0; k < steps; ++k)
+            if ((s2 = mnc_update(ctx, dt, opt))) return s2;
+        MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+        return MNC_OK;
+    };
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    if ((st = save())) { (void)hipFree(bk); ctx->timing = timing; return st; }
+    struct Knob { const char *key; std::vector<int> vals; int *slot; };
+    const int cu = ctx->num_cu;
+    std::vector<Knob> knobs = {
+        {"graph", {0, 1}, &ctx->use_graph},
+        {"rounds", {2, 4, 6, 8, 12}, &ctx->rounds},
+        {"tail_mode", {0, 1, 2}, &ctx->tail_mode},
+        {"tail_blocks", {cu, 2 * cu, 4 * cu}, &ctx->tail_blocks},
+    };
+    if (opt->precision == MNC_PREC_MIXED) knobs.push_back({"mix_blocks", {cu / 2, cu, 2 * cu}, &ctx->mix_blocks});
+    double best = 0.0;
+    if ((st = trial(&best)) == MNC_OK) {
+        char line[160];
+        snprintf(line, sizeof line, "start %.4f ms/step\n", best);
+        rep += line;
+        for (auto &kb : knobs) {   // coordinate descent, one pass
+            const int keep = *kb.slot;
+            int arg = keep;
+            for (int v : kb.vals) {
+                if (v == keep || (v < 1 && std::strcmp(kb.key, "graph") != 0)) continue;
+                *kb.slot = v;
+                ++ctx->epoch;
+                double ms = 0.0;
+                if ((st = trial(&ms))) break;
+                snprintf(line, sizeof line, "%s=%d %.4f ms/step\n", kb.key, v, ms);
+                rep += line;
+                if (ms < best) { best = ms; arg = v; }
+            }
+            *kb.slot = arg;
+            ++ctx->epoch;
+            if (st) break;
+        }
+    }
+    const int st2 = restore();
+    (void)hipFree(bk);
+    ++ctx->epoch;
+    ctx->timing = timing;
+    if (report && report_len > 0) {
+        snprintf(report, size_t(report_len), "%s", rep.c_str());
+    }
+    return st ? st : st2;
+}
 
 int mnc_mlp_num_params(void) { return mnc::mlp_param_count() + mnc::kMlpStats; }
 

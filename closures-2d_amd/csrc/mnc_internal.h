@@ -60,11 +60,21 @@ struct NewtonArgs {
     const int *mgptr, *mgk;   // Gaunt map of the order-2N table
     const double *mgv;
     double cmix;          // condition bound above which a mixed-precision step is redone in FP64
+    // FP64 matrix-core sweep of the high orders (G64Tables, device copies)
+    const double *gz, *gg, *gh;
+    int gnt;              // quadrature tiles of 16 points
+    int gnmm;             // order-2N moments
 };
 
 // Number of control words of a batched solve.
 constexpr int kCtlWords = 8;
 constexpr int kStatWords = 12;
+
+// Result layout of v_mfma_f64_16x16x4f64 on gfx950 (measured with a one-wave
+// probe): result element r of lane l is row 4r + l/16, column l%16.  (The
+// inputs are lane l: A[l%16][l/16], B[l/16][l%16]; the f32/bf16 16x16 forms
+// return row 4(l/16) + r instead.)
+constexpr int mfma_f64_row(int r, int g) { return 4 * r + g; }
 
 // Operand tables of the mixed-precision sweep (built on the host from Tables):
 // the quadrature is padded to a multiple of 32 points (zero weight) and the
@@ -79,6 +89,18 @@ struct MixTables {
 // Returns MNC_OK, or MNC_ERR_ARG when the order/quadrature has no mixed kernel.
 int build_mix_tables(int moment_order, int quad_order, MixTables &m);
 
+// Operand tables of the FP64 matrix-core sweep (the high orders, where the
+// direct Hessian does not fit a lane): quadrature padded to 16-point tiles
+// (zero weight), moments to kz = ceil(n/4) groups of 4, the sums to g tiles
+// of 16 columns (the last column is w), the order-2N moments to h tiles.
+struct G64Tables {
+    int nt = 0, kz = 0, ngt = 0, nht = 0, nmm = 0;
+    std::vector<double> z, g, h;
+    std::vector<int> gptr, gk;
+    std::vector<double> gv;
+};
+int build_g64_tables(int moment_order, int quad_order, G64Tables &m);
+
 // Launchers (mnc_newton.hip).  Return false if (nm, np2, gaunt) has no instance.
 // Enqueues the whole batched solve: prologue, `rounds` lockstep sweep/logic
 // rounds with compaction, then the persistent tail.  No host synchronisation.
@@ -89,6 +111,11 @@ bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int ro
 // is FP64.  mix_blocks: persistent blocks of the mixed sweep.
 bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
                          int tail_mode, int mix_blocks, hipStream_t s);
+// The FP64 matrix-core path (M8: n = 45).  rounds: lockstep rounds per launch
+// batch; the caller repeats batches until control word `lin` reads zero.
+// Returns false for an order without an instance.
+bool launch_newton_g64_init(const NewtonArgs &a, int nm, int np2, hipStream_t s);
+bool launch_newton_g64_rounds(const NewtonArgs &a, int nm, int np2, int &lin, int rounds, int blocks, hipStream_t s);
 constexpr int kMixMaxTiles = 16;   // quadrature tiles of 16 points held in LDS (nq <= 256)
 int mix_tiles(int nm);             // 16-column tiles of the order-2N moments the mixed kernel expects
 int newton_block();

@@ -1281,8 +1281,8 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
 // ---- mixed-precision sweep on the matrix cores ------------------------------
 //
 // A wave evaluates sixteen cells of the active list at once.  Lane l works for
-// cell l%16 on the quadrature points q = 16T + 4(l/16) + r (r = 0..3) of every
-// 16-point tile T, which is the row layout of the MFMA results below, so no
+// cell l%16 on the quadrature points q = 16T + mfma_f64_row(r, l/16) (r = 0..3)
+// of every 16-point tile T, which is where the first product leaves them, so no
 // value crosses lanes:
 //   z = alpha.m       v_mfma_f64_16x16x4f64, (16 points x 16 moments) x (16 moments x 16 cells)
 //   e = exp(z)        FP64, per lane
@@ -1383,17 +1383,22 @@ __global__ __launch_bounds__(64 * kMixWaves) void sweep_mix_kernel(NewtonArgs A,
                 hacc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Alo, Bhi, hacc[n], 0, 0, 0);
             }
         }
-        // Result row 4(l/16) + r belongs to the cell of slot 16 tile + 4(l/16) + r,
-        // held by lane 4(l/16) + r; column l%16.
+        // Result rows are cells: the cell of slot 16 tile + i is held by lane i.
+        // Element r of lane l is row mfma_f64_row(r, l/16) of the FP64 sums and
+        // row 4(l/16) + r of the bf16 products; column l%16 of both.
         const int j = lane & 15;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int cr = __shfl(c, 4 * g + r);
-            if (cr < 0) continue;
-            if (j < NM) S.d(St::F_GS + j, cr) = gacc[r];
-            if (j == 15) S.d(St::F_SF, cr) = gacc[r];
+            const int cg = __shfl(c, mfma_f64_row(r, g));
+            const int ch = __shfl(c, 4 * g + r);
+            if (cg >= 0) {
+                if (j < NM) S.d(St::F_GS + j, cg) = gacc[r];
+                if (j == 15) S.d(St::F_SF, cg) = gacc[r];
+            }
+            if (ch >= 0) {
 #pragma unroll
-            for (int n = 0; n < NNT; ++n) S.d(St::F_ACC + 16 * n + j, cr) = (double)hacc[n][r];
+                for (int n = 0; n < NNT; ++n) S.d(St::F_ACC + 16 * n + j, ch) = (double)hacc[n][r];
+            }
         }
     }
 }

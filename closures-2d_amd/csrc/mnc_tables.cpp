@@ -208,13 +208,16 @@ float bf16_val(unsigned short b) {
 
 // Operand tables of the mixed-precision sweep.  Quadrature point q of a 16-point
 // tile T sits at row q - 16T of the MFMA operands; the lane layouts are those of
-// v_mfma_f64_16x16x4f64 (lane l: A[l%16][l/16], B[l/16][l%16]) and
-// v_mfma_f32_16x16x32_bf16 (lane l: A[l%16][8(l/16)+t], B[8(l/16)+t][l%16]).
-//   z[T][kk][l] = p[16T + l%16][4kk + l/16]                (alpha.m, A operand)
-//   g[T][r][l]  = w_q p[q][l%16], q = 16T + 4(l/16) + r    (sums, B operand;
+// v_mfma_f64_16x16x4f64 (lane l: A[l%16][l/16], B[l/16][l%16], result element r
+// at row R(r, l/16) = mfma_f64_row) and v_mfma_f32_16x16x32_bf16 (lane l:
+// A[l%16][8(l/16)+t], B[8(l/16)+t][l%16]).  The exponent z = alpha.m comes out
+// of the first product with element r of lane l at point 16T + R(r, l/16); the
+// later products take their K index in that order:
+//   z[T][kk][l] = p[16T + l%16][4kk + l/16]                  (alpha.m, A operand)
+//   g[T][r][l]  = w_q p[q][l%16], q = 16T + R(r, l/16)       (sums, B operand;
 //                 column 15 holds w_q: the sum of w e)
 //   h[kb][nt][hl][l][t] = hi/lo bf16 of w_q p2N[q][16nt + l%16],
-//                 q = 16(2kb + t/4) + 4(l/16) + t%4          (Hessian moments, B)
+//                 q = 16(2kb + t/4) + R(t%4, l/16)            (Hessian moments, B)
 int build_mix_tables(int moment_order, int quad_order, MixTables &m) {
     Tables t;
     int st = build_tables(moment_order, quad_order, 1, t);   // order-2N table + Gaunt map
@@ -235,7 +238,7 @@ int build_mix_tables(int moment_order, int quad_order, MixTables &m) {
             for (int l = 0; l < 64; ++l) {
                 const int kz = 4 * k4 + l / 16;
                 m.z[(size_t(T) * 4 + k4) * 64 + l] = kz < nm ? P(16 * T + l % 16, kz) : 0.0;
-                const int q = 16 * T + 4 * (l / 16) + k4, j = l % 16;
+                const int q = 16 * T + mfma_f64_row(k4, l / 16), j = l % 16;
                 m.g[(size_t(T) * 4 + k4) * 64 + l] = j < nm ? W(q) * P(q, j) : (j == 15 ? W(q) : 0.0);
             }
     const int nkb = m.nt / 2;
@@ -244,7 +247,7 @@ int build_mix_tables(int moment_order, int quad_order, MixTables &m) {
         for (int nt = 0; nt < m.nnt; ++nt)
             for (int l = 0; l < 64; ++l)
                 for (int tt = 0; tt < 8; ++tt) {
-                    const int q = 16 * (2 * kb + tt / 4) + 4 * (l / 16) + tt % 4, col = 16 * nt + l % 16;
+                    const int q = 16 * (2 * kb + tt / 4) + mfma_f64_row(tt % 4, l / 16), col = 16 * nt + l % 16;
                     const float v = (float)(col < t.nmm ? W(q) * P(q, col) : 0.0);
                     const unsigned short hi = bf16_rne(v);
                     const unsigned short lo = bf16_rne(v - bf16_val(hi));
@@ -252,6 +255,54 @@ int build_mix_tables(int moment_order, int quad_order, MixTables &m) {
                     m.h[base + size_t(l) * 8 + tt] = hi;
                     m.h[base + 64 * 8 + size_t(l) * 8 + tt] = lo;
                 }
+    m.gptr = t.gptr;
+    m.gk = t.gk;
+    m.gv = t.gv;
+    return MNC_OK;
+}
+
+// Operand tables of the FP64 matrix-core sweep (v_mfma_f64_16x16x4f64 layouts
+// as for the mixed sweep; the exponent of point 16T + R(r, l/16) sits in
+// element r of lane l after the first product):
+//   z[T][kk][l]     = p[16T + l%16][4kk + l/16]                         (A operand)
+//   g[T][r][nt][l]  = w_q p[q][16nt + l%16], column 16 ngt - 1 = w_q      (B operand)
+//   h[T][r][nt][l]  = w_q p2N[q][16nt + l%16]                            (B operand)
+// with q = 16T + R(r, l/16).
+int build_g64_tables(int moment_order, int quad_order, G64Tables &m) {
+    Tables t;
+    int st = build_tables(moment_order, quad_order, 1, t);
+    if (st != MNC_OK) return st;
+    const int nm = t.nm, nq = t.nq, np2 = t.np2;
+    m.nt = (nq + 15) / 16;
+    m.kz = (nm + 3) / 4;
+    m.ngt = (nm + 1 + 15) / 16;
+    m.nht = (t.nmm + 15) / 16;
+    m.nmm = t.nmm;
+    auto P = [&](int q, int k) { return (q < nq && k < np2) ? t.p[size_t(q) * np2 + k] : 0.0; };
+    auto W = [&](int q) { return q < nq ? t.w[q] : 0.0; };
+    m.z.assign(size_t(m.nt) * m.kz * 64, 0.0);
+    m.g.assign(size_t(m.nt) * 4 * m.ngt * 64, 0.0);
+    m.h.assign(size_t(m.nt) * 4 * m.nht * 64, 0.0);
+    for (int T = 0; T < m.nt; ++T) {
+        for (int kk = 0; kk < m.kz; ++kk)
+            for (int l = 0; l < 64; ++l) {
+                const int k = 4 * kk + l / 16;
+                m.z[(size_t(T) * m.kz + kk) * 64 + l] = k < nm ? P(16 * T + l % 16, k) : 0.0;
+            }
+        for (int r = 0; r < 4; ++r)
+            for (int l = 0; l < 64; ++l) {
+                const int q = 16 * T + mfma_f64_row(r, l / 16);
+                for (int nt = 0; nt < m.ngt; ++nt) {
+                    const int j = 16 * nt + l % 16;
+                    m.g[((size_t(T) * 4 + r) * m.ngt + nt) * 64 + l] =
+                        j < nm ? W(q) * P(q, j) : (j == 16 * m.ngt - 1 ? W(q) : 0.0);
+                }
+                for (int nt = 0; nt < m.nht; ++nt) {
+                    const int col = 16 * nt + l % 16;
+                    m.h[((size_t(T) * 4 + r) * m.nht + nt) * 64 + l] = col < t.nmm ? W(q) * P(q, col) : 0.0;
+                }
+            }
+    }
     m.gptr = t.gptr;
     m.gk = t.gk;
     m.gv = t.gv;

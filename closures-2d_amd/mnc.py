@@ -95,6 +95,7 @@ def lib():
         L.mnc_timing_enable.argtypes = [vp, C.c_int]
         L.mnc_timing_get.argtypes = [vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
         L.mnc_tune.argtypes = [vp, C.c_char_p, C.c_int]
+        L.mnc_autotune.argtypes = [vp, C.c_double, C.POINTER(Options), C.c_int, C.c_char_p, C.c_int]
         L.mnc_get_tune.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
         L.mnc_last_error.argtypes = [vp]
         L.mnc_last_error.restype = C.c_char_p
@@ -168,6 +169,12 @@ class Context:
                     "mnc_get_gaunt")
         return ptr, k[:nnz], v[:nnz]
 
+    def opt_batch_device(self, n, u_ptr, alpha_ptr, opt, outs_ptr=None):
+        """mnc_opt_batch on device arrays (e.g. torch tensors' data_ptr()); the
+        caller makes their contents visible to the context's stream first."""
+        self._check(lib().mnc_opt_batch(self.h, n, C.c_void_p(u_ptr), C.c_void_p(alpha_ptr), C.byref(opt),
+                                        C.c_void_p(outs_ptr) if outs_ptr else None), "mnc_opt_batch")
+
     def opt_batch_host(self, u, alpha, opt=None, allow_status=(OK,)):
         """Batched opt() on host arrays; returns (status, u, alpha, iter, r, normG)."""
         opt = opt or options()
@@ -236,6 +243,13 @@ class Context:
 
     def tune(self, key, value):
         self._check(lib().mnc_tune(self.h, key.encode(), int(value)), "mnc_tune")
+
+    def autotune(self, dt, opt, steps=3):
+        """Picks the fastest launch shape for steps of dt on the current grid
+        (state restored afterwards); returns the trial report lines."""
+        buf = C.create_string_buffer(8192)
+        self._check(lib().mnc_autotune(self.h, dt, C.byref(opt), steps, buf, len(buf)), "mnc_autotune")
+        return buf.value.decode().splitlines()
 
     def get_tune(self, key):
         v = C.c_int(0)

@@ -167,11 +167,25 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *                  Hessian only while its condition estimate is <= 10^value
  *                  (0-10, default 4); otherwise the step is redone in FP64
  *   "mix_blocks"   workgroups of the mixed-precision sweep (1-65536)
+ *   "graph"        1: mnc_update steps are captured into hipGraphs (one per
+ *                  distinct dt and options, captured on the second step with
+ *                  that key) and replayed; 0: eager launches (default).  While
+ *                  per-launch timing (mnc_timing_enable) is on, steps run eagerly.
  * mnc_get_tune also reads "has_mixed" (1 if this context's order and
  * quadrature have the mixed-precision kernels: N <= 4, Q <= 16).
  * Returns MNC_ERR_ARG for an unknown key or an out-of-range value. */
 int mnc_tune(mnc_ctx *ctx, const char *key, int value);
 int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value);
+
+/* Autotuner (no counterpart in the reference): times `steps` mnc_update steps
+ * of dt from the current grid state for each candidate launch shape, one
+ * coordinate-descent pass over "graph", "rounds", "tail_mode", "tail_blocks"
+ * and (mixed precision) "mix_blocks", and keeps the fastest.  The grid state
+ * and the statistics are restored after every trial, so the call leaves the
+ * solution where it was.  report (may be NULL) receives one "key=value ms/step"
+ * line per trial.  The tail kernels sum in different orders, so the choice can
+ * move results at rounding level. */
+int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, char *report, int report_len);
 
 /* Synchronises the context's stream; returns the first error seen. */
 int mnc_sync(mnc_ctx *ctx);
