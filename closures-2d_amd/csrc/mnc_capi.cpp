@@ -33,6 +33,10 @@ struct mnc_ctx {
     double *d_mz = nullptr, *d_mg = nullptr, *d_mgv = nullptr;
     unsigned short *d_mh = nullptr;
     int *d_mgptr = nullptr, *d_mgk = nullptr;
+    // FP64 matrix-core operand tables of the high orders (M8)
+    mnc::G64Tables g64;
+    bool has_g64 = false;
+    double *d_gz = nullptr, *d_gg = nullptr, *d_gh = nullptr;
     size_t state_cap = 0;
     int state_nm = 0;
     // tuning
@@ -233,6 +237,8 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     const bool mixed = o->precision == MNC_PREC_MIXED;
     if (mixed && !ctx->has_mix)
         return ctx_fail(ctx, MNC_ERR_ARG, "mixed precision needs moment order <= 4 and quad order <= 16");
+    if (ctx->has_g64 && (mixed || o->warm_start == MNC_WARM_LEARNED))
+        return ctx_fail(ctx, MNC_ERR_ARG, "moment order 8 runs the FP64 form only");
     int st = ensure_state(ctx, size_t(ncells));
     if (st) return st;
     if (o->warm_start == MNC_WARM_LEARNED) {    // the network's guesses for every cell
@@ -278,11 +284,31 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     a.mgk = ctx->d_mgk;
     a.mgv = ctx->d_mgv;
     a.cmix = std::pow(10.0, ctx->mix_cond_log10);
+    a.gz = ctx->d_gz;
+    a.gg = ctx->d_gg;
+    a.gh = ctx->d_gh;
+    a.gnt = ctx->g64.nt;
+    a.gnmm = ctx->g64.nmm;
     const int tail = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
     {
         Timed tm(ctx, MNC_KCLASS_NEWTON);
         mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
-        if (mixed) {
+        if (ctx->has_g64) {
+            // lockstep rounds in batches until the active list is empty (the
+            // count is read back between batches, so this path is not captured)
+            if (!mnc::launch_newton_g64_init(a, t.nm, t.np2, ctx->stream))
+                return ctx_fail(ctx, MNC_ERR_ARG, "no FP64 matrix-core Newton instance for this order");
+            const int blocks = 2 * ctx->num_cu;
+            int lin = 0;
+            for (int batch = 0;; ++batch) {
+                if (batch > 4000) return ctx_fail(ctx, MNC_ERR_STATE, "closure solve did not finish");
+                mnc::launch_newton_g64_rounds(a, t.nm, t.np2, lin, 4, blocks, ctx->stream);
+                int left = 0;
+                MNC_HIP_CHECK(hipMemcpyAsync(&left, ctx->d_ctl + lin, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+                MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+                if (left == 0) break;
+            }
+        } else if (mixed) {
             const int mb = ctx->mix_blocks > 0 ? ctx->mix_blocks : ctx->num_cu;
             if (!mnc::launch_newton_mixed(a, t.nm, t.np2, t.use_gaunt != 0, ctx->rounds, tail, ctx->tail_mode, mb,
                                           ctx->stream))
@@ -387,6 +413,15 @@ int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
         mnc_destroy(ctx);
         return MNC_ERR_HIP;
     }
+    if (t.nm == 45 && mnc::build_g64_tables(setup->moment_order, setup->quad_order, ctx->g64) == MNC_OK &&
+        ctx->g64.kz == 12 && ctx->g64.ngt == 3 && ctx->g64.nht == 10) {   // the instantiated tile counts
+        if (up(&ctx->d_gz, ctx->g64.z) || up(&ctx->d_gg, ctx->g64.g) || up(&ctx->d_gh, ctx->g64.h) ||
+            up(&ctx->d_mgptr, ctx->g64.gptr) || up(&ctx->d_mgk, ctx->g64.gk) || up(&ctx->d_mgv, ctx->g64.gv)) {
+            mnc_destroy(ctx);
+            return MNC_ERR_HIP;
+        }
+        ctx->has_g64 = true;
+    }
     if (mnc::build_mix_tables(setup->moment_order, setup->quad_order, ctx->mix) == MNC_OK &&
         ctx->mix.nnt == mnc::mix_tiles(t.nm)) {
         if (up(&ctx->d_mz, ctx->mix.z) || up(&ctx->d_mg, ctx->mix.g) || up(&ctx->d_mh, ctx->mix.h) ||
@@ -408,7 +443,7 @@ void mnc_destroy(mnc_ctx *ctx) {
     for (auto &g : ctx->graphs) drop_graph(g);
     for (auto &ev : ctx->events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
     void *ptrs[] = {ctx->d_w, ctx->d_p, ctx->d_xi, ctx->d_eta, ctx->d_gv, ctx->d_gptr, ctx->d_gk,
-                    ctx->d_mz, ctx->d_mg, ctx->d_mh, ctx->d_mgptr, ctx->d_mgk, ctx->d_mgv,
+                    ctx->d_mz, ctx->d_mg, ctx->d_mh, ctx->d_mgptr, ctx->d_mgk, ctx->d_mgv, ctx->d_gz, ctx->d_gg, ctx->d_gh,
                     ctx->d_mlp, ctx->d_mlp_scratch, ctx->d_mlp_loss, ctx->d_mlp_io, ctx->d_us, ctx->d_guess,
                     ctx->d_ctl, ctx->d_stats, ctx->d_std, ctx->d_sti, ctx->d_lists, ctx->d_mom, ctx->d_alpha, ctx->d_old,
                     ctx->d_st, ctx->d_ss, ctx->d_flux, ctx->d_ansatz, ctx->d_bu, ctx->d_ba, ctx->d_bo};
@@ -627,7 +662,7 @@ int mnc_update(mnc_ctx *ctx, double dt, const mnc_options *opt) {
     int st = check_opts(ctx, opt);
     if (st) return st;
     MNC_HIP_CHECK(hipSetDevice(ctx->device));
-    if (ctx->use_graph && !ctx->timing) {   // per-launch timing needs the eager path
+    if (ctx->use_graph && !ctx->timing && !ctx->has_g64) {   // timing needs eager launches; M8 syncs per batch
         mnc_ctx::StepGraph *g = nullptr;
         for (auto &c : ctx->graphs)
             if (c.epoch == ctx->epoch && c.dt == dt && same_opts(c.o, *opt)) g = &c;

@@ -1403,6 +1403,273 @@ __global__ __launch_bounds__(64 * kMixWaves) void sweep_mix_kernel(NewtonArgs A,
     }
 }
 
+// ---- FP64 matrix-core path for the high orders (M8: n = 45, nq = 1024) -------
+//
+// The direct Hessian of 45 moments (1035 sums) does not fit a lane, so the
+// sweep runs on the matrix cores with the lane scheme of the mixed sweep, all in
+// FP64, sixteen cells per wave:
+//   z = alpha.m        KZ products per 16-point tile
+//   G = sums w e m_i   NGT column tiles; the last column is sum w e (f)
+//   Hm = sums w e p2N  NHT column tiles: the order-2N moments, from which the
+//                      logic forms H through the Gaunt map
+// f, g and H are then the reference's full evaluation (fobj.cpp:266-287) to
+// rounding: f = sum w e - alpha.u, g_i = sum w e m_i - u_i, and H differs from
+// the direct sum only by the Gaunt least-squares fit (~1e-14).  The operand
+// tables (build_g64_tables, 2 MB at M8/Q32) are read from global memory.
+template <int NM, int NP2, int KZ, int NGT, int NHT>
+__global__ __launch_bounds__(256) void sweep_g64_kernel(NewtonArgs A, int lin) {
+    using St = Store<NM, NP2>;
+    static_assert(16 * NGT - 1 >= NM, "the last sum column holds sum w e");
+    static_assert(16 * NHT <= St::nacc, "order-2N moments exceed the store");
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl[lin ^ 1] = 0;
+    const int count = A.ctl[lin];
+    const int ntiles = (count + 15) >> 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ci = lane & 15, g = lane >> 4;
+    const St S{A.st_d, A.st_i, A.cap};
+    const int nt = A.gnt, nmm = A.gnmm;
+    for (int tile = blockIdx.x * 4 + wave; tile < ntiles; tile += gridDim.x * 4) {
+        const int slot = tile * 16 + ci;
+        const int c = slot < count ? A.lists[size_t(lin) * A.cap + slot] : -1;
+        double ab[KZ];
+#pragma unroll
+        for (int kk = 0; kk < KZ; ++kk) {
+            const int k = 4 * kk + g;
+            ab[kk] = (c >= 0 && k < NM) ? S.d(St::F_AL + k, c) : 0.0;
+        }
+        mf64x4 gacc[NGT], hacc[NHT];
+#pragma unroll
+        for (int n = 0; n < NGT; ++n) gacc[n] = mf64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int n = 0; n < NHT; ++n) hacc[n] = mf64x4{0.0, 0.0, 0.0, 0.0};
+        for (int T = 0; T < nt; ++T) {
+            const double *zt = A.gz + size_t(T) * KZ * 64 + lane;
+            const double *gt = A.gg + size_t(T) * 4 * NGT * 64 + lane;
+            const double *ht = A.gh + size_t(T) * 4 * NHT * 64 + lane;
+            mf64x4 z = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kk = 0; kk < KZ; ++kk) z = __builtin_amdgcn_mfma_f64_16x16x4f64(zt[kk * 64], ab[kk], z, 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double e = exp(z[r]);
+#pragma unroll
+                for (int n = 0; n < NGT; ++n)
+                    gacc[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(e, gt[(r * NGT + n) * 64], gacc[n], 0, 0, 0);
+#pragma unroll
+                for (int n = 0; n < NHT; ++n)
+                    hacc[n] = __builtin_amdgcn_mfma_f64_16x16x4f64(e, ht[(r * NHT + n) * 64], hacc[n], 0, 0, 0);
+            }
+        }
+        const int j = lane & 15;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int cg = __shfl(c, mfma_f64_row(r, g));
+            if (cg < 0) continue;
+#pragma unroll
+            for (int n = 0; n < NGT; ++n) {
+                const int col = 16 * n + j;
+                if (col < NM) S.d(St::F_GS + col, cg) = gacc[n][r];
+                if (col == 16 * NGT - 1) S.d(St::F_SF, cg) = gacc[n][r];
+            }
+#pragma unroll
+            for (int n = 0; n < NHT; ++n) {
+                const int col = 16 * n + j;
+                if (col < nmm) S.d(St::F_ACC + col, cg) = hacc[n][r];
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+template <int NM>
+struct G64Shared {
+    double L[NM][NM];   // H, then its Cholesky factor (lower)
+    double x[NM], gvec[NM], al[NM];
+};
+
+// One step of optScaled's logic per cell after sweep_g64_kernel, one wave per
+// cell (lane i owns row i of H and of its factor; the iterate in LDS).  The
+// steps are those of cell_logic (opt.cpp:209-318, linesearch.cpp:38-98);
+// cells needing another sweep go onto list lin^1.
+template <int NM, int NP2>
+__global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
+    using St = Store<NM, NP2>;
+    __shared__ G64Shared<NM> sh;
+    const St S{A.st_d, A.st_i, A.cap};
+    const int lane = threadIdx.x;
+    const int count = A.ctl[lin];
+    LaneStats st;
+    for (int slot = blockIdx.x; slot < count; slot += gridDim.x) {
+        const int c = A.lists[size_t(lin) * A.cap + slot];
+        int mode = S.i(St::I_MODE, c), iter = S.i(St::I_ITER, c), total = S.i(St::I_TOTAL, c);
+        int reg = S.i(St::I_REG, c), maxit = S.i(St::I_MAXIT, c);
+        double t = S.d(St::F_T, c);
+        const double ul = lane < NM ? S.d(St::F_U + lane, c) : 0.0;
+        if (lane < NM) sh.al[lane] = S.d(St::F_AL + lane, c);
+        __builtin_amdgcn_wave_barrier();
+        bool newton = (mode == ST_NEWTON), do_reg = false, converged = false;
+        double err = 0.0;
+        if (mode == ST_TRIAL) {                                    // linesearch.cpp:45-70
+            if (lane == 0) ++st.trials;
+            const double fnew = S.d(St::F_SF, c) - wave_sum(lane < NM ? sh.al[lane] * ul : 0.0);
+            const double fcur = S.d(St::F_FCUR, c), gd = S.d(St::F_GD, c);
+            bool accept = fnew <= fcur + 1e-3 * t * gd;
+            if (!accept && A.ls_mode == MNC_LS_ROBUST && t == 1.0 && -gd <= 1e3 * DBL_EPSILON * S.d(St::F_FSCALE, c))
+                accept = true;
+            __builtin_amdgcn_wave_barrier();
+            if (accept) {
+                newton = true;
+            } else {
+                t = 0.5 * t;
+                if (t * S.d(St::F_MAXF, c) > DBL_EPSILON / 2) {
+                    if (lane < NM) sh.al[lane] = S.d(St::F_AOLD + lane, c) + t * S.d(St::F_D + lane, c);
+                } else {                                            // exhausted (:74-98)
+                    if (lane < NM) sh.al[lane] = S.d(St::F_AOLD + lane, c);
+                    t = 0.0;
+                    ++iter; ++total;
+                    do_reg = true;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (newton) {
+            ++iter; ++total;                                        // opt.cpp:212-213
+            double rs = 0.0, gi = 0.0;
+            if (lane < NM) {                                        // row `lane` of H (Gaunt map)
+                for (int jj = 0; jj < NM; ++jj) {
+                    const int pr = (lane < jj) ? lane * NM + jj : jj * NM + lane;
+                    double hij = 0.0;
+                    for (int cc = A.mgptr[pr]; cc < A.mgptr[pr + 1]; ++cc)
+                        hij = fma(A.mgv[cc], S.d(St::F_ACC + A.mgk[cc], c), hij);
+                    sh.L[lane][jj] = hij;
+                    rs += fabs(hij);
+                }
+                gi = S.d(St::F_GS + lane, c) - ul;
+                sh.gvec[lane] = gi;
+            }
+            const double au = wave_sum(lane < NM ? sh.al[lane] * ul : 0.0);
+            const double aus = wave_sum(lane < NM ? fabs(sh.al[lane] * ul) : 0.0);
+            const double fcur = S.d(St::F_SF, c) - au;
+            const double fscale = fabs(S.d(St::F_SF, c)) + aus;
+            err = sqrt(wave_sum(gi * gi));                          // dnrm2 (:286)
+            const double anorm = wave_max(rs);                     // utils.cpp:36-52
+            __builtin_amdgcn_wave_barrier();
+            const bool chol_ok = coop_cholesky<NM>(sh.L, lane);
+            double condest = 0.0;
+            if (chol_ok) condest = 1.0 / coop_rcond<NM>(sh.L, sh.x, anorm, lane);
+            if (iter >= maxit || !chol_ok || t == 0.0 || condest > A.cond_h_max) {
+                do_reg = true;                                      // opt.cpp:244
+            } else {
+                if (lane < NM) sh.x[lane] = sh.gvec[lane];
+                __builtin_amdgcn_wave_barrier();
+                coop_apply_inv<NM>(sh.L, sh.x, lane);               // dpotrs (:278-282)
+                if (err <= S.d(St::F_RTOL, c)) {
+                    converged = true;                               // :293-306
+                } else {                                            // linesearch.cpp:28-43
+                    const double di = lane < NM ? -sh.x[lane] : 0.0;
+                    const double ai = lane < NM ? sh.al[lane] : 0.0;
+                    const double maxf = wave_max(lane < NM ? fabs(di) / (fabs(ai) + 1e-100) : 0.0);
+                    const double gd = wave_sum(lane < NM ? sh.gvec[lane] * di : 0.0);
+                    if (lane < NM) {
+                        S.d(St::F_AOLD + lane, c) = ai;
+                        S.d(St::F_D + lane, c) = di;
+                    }
+                    S.d(St::F_MAXF, c) = maxf;
+                    S.d(St::F_GD, c) = gd;
+                    S.d(St::F_FCUR, c) = fcur;
+                    S.d(St::F_FSCALE, c) = fscale;
+                    t = 1.0;
+                    __builtin_amdgcn_wave_barrier();
+                    if (t * maxf > DBL_EPSILON / 2) {
+                        if (lane < NM) sh.al[lane] = ai + t * di;
+                        mode = ST_TRIAL;
+                    } else {
+                        t = 0.0;
+                        ++iter; ++total;
+                        do_reg = true;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        bool done = false;
+        if (do_reg) {                                               // opt.cpp:246-273
+            if (reg < 4) {
+                ++reg;
+                const double r = kReg[reg];
+                const double u0 = S.d(St::F_U0, c);
+                const double uk = (lane > 0 && lane < NM) ? (A.u[size_t(c) * NM + lane] / u0) * (1.0 - r) : 0.0;
+                const double ss = 1.0 + wave_sum(uk * uk);
+                if (lane < NM) {
+                    S.d(St::F_U + lane, c) = lane == 0 ? 1.0 : uk;
+                    sh.al[lane] = lane == 0 ? 2.0 * kSqrtPi * log(0.5 / kSqrtPi) : 0.0;
+                }
+                S.d(St::F_RTOL, c) = sqrt(ss) * A.tol;
+                iter = 0;
+                t = 1.0;
+                if (reg == 4) maxit = 10 * maxit;
+                mode = ST_NEWTON;
+            } else {                                                // "Max regularization achieved."
+                if (lane == 0) {
+                    st.err_bits |= 1 << MNC_ERR_MAXREG;
+                    if (A.outs) { A.outs[c].iter = total; A.outs[c].r = -1.0; A.outs[c].norm_g = err; }
+                }
+                done = true;
+            }
+        } else if (converged) {                                     // in-place reconstruction
+            const double u0 = S.d(St::F_U0, c);
+            const double u0p = S.d(St::F_U, c) + sh.gvec[0];
+            if (lane < NM) {
+                A.u[size_t(c) * NM + lane] = ((ul + sh.gvec[lane]) / u0p) * u0;
+                const double ak = sh.al[lane];
+                A.alpha[size_t(c) * NM + lane] =
+                    lane == 0 ? (ak - log(u0p) * 2.0 * kSqrtPi) + S.d(St::F_ASCALE, c) : ak;
+            }
+            if (lane == 0) {
+                if (A.outs) { A.outs[c].iter = total; A.outs[c].r = kReg[reg]; A.outs[c].norm_g = err; }
+                st.max_iter = total > st.max_iter ? total : st.max_iter;
+                st.sum += total;
+                ++st.n;
+                ++st.hist[reg];
+            }
+            done = true;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (!done) {
+            if (lane < NM) S.d(St::F_AL + lane, c) = sh.al[lane];
+            if (lane == 0) {
+                S.d(St::F_T, c) = t;
+                S.i(St::I_ITER, c) = iter;
+                S.i(St::I_TOTAL, c) = total;
+                S.i(St::I_REG, c) = reg;
+                S.i(St::I_MAXIT, c) = maxit;
+                S.i(St::I_MODE, c) = mode;
+                const int pos = atomicAdd(&A.ctl[lin ^ 1], 1);
+                A.lists[size_t(lin ^ 1) * A.cap + pos] = c;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane == 0 && (st.n || st.trials || st.err_bits)) {
+        atomicMax((unsigned long long *)&A.stats[0], (unsigned long long)st.max_iter);
+        atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)st.sum);
+        atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)st.n);
+        for (int r = 0; r < 5; ++r) atomicAdd((unsigned long long *)&A.stats[3 + r], (unsigned long long)st.hist[r]);
+        atomicAdd((unsigned long long *)&A.stats[8], (unsigned long long)st.trials);
+        if (st.err_bits) atomicOr(&A.ctl[3], st.err_bits);
+    }
+}
+
 template <int NM, int NP2, bool G, int RS>
 void launch_tail(const NewtonArgs &a, int lin, int tail_blocks, int tail_mode, hipStream_t s) {
     if (tail_mode == 0)
@@ -1494,5 +1761,26 @@ bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, int r
 }
 
 int mix_tiles(int nm) { return mix_cols(nm) / 16; }
+
+bool launch_newton_g64_init(const NewtonArgs &a, int nm, int np2, hipStream_t s) {
+    const int nb = (a.ncells + kBlock - 1) / kBlock;
+    if (nm == 45 && np2 == 45) {
+        hipLaunchKernelGGL((init_kernel<45, 45>), dim3(nb), dim3(kBlock), 0, s, a);
+        return true;
+    }
+    return false;
+}
+
+bool launch_newton_g64_rounds(const NewtonArgs &a, int nm, int np2, int &lin, int rounds, int blocks, hipStream_t s) {
+    if (!(nm == 45 && np2 == 45)) return false;
+    constexpr int KZ = 12, NGT = 3, NHT = 10;   // 48 moment rows, 46 sum columns, 153 order-16 moments
+    if (a.gnmm > 16 * NHT) return false;
+    for (int r = 0; r < rounds; ++r) {
+        hipLaunchKernelGGL((sweep_g64_kernel<45, 45, KZ, NGT, NHT>), dim3(blocks), dim3(256), 0, s, a, lin);
+        hipLaunchKernelGGL((logic_g64_kernel<45, 45>), dim3(4 * blocks), dim3(64), 0, s, a, lin);
+        lin ^= 1;
+    }
+    return true;
+}
 
 }  // namespace mnc

@@ -49,3 +49,20 @@ def isotropic_start(u):
     a = np.zeros_like(u)
     a[:, 0] = 2.0 * np.sqrt(np.pi) * (np.log(0.5 / np.sqrt(np.pi)) + np.log(u[:, 0]))
     return a
+
+
+def sine_state(g, nm):
+    """The smooth periodic M8 workload's initial state on a grid_init GridState
+    (moments/alpha replaced in place): isotropic density 1 + 0.5 sin(pi x)
+    sin(pi y) at the cell centres, ghosts included."""
+    NX, NY = g.nx + 4, g.ny + 4
+    i = np.arange(NX)[:, None] * np.ones((1, NY))
+    j = np.ones((NX, 1)) * np.arange(NY)[None, :]
+    x = g.init_x + (i - 2) * g.dx + g.dx / 2.0
+    y = g.init_y + (j - 2) * g.dy + g.dy / 2.0
+    dens = 1.0 + 0.5 * np.sin(np.pi * x) * np.sin(np.pi * y)
+    g.moments = np.zeros((NX, NY, nm))
+    g.alpha = np.zeros((NX, NY, nm))
+    g.moments[:, :, 0] = dens * 2.0 * np.sqrt(np.pi)
+    g.alpha[:, :, 0] = 2.0 * np.sqrt(np.pi) * np.log(dens)
+    return g
