@@ -184,10 +184,9 @@ def form_summary(r, f_iter, f_trial):
 def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
     """The isolated closure batch (n M4 vectors, Q = 16, 20% near-boundary
     lobes, isotropic start, TOL 1e-8, reference line search): converged
-    closures/s of each form with u and alpha resident in HBM (torch tensors as
-    device buffers), and each form's error against the FP64 CPU oracle on the
-    first `oracle_sample` vectors."""
-    import torch
+    closures/s of each form with u and alpha resident in HBM (mnc_batch_set
+    before the clock starts), and each form's error against the FP64 CPU oracle
+    on the first `oracle_sample` vectors."""
     sys.path.insert(0, ROOT)
     import oracle
     wl = _load("workloads")
@@ -206,29 +205,21 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
     ust, ast_ = mnc.scaled_pairs(ut, at)
     ctx.mlp_init(seed)
     loss = ctx.mlp_train(ust, ast_, steps=train_steps, batch=4096, lr=1e-3, seed=seed)
-    dev = torch.device("cuda:0")
-    u_src = torch.from_numpy(u).to(dev)
-    a_src = torch.from_numpy(a0).to(dev)
-    u_d = torch.empty_like(u_src)
-    a_d = torch.empty_like(a_src)
     res = {}
     for name, kw in (("plain-fp64", {}), ("mixed", {"precision": mnc.PREC_MIXED}),
                      ("learned+mixed", {"precision": mnc.PREC_MIXED, "warm_start": mnc.WARM_LEARNED})):
         opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, **kw)
         best = None
         for rep_ in range(3):                      # first run allocates; best of the rest
-            u_d.copy_(u_src)
-            a_d.copy_(a_src)
-            torch.cuda.synchronize()
+            ctx.batch_set(u, a0)                   # inputs resident in HBM before the clock starts
             ctx.reset_stats()
             t1 = time.perf_counter()
-            ctx.opt_batch_device(n, u_d.data_ptr(), a_d.data_ptr(), opt)
+            ctx.batch_solve(opt)
             dt = time.perf_counter() - t1
             if rep_ > 0 and (best is None or dt < best):
                 best = dt
         st = ctx.stats()
-        ug = u_d[:m].cpu().numpy()
-        same_r = None
+        ug = ctx.batch_get()[0][:m]
         err = np.linalg.norm(ug - uo, axis=1) / np.linalg.norm(uo, axis=1)
         res[name] = {"closures_per_s": n / best, "s": best,
                      "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
@@ -241,6 +232,53 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
             "generate_s": gen_s, "oracle_sample": m, "mlp_loss_first_last": [float(loss[:10].mean()),
                                                                            float(loss[-50:].mean())],
             "forms": res}
+
+
+def bench_m8(mnc, gi, steps, n=512, sample=1000):
+    """The M8 workload: 45 moments, Q = 32 (1024 points), n x n periodic cells on
+    [-1,1]^2, sigma_s = 1, isotropic density 1 + 0.5 sin(pi x) sin(pi y), TOL
+    1e-8, direct-Hessian algorithm (FP64 matrix-core path): converged closures/s
+    and s/step over K steps, and the error of GPU closures against the FP64 CPU
+    oracle on `sample` cells of the final state."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    wl = _load("workloads")
+    g = gi.initial_state(n, n, -1.0, 1.0, -1.0, 1.0, 2, 0.0, 1e-4, 1.0, 45, 0.9, 2.0)
+    wl.sine_state(g, 45)
+    ctx = mnc.Context(8, 32, use_gaunt=False)
+    ctx.grid_create(g.nx, g.ny, g.dx, g.dy, g.init_x, g.init_y, 2, 2.0, g.moments, g.alpha, g.sigma_t, g.sigma_s)
+    opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10)
+    ctx.update(g.max_dt, opt)                 # allocations
+    ctx.grid_set(g.moments, g.alpha)
+    ctx.reset_stats()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.update(g.max_dt, opt)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    m, a = ctx.grid_get()
+    # closures of `sample` cells of the final state, GPU vs oracle, from the stored alpha
+    rng = np.random.default_rng(5)
+    cells = rng.choice((n + 4) * (n + 4), sample, replace=False)
+    u = m.reshape(-1, 45)[cells] * (1.0 + 1e-3 * rng.standard_normal((sample, 45)))   # a fresh solve
+    u[:, 0] = m.reshape(-1, 45)[cells, 0]
+    a0 = a.reshape(-1, 45)[cells]
+    _, ug, _, itg, rg, _ = ctx.opt_batch_host(u, a0, opt)
+    ctx.close()
+    tab = oracle.Tables(8, 32, False)
+    _, uo, _, ito, ro, _ = tab.opt_batch(u, a0, max_iter=100, tol=1e-8, cond_h_max=1e10,
+                                         nthreads=len(os.sched_getaffinity(0)))
+    err = np.linalg.norm(ug - uo, axis=1) / np.linalg.norm(uo, axis=1)
+    closures = 2 * (n + 4) * (n + 4) * steps
+    return {"workload": "M8 (n=45), Q=32 (nq=1024), %dx%d periodic on [-1,1]^2, sigma_s=1, density "
+                        "1+0.5 sin(pi x) sin(pi y), TOL 1e-8, Heun dt=%.6g" % (n, n, g.max_dt),
+            "steps": steps, "closures_per_s": closures / wall, "s_per_timestep": wall / steps,
+            "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
+            "trials_per_closure": st.trial_sum / st.num_dual_solves, "reg_hist": list(st.hist_reg),
+            "oracle_sample": sample, "oracle_max_rel_u_err": float(err.max()),
+            "oracle_same_iterations": float(np.mean(itg == ito))}
 
 
 def cpu_baseline(d, seconds):
@@ -276,6 +314,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--isolated", type=int, default=1_000_000,
                     help="vectors of the isolated closure batch (0 skips it)")
+    ap.add_argument("--m8-steps", type=int, default=20, help="steps of the M8 512^2 workload (0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's hot path runs on a single MI355X "
@@ -342,6 +381,8 @@ def main():
     }
     if args.isolated > 0 and not args.plain_only:
         out["config"]["isolated"] = bench_isolated(mnc, args.isolated, 12345, 20000, args.train_steps)
+    if args.m8_steps > 0 and not args.plain_only:
+        out["config"]["m8"] = bench_m8(mnc, gi, args.m8_steps)
     if not args.no_cpu:
         cb = cpu_baseline(d, args.cpu_seconds)
         out["cpu_baseline"] = {"value": cb["value"], "unit": "closures/s", "cores": cb["cores"], "kind": "port",

@@ -45,6 +45,7 @@ struct mnc_ctx {
     int tail_mode = 0;               // 0: wave per cell, 1: lane per cell, 2: wave sweep + serial logic
     int mix_cond_log10 = 4;          // mixed steps need cond(H) <= 10^this, else FP64
     int mix_blocks = 0;              // 0: one per CU
+    int flux_tile = 1;               // 0: ansatz to HBM + flux; 1: fused, 8x8 tiles; 2: fused, 16x16
     // grid
     bool has_grid = false;
     mnc_grid_desc grid{};
@@ -55,6 +56,7 @@ struct mnc_ctx {
     double *d_bu = nullptr, *d_ba = nullptr;
     mnc_outs *d_bo = nullptr;
     size_t batch_cap = 0;
+    int batch_n = 0;                 // cells staged by mnc_batch_set
     // learned warm start
     bool mlp_ready = false, mlp_stats_set = false;
     int mlp_step = 0;
@@ -356,6 +358,13 @@ mnc::GridArgs grid_args(mnc_ctx *ctx) {
 // solveFlux (+ Euler stage when stage > 0) over all quadrature chunks.
 int flux_stage(mnc_ctx *ctx, int stage, double dt) {
     mnc::GridArgs g = grid_args(ctx);
+    if (ctx->flux_tile > 0) {
+        Timed tm(ctx, MNC_KCLASS_FLUX);
+        if (!mnc::launch_flux_fused(g, ctx->flux_tile, stage, dt, ctx->stream))
+            return ctx_fail(ctx, MNC_ERR_ARG, "no flux kernel instance for this moment order");
+        MNC_HIP_CHECK(hipGetLastError());
+        return MNC_OK;
+    }
     const int nq = ctx->tab.nq, ch = ctx->ansatz_chunk;
     for (int q0 = 0; q0 < nq; q0 += ch) {
         const int nqc = std::min(ch, nq - q0);
@@ -513,6 +522,49 @@ int mnc_opt_batch_host(mnc_ctx *ctx, int ncells, double *u, double *alpha, const
                                      ctx->stream));
     MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return st;
+}
+
+int mnc_batch_set(mnc_ctx *ctx, int ncells, const double *u, const double *alpha) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");
+    if (ncells < 1 || !u || !alpha) return ctx_fail(ctx, MNC_ERR_ARG, "bad batch arguments");
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = size_t(ncells) * ctx->tab.nm;
+    int st;
+    if (size_t(ncells) > ctx->batch_cap) {
+        if ((st = dalloc(ctx, &ctx->d_bu, n)) || (st = dalloc(ctx, &ctx->d_ba, n)) ||
+            (st = dalloc(ctx, &ctx->d_bo, size_t(ncells))))
+            return st;
+        ctx->batch_cap = ncells;
+    }
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_bu, u, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_ba, alpha, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->batch_n = ncells;
+    return MNC_OK;
+}
+
+int mnc_batch_solve(mnc_ctx *ctx, const mnc_options *opt) {
+    if (!ctx) return MNC_ERR_ARG;
+    int st = check_opts(ctx, opt);
+    if (st) return st;
+    if (ctx->batch_n < 1) return ctx_fail(ctx, MNC_ERR_STATE, "no staged batch (mnc_batch_set first)");
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    return run_newton(ctx, ctx->batch_n, ctx->d_bu, ctx->d_ba, opt, ctx->d_bo, nullptr, true);
+}
+
+int mnc_batch_get(mnc_ctx *ctx, double *u, double *alpha, mnc_outs *outs) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (ctx->batch_n < 1) return ctx_fail(ctx, MNC_ERR_STATE, "no staged batch (mnc_batch_set first)");
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    const size_t n = size_t(ctx->batch_n) * ctx->tab.nm;
+    if (u) MNC_HIP_CHECK(hipMemcpyAsync(u, ctx->d_bu, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (alpha) MNC_HIP_CHECK(hipMemcpyAsync(alpha, ctx->d_ba, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (outs)
+        MNC_HIP_CHECK(hipMemcpyAsync(outs, ctx->d_bo, size_t(ctx->batch_n) * sizeof(mnc_outs), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MNC_OK;
 }
 
 int mnc_grid_create(mnc_ctx *ctx, const mnc_grid_desc *grid, const double *moments, const double *alpha,
@@ -782,6 +834,11 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         ctx->mix_blocks = value;
         return MNC_OK;
     }
+    if (std::strcmp(key, "flux_tile") == 0) {
+        if (value < 0 || value > 2) return ctx_fail(ctx, MNC_ERR_ARG, "flux_tile must be 0, 1 or 2");
+        ctx->flux_tile = value;
+        return MNC_OK;
+    }
     return ctx_fail(ctx, MNC_ERR_ARG, "unknown tuning key");
 }
 
@@ -796,6 +853,7 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
     }
     if (std::strcmp(key, "has_mixed") == 0) { *value = ctx->has_mix ? 1 : 0; return MNC_OK; }
     if (std::strcmp(key, "graph") == 0) { *value = ctx->use_graph; return MNC_OK; }
+    if (std::strcmp(key, "flux_tile") == 0) { *value = ctx->flux_tile; return MNC_OK; }
     if (std::strcmp(key, "tail_blocks") == 0) {
         *value = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
         return MNC_OK;
@@ -849,6 +907,7 @@ int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, cha
     const int cu = ctx->num_cu;
     std::vector<Knob> knobs = {
         {"graph", {0, 1}, &ctx->use_graph},
+        {"flux_tile", {0, 1, 2}, &ctx->flux_tile},
         {"rounds", {2, 4, 6, 8, 12}, &ctx->rounds},
         {"tail_mode", {0, 1, 2}, &ctx->tail_mode},
         {"tail_blocks", {cu, 2 * cu, 4 * cu}, &ctx->tail_blocks},

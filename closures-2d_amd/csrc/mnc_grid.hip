@@ -123,6 +123,106 @@ __global__ __launch_bounds__(kBlock) void flux_euler_kernel(GridArgs g, int q0, 
     for (int k = 0; k < NM; ++k) u[k] = v[k];
 }
 
+// The ansatz fused into the flux: a block owns a TX x TY tile of interior cells,
+// keeps alpha of the tile and its 2-cell halo in LDS, and for each chunk of QC
+// directions evaluates E = exp(alpha.m_q) of the halo'd tile into LDS (the halo
+// is evaluated by both neighbouring tiles) before the stencil reads it.  E never
+// goes to HBM.  The arithmetic of every cell is that of ansatz_kernel +
+// flux_euler_kernel (same summation order over k and q), so the results are
+// identical.
+template <int NM, int TX, int TY, int QC>
+__global__ __launch_bounds__(TX * TY) void flux_fused_kernel(GridArgs g, int stage, double dt) {
+    constexpr int HX = TX + 4, HY = TY + 4, NH = HX * HY, NT = TX * TY;
+    __shared__ double al_s[NH][NM];
+    __shared__ double E_s[QC][NH];
+    const int tid = threadIdx.x;
+    const int tx0 = blockIdx.x * TX, ty0 = blockIdx.y * TY;   // halo cell (hi, hj) is grid cell (tx0+hi, ty0+hj)
+    const int NX = g.NX, NY = g.NY;
+    for (int idx = tid; idx < NH * NM; idx += NT) {
+        const int h = idx / NM, k = idx % NM;
+        const int gi = tx0 + h / HY, gj = ty0 + h % HY;
+        al_s[h][k] = (gi < NX && gj < NY) ? g.alpha[(size_t(gi) * NY + gj) * NM + k] : 0.0;
+    }
+    const int li = tid / TY, lj = tid % TY;
+    const int i = 2 + tx0 + li, j = 2 + ty0 + lj;
+    const bool active = (tx0 + li < g.nx) && (ty0 + lj < g.ny);
+    const int hc = (li + 2) * HY + (lj + 2);                  // this cell's halo index
+    const double theta = g.theta, dx = g.dx, dy = g.dy;
+    double F[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) F[k] = 0.0;
+    for (int q0 = 0; q0 < g.nq; q0 += QC) {
+        const int nqc = min(QC, g.nq - q0);
+        __syncthreads();
+        for (int qq = 0; qq < nqc; ++qq) {
+            const double *row = g.p + size_t(q0 + qq) * g.np2;
+            for (int h = tid; h < NH; h += NT) {
+                double sa = 0.0;
+#pragma unroll
+                for (int k = 0; k < NM; ++k) sa = fma(al_s[h][k], row[k], sa);
+                E_s[qq][h] = exp(sa);
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
+        for (int qq = 0; qq < nqc; ++qq) {
+            const int q = q0 + qq;
+            const double *E = E_s[qq];
+            const double xi = g.xi[q], eta = g.eta[q], w = g.w[q];
+            double f1 = 0.0, f2 = 0.0, f3 = 0.0, f4 = 0.0;
+            if (xi > 0) {
+                const double k1 = E[hc - 2 * HY], k2 = E[hc - HY], k3 = E[hc], k4 = E[hc + HY];
+                f1 = k3 + 0.5 * slopefit(k2, k3, k4, theta) - k2 - 0.5 * slopefit(k1, k2, k3, theta);
+                f1 = f1 * xi * w;
+            }
+            if (xi < 0) {
+                const double k1 = E[hc - HY], k2 = E[hc], k3 = E[hc + HY], k4 = E[hc + 2 * HY];
+                f2 = k3 - 0.5 * slopefit(k2, k3, k4, theta) - k2 + 0.5 * slopefit(k1, k2, k3, theta);
+                f2 = f2 * xi * w;
+            }
+            if (eta > 0) {
+                const double k1 = E[hc - 2], k2 = E[hc - 1], k3 = E[hc], k4 = E[hc + 1];
+                f3 = k3 + 0.5 * slopefit(k2, k3, k4, theta) - k2 - 0.5 * slopefit(k1, k2, k3, theta);
+                f3 = f3 * eta * w;
+            }
+            if (eta < 0) {
+                const double k1 = E[hc - 1], k2 = E[hc], k3 = E[hc + 1], k4 = E[hc + 2];
+                f4 = k3 - 0.5 * slopefit(k2, k3, k4, theta) - k2 + 0.5 * slopefit(k1, k2, k3, theta);
+                f4 = f4 * eta * w;
+            }
+            const double comb = (f1 + f2) / dx + (f3 + f4) / dy;
+            const double *row = g.p + size_t(q) * g.np2;
+#pragma unroll
+            for (int k = 0; k < NM; ++k) F[k] = fma(row[k], comb, F[k]);
+        }
+    }
+    if (!active) return;
+    const size_t c = size_t(i) * NY + j;
+    if (stage == 0) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) g.flux[c * NM + k] = F[k];
+        return;
+    }
+    double *u = g.moments + c * NM;
+    const double st = g.sigma_t[c], ss = g.sigma_s[c];
+    double v[NM];
+    v[0] = u[0] * (1.0 + dt * ss - dt * st) - dt * F[0];
+#pragma unroll
+    for (int k = 1; k < NM; ++k) v[k] = u[k] * (1.0 - dt * st) - dt * F[k];
+    if (g.init_cond == 1) {                                   // lattice source (:328-334)
+        const double x_i = g.init_x + (i - 2) * dx;
+        const double y_j = g.init_y + (j - 2) * dy;
+        if (fabs(x_i) < 0.5 && fabs(y_j) < 0.5) v[0] = v[0] + dt * 2.0 * sqrt(M_PI);
+    }
+    if (stage == 2) {
+        const double *uo = g.moments_old + c * NM;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) v[k] = 0.5 * (v[k] + uo[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) u[k] = v[k];
+}
+
 // duplicateBoundaries (momopt_boundaries.cpp:190-223): x strips, then y strips.
 __global__ void ghost_x_kernel(GridArgs g) {
     const int idx = blockIdx.x * kBlock + threadIdx.x;   // over 2 * ny * nm
@@ -155,6 +255,19 @@ void launch_flux_t(const GridArgs &g, int q0, int nqc, int stage, double dt, boo
     const int n = g.nx * g.ny;
     hipLaunchKernelGGL(flux_euler_kernel<NM>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g, q0, nqc,
                        stage, dt, int(fc), int(lc));
+}
+
+template <int NM>
+void launch_fused_t(const GridArgs &g, int tile, int stage, double dt, hipStream_t s) {
+    if constexpr (NM <= 15) {   // alpha of a 20 x 20 halo'd tile fits LDS up to n = 15
+        if (tile == 2) {
+            hipLaunchKernelGGL((flux_fused_kernel<NM, 16, 16, 8>), dim3((g.nx + 15) / 16, (g.ny + 15) / 16), dim3(256),
+                               0, s, g, stage, dt);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((flux_fused_kernel<NM, 8, 8, 8>), dim3((g.nx + 7) / 8, (g.ny + 7) / 8), dim3(64), 0, s, g,
+                       stage, dt);
 }
 
 template <int NM>
@@ -194,6 +307,13 @@ void launch_ansatz(const GridArgs &g, int q0, int nqc, hipStream_t s) {
         return true;
     };
     f();
+}
+
+bool launch_flux_fused(const GridArgs &g, int tile, int stage, double dt, hipStream_t s) {
+#define CALLX(N) launch_fused_t<N>(g, tile, stage, dt, s)
+    MNC_NM_SWITCH(g.nm, CALLX)
+#undef CALLX
+    return true;
 }
 
 bool launch_flux_euler(const GridArgs &g, int q0, int nqc, int stage, double dt, bool first_chunk, bool last_chunk,

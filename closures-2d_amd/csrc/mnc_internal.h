@@ -158,6 +158,9 @@ void launch_ansatz(const GridArgs &g, int q0, int nqc, hipStream_t s);
 // stage: 0 = flux only (writes g.flux), 1 = Euler stage, 2 = Euler + Heun average
 bool launch_flux_euler(const GridArgs &g, int q0, int nqc, int stage, double dt, bool first_chunk,
                        bool last_chunk, hipStream_t s);
+// The ansatz fused into the flux/Euler stage (E kept in LDS per tile):
+// tile 1 = 8 x 8 cells, 2 = 16 x 16.
+bool launch_flux_fused(const GridArgs &g, int tile, int stage, double dt, hipStream_t s);
 
 }  // namespace mnc
 

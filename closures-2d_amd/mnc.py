@@ -82,6 +82,9 @@ def lib():
         L.mnc_get_gaunt.argtypes = [vp, _ip, _ip, _dp]
         L.mnc_opt_batch.argtypes = [vp, C.c_int, vp, vp, C.POINTER(Options), vp]
         L.mnc_opt_batch_host.argtypes = [vp, C.c_int, _dp, _dp, C.POINTER(Options), C.POINTER(Outs)]
+        L.mnc_batch_set.argtypes = [vp, C.c_int, _dp, _dp]
+        L.mnc_batch_solve.argtypes = [vp, C.POINTER(Options)]
+        L.mnc_batch_get.argtypes = [vp, _dp, _dp, C.POINTER(Outs)]
         L.mnc_grid_create.argtypes = [vp, C.POINTER(GridDesc), _dp, _dp, _dp, _dp]
         L.mnc_grid_get.argtypes = [vp, _dp, _dp]
         L.mnc_grid_set.argtypes = [vp, _dp, _dp]
@@ -169,11 +172,25 @@ class Context:
                     "mnc_get_gaunt")
         return ptr, k[:nnz], v[:nnz]
 
-    def opt_batch_device(self, n, u_ptr, alpha_ptr, opt, outs_ptr=None):
-        """mnc_opt_batch on device arrays (e.g. torch tensors' data_ptr()); the
-        caller makes their contents visible to the context's stream first."""
-        self._check(lib().mnc_opt_batch(self.h, n, C.c_void_p(u_ptr), C.c_void_p(alpha_ptr), C.byref(opt),
-                                        C.c_void_p(outs_ptr) if outs_ptr else None), "mnc_opt_batch")
+    def batch_set(self, u, alpha):
+        """Stages a batch in device memory (mnc_batch_set)."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, self.nm)
+        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1, self.nm)
+        self._check(lib().mnc_batch_set(self.h, u.shape[0], _p(u), _p(a)), "mnc_batch_set")
+        self._batch_n = u.shape[0]
+
+    def batch_solve(self, opt):
+        self._check(lib().mnc_batch_solve(self.h, C.byref(opt)), "mnc_batch_solve")
+
+    def batch_get(self):
+        n = self._batch_n
+        u = np.zeros((n, self.nm))
+        a = np.zeros((n, self.nm))
+        outs = (Outs * n)()
+        self._check(lib().mnc_batch_get(self.h, _p(u), _p(a), outs), "mnc_batch_get")
+        it = np.array([o.iter for o in outs], dtype=np.int32)
+        r = np.array([o.r for o in outs])
+        return u, a, it, r
 
     def opt_batch_host(self, u, alpha, opt=None, allow_status=(OK,)):
         """Batched opt() on host arrays; returns (status, u, alpha, iter, r, normG)."""

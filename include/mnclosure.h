@@ -134,6 +134,15 @@ int mnc_opt_batch(mnc_ctx *ctx, int ncells, double *u, double *alpha,
 int mnc_opt_batch_host(mnc_ctx *ctx, int ncells, double *u, double *alpha,
                        const mnc_options *opt, mnc_outs *outs);
 
+/* A batch kept in device memory across calls (the staging arrays of
+ * mnc_opt_batch_host): upload it, solve it in place any number of times (each
+ * solve is mnc_opt_batch on those arrays, outs kept on the device), read it
+ * back.  No counterpart in the reference; it serves callers that reuse one
+ * batch and timing of the closure without the PCIe copies. */
+int mnc_batch_set(mnc_ctx *ctx, int ncells, const double *u, const double *alpha);
+int mnc_batch_solve(mnc_ctx *ctx, const mnc_options *opt);
+int mnc_batch_get(mnc_ctx *ctx, double *u, double *alpha, mnc_outs *outs);
+
 /* Grid state (ghost-inclusive, [(nx+4)*(ny+4)][n] moments and alpha;
  * [(nx+4)*(ny+4)] sigma_t, sigma_s), uploaded from host arrays prepared as
  * Solver::initializeGrid + MomOptSolver::init do (main.cpp:257-372,

@@ -134,3 +134,28 @@ def test_lattice_floor_cells_reference_and_robust():
     assert same.mean() >= 0.99
     rel = np.abs(ug - uo).max(axis=1) / np.abs(uo).max(axis=1)
     assert rel.max() <= 1e-12
+
+
+@pytest.mark.gpu
+def test_resident_batch_equals_host_batch():
+    """mnc_batch_set / mnc_batch_solve / mnc_batch_get solve the same batch as
+    mnc_opt_batch_host, and a second solve of the staged (already solved)
+    batch converges at its first iteration."""
+    mnc = load_pkg("mnc")
+    o = oracle()
+    tab = o.Tables(4, 16, False)
+    rng = np.random.default_rng(21)
+    u, _, _ = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, 1000, rng)
+    a0 = isotropic_start(u)
+    ctx = mnc.Context(4, 16)
+    opt = mnc.options(tol=1e-8)
+    st, uh, ah, ith, rh, _ = ctx.opt_batch_host(u, a0, opt)
+    ctx.batch_set(u, a0)
+    ctx.batch_solve(opt)
+    ub, ab, itb, rb = ctx.batch_get()
+    assert np.array_equal(uh, ub) and np.array_equal(ah, ab) and np.array_equal(ith, itb)
+    ctx.batch_solve(opt)
+    _, _, it2, _ = ctx.batch_get()
+    ctx.close()
+    print("second solve iterations: max %d" % it2.max())
+    assert (it2 <= 2).mean() >= 0.99

@@ -174,3 +174,24 @@ def test_driver_reproduces_fixture(tmp_path):
     assert h[:2] == (36, 74) and h[6] == 10
     assert rel_l2(density(m), density(ref)) < 1e-10
     assert (tmp_path / "out_1.030_0.opt").read_text().startswith("Max number of iterations:")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ic", ["lattice", "smooth"])
+def test_fused_flux_tiles_equal_unfused(ic):
+    """The ansatz fused into the flux (8x8 and 16x16 LDS tiles) gives the same
+    state, bit for bit, as the ansatz written to HBM and read back."""
+    mnc, gi = load_pkg("mnc"), load_pkg("grid_init")
+    print("case fused flux %s" % ic, flush=True)
+    d = fixture_deck(ic)
+    states = []
+    for tile in (0, 1, 2):
+        ctx, g = gpu_grid_from_deck(mnc, gi, d)
+        ctx.tune("flux_tile", tile)
+        for _ in range(3):
+            ctx.update(g.max_dt, mn_options(mnc, d))
+        states.append(ctx.grid_get())
+        ctx.close()
+    for m, a in states[1:]:
+        assert np.array_equal(m, states[0][0])
+        assert np.array_equal(a, states[0][1])
