@@ -108,6 +108,7 @@ def train_mlp(mnc, gi, d, pairs_target=2_000_000, train_steps=2000, seed=1):
         idx = np.random.default_rng(seed).choice(len(us), pairs_target, replace=False)
         us, al = us[idx], al[idx]
     t1 = time.perf_counter()
+    progress("harvested %d pairs" % len(us))
     ctx = mnc.Context(d["moment_order"], d["quad_order"], use_gaunt=d["use_gaunt"])
     ctx.mlp_init(seed)
     ctx.timing_enable(True)
@@ -119,6 +120,13 @@ def train_mlp(mnc, gi, d, pairs_target=2_000_000, train_steps=2000, seed=1):
     info = {"pairs": int(len(us)), "harvest_s": t1 - t0, "train_steps": train_steps, "batch": 4096, "lr": 1e-3,
             "train_gpu_ms": ms_mlp, "loss_first": float(loss[:10].mean()), "loss_last": float(loss[-50:].mean())}
     return params, info
+
+
+def progress(msg):
+    print("[bench %.1fs] %s" % (time.perf_counter() - T0, msg), file=sys.stderr, flush=True)
+
+
+T0 = time.perf_counter()
 
 
 def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=60, autotune=True):
@@ -135,6 +143,7 @@ def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=60, a
         ctx.update(dts[k], opt)
     ctx.sync()
     report = ctx.autotune(dts[0], opt, steps=3) if autotune else []
+    progress("autotuned: %s" % "; ".join(report[-3:]))
     tuned = {k: ctx.get_tune(k) for k in ("graph", "rounds", "tail_mode", "tail_blocks", "mix_blocks")}
     ctx.grid_set(g.moments, g.alpha)        # back to t = 0
     ctx.reset_stats()
@@ -146,6 +155,7 @@ def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=60, a
     wall = time.perf_counter() - t0
     st = ctx.stats()
     m, a = ctx.grid_get()
+    progress("timed %d steps in %.2f s" % (steps, wall))
     # eager pass with per-launch timing
     ks = min(steps, kernel_steps)
     ctx.grid_set(g.moments, g.alpha)
@@ -196,6 +206,7 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
     u, astar, lobe = wl.isolated_batch(w, p, xi, eta, 15, n, seed)
     a0 = wl.isotropic_start(u)
     gen_s = time.perf_counter() - t0
+    progress("isolated batch generated (%.1f s)" % gen_s)
     tab = oracle.Tables(4, 16, False)
     m = oracle_sample
     sto, uo, ao, ito, ro, _ = tab.opt_batch(u[:m], a0[:m], max_iter=100, tol=1e-8, cond_h_max=1e10,
@@ -205,6 +216,7 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
     ust, ast_ = mnc.scaled_pairs(ut, at)
     ctx.mlp_init(seed)
     loss = ctx.mlp_train(ust, ast_, steps=train_steps, batch=4096, lr=1e-3, seed=seed)
+    progress("isolated-batch network trained")
     res = {}
     for name, kw in (("plain-fp64", {}), ("mixed", {"precision": mnc.PREC_MIXED}),
                      ("learned+mixed", {"precision": mnc.PREC_MIXED, "warm_start": mnc.WARM_LEARNED})):
@@ -220,6 +232,7 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
                 best = dt
         st = ctx.stats()
         ug = ctx.batch_get()[0][:m]
+        progress("isolated %s: %.3f s" % (name, best))
         err = np.linalg.norm(ug - uo, axis=1) / np.linalg.norm(uo, axis=1)
         res[name] = {"closures_per_s": n / best, "s": best,
                      "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
@@ -237,7 +250,9 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
 def bench_m8(mnc, gi, steps, n=512, sample=1000):
     """The M8 workload: 45 moments, Q = 32 (1024 points), n x n periodic cells on
     [-1,1]^2, sigma_s = 1, isotropic density 1 + 0.5 sin(pi x) sin(pi y), TOL
-    1e-8, direct-Hessian algorithm (FP64 matrix-core path): converged closures/s
+    1e-8 with the robust line search (the reference's Armijo test regularizes
+    ~0.5% of these closures at 1e-8 on rounding), direct-Hessian algorithm
+    (FP64 matrix-core path): converged closures/s
     and s/step over K steps, and the error of GPU closures against the FP64 CPU
     oracle on `sample` cells of the final state."""
     sys.path.insert(0, ROOT)
@@ -247,7 +262,7 @@ def bench_m8(mnc, gi, steps, n=512, sample=1000):
     wl.sine_state(g, 45)
     ctx = mnc.Context(8, 32, use_gaunt=False)
     ctx.grid_create(g.nx, g.ny, g.dx, g.dy, g.init_x, g.init_y, 2, 2.0, g.moments, g.alpha, g.sigma_t, g.sigma_s)
-    opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10)
+    opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, line_search=mnc.LS_ROBUST)
     ctx.update(g.max_dt, opt)                 # allocations
     ctx.grid_set(g.moments, g.alpha)
     ctx.reset_stats()
@@ -269,11 +284,11 @@ def bench_m8(mnc, gi, steps, n=512, sample=1000):
     ctx.close()
     tab = oracle.Tables(8, 32, False)
     _, uo, _, ito, ro, _ = tab.opt_batch(u, a0, max_iter=100, tol=1e-8, cond_h_max=1e10,
-                                         nthreads=len(os.sched_getaffinity(0)))
+                                         nthreads=len(os.sched_getaffinity(0)), line_search=1)
     err = np.linalg.norm(ug - uo, axis=1) / np.linalg.norm(uo, axis=1)
     closures = 2 * (n + 4) * (n + 4) * steps
     return {"workload": "M8 (n=45), Q=32 (nq=1024), %dx%d periodic on [-1,1]^2, sigma_s=1, density "
-                        "1+0.5 sin(pi x) sin(pi y), TOL 1e-8, Heun dt=%.6g" % (n, n, g.max_dt),
+                        "1+0.5 sin(pi x) sin(pi y), TOL 1e-8, robust line search, Heun dt=%.6g" % (n, n, g.max_dt),
             "steps": steps, "closures_per_s": closures / wall, "s_per_timestep": wall / steps,
             "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
             "trials_per_closure": st.trial_sum / st.num_dual_solves, "reg_hist": list(st.hist_reg),
@@ -330,12 +345,16 @@ def main():
         return mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"],
                            line_search=mnc.LS_ROBUST, **kw)
 
+    progress("start")
     plain = run_form(mnc, gi, d, opts(), args.steps, args.warmup)
+    progress("plain form done")
     forms = {"plain-fp64": plain}
     mlp_info = None
     if not args.plain_only:
         forms["mixed"] = run_form(mnc, gi, d, opts(precision=mnc.PREC_MIXED), args.steps, args.warmup)
+        progress("mixed form done")
         params, mlp_info = train_mlp(mnc, gi, d, train_steps=args.train_steps)
+        progress("network trained")
         forms["learned+mixed"] = run_form(mnc, gi, d, opts(precision=mnc.PREC_MIXED, warm_start=mnc.WARM_LEARNED),
                                           args.steps, args.warmup, mlp_params=params)
     form = "learned+mixed" if "learned+mixed" in forms else "plain-fp64"
@@ -381,9 +400,12 @@ def main():
     }
     if args.isolated > 0 and not args.plain_only:
         out["config"]["isolated"] = bench_isolated(mnc, args.isolated, 12345, 20000, args.train_steps)
+        progress("isolated batch done")
     if args.m8_steps > 0 and not args.plain_only:
         out["config"]["m8"] = bench_m8(mnc, gi, args.m8_steps)
+        progress("M8 done")
     if not args.no_cpu:
+        progress("CPU baseline")
         cb = cpu_baseline(d, args.cpu_seconds)
         out["cpu_baseline"] = {"value": cb["value"], "unit": "closures/s", "cores": cb["cores"], "kind": "port",
                                "sample": "oracle (FP64 C restatement, OpenMP dynamic,1) on the first %d steps "

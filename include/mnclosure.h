@@ -176,6 +176,9 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *                  Hessian only while its condition estimate is <= 10^value
  *                  (0-10, default 4); otherwise the step is redone in FP64
  *   "mix_blocks"   workgroups of the mixed-precision sweep (1-65536)
+ *   "flux_tile"    0: ansatz written to HBM, then the flux; 1: ansatz fused
+ *                  into the flux with 8x8-cell LDS tiles (default); 2: 16x16
+ *                  tiles (n <= 15).  The results are identical.
  *   "graph"        1: mnc_update steps are captured into hipGraphs (one per
  *                  distinct dt and options, captured on the second step with
  *                  that key) and replayed; 0: eager launches (default).  While
