@@ -107,11 +107,12 @@ def test_known_answer_alpha():
 
 @pytest.mark.gpu
 def test_lattice_floor_cells_reference_and_robust():
-    """Near-isotropic floor cells of the M4 lattice at TOL 1e-8 (captured from
-    a lattice run at step 34, tests/golden/lattice_state): the reference line
-    search cannot resolve their last decrease and climbs the regularization
-    ladder, which the oracle shows too; the robust rule solves them at r = 0,
-    matching the oracle's robust mode."""
+    """Near-isotropic floor cells of the M4 lattice at TOL 1e-8 (two 24 x 24
+    patches of the state at the start of the first step in which the oracle
+    aborts with "Max regularization", tests/golden/make_lattice_state.py): the
+    reference line search cannot resolve their last decrease and climbs the
+    regularization ladder, which the oracle shows too; the robust rule solves
+    them at r = 0, matching the oracle's robust mode."""
     import os
     from helpers import GOLDEN
     mnc = load_pkg("mnc")
@@ -122,7 +123,7 @@ def test_lattice_floor_cells_reference_and_robust():
     tab = o.Tables(4, 16, False)
     sto, uo, ao, ito, ro, _ = tab.opt_batch(u, a, tol=1e-8, nthreads=8)
     print("reference mode, oracle: r hist", [int((ro == x).sum()) for x in o.REGULARIZATIONS])
-    assert (ro >= 1e-2).sum() >= 2         # the ladder is climbed for the captured cells
+    assert (ro > 0).sum() >= 2 and (ro >= 1e-2).sum() >= 1   # the ladder is climbed
     opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, line_search=mnc.LS_ROBUST)
     st, ug, ag, itg, rg, _ = ctx.opt_batch_host(u, a, opt)
     sto, uo, ao, ito, ro, _ = tab.opt_batch(u, a, tol=1e-8, nthreads=8, line_search=1)
