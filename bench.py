@@ -326,6 +326,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true", help="only the plain FP64 form")
+    ap.add_argument("--only", choices=["plain-fp64", "mixed", "learned+mixed"], default=None,
+                    help="run one lattice form (for profiling); value is that form's rate")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--isolated", type=int, default=1_000_000,
                     help="vectors of the isolated closure batch (0 skips it)")
@@ -346,18 +348,23 @@ def main():
                            line_search=mnc.LS_ROBUST, **kw)
 
     progress("start")
-    plain = run_form(mnc, gi, d, opts(), args.steps, args.warmup)
-    progress("plain form done")
-    forms = {"plain-fp64": plain}
+    want = ["plain-fp64"] if args.plain_only else ([args.only] if args.only else
+                                                    ["plain-fp64", "mixed", "learned+mixed"])
+    forms = {}
     mlp_info = None
-    if not args.plain_only:
+    if "plain-fp64" in want:
+        forms["plain-fp64"] = run_form(mnc, gi, d, opts(), args.steps, args.warmup)
+        progress("plain form done")
+    if "mixed" in want:
         forms["mixed"] = run_form(mnc, gi, d, opts(precision=mnc.PREC_MIXED), args.steps, args.warmup)
         progress("mixed form done")
+    if "learned+mixed" in want:
         params, mlp_info = train_mlp(mnc, gi, d, train_steps=args.train_steps)
         progress("network trained")
         forms["learned+mixed"] = run_form(mnc, gi, d, opts(precision=mnc.PREC_MIXED, warm_start=mnc.WARM_LEARNED),
                                           args.steps, args.warmup, mlp_params=params)
-    form = "learned+mixed" if "learned+mixed" in forms else "plain-fp64"
+    form = want[-1]
+    plain = forms.get("plain-fp64", forms[form])
     best = forms[form]
     value = best["closures"] / best["wall"]
     kn, kms = best["newton"]
@@ -366,8 +373,8 @@ def main():
     achieved = alg_flops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
     # drift of each form's final state from the plain form's (density rel-L2)
     dens = {k: v["moments"][2:-2, 2:-2, 0] for k, v in forms.items()}
-    drift = {k: float(np.sqrt(((dens[k] - dens["plain-fp64"]) ** 2).sum() / (dens["plain-fp64"] ** 2).sum()))
-             for k in forms}
+    ref = dens.get("plain-fp64", dens[form])
+    drift = {k: float(np.sqrt(((dens[k] - ref) ** 2).sum() / (ref ** 2).sum())) for k in forms}
     out = {
         "metric": "converged Mn closures/s (M4 lattice 280^2, t=0..3.2, TOL 1e-8), learned+mixed vs plain",
         "value": value,
@@ -398,10 +405,10 @@ def main():
                      "algorithmic_flops_per_launch": alg_flops / max(kn, 1),
                      "avg_launch_ms": kms / max(kn, 1)},
     }
-    if args.isolated > 0 and not args.plain_only:
+    if args.isolated > 0 and not args.plain_only and not args.only:
         out["config"]["isolated"] = bench_isolated(mnc, args.isolated, 12345, 20000, args.train_steps)
         progress("isolated batch done")
-    if args.m8_steps > 0 and not args.plain_only:
+    if args.m8_steps > 0 and not args.plain_only and not args.only:
         out["config"]["m8"] = bench_m8(mnc, gi, args.m8_steps)
         progress("M8 done")
     if not args.no_cpu:

@@ -195,7 +195,19 @@ __device__ __forceinline__ unsigned long long lane_mask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
 enum : int { ST_NEWTON = 0, ST_TRIAL = 1 };
+constexpr int kHalvings = 8;   // line-search halvings evaluated per wave sweep in the tail
 enum : int { LOGIC_CONTINUE = 0, LOGIC_DONE = 1, LOGIC_DEFER = 2 };
 
 // The table is read-only for the whole kernel; reading it through the constant
@@ -495,7 +507,8 @@ __device__ bool cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
 // the iteration is not taken: the cell returns LOGIC_DEFER at the point where
 // that Newton iteration starts, to be redone by an FP64 sweep.
 template <int NM, int NP2, bool GAUNT, bool MIX = false>
-__device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, LaneStats &st) {
+__device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, LaneStats &st,
+                          float *gstage = nullptr) {
     using St = Store<NM, NP2>;
     const double p0 = A.p[0];
     int mode = S.i(St::I_MODE, c), iter = S.i(St::I_ITER, c), total = S.i(St::I_TOTAL, c);
@@ -526,7 +539,7 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
             t = 0.5 * t;
             if (t * S.d(St::F_MAXF, c) > DBL_EPSILON / 2) {
 #pragma unroll
-                for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AOLD + k, c) + t * S.d(St::F_D + k, c);
+                for (int k = 0; k < NM; ++k) al[k] = fma(t, S.d(St::F_D + k, c), S.d(St::F_AOLD + k, c));
             } else {                                               // exhausted: t = 0 (:74-98)
 #pragma unroll
                 for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AOLD + k, c);
@@ -550,14 +563,20 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
         using HT = typename std::conditional<MIX, float, double>::type;
         HT h[Tri<NM>::size];
         if constexpr (MIX) {                                       // order-2N moments -> H
+            // The moments are staged in this thread's LDS column (stride
+            // kBlock), so the Gaunt terms, whose indices are the same for every
+            // cell (scalar loads), read LDS instead of the store.
+            constexpr int NMM = mix_cols(NM);
+#pragma unroll
+            for (int k = 0; k < NMM; ++k) gstage[k * kBlock] = float(S.d(St::F_ACC + k, c));
 #pragma unroll
             for (int i = 0; i < NM; ++i) {
 #pragma unroll
                 for (int j = i; j < NM; ++j) {
-                    double hij = 0.0;
+                    float hij = 0.0f;
                     const int pr = i * NM + j;
                     for (int cc = A.mgptr[pr]; cc < A.mgptr[pr + 1]; ++cc)
-                        hij = fma(A.mgv[cc], S.d(St::F_ACC + A.mgk[cc], c), hij);
+                        hij = fmaf(float(A.mgv[cc]), gstage[A.mgk[cc] * kBlock], hij);
                     h[Tri<NM>::at(j, i)] = HT(hij);
                 }
             }
@@ -793,12 +812,13 @@ template <int NM, int NP2, bool GAUNT, bool MIX = false>
 __global__ __launch_bounds__(kBlock) void logic_kernel(NewtonArgs A, int lin) {
     const int slot = blockIdx.x * kBlock + threadIdx.x;
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
+    __shared__ float gstage[MIX ? mix_cols(NM) : 1][kBlock];
     LaneStats st;
     int res = LOGIC_DONE;
     int c = -1;
     if (slot < A.ctl[lin]) {
         c = A.lists[size_t(lin) * A.cap + slot];
-        res = cell_logic<NM, NP2, GAUNT, MIX>(A, S, c, st);
+        res = cell_logic<NM, NP2, GAUNT, MIX>(A, S, c, st, &gstage[0][threadIdx.x]);
     }
     list_append(A, lin ^ 1, lin ^ 1, res == LOGIC_CONTINUE, c);
     if constexpr (MIX) list_append(A, 2, 4, res == LOGIC_DEFER, c);
@@ -1043,7 +1063,8 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
         int s = 0;
         if (lane == 0) s = atomicAdd(&A.ctl[2], 1);
         s = __shfl(s, 0);
-        const int c = tail_cell(A, lin, count, dcount, s);
+        // wave-uniform: state loads become scalar loads
+        const int c = __builtin_amdgcn_readfirstlane(tail_cell(A, lin, count, dcount, s));
         if (c < 0) break;
         for (;;) {
             double al[NM];
@@ -1115,10 +1136,64 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
                 if (accept) {
                     newton = true;
                 } else {
-                    t = 0.5 * t;
-                    if (t * S.d(St::F_MAXF, c) > DBL_EPSILON / 2) {
+                    // The reference halves t and evaluates f again until
+                    // Armijo holds or t * maxf <= eps/2 (linesearch.cpp:45-98).
+                    // The next kHalvings candidates are evaluated in one wave
+                    // sweep (f only, the same sums in the same order as a
+                    // trial sweep) and the first accepted one is taken: the
+                    // reference's decisions without one pass per halving.
+                    const double maxf = S.d(St::F_MAXF, c);
+                    if (lane < NM) {
+                        sh.x[lane] = S.d(St::F_AOLD + lane, c);
+                        sh.gvec[lane] = S.d(St::F_D + lane, c);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    int state = 0;                 // 0 searching, 1 accepted, 2 exhausted
+                    while (state == 0) {
+                        int nval = 0;
 #pragma unroll
-                        for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AOLD + k, c) + t * S.d(St::F_D + k, c);
+                        for (int jj = 0; jj < kHalvings; ++jj)
+                            if (nval == jj && ldexp(t, -(jj + 1)) * maxf > DBL_EPSILON / 2) nval = jj + 1;
+                        if (nval == 0) { state = 2; break; }
+                        double fj[kHalvings];
+#pragma unroll
+                        for (int jj = 0; jj < kHalvings; ++jj) fj[jj] = 0.0;
+                        for (int q = lane; q < A.nq; q += 64) {
+                            const double *row = A.p + size_t(q) * NP2;
+                            const double wq = A.w[q];
+#pragma unroll
+                            for (int jj = 0; jj < kHalvings; ++jj) {
+                                if (jj >= nval) break;
+                                const double tj = ldexp(t, -(jj + 1));
+                                double am = 0.0;
+                                for (int k = 0; k < NM; ++k) am = fma(fma(tj, sh.gvec[k], sh.x[k]), row[k], am);
+                                fj[jj] += wq * exp(am);
+                            }
+                        }
+                        int acc = -1;
+                        for (int jj = 0; jj < kHalvings; ++jj) {
+                            if (jj >= nval) break;
+                            double fv = fj[jj];
+#pragma unroll
+                            for (int o = 32; o > 0; o >>= 1) fv += __shfl_xor(fv, o);
+                            const double tj = ldexp(t, -(jj + 1));
+                            for (int k = 0; k < NM; ++k)
+                                fv = fv - fma(tj, sh.gvec[k], sh.x[k]) * S.d(St::F_U + k, c);
+                            if (acc < 0 && fv <= fcur + 1e-3 * tj * gd) acc = jj;
+                        }
+                        if (lane == 0) st.trials += (acc >= 0) ? acc + 1 : nval;
+                        if (acc >= 0) {
+                            t = ldexp(t, -(acc + 1));
+                            state = 1;
+                        } else {
+                            t = ldexp(t, -nval);
+                            if (nval < kHalvings) state = 2;     // the next halving fails the bound
+                        }
+                    }
+                    if (state == 1) {                  // a full sweep at the accepted point next
+#pragma unroll
+                        for (int k = 0; k < NM; ++k) al[k] = fma(t, S.d(St::F_D + k, c), S.d(St::F_AOLD + k, c));
+                        mode = ST_NEWTON;
                     } else {
 #pragma unroll
                         for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AOLD + k, c);
@@ -1126,6 +1201,7 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
                         ++iter; ++total;
                         do_reg = true;
                     }
+                    __builtin_amdgcn_wave_barrier();
                 }
             }
             if (newton) {
@@ -1147,7 +1223,7 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
                     }
                 }
                 __builtin_amdgcn_wave_barrier();
-                double fcur = sh.L[0][0] / (p0 * p0), fscale = fabs(fcur), gg = 0.0, anorm = 0.0;
+                double fcur = sh.L[0][0] / (p0 * p0), fscale = fabs(fcur), gg = 0.0;
                 for (int i = 0; i < NM; ++i) {
                     const double ui = S.d(St::F_U + i, c);
                     const double gi = sh.L[i][0] / p0 - ui;
@@ -1155,10 +1231,11 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
                     fcur = fcur - al[i] * ui;
                     fscale += fabs(al[i] * ui);
                     if (lane == i) sh.gvec[i] = gi;
-                    double rs = 0.0;
-                    for (int jj = 0; jj < NM; ++jj) rs += fabs(sh.L[i][jj]);
-                    anorm = rs > anorm ? rs : anorm;
                 }
+                double rs = 0.0;                       // row sums, one row per lane (utils.cpp:36-52)
+                if (lane < NM)
+                    for (int jj = 0; jj < NM; ++jj) rs += fabs(sh.L[lane][jj]);
+                const double anorm = wave_max(rs);
                 err = sqrt(gg);
                 __builtin_amdgcn_wave_barrier();
                 const bool chol_ok = coop_cholesky<NM>(sh.L, lane);
@@ -1480,17 +1557,6 @@ __global__ __launch_bounds__(256) void sweep_g64_kernel(NewtonArgs A, int lin) {
     }
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
-}
-
 template <int NM>
 struct G64Shared {
     double L[NM][NM];   // H, then its Cholesky factor (lower)
@@ -1532,7 +1598,7 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
             } else {
                 t = 0.5 * t;
                 if (t * S.d(St::F_MAXF, c) > DBL_EPSILON / 2) {
-                    if (lane < NM) sh.al[lane] = S.d(St::F_AOLD + lane, c) + t * S.d(St::F_D + lane, c);
+                    if (lane < NM) sh.al[lane] = fma(t, S.d(St::F_D + lane, c), S.d(St::F_AOLD + lane, c));
                 } else {                                            // exhausted (:74-98)
                     if (lane < NM) sh.al[lane] = S.d(St::F_AOLD + lane, c);
                     t = 0.0;

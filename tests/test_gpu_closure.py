@@ -160,3 +160,36 @@ def test_resident_batch_equals_host_batch():
     ctx.close()
     print("second solve iterations: max %d" % it2.max())
     assert (it2 <= 2).mean() >= 0.99
+
+
+@pytest.mark.gpu
+def test_tail_modes_take_the_same_path():
+    """rounds = 0 sends every cell through the persistent tail.  The wave tail
+    evaluates line-search halvings eight at a time; the lane tail one at a
+    time.  Both take the reference's decisions, so the iteration counts,
+    regularization levels and trial counts agree, and the answers agree to the
+    rounding of their different summation orders."""
+    mnc = load_pkg("mnc")
+    o = oracle()
+    tab = o.Tables(4, 16, False)
+    rng = np.random.default_rng(33)
+    u, _, _ = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, 2000, rng, near_boundary_frac=0.5)
+    a0 = isotropic_start(u)
+    res = {}
+    for mode in (0, 1):
+        ctx = mnc.Context(4, 16)
+        ctx.tune("rounds", 0)
+        ctx.tune("tail_mode", mode)
+        ctx.reset_stats()
+        st, ug, ag, itg, rg, _ = ctx.opt_batch_host(u, a0, mnc.options(tol=1e-8))
+        res[mode] = (st, ug, itg, rg, ctx.stats())
+        ctx.close()
+    same = (res[0][2] == res[1][2]) & (res[0][3] == res[1][3])
+    print("wave vs lane tail: identical (iter, r) %.4f; trials %d vs %d"
+          % (same.mean(), res[0][4].trial_sum, res[1][4].trial_sum))
+    assert res[0][0] == res[1][0] == 0
+    assert same.mean() >= 0.97
+    un = np.linalg.norm(res[0][1][same] - res[1][1][same], axis=1) / np.linalg.norm(res[1][1][same], axis=1)
+    assert np.median(un) <= 1e-13
+    rel = abs(res[0][4].trial_sum - res[1][4].trial_sum) / res[1][4].trial_sum
+    assert rel <= 0.05
