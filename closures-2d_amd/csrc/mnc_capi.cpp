@@ -46,6 +46,7 @@ struct mnc_ctx {
     int mix_cond_log10 = 4;          // mixed steps need cond(H) <= 10^this, else FP64
     int mix_blocks = 0;              // 0: one per CU
     int flux_tile = 1;               // 0: ansatz to HBM + flux; 1: fused, 8x8 tiles; 2: fused, 16x16
+    int learned_gate_log10 = -4;     // the network is consulted when ||g(best start)|| > 10^this ||u||
     // grid
     bool has_grid = false;
     mnc_grid_desc grid{};
@@ -243,16 +244,14 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
         return ctx_fail(ctx, MNC_ERR_ARG, "moment order 8 runs the FP64 form only");
     int st = ensure_state(ctx, size_t(ncells));
     if (st) return st;
-    if (o->warm_start == MNC_WARM_LEARNED) {    // the network's guesses for every cell
+    const bool learned = o->warm_start == MNC_WARM_LEARNED;
+    if (learned) {                               // buffers of the network's guesses
         if ((st = mlp_check(ctx, true))) return st;
         if (size_t(ncells) > ctx->guess_cap) {
-            if ((st = dalloc(ctx, &ctx->d_us, size_t(ncells) * 15)) || (st = dalloc(ctx, &ctx->d_guess, size_t(ncells) * 15)))
-                return st;
+            if ((st = dalloc(ctx, &ctx->d_guess, size_t(ncells) * 15))) return st;
             ctx->guess_cap = size_t(ncells);
         }
-        Timed tm(ctx, MNC_KCLASS_MLP);
-        mnc::mlp_scale_moments(u, ncells, ctx->d_us, ctx->stream);
-        if ((st = mlp_predict_dev(ctx, ncells, ctx->d_us, ctx->d_guess))) return st;
+        if ((st = mlp_alloc(ctx, ncells, 1))) return st;
         alpha_guess = ctx->d_guess;
     }
     mnc::NewtonArgs a{};
@@ -286,15 +285,36 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     a.mgk = ctx->d_mgk;
     a.mgv = ctx->d_mgv;
     a.cmix = std::pow(10.0, ctx->mix_cond_log10);
+    a.gate = std::pow(10.0, ctx->learned_gate_log10);
     a.gz = ctx->d_gz;
     a.gg = ctx->d_gg;
     a.gh = ctx->d_gh;
     a.gnt = ctx->g64.nt;
     a.gnmm = ctx->g64.nmm;
     const int tail = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
+    if (learned) {
+        // prologue (cells whose best classical start is poor go to list 2), the
+        // network on those cells only (row count read on the device), judge
+        {
+            Timed tm(ctx, MNC_KCLASS_NEWTON);
+            mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
+            if (!mnc::launch_newton_init(a, t.nm, t.np2, t.use_gaunt != 0, ctx->stream))
+                return ctx_fail(ctx, MNC_ERR_ARG, "no learned warm start for this order");
+        }
+        {
+            Timed tm(ctx, MNC_KCLASS_MLP);
+            mnc::mlp_predict_listed(mlp_buffers(ctx), u, ctx->d_lists + 2 * ctx->state_cap, ctx->d_ctl + 4, ncells,
+                                    ctx->d_guess, ctx->stream);
+        }
+        {
+            Timed tm(ctx, MNC_KCLASS_NEWTON);
+            mnc::launch_newton_judge(a, t.nm, t.np2, t.use_gaunt != 0, ctx->stream);
+        }
+        a.skip_init = 1;
+    }
     {
         Timed tm(ctx, MNC_KCLASS_NEWTON);
-        mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
+        if (!learned) mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
         if (ctx->has_g64) {
             // lockstep rounds in batches until the active list is empty (the
             // count is read back between batches, so this path is not captured)
@@ -839,6 +859,11 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         ctx->flux_tile = value;
         return MNC_OK;
     }
+    if (std::strcmp(key, "learned_gate_log10") == 0) {
+        if (value < -16 || value > 0) return ctx_fail(ctx, MNC_ERR_ARG, "learned_gate_log10 out of range");
+        ctx->learned_gate_log10 = value;
+        return MNC_OK;
+    }
     return ctx_fail(ctx, MNC_ERR_ARG, "unknown tuning key");
 }
 
@@ -854,6 +879,7 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
     if (std::strcmp(key, "has_mixed") == 0) { *value = ctx->has_mix ? 1 : 0; return MNC_OK; }
     if (std::strcmp(key, "graph") == 0) { *value = ctx->use_graph; return MNC_OK; }
     if (std::strcmp(key, "flux_tile") == 0) { *value = ctx->flux_tile; return MNC_OK; }
+    if (std::strcmp(key, "learned_gate_log10") == 0) { *value = ctx->learned_gate_log10; return MNC_OK; }
     if (std::strcmp(key, "tail_blocks") == 0) {
         *value = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
         return MNC_OK;

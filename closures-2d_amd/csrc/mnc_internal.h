@@ -46,6 +46,9 @@ struct NewtonArgs {
                           // [9..11] warm starts taken: learned, isotropic, previous
     int ls_mode;          // MNC_LS_*
     const double *alpha_guess; // [ncells][nm] scaled-problem guess (learned warm start) or nullptr
+    double gate;          // learned warm start: the network is consulted when the best of isotropic
+                          // and previous alpha has ||g|| > gate * ||u|| (scaled problem)
+    int skip_init;        // the prologue (init_kernel) was already enqueued
     // per-cell solver state and the active lists
     double *st_d;         // [dfields][cap]
     int *st_i;            // [ifields][cap]
@@ -107,6 +110,11 @@ int build_g64_tables(int moment_order, int quad_order, G64Tables &m);
 // tail_mode 0: one wave per remaining cell (cooperative sweeps); 1: one lane per cell.
 bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
                         int tail_mode, hipStream_t s);
+// The learned warm start's two halves (M4): the prologue, which parks the cells
+// whose best classical start is poor on list 2, and the judge, which compares
+// the network's guesses for them and moves them to list 0.
+bool launch_newton_init(const NewtonArgs &a, int nm, int np2, bool gaunt, hipStream_t s);
+bool launch_newton_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, hipStream_t s);
 // The same with the lockstep rounds in mixed precision (MFMA sweeps); the tail
 // is FP64.  mix_blocks: persistent blocks of the mixed sweep.
 bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
@@ -136,12 +144,18 @@ size_t mlp_scratch_floats(int rows);
 // order; as == nullptr: inputs only) into the scratch's input/target slots.
 void mlp_gather(const MlpBuffers &b, const double *us, const double *as, long long npairs, int rows,
                 unsigned long long seed, long long i0, hipStream_t s);
-void mlp_forward(const MlpBuffers &b, int rows, hipStream_t s);
+// mdev: if set, the row count is *mdev (at most rows), read on the device.
+void mlp_forward(const MlpBuffers &b, int rows, hipStream_t s, const int *mdev = nullptr);
 void mlp_backward(const MlpBuffers &b, int rows, hipStream_t s);
 void mlp_adam(const MlpBuffers &b, float lr, int step, hipStream_t s);
 // Scaled multipliers of the last forward pass into alpha[rows][15] (FP64).
 void mlp_unscale(const MlpBuffers &b, int rows, double *alpha, hipStream_t s);
 void mlp_scale_moments(const double *u, int ncells, double *us, hipStream_t s);
+// The network for the cells listed in list[0 .. *count) (a device count; at
+// most max_rows): scaled moments gathered from u (unscaled), guesses scattered
+// into alpha[cell][15].
+void mlp_predict_listed(const MlpBuffers &b, const double *u, const int *list, const int *count, int max_rows,
+                        double *alpha, hipStream_t s);
 float *mlp_output(const MlpBuffers &b, int rows);
 
 // Arguments of the grid kernels (mnc_grid.hip).

@@ -50,6 +50,7 @@ struct GemmArgs {
     const float *A, *B, *bias, *aux;
     float *C, *C2;
     int M, N, K;
+    const int *mdev;      // if set, M = min(M, *mdev) (row count known only on the device)
     long long sai, sak, sbk, sbj;
     int ldc;
     int epi;
@@ -59,6 +60,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
     const int lane = threadIdx.x & 63;
     const int tiles_n = (g.N + 15) >> 4;
     const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g.mdev) g.M = min(g.M, *g.mdev);
     const int tiles = ((g.M + 15) >> 4) * tiles_n;
     if (tile >= tiles) return;
     const int i0 = (tile / tiles_n) * 16, j0 = (tile % tiles_n) * 16;
@@ -168,6 +170,26 @@ __global__ void adam_kernel(float *w, const float *grad, float *m, float *v, int
     w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+// Listed rows: X[i] = standardised u[cell]/u[cell][0], cell = list[i], i < *count.
+__global__ void gather_list_kernel(const double *u, const int *list, const int *count, const float *stats,
+                                   float *X) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count) return;
+    const double *uc = u + size_t(list[i]) * 15;
+    const double r = uc[0] > 0.0 ? 1.0 / uc[0] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 15; ++k) X[i * 15 + k] = float((uc[k] * r - stats[k]) / stats[15 + k]);
+}
+
+// Listed rows: alpha[cell] = Y[i] * out_std + out_mean, cell = list[i].
+__global__ void unscale_list_kernel(const float *Y, const int *list, const int *count, const float *stats,
+                                    double *alpha) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *count * 15) return;
+    const int row = i / 15, k = i % 15;
+    alpha[size_t(list[row]) * 15 + k] = double(Y[i]) * double(stats[45 + k]) + double(stats[30 + k]);
+}
+
 // Scaled multipliers from the network output: alpha_s = Y * out_std + out_mean.
 __global__ void unscale_kernel(const float *Y, int rows, const float *stats, double *alpha) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -197,8 +219,9 @@ constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + kIn * kH, OFF_W2 = OFF_B1 + kH, OFF_
 static_assert(NPARAM == 37007, "parameter count");
 
 void gemm(hipStream_t s, const float *A, long long sai, long long sak, const float *B, long long sbk, long long sbj,
-          float *C, int ldc, int M, int N, int K, const float *bias, int epi, float *C2, const float *aux) {
-    GemmArgs g{A, B, bias, aux, C, C2, M, N, K, sai, sak, sbk, sbj, ldc, epi};
+          float *C, int ldc, int M, int N, int K, const float *bias, int epi, float *C2, const float *aux,
+          const int *mdev = nullptr) {
+    GemmArgs g{A, B, bias, aux, C, C2, M, N, K, mdev, sai, sak, sbk, sbj, ldc, epi};
     const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
     hipLaunchKernelGGL(gemm_kernel, dim3((tiles + 3) / 4), dim3(256), 0, s, g);
 }
@@ -210,15 +233,25 @@ int mlp_param_count() { return NPARAM; }
 // Y, dY [rows*15], dA/dZ [2][rows*128].
 size_t mlp_scratch_floats(int rows) { return size_t(rows) * (15 * 4 + 128 * 8); }
 
-void mlp_forward(const MlpBuffers &b, int rows, hipStream_t s) {
+// rows: the scratch layout's row count (and the GEMMs' M unless mdev gives fewer).
+void mlp_forward(const MlpBuffers &b, int rows, hipStream_t s, const int *mdev) {
     float *X = b.scratch, *Z1 = X + size_t(rows) * 30, *A1 = Z1 + size_t(rows) * kH, *Z2 = A1 + size_t(rows) * kH,
           *A2 = Z2 + size_t(rows) * kH, *Z3 = A2 + size_t(rows) * kH, *A3 = Z3 + size_t(rows) * kH,
           *Y = A3 + size_t(rows) * kH;
     const float *W = b.w;
-    gemm(s, X, kIn, 1, W + OFF_W1, kH, 1, Z1, kH, rows, kH, kIn, W + OFF_B1, EPI_GELU, A1, nullptr);
-    gemm(s, A1, kH, 1, W + OFF_W2, kH, 1, Z2, kH, rows, kH, kH, W + OFF_B2, EPI_GELU, A2, nullptr);
-    gemm(s, A2, kH, 1, W + OFF_W3, kH, 1, Z3, kH, rows, kH, kH, W + OFF_B3, EPI_GELU, A3, nullptr);
-    gemm(s, A3, kH, 1, W + OFF_W4, kOut, 1, Y, kOut, rows, kOut, kH, W + OFF_B4, EPI_STORE, nullptr, nullptr);
+    gemm(s, X, kIn, 1, W + OFF_W1, kH, 1, Z1, kH, rows, kH, kIn, W + OFF_B1, EPI_GELU, A1, nullptr, mdev);
+    gemm(s, A1, kH, 1, W + OFF_W2, kH, 1, Z2, kH, rows, kH, kH, W + OFF_B2, EPI_GELU, A2, nullptr, mdev);
+    gemm(s, A2, kH, 1, W + OFF_W3, kH, 1, Z3, kH, rows, kH, kH, W + OFF_B3, EPI_GELU, A3, nullptr, mdev);
+    gemm(s, A3, kH, 1, W + OFF_W4, kOut, 1, Y, kOut, rows, kOut, kH, W + OFF_B4, EPI_STORE, nullptr, nullptr, mdev);
+}
+
+void mlp_predict_listed(const MlpBuffers &b, const double *u, const int *list, const int *count, int max_rows,
+                        double *alpha, hipStream_t s) {
+    hipLaunchKernelGGL(gather_list_kernel, dim3((max_rows + 255) / 256), dim3(256), 0, s, u, list, count,
+                       b.w + NPARAM, b.scratch);
+    mlp_forward(b, max_rows, s, count);
+    hipLaunchKernelGGL(unscale_list_kernel, dim3((max_rows * 15 + 255) / 256), dim3(256), 0, s,
+                       mlp_output(b, max_rows), list, count, b.w + NPARAM, alpha);
 }
 
 float *mlp_output(const MlpBuffers &b, int rows) { return b.scratch + size_t(rows) * (30 + 6 * kH); }

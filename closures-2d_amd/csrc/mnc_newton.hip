@@ -399,15 +399,24 @@ __device__ __forceinline__ void list_append(const NewtonArgs &A, int dst, int cw
     if (want) A.lists[size_t(dst) * A.cap + base + __popcll(m & lane_mask_lt())] = c;
 }
 
+enum : int { INIT_INVALID = 0, INIT_READY = 1, INIT_NEEDS_MLP = 2 };
+
 // opt() prologue (opt.cpp:87-101) and optScaled set-up (:170-205) for cell c.
-// Returns false if u[0] <= 0 (the reference aborts: opt.cpp:97-101).
+// Returns INIT_INVALID if u[0] <= 0 (the reference aborts: opt.cpp:97-101).
+// MNC_WARM_LEARNED: the start is the better (smaller FP64 gradient norm) of
+// isotropicAlpha and the alpha passed in; if that norm is above gate * ||u||,
+// the cell returns INIT_NEEDS_MLP with the norm^2 parked in F_FCUR and the
+// pick in I_REG, and judge_kernel later compares the network's guess.
 template <int NM, int NP2>
-__device__ bool cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, LaneStats &st) {
+__device__ int cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, LaneStats &st) {
     using St = Store<NM, NP2>;
     const double *uin = A.u + size_t(c) * NM;
     const double *ain = A.alpha + size_t(c) * NM;
     const double u0 = uin[0];
     const double ascale = log(u0) * 2.0 * kSqrtPi;
+    bool needs_mlp = false;
+    double park_best = 0.0;
+    int park_pick = 0;
     double ss = 0.0;
 #pragma unroll
     for (int k = 0; k < NM; ++k) {
@@ -420,17 +429,18 @@ __device__ bool cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
         al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
 #pragma unroll
         for (int k = 1; k < NM; ++k) al[k] = 0.0;
-    } else if (A.warm == MNC_WARM_LEARNED && A.alpha_guess) {
+    } else if (A.warm == MNC_WARM_LEARNED) {
         // Judge: the candidate with the smallest FP64 gradient norm.  At
         // isotropicAlpha the integrals are (1, 0, ...) up to rounding (the
         // basis is orthonormal and <m0 e> = 1 there), so its gradient norm is
         // sum_{k>0} us_k^2 without a sweep.
         double us[NM];
-        double best = 0.0;
+        double best = 0.0, unorm2 = 0.0;
 #pragma unroll
         for (int k = 0; k < NM; ++k) us[k] = uin[k] / u0;
 #pragma unroll
         for (int k = 1; k < NM; ++k) best = fma(us[k], us[k], best);
+        unorm2 = 1.0 + best;
         al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
 #pragma unroll
         for (int k = 1; k < NM; ++k) al[k] = 0.0;
@@ -452,22 +462,13 @@ __device__ bool cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
                 for (int k = 0; k < NM; ++k) al[k] = cand[k];
             }
         }
-        finite = true;
-#pragma unroll
-        for (int k = 0; k < NM; ++k) {
-            cand[k] = A.alpha_guess[size_t(c) * NM + k];
-            finite = finite && isfinite(cand[k]);
+        if (u0 > 0.0 && best > A.gate * A.gate * unorm2) {   // the network may do better
+            needs_mlp = true;
+            park_best = best;
+            park_pick = pick;
+        } else if (u0 > 0.0) {
+            ++st.warm[pick];
         }
-        if (finite && u0 > 0.0) {
-            const double n2 = grad_norm2<NM, NP2>(A, cand, us);
-            if (n2 < best) {
-                best = n2;
-                pick = 0;
-#pragma unroll
-                for (int k = 0; k < NM; ++k) al[k] = cand[k];
-            }
-        }
-        if (u0 > 0.0) ++st.warm[pick];
     } else {
 #pragma unroll
         for (int k = 0; k < NM; ++k) al[k] = ain[k];
@@ -478,7 +479,7 @@ __device__ bool cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
     if (!(u0 > 0.0)) {
         st.err_bits |= 1 << MNC_ERR_U0;
         if (A.outs) { A.outs[c].iter = -1; A.outs[c].r = 0.0; A.outs[c].norm_g = 0.0; }
-        return false;
+        return INIT_INVALID;
     }
     S.d(St::F_U0, c) = u0;
     S.d(St::F_ASCALE, c) = ascale;
@@ -486,10 +487,14 @@ __device__ bool cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
     S.d(St::F_T, c) = 1.0;
     S.i(St::I_ITER, c) = 0;
     S.i(St::I_TOTAL, c) = 0;
-    S.i(St::I_REG, c) = 0;
+    S.i(St::I_REG, c) = needs_mlp ? park_pick : 0;
     S.i(St::I_MAXIT, c) = A.max_iter;
     S.i(St::I_MODE, c) = ST_NEWTON;
-    return true;
+    if (needs_mlp) {
+        S.d(St::F_FCUR, c) = park_best;
+        return INIT_NEEDS_MLP;
+    }
+    return INIT_READY;
 }
 
 // One step of optScaled's loop for cell c after a sweep at its evaluation
@@ -786,10 +791,50 @@ __global__ __launch_bounds__(kBlock) void init_kernel(NewtonArgs A) {
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
     const int c = blockIdx.x * kBlock + threadIdx.x;
     LaneStats st;
+    int res = INIT_INVALID;
+    if (c < A.ncells) res = cell_init<NM, NP2>(A, S, c, st);
+    list_append(A, 0, 0, res == INIT_READY, c);
+    list_append(A, 2, 4, res == INIT_NEEDS_MLP, c);     // (only with MNC_WARM_LEARNED)
+    flush_stats(A, st);
+}
+
+// The learned warm start's second half: for the cells init_kernel parked on
+// list 2, the network's guess (alpha_guess, scaled problem) replaces the start
+// if its FP64 gradient norm is smaller; the cells go onto list 0.
+template <int NM, int NP2>
+__global__ __launch_bounds__(kBlock) void judge_kernel(NewtonArgs A) {
+    using St = Store<NM, NP2>;
+    const St S{A.st_d, A.st_i, A.cap};
+    const int slot = blockIdx.x * kBlock + threadIdx.x;
+    LaneStats st;
     bool ok = false;
-    if (c < A.ncells) ok = cell_init<NM, NP2>(A, S, c, st);
+    int c = -1;
+    if (slot < A.ctl[4]) {
+        c = A.lists[2 * A.cap + slot];
+        double us[NM], cand[NM];
+        bool finite = true;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            us[k] = S.d(St::F_U + k, c);
+            cand[k] = A.alpha_guess[size_t(c) * NM + k];
+            finite = finite && isfinite(cand[k]);
+        }
+        int pick = S.i(St::I_REG, c);
+        if (finite && grad_norm2<NM, NP2>(A, cand, us) < S.d(St::F_FCUR, c)) {
+            pick = 0;
+#pragma unroll
+            for (int k = 0; k < NM; ++k) S.d(St::F_AL + k, c) = cand[k];
+        }
+        ++st.warm[pick];
+        S.i(St::I_REG, c) = 0;
+        ok = true;
+    }
     list_append(A, 0, 0, ok, c);
     flush_stats(A, st);
+}
+
+__global__ void ctl_clear_kernel(int *ctl, int w) {
+    if (threadIdx.x == 0) ctl[w] = 0;
 }
 
 // One sweep for every cell on list `lin`; also clears the count of the list
@@ -929,6 +974,119 @@ __global__ __launch_bounds__(kBlock) void tail_wave_kernel(NewtonArgs A, int lin
 // order of every sum is the serial one (the column-oriented back substitution
 // subtracts x_{n-1}, x_{n-2}, ... in the order of the dot-product form), so a
 // cell takes the same path here as in the lockstep kernels up to FMA rounding.
+// ---- wave-parallel dense algebra with the rows in registers ----------------
+//
+// Lane i (< NM) holds row i of the symmetric matrix (row[k] = A[i][k]); after
+// reg_cholesky, row[k] = L[i][k] for k <= i, and col[k] = L[k][i] (k > i) is
+// filled from it for the back substitution.  Entries of other lanes are read
+// with v_readlane (wave-uniform lane index), so each dependent step of the
+// factorisation and of the triangular solves is a register operation rather
+// than an LDS round trip.  The arithmetic and its order are those of
+// cholesky<N>, apply_inv<N> and rcond<N>.
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffu), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <int NM>
+__device__ __forceinline__ bool reg_cholesky(double (&row)[NM], int lane) {
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        double dot = 0.0;
+#pragma unroll
+        for (int k = 0; k < j; ++k) dot = fma(row[k], row[k], dot);
+        double ajj = readlane_d(row[j] - dot, j);
+        ok = ok && (ajj > 0.0);
+        ajj = sqrt(ajj > 0.0 ? ajj : 1.0);
+        double y = row[j];
+#pragma unroll
+        for (int k = 0; k < j; ++k) y = fma(-readlane_d(row[k], j), row[k], y);
+        const double r = 1.0 / ajj;
+        if (lane == j) row[j] = ajj;
+        else if (lane > j && lane < NM) row[j] = y * r;
+    }
+    return ok;
+}
+
+// x <- (L L^T)^{-1} x, x_i on lane i
+template <int NM>
+__device__ __forceinline__ void reg_apply_inv(const double (&row)[NM], const double (&col)[NM], double &x, int lane) {
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        const double xj = readlane_d(x, j) / readlane_d(row[j], j);
+        if (lane == j) x = xj;
+        else if (lane > j && lane < NM) x = fma(-xj, row[j], x);
+    }
+#pragma unroll
+    for (int j = NM - 1; j >= 0; --j) {
+        const double xj = readlane_d(x, j) / readlane_d(row[j], j);
+        if (lane == j) x = xj;
+        else if (lane < j) x = fma(-col[j], xj, x);
+    }
+}
+
+// Reciprocal 1-norm condition estimate (dpocon's role; Higham 1988), x_i on lane i.
+template <int NM>
+__device__ __forceinline__ double reg_rcond(const double (&row)[NM], const double (&col)[NM], double anorm, int lane) {
+    if (anorm == 0.0) return 0.0;
+    unsigned isgn = 0;
+    double est = 0.0;
+    int jump = 1, j = 0, iter = 0;
+    double x = lane < NM ? 1.0 / NM : 0.0;
+#pragma unroll 1
+    for (int guard = 0; guard < 16; ++guard) {
+        reg_apply_inv<NM>(row, col, x, lane);
+        double s = 0.0, m = -1.0;
+        int im = 0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {                 // dasum / idamax in the serial order
+            const double v = fabs(readlane_d(x, i));
+            s += v;
+            if (v > m) { m = v; im = i; }
+        }
+        const unsigned sg = (unsigned)(__ballot(lane < NM && !(x >= 0.0)) & ((NM >= 32) ? 0xffffffffull : ((1ull << NM) - 1)));
+        const double xj_old = readlane_d(x, j);
+        double nx = 0.0;
+        bool done = false;
+        if (jump == 1) {
+            if (NM == 1) { est = fabs(readlane_d(x, 0)); done = true; }
+            else { est = s; isgn = sg; nx = (lane < NM && (sg >> lane & 1u)) ? -1.0 : 1.0; jump = 2; }
+        } else if (jump == 2) {
+            j = im; iter = 2; nx = (lane == j) ? 1.0 : 0.0; jump = 3;
+        } else if (jump == 3) {
+            const double estold = est;
+            est = s;
+            if (sg == isgn || est <= estold) {
+                nx = ((lane & 1) ? -1.0 : 1.0) * (1.0 + double(lane) / double(NM - 1));
+                jump = 5;
+            } else {
+                nx = (lane < NM && (sg >> lane & 1u)) ? -1.0 : 1.0;
+                isgn = sg;
+                jump = 4;
+            }
+        } else if (jump == 4) {
+            j = im;
+            if (xj_old != m && iter < 5) {
+                ++iter; nx = (lane == j) ? 1.0 : 0.0; jump = 3;
+            } else {
+                nx = ((lane & 1) ? -1.0 : 1.0) * (1.0 + double(lane) / double(NM - 1));
+                jump = 5;
+            }
+        } else {
+            const double temp = 2.0 * (s / double(3 * NM));
+            if (temp > est) est = temp;
+            done = true;
+        }
+        if (done) break;
+        x = lane < NM ? nx : 0.0;
+    }
+    if (!isfinite(est) || est == 0.0) return 0.0;
+    return (1.0 / est) / anorm;
+}
+
 template <int NM, int NP2, bool GAUNT, int RS>
 struct CoopShared {
     static constexpr int NA = GAUNT ? NP2 : Tri<NM>::at(RS, 0);
@@ -1206,60 +1364,73 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
             }
             if (newton) {
                 ++iter; ++total;
-                // H into LDS (full symmetric)
-                if (lane < NM) {
-                    for (int jj = 0; jj < NM; ++jj) {
-                        const int i0 = lane > jj ? lane : jj, j0 = lane > jj ? jj : lane;
-                        double hij;
+                // H: row i in the registers of lane i (full symmetric row)
+                double row[NM];
+#pragma unroll
+                for (int jj = 0; jj < NM; ++jj) {
+                    const int i0 = lane > jj ? lane : jj, j0 = lane > jj ? jj : lane;
+                    double hij = 0.0;
+                    if (lane < NM) {
                         if constexpr (GAUNT) {
                             const int pr = j0 * NM + i0;
-                            hij = 0.0;
                             for (int cc = A.gptr[pr]; cc < A.gptr[pr + 1]; ++cc)
                                 hij = fma(A.gv[cc], sh.accs[A.gk[cc]], hij);
                         } else {
                             hij = sh.accs[Tri<NM>::at(i0, j0)];
                         }
-                        sh.L[lane][jj] = hij;
                     }
+                    row[jj] = hij;
                 }
-                __builtin_amdgcn_wave_barrier();
-                double fcur = sh.L[0][0] / (p0 * p0), fscale = fabs(fcur), gg = 0.0;
+                // g_i = H_i0 / m0 - u_i on lane i; f, its scale and ||g|| in the serial order
+                const double ul = lane < NM ? S.d(St::F_U + lane, c) : 0.0;
+                const double gv = lane < NM ? row[0] / p0 - ul : 0.0;
+                if (lane < NM) sh.gvec[lane] = gv;
+                double fcur = readlane_d(row[0], 0) / (p0 * p0), fscale = fabs(fcur), gg = 0.0;
+#pragma unroll
                 for (int i = 0; i < NM; ++i) {
                     const double ui = S.d(St::F_U + i, c);
-                    const double gi = sh.L[i][0] / p0 - ui;
+                    const double gi = readlane_d(gv, i);
                     gg = fma(gi, gi, gg);
                     fcur = fcur - al[i] * ui;
                     fscale += fabs(al[i] * ui);
-                    if (lane == i) sh.gvec[i] = gi;
                 }
-                double rs = 0.0;                       // row sums, one row per lane (utils.cpp:36-52)
-                if (lane < NM)
-                    for (int jj = 0; jj < NM; ++jj) rs += fabs(sh.L[lane][jj]);
-                const double anorm = wave_max(rs);
+                double rs = 0.0;                       // row sums (utils.cpp:36-52)
+#pragma unroll
+                for (int jj = 0; jj < NM; ++jj) rs += fabs(row[jj]);
+                const double anorm = wave_max(lane < NM ? rs : 0.0);
                 err = sqrt(gg);
+                const bool chol_ok = reg_cholesky<NM>(row, lane);
+                // the columns of the factor (lane i: L[k][i], k > i) for the back substitution
+                double col[NM];
+                if (lane < NM) {
+#pragma unroll
+                    for (int k = 0; k < NM; ++k) sh.L[lane][k] = row[k];
+                }
                 __builtin_amdgcn_wave_barrier();
-                const bool chol_ok = coop_cholesky<NM>(sh.L, lane);
+#pragma unroll
+                for (int k = 0; k < NM; ++k) col[k] = (lane < NM && k > lane) ? sh.L[k][lane] : 0.0;
+                __builtin_amdgcn_wave_barrier();
                 double condest = 0.0;
-                if (chol_ok) condest = 1.0 / coop_rcond<NM>(sh.L, sh.x, anorm, lane);
+                if (chol_ok) condest = 1.0 / reg_rcond<NM>(row, col, anorm, lane);
                 if (iter >= maxit || !chol_ok || t == 0.0 || condest > A.cond_h_max) {
                     do_reg = true;
                 } else {
-                    if (lane < NM) sh.x[lane] = sh.gvec[lane];
-                    __builtin_amdgcn_wave_barrier();
-                    coop_apply_inv<NM>(sh.L, sh.x, lane);
+                    double xv = gv;
+                    reg_apply_inv<NM>(row, col, xv, lane);
                     if (err <= S.d(St::F_RTOL, c)) {
                         converged = true;
                     } else {
-                        double maxf = 0.0, gd = 0.0;
-                        for (int i = 0; i < NM; ++i) {
-                            const double di = -sh.x[i];
-                            const double tmp = fabs(di) / (fabs(al[i]) + 1e-100);
-                            maxf = tmp > maxf ? tmp : maxf;
-                            gd = fma(sh.gvec[i], di, gd);
-                        }
-                        for (int i = 0; i < NM; ++i) {
-                            S.d(St::F_AOLD + i, c) = al[i];
-                            S.d(St::F_D + i, c) = -sh.x[i];
+                        const double di = -xv;
+                        double al_l = 0.0;
+#pragma unroll
+                        for (int k = 0; k < NM; ++k) al_l = (k == lane) ? al[k] : al_l;
+                        const double maxf = wave_max(lane < NM ? fabs(di) / (fabs(al_l) + 1e-100) : 0.0);
+                        double gd = 0.0;
+#pragma unroll
+                        for (int i = 0; i < NM; ++i) gd = fma(readlane_d(gv, i), readlane_d(di, i), gd);
+                        if (lane < NM) {
+                            S.d(St::F_AOLD + lane, c) = al_l;
+                            S.d(St::F_D + lane, c) = di;
                         }
                         S.d(St::F_MAXF, c) = maxf;
                         S.d(St::F_GD, c) = gd;
@@ -1268,7 +1439,7 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
                         t = 1.0;
                         if (t * maxf > DBL_EPSILON / 2) {
 #pragma unroll
-                            for (int i = 0; i < NM; ++i) al[i] = al[i] + t * (-sh.x[i]);
+                            for (int i = 0; i < NM; ++i) al[i] = al[i] + t * readlane_d(di, i);
                             mode = ST_TRIAL;
                         } else {
                             t = 0.0;
@@ -1749,7 +1920,7 @@ void launch_tail(const NewtonArgs &a, int lin, int tail_blocks, int tail_mode, h
 template <int NM, int NP2, bool G, int RS>
 void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode, hipStream_t s) {
     const int nb = (a.ncells + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
+    if (!a.skip_init) hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
     int lin = 0;
     for (int r = 0; r < rounds; ++r) {
         hipLaunchKernelGGL((sweep_kernel<NM, NP2, G, RS>), dim3(nb), dim3(kBlock), 0, s, a, lin);
@@ -1764,7 +1935,7 @@ void launch_mixed_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail
                       hipStream_t s) {
     constexpr int NNT = mix_cols(NM) / 16;
     const int nb = (a.ncells + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
+    if (!a.skip_init) hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
     int lin = 0;
     for (int r = 0; r < rounds; ++r) {
         hipLaunchKernelGGL((sweep_mix_kernel<NM, NP2, NNT>), dim3(mix_blocks), dim3(64 * kMixWaves), 0, s, a, lin);
@@ -1846,6 +2017,22 @@ bool launch_newton_g64_rounds(const NewtonArgs &a, int nm, int np2, int &lin, in
         hipLaunchKernelGGL((logic_g64_kernel<45, 45>), dim3(4 * blocks), dim3(64), 0, s, a, lin);
         lin ^= 1;
     }
+    return true;
+}
+
+bool launch_newton_init(const NewtonArgs &a, int nm, int np2, bool gaunt, hipStream_t s) {
+    const int nb = (a.ncells + kBlock - 1) / kBlock;
+    if (nm == 15 && !gaunt && np2 == 15) { hipLaunchKernelGGL((init_kernel<15, 15>), dim3(nb), dim3(kBlock), 0, s, a); return true; }
+    if (nm == 15 && gaunt && np2 == 45) { hipLaunchKernelGGL((init_kernel<15, 45>), dim3(nb), dim3(kBlock), 0, s, a); return true; }
+    return false;
+}
+
+bool launch_newton_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, hipStream_t s) {
+    const int nb = (a.ncells + kBlock - 1) / kBlock;
+    if (nm == 15 && !gaunt && np2 == 15) hipLaunchKernelGGL((judge_kernel<15, 15>), dim3(nb), dim3(kBlock), 0, s, a);
+    else if (nm == 15 && gaunt && np2 == 45) hipLaunchKernelGGL((judge_kernel<15, 45>), dim3(nb), dim3(kBlock), 0, s, a);
+    else return false;
+    hipLaunchKernelGGL(ctl_clear_kernel, dim3(1), dim3(64), 0, s, a.ctl, 4);
     return true;
 }
 

@@ -148,3 +148,32 @@ def test_learned_warm_start_solves_like_oracle(precision):
     un = np.linalg.norm(ug[rs] - uo[rs], axis=1) / np.linalg.norm(uo[rs], axis=1)
     print("same-r cells: max ||u - u_oracle|| / ||u|| = %.3e" % un.max())
     assert un.max() <= 4 * tol
+
+
+@pytest.mark.gpu
+def test_learned_start_gate_keeps_a_good_previous_alpha():
+    """When the alpha passed in already nearly solves the problem (the time-
+    stepping case), the prologue keeps it and the network is not consulted:
+    every cell picks 'previous' and converges in one or two iterations."""
+    print("case learned gate", flush=True)
+    mnc, ctx = _ctx()
+    o = oracle()
+    tab = o.Tables(4, 16, False)
+    us, al = _smooth_pairs(mnc, tab, 50000, 16)
+    ctx.mlp_init(seed=3)
+    ctx.mlp_train(us, al, steps=200, batch=4096, lr=1e-3, seed=4)
+    rng = np.random.default_rng(17)
+    u, astar, lobe = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, 2000, rng, near_boundary_frac=0.0)
+    a0 = isotropic_start(u)
+    opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, warm_start=mnc.WARM_LEARNED)
+    st, u1, a1, it1, r1, _ = ctx.opt_batch_host(u, a0, opt)
+    assert st == 0
+    ctx.reset_stats()
+    st, u2, a2, it2, r2, _ = ctx.opt_batch_host(u1, a1, opt)       # already solved: the start is kept
+    s = ctx.stats()
+    ctx.close()
+    print("second solve: picks mlp/iso/prev %s, max iterations %d" % (list(s.warm_pick), it2.max()))
+    assert st == 0
+    ok = r1 == 0
+    assert s.warm_pick[2] >= ok.sum()
+    assert it2[ok].max() <= 2
