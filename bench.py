@@ -129,6 +129,9 @@ def progress(msg):
 T0 = time.perf_counter()
 
 
+FORCE_TUNE = {}
+
+
 def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=60, autotune=True):
     """W warm-up steps, the library's autotuner on the state they reach, then
     (from t = 0 again) K steps timed by wall clock with the tuned launch shape
@@ -144,7 +147,10 @@ def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=60, a
     ctx.sync()
     report = ctx.autotune(dts[0], opt, steps=3) if autotune else []
     progress("autotuned: %s" % "; ".join(report[-3:]))
-    tuned = {k: ctx.get_tune(k) for k in ("graph", "rounds", "tail_mode", "tail_blocks", "mix_blocks")}
+    for k, v in FORCE_TUNE.items():
+        ctx.tune(k, v)
+    tuned = {k: ctx.get_tune(k) for k in ("graph", "rounds", "tail_mode", "tail_blocks", "mix_blocks", "flux_tile",
+                                          "mix_cond_log10", "learned_gate_log10")}
     ctx.grid_set(g.moments, g.alpha)        # back to t = 0
     ctx.reset_stats()
     ctx.sync()
@@ -184,6 +190,7 @@ def form_summary(r, f_iter, f_trial):
             "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
             "trials_per_closure": st.trial_sum / st.num_dual_solves, "max_iter": st.max_iter,
             "reg_hist": list(st.hist_reg), "warm_pick_mlp_iso_prev": list(st.warm_pick),
+            "mixed_steps_redone_fp64": st.mixed_deferred,
             "tuned": r["tuned"],
             "kernel_timed_steps": r["kernel_steps"],
             "kernel_ms": {"newton": kms, "flux": r["flux"][1], "other": r["other"][1], "mlp": r["mlp"][1]},
@@ -332,11 +339,16 @@ def main():
     ap.add_argument("--isolated", type=int, default=1_000_000,
                     help="vectors of the isolated closure batch (0 skips it)")
     ap.add_argument("--m8-steps", type=int, default=20, help="steps of the M8 512^2 workload (0 skips it)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="force a launch/tuning setting after the autotuner (repeatable)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's hot path runs on a single MI355X "
               "(multi-GPU sharding is a design note only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    for kv in args.tune:
+        k, v = kv.split("=")
+        FORCE_TUNE[k] = int(v)
     mnc, gi = _load("mnc"), _load("grid_init")
     d = dict(LATTICE)
     n = (d["moment_order"] + 1) * (d["moment_order"] + 2) // 2

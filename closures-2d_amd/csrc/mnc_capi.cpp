@@ -46,7 +46,7 @@ struct mnc_ctx {
     int mix_cond_log10 = 4;          // mixed steps need cond(H) <= 10^this, else FP64
     int mix_blocks = 0;              // 0: one per CU
     int flux_tile = 1;               // 0: ansatz to HBM + flux; 1: fused, 8x8 tiles; 2: fused, 16x16
-    int learned_gate_log10 = -4;     // the network is consulted when ||g(best start)|| > 10^this ||u||
+    int learned_gate_log10 = -2;     // the network is consulted when ||g(best start)|| > 10^this ||u||
     // grid
     bool has_grid = false;
     mnc_grid_desc grid{};
@@ -144,7 +144,7 @@ int ensure_state(mnc_ctx *ctx, size_t ncells) {
     const size_t cap = std::max<size_t>(ncells, 64);
     int st;
     if ((st = dalloc(ctx, &ctx->d_std, size_t(df) * cap)) || (st = dalloc(ctx, &ctx->d_sti, size_t(ifl) * cap)) ||
-        (st = dalloc(ctx, &ctx->d_lists, 3 * cap)))
+        (st = dalloc(ctx, &ctx->d_lists, 4 * cap)))
         return st;
     ctx->state_cap = cap;
     return MNC_OK;
@@ -773,6 +773,7 @@ int mnc_get_stats(mnc_ctx *ctx, mnc_stats *st) {
     for (int r = 0; r < 5; ++r) st->hist_reg[r] = h[3 + r];
     st->trial_sum = h[8];
     for (int r = 0; r < 3; ++r) st->warm_pick[r] = h[9 + r];
+    st->mixed_deferred = h[12];
     return MNC_OK;
 }
 

@@ -43,7 +43,8 @@ struct NewtonArgs {
     double cond_h_max;
     int warm;             // MNC_WARM_*
     long long *stats;     // [0] max iter, [1] iter sum, [2] solves, [3..7] hist, [8] trials,
-                          // [9..11] warm starts taken: learned, isotropic, previous
+                          // [9..11] warm starts taken: learned, isotropic, previous,
+                          // [12] mixed-precision steps redone in FP64
     int ls_mode;          // MNC_LS_*
     const double *alpha_guess; // [ncells][nm] scaled-problem guess (learned warm start) or nullptr
     double gate;          // learned warm start: the network is consulted when the best of isotropic
@@ -52,8 +53,8 @@ struct NewtonArgs {
     // per-cell solver state and the active lists
     double *st_d;         // [dfields][cap]
     int *st_i;            // [ifields][cap]
-    int *lists;           // [3][cap]: two ping-pong active lists, then the deferred list
-    int *ctl;             // [0],[1] list counts, [2] tail queue, [3] error bits, [4] deferred count
+    int *lists;           // [4][cap]: the lockstep pair 0/1, the FP64 pair 2/3 of a mixed solve
+    int *ctl;             // [0],[1] counts of lists 0/1, [2] tail queue, [3] error bits, [4],[5] lists 2/3
     size_t cap;
     // mixed precision (MixTables, device copies)
     const double *mz;     // [nt][4][64]  alpha.m MFMA operand
@@ -71,7 +72,7 @@ struct NewtonArgs {
 
 // Number of control words of a batched solve.
 constexpr int kCtlWords = 8;
-constexpr int kStatWords = 12;
+constexpr int kStatWords = 13;
 
 // Result layout of v_mfma_f64_16x16x4f64 on gfx950 (measured with a one-wave
 // probe): result element r of lane l is row 4r + l/16, column l%16.  (The

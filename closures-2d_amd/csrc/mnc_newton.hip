@@ -316,11 +316,11 @@ struct Store {
 // Per-lane statistics, reduced per block and flushed with one atomic each.
 struct LaneStats {
     int max_iter = 0, err_bits = 0;
-    long long sum = 0, n = 0, trials = 0, hist[5] = {0, 0, 0, 0, 0}, warm[3] = {0, 0, 0};
+    long long sum = 0, n = 0, trials = 0, hist[5] = {0, 0, 0, 0, 0}, warm[3] = {0, 0, 0}, deferred = 0;
 };
 
 __device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
-    __shared__ long long red[kBlock / 64][13];
+    __shared__ long long red[kBlock / 64][14];
     for (int off = 32; off > 0; off >>= 1) {
         s.max_iter = max(s.max_iter, __shfl_xor(s.max_iter, off));
         s.err_bits |= __shfl_xor(s.err_bits, off);
@@ -331,28 +331,31 @@ __device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
         for (int r = 0; r < 5; ++r) s.hist[r] += __shfl_xor(s.hist[r], off);
 #pragma unroll
         for (int r = 0; r < 3; ++r) s.warm[r] += __shfl_xor(s.warm[r], off);
+        s.deferred += __shfl_xor(s.deferred, off);
     }
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) {
         red[w][0] = s.max_iter; red[w][1] = s.err_bits; red[w][2] = s.sum; red[w][3] = s.n; red[w][4] = s.trials;
         for (int r = 0; r < 5; ++r) red[w][5 + r] = s.hist[r];
         for (int r = 0; r < 3; ++r) red[w][10 + r] = s.warm[r];
+        red[w][13] = s.deferred;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        long long v[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        long long v[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int ww = 0; ww < kBlock / 64; ++ww) {
             v[0] = max(v[0], red[ww][0]);
             v[1] |= red[ww][1];
-            for (int k = 2; k < 13; ++k) v[k] += red[ww][k];
+            for (int k = 2; k < 14; ++k) v[k] += red[ww][k];
         }
-        if (v[3] || v[4] || v[1] || v[10] || v[11] || v[12]) {
+        if (v[3] || v[4] || v[1] || v[10] || v[11] || v[12] || v[13]) {
             atomicMax((unsigned long long *)&A.stats[0], (unsigned long long)v[0]);
             atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)v[2]);
             atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)v[3]);
             for (int r = 0; r < 5; ++r) atomicAdd((unsigned long long *)&A.stats[3 + r], (unsigned long long)v[5 + r]);
             atomicAdd((unsigned long long *)&A.stats[8], (unsigned long long)v[4]);
             for (int r = 0; r < 3; ++r) atomicAdd((unsigned long long *)&A.stats[9 + r], (unsigned long long)v[10 + r]);
+            atomicAdd((unsigned long long *)&A.stats[12], (unsigned long long)v[13]);
             if (v[1]) atomicOr(&A.ctl[3], int(v[1]));
         }
     }
@@ -385,6 +388,12 @@ __device__ double grad_norm2(const NewtonArgs &A, const double (&al)[NM], const 
     }
     return s;
 }
+
+// Control word holding the count of list l: lists 0 and 1 (the lockstep
+// ping-pong pair) use words 0 and 1, lists 2 and 3 (the FP64 pair of a
+// mixed-precision solve; list 2 also carries the learned prologue's cells)
+// use words 4 and 5.  Word 2 is the tail's queue head, word 3 the error bits.
+__device__ __forceinline__ int cw(int l) { return l < 2 ? l : l + 2; }
 
 // Appends the calling lanes with `want` set to list `dst`, whose count is
 // control word `cw` (wave-aggregated).
@@ -841,9 +850,9 @@ __global__ void ctl_clear_kernel(int *ctl, int w) {
 // the following logic kernel fills.
 template <int NM, int NP2, bool GAUNT, int RS>
 __global__ __launch_bounds__(kBlock) void sweep_kernel(NewtonArgs A, int lin) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl[lin ^ 1] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl[cw(lin ^ 1)] = 0;
     const int slot = blockIdx.x * kBlock + threadIdx.x;
-    if (slot >= A.ctl[lin]) return;
+    if (slot >= A.ctl[cw(lin)]) return;
     const int c = A.lists[size_t(lin) * A.cap + slot];
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
     cell_sweep<NM, NP2, GAUNT, RS>(A, S, c);
@@ -851,30 +860,31 @@ __global__ __launch_bounds__(kBlock) void sweep_kernel(NewtonArgs A, int lin) {
 
 // Newton / Armijo logic for every cell on list `lin`; cells needing another
 // sweep are compacted onto list lin^1.
-// MIX: after a mixed-precision sweep; deferred cells go to list 2 (count in
-// control word 4) for the FP64 tail.
+// MIX: after a mixed-precision sweep; deferred cells go to list defer_to
+// (the FP64 pair's next list).
 template <int NM, int NP2, bool GAUNT, bool MIX = false>
-__global__ __launch_bounds__(kBlock) void logic_kernel(NewtonArgs A, int lin) {
+__global__ __launch_bounds__(kBlock) void logic_kernel(NewtonArgs A, int lin, int defer_to) {
     const int slot = blockIdx.x * kBlock + threadIdx.x;
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
     __shared__ float gstage[MIX ? mix_cols(NM) : 1][kBlock];
     LaneStats st;
     int res = LOGIC_DONE;
     int c = -1;
-    if (slot < A.ctl[lin]) {
+    if (slot < A.ctl[cw(lin)]) {
         c = A.lists[size_t(lin) * A.cap + slot];
         res = cell_logic<NM, NP2, GAUNT, MIX>(A, S, c, st, &gstage[0][threadIdx.x]);
+        if (res == LOGIC_DEFER) ++st.deferred;
     }
-    list_append(A, lin ^ 1, lin ^ 1, res == LOGIC_CONTINUE, c);
-    if constexpr (MIX) list_append(A, 2, 4, res == LOGIC_DEFER, c);
+    list_append(A, lin ^ 1, cw(lin ^ 1), res == LOGIC_CONTINUE, c);
+    if constexpr (MIX) list_append(A, defer_to, cw(defer_to), res == LOGIC_DEFER, c);
     flush_stats(A, st);
 }
 
-// Entry s of the tail's work: the cells left on list `lin`, then the cells a
-// mixed-precision round deferred to FP64.  -1 past the end.
-__device__ __forceinline__ int tail_cell(const NewtonArgs &A, int lin, int count, int dcount, int s) {
+// Entry s of the tail's work: the cells left on list `lin`, then those left
+// on list `lin2` (the FP64 list of a mixed-precision solve).  -1 past the end.
+__device__ __forceinline__ int tail_cell(const NewtonArgs &A, int lin, int count, int lin2, int count2, int s) {
     if (s < count) return A.lists[size_t(lin) * A.cap + s];
-    if (s < count + dcount) return A.lists[2 * A.cap + (s - count)];
+    if (s < count + count2) return A.lists[size_t(lin2) * A.cap + (s - count)];
     return -1;
 }
 
@@ -882,10 +892,10 @@ __device__ __forceinline__ int tail_cell(const NewtonArgs &A, int lin, int count
 // and iterate each to completion, so a few cells needing tens of iterations
 // near the realizability boundary cost one launch, not one per iteration.
 template <int NM, int NP2, bool GAUNT, int RS>
-__global__ __launch_bounds__(kBlock) void tail_kernel(NewtonArgs A, int lin) {
+__global__ __launch_bounds__(kBlock) void tail_kernel(NewtonArgs A, int lin, int lin2) {
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
     const int lane = threadIdx.x & 63;
-    const int count = A.ctl[lin], dcount = A.ctl[4];
+    const int count = A.ctl[cw(lin)], dcount = A.ctl[cw(lin2)];
     LaneStats st;
     int c = -1;          // -1 wants work, -2 finished
     for (;;) {
@@ -898,7 +908,7 @@ __global__ __launch_bounds__(kBlock) void tail_kernel(NewtonArgs A, int lin) {
             base = __shfl(base, leader);
             if (need) {
                 const int s = base + __popcll(m & lane_mask_lt());
-                c = tail_cell(A, lin, count, dcount, s);
+                c = tail_cell(A, lin, count, lin2, dcount, s);
                 if (c < 0) c = -2;
             }
         }
@@ -917,19 +927,19 @@ __global__ __launch_bounds__(kBlock) void tail_kernel(NewtonArgs A, int lin) {
 // iteration instead of a single lane's full sweep.  All lanes run the per-cell
 // logic on identical data (uniform control flow; stores write equal values).
 template <int NM, int NP2, bool GAUNT, int RS>
-__global__ __launch_bounds__(kBlock) void tail_wave_kernel(NewtonArgs A, int lin) {
+__global__ __launch_bounds__(kBlock) void tail_wave_kernel(NewtonArgs A, int lin, int lin2) {
     using St = Store<NM, NP2>;
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
     constexpr int NA = GAUNT ? NP2 : Tri<NM>::at(RS, 0);
     constexpr int NB = (GAUNT || RS >= NM) ? 1 : Tri<NM>::size - Tri<NM>::at(RS, 0);
     const int lane = threadIdx.x & 63;
-    const int count = A.ctl[lin], dcount = A.ctl[4];
+    const int count = A.ctl[cw(lin)], dcount = A.ctl[cw(lin2)];
     LaneStats st;
     for (;;) {
         int s = 0;
         if (lane == 0) s = atomicAdd(&A.ctl[2], 1);
         s = __shfl(s, 0);
-        const int c = tail_cell(A, lin, count, dcount, s);
+        const int c = tail_cell(A, lin, count, lin2, dcount, s);
         if (c < 0) break;
         for (;;) {
             double al[NM];
@@ -1207,14 +1217,14 @@ __device__ __forceinline__ double coop_rcond(double (*L)[NM], double *x, double 
 }
 
 template <int NM, int NP2, bool GAUNT, int RS>
-__global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
+__global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, int lin2) {
     using St = Store<NM, NP2>;
     using CS = CoopShared<NM, NP2, GAUNT, RS>;
     constexpr int NA = CS::NA, NB = CS::NB;
     __shared__ CS sh;
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
     const int lane = threadIdx.x & 63;
-    const int count = A.ctl[lin], dcount = A.ctl[4];
+    const int count = A.ctl[cw(lin)], dcount = A.ctl[cw(lin2)];
     const double p0 = A.p[0];
     LaneStats st;
     for (;;) {
@@ -1222,7 +1232,7 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin) {
         if (lane == 0) s = atomicAdd(&A.ctl[2], 1);
         s = __shfl(s, 0);
         // wave-uniform: state loads become scalar loads
-        const int c = __builtin_amdgcn_readfirstlane(tail_cell(A, lin, count, dcount, s));
+        const int c = __builtin_amdgcn_readfirstlane(tail_cell(A, lin, count, lin2, dcount, s));
         if (c < 0) break;
         for (;;) {
             double al[NM];
@@ -1908,13 +1918,13 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
 }
 
 template <int NM, int NP2, bool G, int RS>
-void launch_tail(const NewtonArgs &a, int lin, int tail_blocks, int tail_mode, hipStream_t s) {
+void launch_tail(const NewtonArgs &a, int lin, int lin2, int tail_blocks, int tail_mode, hipStream_t s) {
     if (tail_mode == 0)
-        hipLaunchKernelGGL((tail_coop_kernel<NM, NP2, G, RS>), dim3(4 * tail_blocks), dim3(64), 0, s, a, lin);
+        hipLaunchKernelGGL((tail_coop_kernel<NM, NP2, G, RS>), dim3(4 * tail_blocks), dim3(64), 0, s, a, lin, lin2);
     else if (tail_mode == 2)
-        hipLaunchKernelGGL((tail_wave_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin);
+        hipLaunchKernelGGL((tail_wave_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin, lin2);
     else
-        hipLaunchKernelGGL((tail_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin);
+        hipLaunchKernelGGL((tail_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin, lin2);
 }
 
 template <int NM, int NP2, bool G, int RS>
@@ -1924,10 +1934,10 @@ void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode,
     int lin = 0;
     for (int r = 0; r < rounds; ++r) {
         hipLaunchKernelGGL((sweep_kernel<NM, NP2, G, RS>), dim3(nb), dim3(kBlock), 0, s, a, lin);
-        hipLaunchKernelGGL((logic_kernel<NM, NP2, G>), dim3(nb), dim3(kBlock), 0, s, a, lin);
+        hipLaunchKernelGGL((logic_kernel<NM, NP2, G>), dim3(nb), dim3(kBlock), 0, s, a, lin, 2);
         lin ^= 1;
     }
-    launch_tail<NM, NP2, G, RS>(a, lin, tail_blocks, tail_mode, s);
+    launch_tail<NM, NP2, G, RS>(a, lin, 2, tail_blocks, tail_mode, s);   // list 2 is empty here
 }
 
 template <int NM, int NP2, bool G, int RS>
@@ -1936,13 +1946,19 @@ void launch_mixed_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail
     constexpr int NNT = mix_cols(NM) / 16;
     const int nb = (a.ncells + kBlock - 1) / kBlock;
     if (!a.skip_init) hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
-    int lin = 0;
+    // Mixed-precision cells on lists 0/1; a cell whose step must be redone in
+    // FP64 moves to the FP64 pair (lists 2/3), which the same rounds sweep with
+    // the FP64 lane kernels; what is left after the rounds goes to the tail.
+    int lin = 0, flin = 2;
     for (int r = 0; r < rounds; ++r) {
         hipLaunchKernelGGL((sweep_mix_kernel<NM, NP2, NNT>), dim3(mix_blocks), dim3(64 * kMixWaves), 0, s, a, lin);
-        hipLaunchKernelGGL((logic_kernel<NM, NP2, G, true>), dim3(nb), dim3(kBlock), 0, s, a, lin);
+        hipLaunchKernelGGL((sweep_kernel<NM, NP2, G, RS>), dim3(nb), dim3(kBlock), 0, s, a, flin);
+        hipLaunchKernelGGL((logic_kernel<NM, NP2, G, true>), dim3(nb), dim3(kBlock), 0, s, a, lin, flin ^ 1);
+        hipLaunchKernelGGL((logic_kernel<NM, NP2, G>), dim3(nb), dim3(kBlock), 0, s, a, flin, 2);
         lin ^= 1;
+        flin ^= 1;
     }
-    launch_tail<NM, NP2, G, RS>(a, lin, tail_blocks, tail_mode, s);
+    launch_tail<NM, NP2, G, RS>(a, lin, flin, tail_blocks, tail_mode, s);
 }
 
 }  // namespace

@@ -47,7 +47,8 @@ class GridDesc(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("num_dual_solves", C.c_longlong),
                 ("iter_sum", C.c_longlong), ("hist_reg", C.c_longlong * 5),
-                ("trial_sum", C.c_longlong), ("warm_pick", C.c_longlong * 3)]
+                ("trial_sum", C.c_longlong), ("warm_pick", C.c_longlong * 3),
+                ("mixed_deferred", C.c_longlong)]
 
 
 class MncError(RuntimeError):
