@@ -31,6 +31,7 @@ struct mnc_ctx {
     mnc::MixTables mix;
     bool has_mix = false;
     double *d_mz = nullptr, *d_mg = nullptr, *d_mgv = nullptr;
+    float *d_mpv = nullptr;          // Gaunt map in compile-time pattern order (mixed logic)
     unsigned short *d_mh = nullptr;
     int *d_mgptr = nullptr, *d_mgk = nullptr;
     // FP64 matrix-core operand tables of the high orders (M8)
@@ -43,7 +44,8 @@ struct mnc_ctx {
     int rounds = 6;
     int tail_blocks = 0;             // 0: 2 per CU
     int tail_mode = 0;               // 0: wave per cell, 1: lane per cell, 2: wave sweep + serial logic
-    int mix_cond_log10 = 4;          // mixed steps need cond(H) <= 10^this, else FP64
+    int mix_cond_log10 = 5;          // mixed steps need cond(H) <= 10^this, else FP64
+    int mix_defer = 0;               // 0: redone steps wait for the FP64 tail; 1: FP64 lockstep rounds
     int mix_blocks = 0;              // 0: one per CU
     int flux_tile = 1;               // 0: ansatz to HBM + flux; 1: fused, 8x8 tiles; 2: fused, 16x16
     int learned_gate_log10 = -2;     // the network is consulted when ||g(best start)|| > 10^this ||u||
@@ -284,6 +286,7 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     a.mgptr = ctx->d_mgptr;
     a.mgk = ctx->d_mgk;
     a.mgv = ctx->d_mgv;
+    a.mpv = ctx->d_mpv;
     a.cmix = std::pow(10.0, ctx->mix_cond_log10);
     a.gate = std::pow(10.0, ctx->learned_gate_log10);
     a.gz = ctx->d_gz;
@@ -333,7 +336,7 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
         } else if (mixed) {
             const int mb = ctx->mix_blocks > 0 ? ctx->mix_blocks : ctx->num_cu;
             if (!mnc::launch_newton_mixed(a, t.nm, t.np2, t.use_gaunt != 0, ctx->rounds, tail, ctx->tail_mode, mb,
-                                          ctx->stream))
+                                          ctx->mix_defer, ctx->stream))
                 return ctx_fail(ctx, MNC_ERR_ARG, "no mixed-precision Newton kernel instance for this order");
         } else if (!mnc::launch_newton_fp64(a, t.nm, t.np2, t.use_gaunt != 0, ctx->rounds, tail, ctx->tail_mode,
                                             ctx->stream)) {
@@ -454,7 +457,8 @@ int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
     if (mnc::build_mix_tables(setup->moment_order, setup->quad_order, ctx->mix) == MNC_OK &&
         ctx->mix.nnt == mnc::mix_tiles(t.nm)) {
         if (up(&ctx->d_mz, ctx->mix.z) || up(&ctx->d_mg, ctx->mix.g) || up(&ctx->d_mh, ctx->mix.h) ||
-            up(&ctx->d_mgptr, ctx->mix.gptr) || up(&ctx->d_mgk, ctx->mix.gk) || up(&ctx->d_mgv, ctx->mix.gv)) {
+            up(&ctx->d_mgptr, ctx->mix.gptr) || up(&ctx->d_mgk, ctx->mix.gk) || up(&ctx->d_mgv, ctx->mix.gv) ||
+            (!ctx->mix.pv.empty() && up(&ctx->d_mpv, ctx->mix.pv))) {
             mnc_destroy(ctx);
             return MNC_ERR_HIP;
         }
@@ -472,7 +476,7 @@ void mnc_destroy(mnc_ctx *ctx) {
     for (auto &g : ctx->graphs) drop_graph(g);
     for (auto &ev : ctx->events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
     void *ptrs[] = {ctx->d_w, ctx->d_p, ctx->d_xi, ctx->d_eta, ctx->d_gv, ctx->d_gptr, ctx->d_gk,
-                    ctx->d_mz, ctx->d_mg, ctx->d_mh, ctx->d_mgptr, ctx->d_mgk, ctx->d_mgv, ctx->d_gz, ctx->d_gg, ctx->d_gh,
+                    ctx->d_mz, ctx->d_mg, ctx->d_mh, ctx->d_mgptr, ctx->d_mgk, ctx->d_mgv, ctx->d_mpv, ctx->d_gz, ctx->d_gg, ctx->d_gh,
                     ctx->d_mlp, ctx->d_mlp_scratch, ctx->d_mlp_loss, ctx->d_mlp_io, ctx->d_us, ctx->d_guess,
                     ctx->d_ctl, ctx->d_stats, ctx->d_std, ctx->d_sti, ctx->d_lists, ctx->d_mom, ctx->d_alpha, ctx->d_old,
                     ctx->d_st, ctx->d_ss, ctx->d_flux, ctx->d_ansatz, ctx->d_bu, ctx->d_ba, ctx->d_bo};
@@ -850,6 +854,11 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         ctx->mix_cond_log10 = value;
         return MNC_OK;
     }
+    if (std::strcmp(key, "mix_defer") == 0) {
+        if (value < 0 || value > 1) return ctx_fail(ctx, MNC_ERR_ARG, "mix_defer is 0 or 1");
+        ctx->mix_defer = value;
+        return MNC_OK;
+    }
     if (std::strcmp(key, "mix_blocks") == 0) {
         if (value < 1 || value > 65536) return ctx_fail(ctx, MNC_ERR_ARG, "mix_blocks out of range");
         ctx->mix_blocks = value;
@@ -873,6 +882,7 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
     if (std::strcmp(key, "rounds") == 0) { *value = ctx->rounds; return MNC_OK; }
     if (std::strcmp(key, "tail_mode") == 0) { *value = ctx->tail_mode; return MNC_OK; }
     if (std::strcmp(key, "mix_cond_log10") == 0) { *value = ctx->mix_cond_log10; return MNC_OK; }
+    if (std::strcmp(key, "mix_defer") == 0) { *value = ctx->mix_defer; return MNC_OK; }
     if (std::strcmp(key, "mix_blocks") == 0) {
         *value = ctx->mix_blocks > 0 ? ctx->mix_blocks : ctx->num_cu;
         return MNC_OK;
@@ -939,7 +949,11 @@ int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, cha
         {"tail_mode", {0, 1, 2}, &ctx->tail_mode},
         {"tail_blocks", {cu, 2 * cu, 4 * cu}, &ctx->tail_blocks},
     };
-    if (opt->precision == MNC_PREC_MIXED) knobs.push_back({"mix_blocks", {cu / 2, cu, 2 * cu}, &ctx->mix_blocks});
+    if (opt->precision == MNC_PREC_MIXED) {
+        knobs.push_back({"mix_cond_log10", {4, 5, 6}, &ctx->mix_cond_log10});
+        knobs.push_back({"mix_defer", {0, 1}, &ctx->mix_defer});
+        knobs.push_back({"mix_blocks", {cu / 2, cu, 2 * cu}, &ctx->mix_blocks});
+    }
     double best = 0.0;
     if ((st = trial(&best)) == MNC_OK) {
         char line[160];
@@ -949,7 +963,7 @@ int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, cha
             const int keep = *kb.slot;
             int arg = keep;
             for (int v : kb.vals) {
-                if (v == keep || (v < 1 && std::strcmp(kb.key, "graph") != 0)) continue;
+                if (v == keep || (v < 1 && std::strstr(kb.key, "_blocks") != nullptr)) continue;
                 *kb.slot = v;
                 ++ctx->epoch;
                 double ms = 0.0;

@@ -63,6 +63,7 @@ struct NewtonArgs {
     int mnt;              // quadrature tiles of 16 points (even)
     const int *mgptr, *mgk;   // Gaunt map of the order-2N table
     const double *mgv;
+    const float *mpv;         // the same map in compile-time pattern order, or nullptr
     double cmix;          // condition bound above which a mixed-precision step is redone in FP64
     // FP64 matrix-core sweep of the high orders (G64Tables, device copies)
     const double *gz, *gg, *gh;
@@ -89,7 +90,13 @@ struct MixTables {
     std::vector<unsigned short> h;
     std::vector<int> gptr, gk;
     std::vector<double> gv;
+    std::vector<float> pv;   // the Gaunt map in compile-time pattern order (empty: pattern does not cover it)
 };
+// The fitted Gaunt map (CSR over nm*nm pairs) in the order of the compile-time
+// pattern of moment order n (mnc_gaunt_pattern.h), 0 where the fit has no
+// entry.  False when the map has an entry outside the pattern.
+bool gaunt_pattern_values(int n, const std::vector<int> &gptr, const std::vector<int> &gk,
+                          const std::vector<double> &gv, std::vector<float> &out);
 // Returns MNC_OK, or MNC_ERR_ARG when the order/quadrature has no mixed kernel.
 int build_mix_tables(int moment_order, int quad_order, MixTables &m);
 
@@ -119,7 +126,7 @@ bool launch_newton_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, hipSt
 // The same with the lockstep rounds in mixed precision (MFMA sweeps); the tail
 // is FP64.  mix_blocks: persistent blocks of the mixed sweep.
 bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
-                         int tail_mode, int mix_blocks, hipStream_t s);
+                         int tail_mode, int mix_blocks, int defer_rounds, hipStream_t s);
 // The FP64 matrix-core path (M8: n = 45).  rounds: lockstep rounds per launch
 // batch; the caller repeats batches until control word `lin` reads zero.
 // Returns false for an order without an instance.

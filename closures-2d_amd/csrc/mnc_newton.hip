@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "mnc_internal.h"
+#include "mnc_gaunt_pattern.h"
 
 namespace mnc {
 namespace {
@@ -203,6 +204,23 @@ __device__ __forceinline__ double wave_sum(double v) {
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// True when the condition test `1/rcond > limit` (opt.cpp:231-244) is false
+// whatever the estimate: Higham's estimate never exceeds ||A^-1||_1 <=
+// sqrt(n) / lambda_min, and Gershgorin gives lambda_min >= gersh =
+// min_i (2 a_ii - sum_j |a_ij|).  With a factor 2 of margin for rounding, the
+// estimate (the condition number's main cost after the factorisation) is
+// skipped when anorm sqrt(n) / gersh cannot reach the limit; the decision is
+// the reference's either way.
+__device__ __forceinline__ bool cond_proven_below(double anorm, double gersh, int n, double limit) {
+    return gersh > 0.0 && 2.0 * anorm * sqrt(double(n)) <= limit * gersh;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
     return v;
 }
 
@@ -583,15 +601,29 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
             constexpr int NMM = mix_cols(NM);
 #pragma unroll
             for (int k = 0; k < NMM; ++k) gstage[k * kBlock] = float(S.d(St::F_ACC + k, c));
+            if (A.mpv != nullptr) {
+                // the map's pattern fixed at compile time: every index folds,
+                // the coefficients come in as batched scalar loads; the same
+                // products in the same order as the CSR walk below
+                constexpr GauntPattern<order_of(NM)> P = gaunt_pattern<order_of(NM)>();
 #pragma unroll
-            for (int i = 0; i < NM; ++i) {
+                for (int k = 0; k < Tri<NM>::size; ++k) h[k] = 0.0f;
 #pragma unroll
-                for (int j = i; j < NM; ++j) {
-                    float hij = 0.0f;
-                    const int pr = i * NM + j;
-                    for (int cc = A.mgptr[pr]; cc < A.mgptr[pr + 1]; ++cc)
-                        hij = fmaf(float(A.mgv[cc]), gstage[A.mgk[cc] * kBlock], hij);
-                    h[Tri<NM>::at(j, i)] = HT(hij);
+                for (int e = 0; e < P.count; ++e) {
+                    const int hk = Tri<NM>::at(P.j[e], P.i[e]);
+                    h[hk] = fmaf(A.mpv[e], gstage[P.k[e] * kBlock], h[hk]);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < NM; ++i) {
+#pragma unroll
+                    for (int j = i; j < NM; ++j) {
+                        float hij = 0.0f;
+                        const int pr = i * NM + j;
+                        for (int cc = A.mgptr[pr]; cc < A.mgptr[pr + 1]; ++cc)
+                            hij = fmaf(float(A.mgv[cc]), gstage[A.mgk[cc] * kBlock], hij);
+                        h[Tri<NM>::at(j, i)] = HT(hij);
+                    }
                 }
             }
         } else if constexpr (GAUNT) {                              // fobj.cpp:99-113
@@ -629,19 +661,20 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
             S.d(St::F_FSCALE, c) = fscale;
             err = sqrt(gg);                                        // dnrm2 (:286)
         }
-        double anorm = 0.0;
+        double anorm = 0.0, gersh = DBL_MAX;
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
             double s = 0.0;
 #pragma unroll
             for (int j = 0; j < NM; ++j) s += fabs(h[i >= j ? Tri<NM>::at(i, j) : Tri<NM>::at(j, i)]);
             anorm = s > anorm ? s : anorm;
+            gersh = fmin(gersh, 2.0 * double(h[Tri<NM>::at(i, i)]) - s);
         }
         // Phase 2: factor, condition estimate, direction (dpotrf, dpocon, dpotrs).
+        const double climit = MIX ? fmin(A.cmix, A.cond_h_max) : A.cond_h_max;
         const bool chol_ok = cholesky<NM, HT>(h);
         double condest = 0.0;
-        if (chol_ok) condest = 1.0 / rcond<NM, HT>(h, anorm);
-        const double climit = MIX ? fmin(A.cmix, A.cond_h_max) : A.cond_h_max;
+        if (chol_ok && !cond_proven_below(anorm, gersh, NM, climit)) condest = 1.0 / rcond<NM, HT>(h, anorm);
         if (iter >= maxit || !chol_ok || t == 0.0 || condest > climit) {
             if constexpr (MIX) {
                 --iter; --total;
@@ -1404,10 +1437,14 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                     fcur = fcur - al[i] * ui;
                     fscale += fabs(al[i] * ui);
                 }
-                double rs = 0.0;                       // row sums (utils.cpp:36-52)
+                double rs = 0.0, dg = 0.0;             // row sums (utils.cpp:36-52), diagonal
 #pragma unroll
-                for (int jj = 0; jj < NM; ++jj) rs += fabs(row[jj]);
+                for (int jj = 0; jj < NM; ++jj) {
+                    rs += fabs(row[jj]);
+                    dg = (jj == lane) ? row[jj] : dg;
+                }
                 const double anorm = wave_max(lane < NM ? rs : 0.0);
+                const double gersh = wave_min(lane < NM ? 2.0 * dg - rs : DBL_MAX);
                 err = sqrt(gg);
                 const bool chol_ok = reg_cholesky<NM>(row, lane);
                 // the columns of the factor (lane i: L[k][i], k > i) for the back substitution
@@ -1421,7 +1458,8 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                 for (int k = 0; k < NM; ++k) col[k] = (lane < NM && k > lane) ? sh.L[k][lane] : 0.0;
                 __builtin_amdgcn_wave_barrier();
                 double condest = 0.0;
-                if (chol_ok) condest = 1.0 / reg_rcond<NM>(row, col, anorm, lane);
+                if (chol_ok && !cond_proven_below(anorm, gersh, NM, A.cond_h_max))
+                    condest = 1.0 / reg_rcond<NM>(row, col, anorm, lane);
                 if (iter >= maxit || !chol_ok || t == 0.0 || condest > A.cond_h_max) {
                     do_reg = true;
                 } else {
@@ -1791,7 +1829,7 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
         }
         if (newton) {
             ++iter; ++total;                                        // opt.cpp:212-213
-            double rs = 0.0, gi = 0.0;
+            double rs = 0.0, gi = 0.0, dg = 0.0;
             if (lane < NM) {                                        // row `lane` of H (Gaunt map)
                 for (int jj = 0; jj < NM; ++jj) {
                     const int pr = (lane < jj) ? lane * NM + jj : jj * NM + lane;
@@ -1800,6 +1838,7 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
                         hij = fma(A.mgv[cc], S.d(St::F_ACC + A.mgk[cc], c), hij);
                     sh.L[lane][jj] = hij;
                     rs += fabs(hij);
+                    if (jj == lane) dg = hij;
                 }
                 gi = S.d(St::F_GS + lane, c) - ul;
                 sh.gvec[lane] = gi;
@@ -1810,10 +1849,12 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
             const double fscale = fabs(S.d(St::F_SF, c)) + aus;
             err = sqrt(wave_sum(gi * gi));                          // dnrm2 (:286)
             const double anorm = wave_max(rs);                     // utils.cpp:36-52
+            const double gersh = wave_min(lane < NM ? 2.0 * dg - rs : DBL_MAX);
             __builtin_amdgcn_wave_barrier();
             const bool chol_ok = coop_cholesky<NM>(sh.L, lane);
             double condest = 0.0;
-            if (chol_ok) condest = 1.0 / coop_rcond<NM>(sh.L, sh.x, anorm, lane);
+            if (chol_ok && !cond_proven_below(anorm, gersh, NM, A.cond_h_max))
+                condest = 1.0 / coop_rcond<NM>(sh.L, sh.x, anorm, lane);
             if (iter >= maxit || !chol_ok || t == 0.0 || condest > A.cond_h_max) {
                 do_reg = true;                                      // opt.cpp:244
             } else {
@@ -1942,21 +1983,26 @@ void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode,
 
 template <int NM, int NP2, bool G, int RS>
 void launch_mixed_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode, int mix_blocks,
-                      hipStream_t s) {
+                      int defer_rounds, hipStream_t s) {
     constexpr int NNT = mix_cols(NM) / 16;
     const int nb = (a.ncells + kBlock - 1) / kBlock;
     if (!a.skip_init) hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
-    // Mixed-precision cells on lists 0/1; a cell whose step must be redone in
-    // FP64 moves to the FP64 pair (lists 2/3), which the same rounds sweep with
-    // the FP64 lane kernels; what is left after the rounds goes to the tail.
+    // Mixed-precision cells on lists 0/1.  A cell whose step must be redone in
+    // FP64 either waits on list 2 for the FP64 tail (defer_rounds 0) or moves
+    // to the FP64 pair (lists 2/3), which the same rounds sweep with the FP64
+    // lane kernels (defer_rounds 1); what is left after the rounds goes to the tail.
     int lin = 0, flin = 2;
     for (int r = 0; r < rounds; ++r) {
         hipLaunchKernelGGL((sweep_mix_kernel<NM, NP2, NNT>), dim3(mix_blocks), dim3(64 * kMixWaves), 0, s, a, lin);
-        hipLaunchKernelGGL((sweep_kernel<NM, NP2, G, RS>), dim3(nb), dim3(kBlock), 0, s, a, flin);
-        hipLaunchKernelGGL((logic_kernel<NM, NP2, G, true>), dim3(nb), dim3(kBlock), 0, s, a, lin, flin ^ 1);
-        hipLaunchKernelGGL((logic_kernel<NM, NP2, G>), dim3(nb), dim3(kBlock), 0, s, a, flin, 2);
+        if (defer_rounds) {
+            hipLaunchKernelGGL((sweep_kernel<NM, NP2, G, RS>), dim3(nb), dim3(kBlock), 0, s, a, flin);
+            hipLaunchKernelGGL((logic_kernel<NM, NP2, G, true>), dim3(nb), dim3(kBlock), 0, s, a, lin, flin ^ 1);
+            hipLaunchKernelGGL((logic_kernel<NM, NP2, G>), dim3(nb), dim3(kBlock), 0, s, a, flin, 2);
+            flin ^= 1;
+        } else {
+            hipLaunchKernelGGL((logic_kernel<NM, NP2, G, true>), dim3(nb), dim3(kBlock), 0, s, a, lin, 2);
+        }
         lin ^= 1;
-        flin ^= 1;
     }
     launch_tail<NM, NP2, G, RS>(a, lin, flin, tail_blocks, tail_mode, s);
 }
@@ -1996,12 +2042,12 @@ bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int ro
 }
 
 bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
-                         int tail_mode, int mix_blocks, hipStream_t s) {
+                         int tail_mode, int mix_blocks, int defer_rounds, hipStream_t s) {
 #define MNC_CASE(NMV, NP2D, NP2G, RSPLIT)                                                                \
     if (nm == NMV) {                                                                                      \
         if (a.mnt < 2 || a.mnt > kMixMaxTiles || (a.mnt & 1)) return false;                              \
-        if (gaunt && np2 == NP2G) { launch_mixed_all<NMV, NP2G, true, NMV>(a, rounds, tail_blocks, tail_mode, mix_blocks, s); return true; } \
-        if (!gaunt && np2 == NP2D) { launch_mixed_all<NMV, NP2D, false, RSPLIT>(a, rounds, tail_blocks, tail_mode, mix_blocks, s); return true; } \
+        if (gaunt && np2 == NP2G) { launch_mixed_all<NMV, NP2G, true, NMV>(a, rounds, tail_blocks, tail_mode, mix_blocks, defer_rounds, s); return true; } \
+        if (!gaunt && np2 == NP2D) { launch_mixed_all<NMV, NP2D, false, RSPLIT>(a, rounds, tail_blocks, tail_mode, mix_blocks, defer_rounds, s); return true; } \
         return false;                                                                                     \
     }
     MNC_CASE(1, 1, 1, 1)

@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "mnc_internal.h"
+#include "mnc_gaunt_pattern.h"
 
 namespace mnc {
 namespace {
@@ -258,7 +259,27 @@ int build_mix_tables(int moment_order, int quad_order, MixTables &m) {
     m.gptr = t.gptr;
     m.gk = t.gk;
     m.gv = t.gv;
+    if (!gaunt_pattern_values(moment_order, t.gptr, t.gk, t.gv, m.pv)) m.pv.clear();
     return MNC_OK;
+}
+
+bool gaunt_pattern_values(int n, const std::vector<int> &gptr, const std::vector<int> &gk,
+                          const std::vector<double> &gv, std::vector<float> &out) {
+    const int nm = gp_nm(n);
+    out.clear();
+    if (gptr.size() != size_t(nm) * nm + 1) return false;
+    size_t matched = 0, upper = 0;
+    for (int i = 0; i < nm; ++i)
+        for (int j = i; j < nm; ++j) upper += size_t(gptr[size_t(i) * nm + j + 1] - gptr[size_t(i) * nm + j]);
+    gp_walk(n, [&](int i, int j, int k) {
+        float v = 0.0f;
+        const int pr = i * nm + j;
+        for (int cc = gptr[pr]; cc < gptr[pr + 1]; ++cc)
+            if (gk[cc] == k) { v = float(gv[cc]); ++matched; }
+        out.push_back(v);
+    });
+    if (matched != upper) { out.clear(); return false; }
+    return true;
 }
 
 // Operand tables of the FP64 matrix-core sweep (v_mfma_f64_16x16x4f64 layouts

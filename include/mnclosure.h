@@ -102,6 +102,7 @@ typedef struct {
     long long trial_sum;         /* Armijo trial evaluations (f only)             */
     long long warm_pick[3];      /* MNC_WARM_LEARNED starts taken from: [0] the
                                     MLP, [1] isotropicAlpha, [2] the alpha passed in */
+    long long mixed_deferred;    /* MNC_PREC_MIXED Newton steps redone in FP64       */
 } mnc_stats;
 
 /* Creates a context: builds the quadrature, the real spherical-harmonic table
@@ -174,7 +175,10 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *                  1: one lane per cell; 2: one wave per cell, algebra per lane
  *   "mix_cond_log10"  MNC_PREC_MIXED: a Newton step uses the mixed-precision
  *                  Hessian only while its condition estimate is <= 10^value
- *                  (0-10, default 4); otherwise the step is redone in FP64
+ *                  (0-10, default 5); otherwise the step is redone in FP64
+ *   "mix_defer"    MNC_PREC_MIXED: 0: a step to redo in FP64 waits for the
+ *                  FP64 tail (default); 1: it joins FP64 lockstep rounds
+ *                  that run beside the mixed-precision ones
  *   "mix_blocks"   workgroups of the mixed-precision sweep (1-65536)
  *   "flux_tile"    0: ansatz written to HBM, then the flux; 1: ansatz fused
  *                  into the flux with 8x8-cell LDS tiles (default); 2: 16x16
@@ -191,8 +195,11 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value);
 
 /* Autotuner (no counterpart in the reference): times `steps` mnc_update steps
  * of dt from the current grid state for each candidate launch shape, one
- * coordinate-descent pass over "graph", "rounds", "tail_mode", "tail_blocks"
- * and (mixed precision) "mix_blocks", and keeps the fastest.  The grid state
+ * coordinate-descent pass over "graph", "flux_tile", "rounds", "tail_mode",
+ * "tail_blocks" and (mixed precision) "mix_cond_log10", "mix_defer" and
+ * "mix_blocks", and keeps the fastest.  Every candidate converges to the
+ * same tolerance; "mix_cond_log10" trades mixed-precision steps against
+ * FP64 ones.  The grid state
  * and the statistics are restored after every trial, so the call leaves the
  * solution where it was.  report (may be NULL) receives one "key=value ms/step"
  * line per trial.  The tail kernels sum in different orders, so the choice can
