@@ -132,9 +132,10 @@ T0 = time.perf_counter()
 FORCE_TUNE = {}
 
 
-def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=60, autotune=True):
-    """W warm-up steps, the library's autotuner on the state they reach, then
-    (from t = 0 again) K steps timed by wall clock with the tuned launch shape
+def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=60, autotune=True, tune_step=None):
+    """W warm-up steps, then (untimed) on to step `tune_step` (default: the
+    middle of the run) and the library's autotuner on that state, then (from
+    t = 0 again) K steps timed by wall clock with the tuned launch shape
     (hipGraph replay if the tuner picked it; per-launch timing off), then the
     first min(K, kernel_steps) steps again, eagerly, with HIP-event timing of
     every launch for the per-kernel-class split and the roofline."""
@@ -142,10 +143,12 @@ def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=60, a
     if mlp_params is not None:
         ctx.mlp_set_params(mlp_params)
     dts = gi.time_steps(g.max_dt, T_FINAL)
-    for k in range(min(warmup, len(dts))):
+    if tune_step is None:
+        tune_step = len(dts) // 2
+    for k in range(min(max(warmup, tune_step if autotune else 0), len(dts))):
         ctx.update(dts[k], opt)
     ctx.sync()
-    report = ctx.autotune(dts[0], opt, steps=3) if autotune else []
+    report = ctx.autotune(dts[0], opt, steps=5) if autotune else []
     progress("autotuned: %s" % "; ".join(report[-3:]))
     for k, v in FORCE_TUNE.items():
         ctx.tune(k, v)

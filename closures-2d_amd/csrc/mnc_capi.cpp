@@ -19,14 +19,14 @@ struct mnc_ctx {
     mnc::Tables tab;
     int num_cu = 256;
     // tables on device
-    double *d_w = nullptr, *d_p = nullptr, *d_xi = nullptr, *d_eta = nullptr, *d_gv = nullptr;
+    double *d_w = nullptr, *d_p = nullptr, *d_pT = nullptr, *d_xi = nullptr, *d_eta = nullptr, *d_gv = nullptr;
     int *d_gptr = nullptr, *d_gk = nullptr;
     // solver state / stats
     int *d_ctl = nullptr;            // [0],[1] list counts, [2] tail queue, [3] error bits, [4] deferred
     long long *d_stats = nullptr;    // [kStatWords]
     double *d_std = nullptr;         // per-cell solver state (doubles)
     int *d_sti = nullptr;            // per-cell solver state (ints)
-    int *d_lists = nullptr;          // two active lists and the deferred list
+    int *d_lists = nullptr;          // mnc::kNumLists lists of cell indices (see mnc_internal.h)
     // mixed precision operand tables (absent when the order/quadrature has none)
     mnc::MixTables mix;
     bool has_mix = false;
@@ -146,7 +146,7 @@ int ensure_state(mnc_ctx *ctx, size_t ncells) {
     const size_t cap = std::max<size_t>(ncells, 64);
     int st;
     if ((st = dalloc(ctx, &ctx->d_std, size_t(df) * cap)) || (st = dalloc(ctx, &ctx->d_sti, size_t(ifl) * cap)) ||
-        (st = dalloc(ctx, &ctx->d_lists, 4 * cap)))
+        (st = dalloc(ctx, &ctx->d_lists, size_t(mnc::kNumLists) * cap)))
         return st;
     ctx->state_cap = cap;
     return MNC_OK;
@@ -263,6 +263,7 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     a.outs = outs;
     a.w = ctx->d_w;
     a.p = ctx->d_p;
+    a.pT = ctx->d_pT;
     a.nq = t.nq;
     a.gptr = ctx->d_gptr;
     a.gk = ctx->d_gk;
@@ -294,30 +295,55 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     a.gh = ctx->d_gh;
     a.gnt = ctx->g64.nt;
     a.gnmm = ctx->g64.nmm;
-    const int tail = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
+    mnc::LaunchPlan lp;
+    lp.r0 = 0;
+    lp.r1 = ctx->rounds;
+    lp.tail_blocks = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
+    lp.tail_mode = ctx->tail_mode;
+    lp.mix_blocks = ctx->mix_blocks > 0 ? ctx->mix_blocks : ctx->num_cu;
+    lp.defer_rounds = ctx->mix_defer;
+    const bool gaunt = t.use_gaunt != 0;
+    auto launch = [&](const mnc::NewtonArgs &aa, const mnc::LaunchPlan &p) -> int {
+        if (mixed) {
+            if (!mnc::launch_newton_mixed(aa, t.nm, t.np2, gaunt, p, ctx->stream))
+                return ctx_fail(ctx, MNC_ERR_ARG, "no mixed-precision Newton kernel instance for this order");
+        } else if (!mnc::launch_newton_fp64(aa, t.nm, t.np2, gaunt, p, ctx->stream)) {
+            return ctx_fail(ctx, MNC_ERR_ARG, "no FP64 Newton kernel instance for this moment order");
+        }
+        return MNC_OK;
+    };
     if (learned) {
-        // prologue (cells whose best classical start is poor go to list 2), the
-        // network on those cells only (row count read on the device), judge
+        // Round 1 evaluates every cell at its start and parks those failing the
+        // gate on the network's list; the network runs on those rows only (the
+        // count is read on the device), the judge moves them to round 2's list.
+        if (t.nm != 15) return ctx_fail(ctx, MNC_ERR_ARG, "the learned warm start is for moment order 4");
+        mnc::LaunchPlan first = lp, rest = lp;
+        first.r1 = 1;
+        first.tail = false;
+        rest.r0 = 1;
+        rest.r1 = std::max(ctx->rounds, 1);
+        rest.init = false;
+        mnc::NewtonArgs ad = a;
+        ad.divert = 1;
         {
             Timed tm(ctx, MNC_KCLASS_NEWTON);
             mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
-            if (!mnc::launch_newton_init(a, t.nm, t.np2, t.use_gaunt != 0, ctx->stream))
-                return ctx_fail(ctx, MNC_ERR_ARG, "no learned warm start for this order");
+            if ((st = launch(ad, first))) return st;
         }
         {
             Timed tm(ctx, MNC_KCLASS_MLP);
-            mnc::mlp_predict_listed(mlp_buffers(ctx), u, ctx->d_lists + 2 * ctx->state_cap, ctx->d_ctl + 4, ncells,
-                                    ctx->d_guess, ctx->stream);
+            mnc::mlp_predict_listed(mlp_buffers(ctx), u, ctx->d_lists + size_t(mnc::kMlpListIndex) * ctx->state_cap,
+                                    ctx->d_ctl + mnc::kMlpListWord, ncells, ctx->d_guess, ctx->stream);
         }
         {
             Timed tm(ctx, MNC_KCLASS_NEWTON);
-            mnc::launch_newton_judge(a, t.nm, t.np2, t.use_gaunt != 0, ctx->stream);
+            if (!mnc::launch_newton_judge(a, t.nm, t.np2, gaunt, ctx->stream))
+                return ctx_fail(ctx, MNC_ERR_ARG, "no learned warm start for this order");
+            if ((st = launch(a, rest))) return st;
         }
-        a.skip_init = 1;
-    }
-    {
+    } else {
         Timed tm(ctx, MNC_KCLASS_NEWTON);
-        if (!learned) mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
+        mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
         if (ctx->has_g64) {
             // lockstep rounds in batches until the active list is empty (the
             // count is read back between batches, so this path is not captured)
@@ -333,14 +359,8 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
                 MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
                 if (left == 0) break;
             }
-        } else if (mixed) {
-            const int mb = ctx->mix_blocks > 0 ? ctx->mix_blocks : ctx->num_cu;
-            if (!mnc::launch_newton_mixed(a, t.nm, t.np2, t.use_gaunt != 0, ctx->rounds, tail, ctx->tail_mode, mb,
-                                          ctx->mix_defer, ctx->stream))
-                return ctx_fail(ctx, MNC_ERR_ARG, "no mixed-precision Newton kernel instance for this order");
-        } else if (!mnc::launch_newton_fp64(a, t.nm, t.np2, t.use_gaunt != 0, ctx->rounds, tail, ctx->tail_mode,
-                                            ctx->stream)) {
-            return ctx_fail(ctx, MNC_ERR_ARG, "no FP64 Newton kernel instance for this moment order");
+        } else if ((st = launch(a, lp))) {
+            return st;
         }
     }
     MNC_HIP_CHECK(hipGetLastError());
@@ -436,7 +456,10 @@ int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
             return MNC_ERR_HIP;
         return MNC_OK;
     };
-    if (up(&ctx->d_w, t.w) || up(&ctx->d_p, t.p) || up(&ctx->d_xi, t.xi) || up(&ctx->d_eta, t.eta) ||
+    std::vector<double> pT(t.p.size());
+    for (int q = 0; q < t.nq; ++q)
+        for (int k = 0; k < t.np2; ++k) pT[size_t(k) * t.nq + q] = t.p[size_t(q) * t.np2 + k];
+    if (up(&ctx->d_w, t.w) || up(&ctx->d_p, t.p) || up(&ctx->d_pT, pT) || up(&ctx->d_xi, t.xi) || up(&ctx->d_eta, t.eta) ||
         up(&ctx->d_gptr, t.gptr) || up(&ctx->d_gk, t.gk) || up(&ctx->d_gv, t.gv) ||
         hipMalloc((void **)&ctx->d_ctl, mnc::kCtlWords * sizeof(int)) != hipSuccess ||
         hipMemset(ctx->d_ctl, 0, mnc::kCtlWords * sizeof(int)) != hipSuccess ||
@@ -475,7 +498,7 @@ void mnc_destroy(mnc_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto &g : ctx->graphs) drop_graph(g);
     for (auto &ev : ctx->events) { (void)hipEventDestroy(ev.a); (void)hipEventDestroy(ev.b); }
-    void *ptrs[] = {ctx->d_w, ctx->d_p, ctx->d_xi, ctx->d_eta, ctx->d_gv, ctx->d_gptr, ctx->d_gk,
+    void *ptrs[] = {ctx->d_w, ctx->d_p, ctx->d_pT, ctx->d_xi, ctx->d_eta, ctx->d_gv, ctx->d_gptr, ctx->d_gk,
                     ctx->d_mz, ctx->d_mg, ctx->d_mh, ctx->d_mgptr, ctx->d_mgk, ctx->d_mgv, ctx->d_mpv, ctx->d_gz, ctx->d_gg, ctx->d_gh,
                     ctx->d_mlp, ctx->d_mlp_scratch, ctx->d_mlp_loss, ctx->d_mlp_io, ctx->d_us, ctx->d_guess,
                     ctx->d_ctl, ctx->d_stats, ctx->d_std, ctx->d_sti, ctx->d_lists, ctx->d_mom, ctx->d_alpha, ctx->d_old,
@@ -870,7 +893,7 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         return MNC_OK;
     }
     if (std::strcmp(key, "learned_gate_log10") == 0) {
-        if (value < -16 || value > 0) return ctx_fail(ctx, MNC_ERR_ARG, "learned_gate_log10 out of range");
+        if (value < -16 || value > 4) return ctx_fail(ctx, MNC_ERR_ARG, "learned_gate_log10 out of range");
         ctx->learned_gate_log10 = value;
         return MNC_OK;
     }
@@ -949,6 +972,8 @@ int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, cha
         {"tail_mode", {0, 1, 2}, &ctx->tail_mode},
         {"tail_blocks", {cu, 2 * cu, 4 * cu}, &ctx->tail_blocks},
     };
+    if (opt->warm_start == MNC_WARM_LEARNED)
+        knobs.push_back({"learned_gate_log10", {-3, -2, -1, 0, 1}, &ctx->learned_gate_log10});
     if (opt->precision == MNC_PREC_MIXED) {
         knobs.push_back({"mix_cond_log10", {4, 5, 6}, &ctx->mix_cond_log10});
         knobs.push_back({"mix_defer", {0, 1}, &ctx->mix_defer});

@@ -34,6 +34,7 @@ struct NewtonArgs {
     mnc_outs *outs;       // [ncells] or nullptr
     const double *w;      // [nq]
     const double *p;      // [nq][np2]
+    const double *pT;     // [np2][nq], the same table transposed
     int nq;
     const int *gptr;      // Gaunt CSR (nm*nm+1)
     const int *gk;
@@ -49,7 +50,7 @@ struct NewtonArgs {
     const double *alpha_guess; // [ncells][nm] scaled-problem guess (learned warm start) or nullptr
     double gate;          // learned warm start: the network is consulted when the best of isotropic
                           // and previous alpha has ||g|| > gate * ||u|| (scaled problem)
-    int skip_init;        // the prologue (init_kernel) was already enqueued
+    int divert;           // learned warm start: this launch's logic holds the gate (first round)
     // per-cell solver state and the active lists
     double *st_d;         // [dfields][cap]
     int *st_i;            // [ifields][cap]
@@ -71,8 +72,13 @@ struct NewtonArgs {
     int gnmm;             // order-2N moments
 };
 
-// Number of control words of a batched solve.
+// Control words of a batched solve: [0], [1] counts of lists 0/1 (the
+// lockstep pair), [2] the tail's queue head, [3] error bits, [4], [5] counts of
+// lists 2/3 (cells redone in FP64 by a mixed-precision solve), [6] count of
+// list 4 (cells waiting for the learned warm start's network).
 constexpr int kCtlWords = 8;
+constexpr int kNumLists = 5;
+constexpr int kMlpListIndex = 4, kMlpListWord = 6;
 constexpr int kStatWords = 13;
 
 // Result layout of v_mfma_f64_16x16x4f64 on gfx950 (measured with a one-wave
@@ -113,20 +119,25 @@ struct G64Tables {
 int build_g64_tables(int moment_order, int quad_order, G64Tables &m);
 
 // Launchers (mnc_newton.hip).  Return false if (nm, np2, gaunt) has no instance.
-// Enqueues the whole batched solve: prologue, `rounds` lockstep sweep/logic
-// rounds with compaction, then the persistent tail.  No host synchronisation.
-// tail_mode 0: one wave per remaining cell (cooperative sweeps); 1: one lane per cell.
-bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
-                        int tail_mode, hipStream_t s);
-// The learned warm start's two halves (M4): the prologue, which parks the cells
-// whose best classical start is poor on list 2, and the judge, which compares
-// the network's guesses for them and moves them to list 0.
-bool launch_newton_init(const NewtonArgs &a, int nm, int np2, bool gaunt, hipStream_t s);
-bool launch_newton_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, hipStream_t s);
+// One call enqueues (no host synchronisation) the prologue if `init`, the
+// lockstep sweep/logic rounds r0..r1-1 with compaction (round r reads list
+// r & 1) and, if `tail`, the persistent tail.  tail_mode 0: one wave per
+// remaining cell (cooperative sweeps); 1: one lane per cell; 2: one wave per
+// cell, algebra per lane.
+struct LaunchPlan {
+    int r0 = 0, r1 = 0;
+    bool init = true, tail = true;
+    int tail_blocks = 0, tail_mode = 0;
+    int mix_blocks = 0, defer_rounds = 0;   // mixed precision only
+};
+bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, const LaunchPlan &lp, hipStream_t s);
 // The same with the lockstep rounds in mixed precision (MFMA sweeps); the tail
-// is FP64.  mix_blocks: persistent blocks of the mixed sweep.
-bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
-                         int tail_mode, int mix_blocks, int defer_rounds, hipStream_t s);
+// is FP64.
+bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, const LaunchPlan &lp, hipStream_t s);
+// The learned warm start (M4): with NewtonArgs::divert set, the first round's
+// logic parks the cells whose start fails the gate on the network's list; the
+// judge compares the network's guesses for them and moves them to list 1.
+bool launch_newton_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, hipStream_t s);
 // The FP64 matrix-core path (M8: n = 45).  rounds: lockstep rounds per launch
 // batch; the caller repeats batches until control word `lin` reads zero.
 // Returns false for an order without an instance.

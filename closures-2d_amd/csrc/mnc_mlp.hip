@@ -14,6 +14,8 @@
 // W[in][out] (y = x W + b).  The parameter vector is
 //   W1[15][128] b1[128] W2[128][128] b2[128] W3[128][128] b3[128] W4[128][15] b4[15]
 // (37,007 values), followed by in_mean[15] in_std[15] out_mean[15] out_std[15].
+#include <algorithm>
+
 #include "mnc_internal.h"
 
 namespace mnc {
@@ -56,13 +58,7 @@ struct GemmArgs {
     int epi;
 };
 
-__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-    const int lane = threadIdx.x & 63;
-    const int tiles_n = (g.N + 15) >> 4;
-    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (g.mdev) g.M = min(g.M, *g.mdev);
-    const int tiles = ((g.M + 15) >> 4) * tiles_n;
-    if (tile >= tiles) return;
+__device__ __forceinline__ void gemm_tile(const GemmArgs &g, int tile, int tiles_n, int lane) {
     const int i0 = (tile / tiles_n) * 16, j0 = (tile % tiles_n) * 16;
     const int ia = i0 + (lane & 15), jb = j0 + (lane & 15), kq = 8 * (lane >> 4);
     mf32x4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -95,6 +91,17 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
             if (g.epi == EPI_GELU) g.C2[o] = gelu(v);
         }
     }
+}
+
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
+    const int lane = threadIdx.x & 63;
+    const int tiles_n = (g.N + 15) >> 4;
+    if (g.mdev) g.M = min(g.M, *g.mdev);
+    const int tiles = ((g.M + 15) >> 4) * tiles_n;
+    // grid-stride over the tiles: with the row count known only on the device
+    // the grid is sized for the maximum, so blocks past the work leave at once
+    for (int tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < tiles; tile += gridDim.x * 4)
+        gemm_tile(g, tile, tiles_n, lane);
 }
 
 // db[j] = sum_i dZ[i][j]  (one block per column, tree sum)
@@ -223,7 +230,8 @@ void gemm(hipStream_t s, const float *A, long long sai, long long sak, const flo
           const int *mdev = nullptr) {
     GemmArgs g{A, B, bias, aux, C, C2, M, N, K, mdev, sai, sak, sbk, sbj, ldc, epi};
     const int tiles = ((M + 15) / 16) * ((N + 15) / 16);
-    hipLaunchKernelGGL(gemm_kernel, dim3((tiles + 3) / 4), dim3(256), 0, s, g);
+    const int blocks = std::min((tiles + 3) / 4, 2048);     // 8 blocks per CU at 256 CUs; tiles beyond loop
+    hipLaunchKernelGGL(gemm_kernel, dim3(blocks), dim3(256), 0, s, g);
 }
 }  // namespace
 

@@ -86,6 +86,26 @@ __device__ __forceinline__ void apply_inv(const T (&l)[Tri<N>::size], T (&x)[N])
     }
 }
 
+// apply_inv with the reciprocals of the diagonal precomputed (rinv): the
+// condition estimate's repeated solves, where a division per step would sit on
+// the dependency chain; the estimate moves only at rounding level.
+template <int N, class T = double>
+__device__ __forceinline__ void apply_inv_rcp(const T (&l)[Tri<N>::size], const T (&rinv)[N], T (&x)[N]) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        x[j] = x[j] * rinv[j];
+#pragma unroll
+        for (int i = j + 1; i < N; ++i) x[i] = fma(-x[j], l[Tri<N>::at(i, j)], x[i]);
+    }
+#pragma unroll
+    for (int j = N - 1; j >= 0; --j) {
+        T t = x[j];
+#pragma unroll
+        for (int i = N - 1; i > j; --i) t = fma(-l[Tri<N>::at(i, j)], x[i], t);
+        x[j] = t * rinv[j];
+    }
+}
+
 template <int N, class T>
 __device__ __forceinline__ T dasum(const T (&x)[N]) {
     T s = T(0);
@@ -116,11 +136,15 @@ __device__ __forceinline__ double rcond(const T (&l)[Tri<N>::size], double anorm
     unsigned isgn = 0;   // bit i set <=> isgn_i = -1
     T est = T(0);
     int jump = 1, j = 0, iter = 0;
+    T rinv[N];
 #pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = T(1) / T(N);
+    for (int i = 0; i < N; ++i) {
+        x[i] = T(1) / T(N);
+        rinv[i] = T(1) / l[Tri<N>::at(i, i)];
+    }
 #pragma unroll 1
     for (int guard = 0; guard < 16; ++guard) {
-        apply_inv<N, T>(l, x);
+        apply_inv_rcp<N, T>(l, rinv, x);
         if (jump == 1) {                       // first product
             if (N == 1) { est = fabs(x[0]); break; }
             est = dasum<N, T>(x);
@@ -226,7 +250,10 @@ __device__ __forceinline__ double wave_min(double v) {
 
 enum : int { ST_NEWTON = 0, ST_TRIAL = 1 };
 constexpr int kHalvings = 8;   // line-search halvings evaluated per wave sweep in the tail
-enum : int { LOGIC_CONTINUE = 0, LOGIC_DONE = 1, LOGIC_DEFER = 2 };
+constexpr int kTailPts = 4;    // quadrature points per lane whose w e the tail keeps between passes
+enum : int { LOGIC_CONTINUE = 0, LOGIC_DONE = 1, LOGIC_DEFER = 2, LOGIC_MLP = 3 };
+constexpr int kMlpList = kMlpListIndex;   // cells waiting for the learned warm start's network
+static_assert(kMlpListWord == kMlpListIndex + 2, "list word of cw()");
 
 // The table is read-only for the whole kernel; reading it through the constant
 // address space lets wave-uniform row reads become scalar loads (SGPR operands).
@@ -323,7 +350,7 @@ struct Store {
         F_GD, F_MAXF, F_FSCALE, F_SF, F_GS, F_ACC = F_GS + NM
     };
     static constexpr int dfields = F_ACC + nacc;
-    enum : int { I_ITER = 0, I_TOTAL, I_REG, I_MAXIT, I_MODE, ifields };
+    enum : int { I_ITER = 0, I_TOTAL, I_REG, I_MAXIT, I_MODE, I_WARM, ifields };
     double *D;
     int *I;
     size_t cap;
@@ -409,8 +436,8 @@ __device__ double grad_norm2(const NewtonArgs &A, const double (&al)[NM], const 
 
 // Control word holding the count of list l: lists 0 and 1 (the lockstep
 // ping-pong pair) use words 0 and 1, lists 2 and 3 (the FP64 pair of a
-// mixed-precision solve; list 2 also carries the learned prologue's cells)
-// use words 4 and 5.  Word 2 is the tail's queue head, word 3 the error bits.
+// mixed-precision solve) words 4 and 5, list 4 (the learned warm start's
+// network rows) word 6.  Word 2 is the tail's queue head, word 3 the error bits.
 __device__ __forceinline__ int cw(int l) { return l < 2 ? l : l + 2; }
 
 // Appends the calling lanes with `want` set to list `dst`, whose count is
@@ -426,14 +453,13 @@ __device__ __forceinline__ void list_append(const NewtonArgs &A, int dst, int cw
     if (want) A.lists[size_t(dst) * A.cap + base + __popcll(m & lane_mask_lt())] = c;
 }
 
-enum : int { INIT_INVALID = 0, INIT_READY = 1, INIT_NEEDS_MLP = 2 };
+enum : int { INIT_INVALID = 0, INIT_READY = 1 };
 
 // opt() prologue (opt.cpp:87-101) and optScaled set-up (:170-205) for cell c.
 // Returns INIT_INVALID if u[0] <= 0 (the reference aborts: opt.cpp:97-101).
-// MNC_WARM_LEARNED: the start is the better (smaller FP64 gradient norm) of
-// isotropicAlpha and the alpha passed in; if that norm is above gate * ||u||,
-// the cell returns INIT_NEEDS_MLP with the norm^2 parked in F_FCUR and the
-// pick in I_REG, and judge_kernel later compares the network's guess.
+// MNC_WARM_LEARNED starts from the alpha passed in (isotropicAlpha if it is not
+// finite; I_WARM records which); the first round's logic then decides whether
+// the network is consulted (cell_logic, divert).
 template <int NM, int NP2>
 __device__ int cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, LaneStats &st) {
     using St = Store<NM, NP2>;
@@ -441,9 +467,6 @@ __device__ int cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, La
     const double *ain = A.alpha + size_t(c) * NM;
     const double u0 = uin[0];
     const double ascale = log(u0) * 2.0 * kSqrtPi;
-    bool needs_mlp = false;
-    double park_best = 0.0;
-    int park_pick = 0;
     double ss = 0.0;
 #pragma unroll
     for (int k = 0; k < NM; ++k) {
@@ -452,50 +475,19 @@ __device__ int cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, La
         ss = fma(uk, uk, ss);
     }
     double al[NM];
-    if (A.warm == MNC_WARM_ISOTROPIC) {
-        al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
-#pragma unroll
-        for (int k = 1; k < NM; ++k) al[k] = 0.0;
-    } else if (A.warm == MNC_WARM_LEARNED) {
-        // Judge: the candidate with the smallest FP64 gradient norm.  At
-        // isotropicAlpha the integrals are (1, 0, ...) up to rounding (the
-        // basis is orthonormal and <m0 e> = 1 there), so its gradient norm is
-        // sum_{k>0} us_k^2 without a sweep.
-        double us[NM];
-        double best = 0.0, unorm2 = 0.0;
-#pragma unroll
-        for (int k = 0; k < NM; ++k) us[k] = uin[k] / u0;
-#pragma unroll
-        for (int k = 1; k < NM; ++k) best = fma(us[k], us[k], best);
-        unorm2 = 1.0 + best;
-        al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
-#pragma unroll
-        for (int k = 1; k < NM; ++k) al[k] = 0.0;
-        int pick = 1;
-        double cand[NM];
+    int warm = 2;
+    bool iso = A.warm == MNC_WARM_ISOTROPIC;
+    if (A.warm == MNC_WARM_LEARNED) {
         bool finite = true;
 #pragma unroll
-        for (int k = 0; k < NM; ++k) {
-            cand[k] = ain[k];
-            finite = finite && isfinite(cand[k]);
-        }
-        cand[0] = cand[0] - ascale;
-        if (finite && u0 > 0.0) {
-            const double n2 = grad_norm2<NM, NP2>(A, cand, us);
-            if (n2 < best) {
-                best = n2;
-                pick = 2;
+        for (int k = 0; k < NM; ++k) finite = finite && isfinite(ain[k]);
+        iso = !finite;
+        warm = finite ? 2 : 1;
+    }
+    if (iso) {
+        al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
 #pragma unroll
-                for (int k = 0; k < NM; ++k) al[k] = cand[k];
-            }
-        }
-        if (u0 > 0.0 && best > A.gate * A.gate * unorm2) {   // the network may do better
-            needs_mlp = true;
-            park_best = best;
-            park_pick = pick;
-        } else if (u0 > 0.0) {
-            ++st.warm[pick];
-        }
+        for (int k = 1; k < NM; ++k) al[k] = 0.0;
     } else {
 #pragma unroll
         for (int k = 0; k < NM; ++k) al[k] = ain[k];
@@ -514,13 +506,10 @@ __device__ int cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, La
     S.d(St::F_T, c) = 1.0;
     S.i(St::I_ITER, c) = 0;
     S.i(St::I_TOTAL, c) = 0;
-    S.i(St::I_REG, c) = needs_mlp ? park_pick : 0;
+    S.i(St::I_REG, c) = 0;
     S.i(St::I_MAXIT, c) = A.max_iter;
     S.i(St::I_MODE, c) = ST_NEWTON;
-    if (needs_mlp) {
-        S.d(St::F_FCUR, c) = park_best;
-        return INIT_NEEDS_MLP;
-    }
+    S.i(St::I_WARM, c) = warm;
     return INIT_READY;
 }
 
@@ -550,7 +539,7 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
 #pragma unroll
     for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
     bool newton = (mode == ST_NEWTON);
-    bool do_reg = false, converged = false, defer = false;
+    bool do_reg = false, converged = false, defer = false, to_mlp = false;
     double err = 0.0;
     if (mode == ST_TRIAL) {                                        // linesearch.cpp:45-70
         ++st.trials;
@@ -661,83 +650,125 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
             S.d(St::F_FSCALE, c) = fscale;
             err = sqrt(gg);                                        // dnrm2 (:286)
         }
-        double anorm = 0.0, gersh = DBL_MAX;
+        bool diverted = false;
+        if (A.divert) {
+            // The learned warm start's gate at the first evaluation: the start
+            // (the alpha passed in, or isotropicAlpha if that was not finite)
+            // is kept while ||g|| <= gate * ||u||.  Otherwise the better of it
+            // and isotropicAlpha -- whose gradient norm is sum_{k>0} u_k^2
+            // without a sweep: there the integrals are (1, 0, ...) up to
+            // rounding -- is parked (norm^2 in F_FCUR, label in I_WARM) for
+            // the network's guess to be judged against (judge_kernel), and
+            // the evaluation does not count as an iteration.
+            const int warm = S.i(St::I_WARM, c);
+            double iso2 = 0.0;
 #pragma unroll
-        for (int i = 0; i < NM; ++i) {
-            double s = 0.0;
-#pragma unroll
-            for (int j = 0; j < NM; ++j) s += fabs(h[i >= j ? Tri<NM>::at(i, j) : Tri<NM>::at(j, i)]);
-            anorm = s > anorm ? s : anorm;
-            gersh = fmin(gersh, 2.0 * double(h[Tri<NM>::at(i, i)]) - s);
-        }
-        // Phase 2: factor, condition estimate, direction (dpotrf, dpocon, dpotrs).
-        const double climit = MIX ? fmin(A.cmix, A.cond_h_max) : A.cond_h_max;
-        const bool chol_ok = cholesky<NM, HT>(h);
-        double condest = 0.0;
-        if (chol_ok && !cond_proven_below(anorm, gersh, NM, climit)) condest = 1.0 / rcond<NM, HT>(h, anorm);
-        if (iter >= maxit || !chol_ok || t == 0.0 || condest > climit) {
-            if constexpr (MIX) {
-                --iter; --total;
-                defer = true;
-            } else {
-                do_reg = true;                                     // opt.cpp:244
+            for (int k = 1; k < NM; ++k) {
+                const double uk = S.d(St::F_U + k, c);
+                iso2 = fma(uk, uk, iso2);
             }
-        } else {
-            double d[NM];
-            {
-                HT dh[NM];
+            const double g2 = err * err;
+            if (g2 > A.gate * A.gate * (1.0 + iso2)) {
+                int pick = warm;
+                double best = g2;
+                if (warm == 2 && iso2 < g2) {
+                    pick = 1;
+                    best = iso2;
+                    al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
 #pragma unroll
-                for (int i = 0; i < NM; ++i) dh[i] = HT(S.d(St::F_D + i, c));
-                apply_inv<NM, HT>(h, dh);                          // dpotrs (:278-282)
-#pragma unroll
-                for (int i = 0; i < NM; ++i) d[i] = double(dh[i]);
-            }
-            // Phase 3: stop test, or the line search set-up (linesearch.cpp:28-43)
-            if (err <= S.d(St::F_RTOL, c)) {
-                converged = true;                                  // :293-306
-#pragma unroll
-                for (int i = 0; i < NM; ++i) g0[i] = S.d(St::F_D + i, c);
-            } else {
-                double maxf = 0.0, gd = 0.0;
-#pragma unroll
-                for (int i = 0; i < NM; ++i) {
-                    const double ai = S.d(St::F_AL + i, c);
-                    const double gi = S.d(St::F_D + i, c);
-                    const double di = -d[i];
-                    const double tmp = fabs(di) / (fabs(ai) + 1e-100);
-                    maxf = tmp > maxf ? tmp : maxf;
-                    gd = fma(gi, di, gd);
-                    S.d(St::F_AOLD + i, c) = ai;
-                    S.d(St::F_D + i, c) = di;
-                    al[i] = ai;
-                    d[i] = di;
+                    for (int k = 1; k < NM; ++k) al[k] = 0.0;
                 }
-                S.d(St::F_MAXF, c) = maxf;
-                S.d(St::F_GD, c) = gd;
-                t = 1.0;
-                if (t * maxf > DBL_EPSILON / 2) {
-#pragma unroll
-                    for (int i = 0; i < NM; ++i) al[i] = al[i] + t * d[i];
-                    mode = ST_TRIAL;
-                } else if constexpr (MIX) {
+                S.d(St::F_FCUR, c) = best;
+                S.i(St::I_WARM, c) = pick;
+                --iter;
+                --total;
+                diverted = true;
+                to_mlp = true;
+            } else {
+                ++st.warm[warm];
+            }
+        }
+        if (!diverted) {
+            double anorm = 0.0, gersh = DBL_MAX;
+    #pragma unroll
+            for (int i = 0; i < NM; ++i) {
+                double s = 0.0;
+    #pragma unroll
+                for (int j = 0; j < NM; ++j) s += fabs(h[i >= j ? Tri<NM>::at(i, j) : Tri<NM>::at(j, i)]);
+                anorm = s > anorm ? s : anorm;
+                gersh = fmin(gersh, 2.0 * double(h[Tri<NM>::at(i, i)]) - s);
+            }
+            // Phase 2: factor, condition estimate, direction (dpotrf, dpocon, dpotrs).
+            const double climit = MIX ? fmin(A.cmix, A.cond_h_max) : A.cond_h_max;
+            const bool chol_ok = cholesky<NM, HT>(h);
+            double condest = 0.0;
+            if (chol_ok && !cond_proven_below(anorm, gersh, NM, climit)) condest = 1.0 / rcond<NM, HT>(h, anorm);
+            if (iter >= maxit || !chol_ok || t == 0.0 || condest > climit) {
+                if constexpr (MIX) {
                     --iter; --total;
                     defer = true;
                 } else {
-                    // the Armijo loop cannot start: linesearch returns 0 and the
-                    // next iteration regularizes (opt.cpp:212-213,244)
-                    t = 0.0;
-                    ++iter; ++total;
-                    do_reg = true;
+                    do_reg = true;                                     // opt.cpp:244
+                }
+            } else {
+                double d[NM];
+                {
+                    HT dh[NM];
+    #pragma unroll
+                    for (int i = 0; i < NM; ++i) dh[i] = HT(S.d(St::F_D + i, c));
+                    apply_inv<NM, HT>(h, dh);                          // dpotrs (:278-282)
+    #pragma unroll
+                    for (int i = 0; i < NM; ++i) d[i] = double(dh[i]);
+                }
+                // Phase 3: stop test, or the line search set-up (linesearch.cpp:28-43)
+                if (err <= S.d(St::F_RTOL, c)) {
+                    converged = true;                                  // :293-306
+    #pragma unroll
+                    for (int i = 0; i < NM; ++i) g0[i] = S.d(St::F_D + i, c);
+                } else {
+                    double maxf = 0.0, gd = 0.0;
+    #pragma unroll
+                    for (int i = 0; i < NM; ++i) {
+                        const double ai = S.d(St::F_AL + i, c);
+                        const double gi = S.d(St::F_D + i, c);
+                        const double di = -d[i];
+                        const double tmp = fabs(di) / (fabs(ai) + 1e-100);
+                        maxf = tmp > maxf ? tmp : maxf;
+                        gd = fma(gi, di, gd);
+                        S.d(St::F_AOLD + i, c) = ai;
+                        S.d(St::F_D + i, c) = di;
+                        al[i] = ai;
+                        d[i] = di;
+                    }
+                    S.d(St::F_MAXF, c) = maxf;
+                    S.d(St::F_GD, c) = gd;
+                    t = 1.0;
+                    if (t * maxf > DBL_EPSILON / 2) {
+    #pragma unroll
+                        for (int i = 0; i < NM; ++i) al[i] = al[i] + t * d[i];
+                        mode = ST_TRIAL;
+                    } else if constexpr (MIX) {
+                        --iter; --total;
+                        defer = true;
+                    } else {
+                        // the Armijo loop cannot start: linesearch returns 0 and the
+                        // next iteration regularizes (opt.cpp:212-213,244)
+                        t = 0.0;
+                        ++iter; ++total;
+                        do_reg = true;
+                    }
                 }
             }
-        }
-        if (!converged && mode != ST_TRIAL) {
-#pragma unroll
-            for (int i = 0; i < NM; ++i) al[i] = S.d(St::F_AL + i, c);
+            if (!converged && mode != ST_TRIAL) {
+    #pragma unroll
+                for (int i = 0; i < NM; ++i) al[i] = S.d(St::F_AL + i, c);
+            }
         }
     }
     int result = LOGIC_CONTINUE;
-    if (defer) {
+    if (to_mlp) {
+        result = LOGIC_MLP;
+    } else if (defer) {
         mode = ST_NEWTON;
         result = LOGIC_DEFER;
     } else if (do_reg) {                                           // opt.cpp:246-273
@@ -836,13 +867,13 @@ __global__ __launch_bounds__(kBlock) void init_kernel(NewtonArgs A) {
     int res = INIT_INVALID;
     if (c < A.ncells) res = cell_init<NM, NP2>(A, S, c, st);
     list_append(A, 0, 0, res == INIT_READY, c);
-    list_append(A, 2, 4, res == INIT_NEEDS_MLP, c);     // (only with MNC_WARM_LEARNED)
     flush_stats(A, st);
 }
 
-// The learned warm start's second half: for the cells init_kernel parked on
-// list 2, the network's guess (alpha_guess, scaled problem) replaces the start
-// if its FP64 gradient norm is smaller; the cells go onto list 0.
+// The learned warm start's second half: for the cells the first round's
+// logic parked on list kMlpList, the network's guess (alpha_guess, scaled
+// problem) replaces the parked start if its FP64 gradient norm is smaller; the
+// cells join list 1, the second round's input.
 template <int NM, int NP2>
 __global__ __launch_bounds__(kBlock) void judge_kernel(NewtonArgs A) {
     using St = Store<NM, NP2>;
@@ -851,8 +882,8 @@ __global__ __launch_bounds__(kBlock) void judge_kernel(NewtonArgs A) {
     LaneStats st;
     bool ok = false;
     int c = -1;
-    if (slot < A.ctl[4]) {
-        c = A.lists[2 * A.cap + slot];
+    if (slot < A.ctl[cw(kMlpList)]) {
+        c = A.lists[size_t(kMlpList) * A.cap + slot];
         double us[NM], cand[NM];
         bool finite = true;
 #pragma unroll
@@ -861,22 +892,17 @@ __global__ __launch_bounds__(kBlock) void judge_kernel(NewtonArgs A) {
             cand[k] = A.alpha_guess[size_t(c) * NM + k];
             finite = finite && isfinite(cand[k]);
         }
-        int pick = S.i(St::I_REG, c);
+        int pick = S.i(St::I_WARM, c);
         if (finite && grad_norm2<NM, NP2>(A, cand, us) < S.d(St::F_FCUR, c)) {
             pick = 0;
 #pragma unroll
             for (int k = 0; k < NM; ++k) S.d(St::F_AL + k, c) = cand[k];
         }
         ++st.warm[pick];
-        S.i(St::I_REG, c) = 0;
         ok = true;
     }
-    list_append(A, 0, 0, ok, c);
+    list_append(A, 1, cw(1), ok, c);
     flush_stats(A, st);
-}
-
-__global__ void ctl_clear_kernel(int *ctl, int w) {
-    if (threadIdx.x == 0) ctl[w] = 0;
 }
 
 // One sweep for every cell on list `lin`; also clears the count of the list
@@ -910,6 +936,7 @@ __global__ __launch_bounds__(kBlock) void logic_kernel(NewtonArgs A, int lin, in
     }
     list_append(A, lin ^ 1, cw(lin ^ 1), res == LOGIC_CONTINUE, c);
     if constexpr (MIX) list_append(A, defer_to, cw(defer_to), res == LOGIC_DEFER, c);
+    if (A.divert) list_append(A, kMlpList, cw(kMlpList), res == LOGIC_MLP, c);
     flush_stats(A, st);
 }
 
@@ -1071,6 +1098,25 @@ __device__ __forceinline__ void reg_apply_inv(const double (&row)[NM], const dou
     }
 }
 
+// reg_apply_inv with the reciprocals of the diagonal (wave-uniform rd[j]):
+// the condition estimate's repeated solves (see apply_inv_rcp).
+template <int NM>
+__device__ __forceinline__ void reg_apply_inv_rcp(const double (&row)[NM], const double (&col)[NM],
+                                                  const double (&rd)[NM], double &x, int lane) {
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        const double xj = readlane_d(x, j) * rd[j];
+        if (lane == j) x = xj;
+        else if (lane > j && lane < NM) x = fma(-xj, row[j], x);
+    }
+#pragma unroll
+    for (int j = NM - 1; j >= 0; --j) {
+        const double xj = readlane_d(x, j) * rd[j];
+        if (lane == j) x = xj;
+        else if (lane < j) x = fma(-col[j], xj, x);
+    }
+}
+
 // Reciprocal 1-norm condition estimate (dpocon's role; Higham 1988), x_i on lane i.
 template <int NM>
 __device__ __forceinline__ double reg_rcond(const double (&row)[NM], const double (&col)[NM], double anorm, int lane) {
@@ -1079,9 +1125,12 @@ __device__ __forceinline__ double reg_rcond(const double (&row)[NM], const doubl
     double est = 0.0;
     int jump = 1, j = 0, iter = 0;
     double x = lane < NM ? 1.0 / NM : 0.0;
+    double rd[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) rd[k] = 1.0 / readlane_d(row[k], k);
 #pragma unroll 1
     for (int guard = 0; guard < 16; ++guard) {
-        reg_apply_inv<NM>(row, col, x, lane);
+        reg_apply_inv_rcp<NM>(row, col, rd, x, lane);
         double s = 0.0, m = -1.0;
         int im = 0;
 #pragma unroll
@@ -1273,6 +1322,7 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
             for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
             // ---- sweep(s): partial sums per lane, reduced through LDS ----
             double sf = 0.0;
+            double wes[kTailPts];
 #pragma unroll
             for (int pass = 0; pass < ((!GAUNT && RS < NM) ? 2 : 1); ++pass) {
                 const int R0 = pass == 0 ? 0 : RS, R1 = pass == 0 ? (GAUNT ? NM : RS) : NM;
@@ -1281,25 +1331,43 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                 double acc[CS::NPART];
 #pragma unroll
                 for (int k = 0; k < CS::NPART; ++k) acc[k] = 0.0;
-                for (int q = lane; q < A.nq; q += 64) {
-                    const double *row = A.p + size_t(q) * NP2;
-                    double am = 0.0;
+                // The moment values come from the transposed table (pT[k][q]):
+                // for each k the wave reads 64 consecutive doubles.  With at
+                // most kTailPts points per lane, w e of the first pass is kept
+                // for the second.
+                const bool cached = A.nq <= 64 * kTailPts;
+                for (int q0 = 0; q0 < A.nq; q0 += 64 * kTailPts) {
 #pragma unroll
-                    for (int k = 0; k < NM; ++k) am = fma(al[k], row[k], am);
-                    const double we = A.w[q] * exp(am);
-                    if (pass == 0) sf += we;
-                    if constexpr (GAUNT) {
+                    for (int qi = 0; qi < kTailPts; ++qi) {
+                        const int q = q0 + 64 * qi + lane;
+                        if (q >= A.nq) break;
+                        double row[GAUNT ? NP2 : NM];
 #pragma unroll
-                        for (int k = 0; k < NP2; ++k) acc[k] = fma(we, row[k], acc[k]);
-                    } else {
+                        for (int k = 0; k < (GAUNT ? NP2 : NM); ++k) row[k] = A.pT[size_t(k) * A.nq + q];
+                        double we;
+                        if (pass > 0 && cached) {
+                            we = wes[qi];
+                        } else {
+                            double am = 0.0;
 #pragma unroll
-                        for (int i = 0; i < NM; ++i) {
-                            if (i < R0 || i >= R1) continue;
-                            const double v = we * row[i];
+                            for (int k = 0; k < NM; ++k) am = fma(al[k], row[k], am);
+                            we = A.w[q] * exp(am);
+                            wes[qi] = we;
+                        }
+                        if (pass == 0) sf += we;
+                        if constexpr (GAUNT) {
 #pragma unroll
-                            for (int jj = 0; jj <= i; ++jj) {
-                                const int k = Tri<NM>::at(i, jj) - off;
-                                if (k >= 0 && k < CS::NPART) acc[k] = fma(v, row[jj], acc[k]);
+                            for (int k = 0; k < NP2; ++k) acc[k] = fma(we, row[k], acc[k]);
+                        } else {
+#pragma unroll
+                            for (int i = 0; i < NM; ++i) {
+                                if (i < R0 || i >= R1) continue;
+                                const double v = we * row[i];
+#pragma unroll
+                                for (int jj = 0; jj <= i; ++jj) {
+                                    const int k = Tri<NM>::at(i, jj) - off;
+                                    if (k >= 0 && k < CS::NPART) acc[k] = fma(v, row[jj], acc[k]);
+                                }
                             }
                         }
                     }
@@ -1308,10 +1376,14 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                 for (int k = 0; k < CS::NPART; ++k)
                     if (k < nacc) sh.part[k][lane] = acc[k];
                 __builtin_amdgcn_wave_barrier();
-                for (int k = lane; k < nacc; k += 64) {
-                    double tot = 0.0;
-                    for (int l2 = 0; l2 < 64; ++l2) tot += sh.part[k][l2];
-                    sh.accs[off + k] = tot;
+                for (int k = lane; k < nacc; k += 64) {       // four partial sums per column
+                    double t4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int l2 = 0; l2 < 64; l2 += 4) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) t4[u] += sh.part[k][l2 + u];
+                    }
+                    sh.accs[off + k] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
                 }
                 __builtin_amdgcn_wave_barrier();
             }
@@ -1360,7 +1432,9 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
 #pragma unroll
                         for (int jj = 0; jj < kHalvings; ++jj) fj[jj] = 0.0;
                         for (int q = lane; q < A.nq; q += 64) {
-                            const double *row = A.p + size_t(q) * NP2;
+                            double row[NM];
+#pragma unroll
+                            for (int k = 0; k < NM; ++k) row[k] = A.pT[size_t(k) * A.nq + q];
                             const double wq = A.w[q];
 #pragma unroll
                             for (int jj = 0; jj < kHalvings; ++jj) {
@@ -1968,31 +2042,33 @@ void launch_tail(const NewtonArgs &a, int lin, int lin2, int tail_blocks, int ta
         hipLaunchKernelGGL((tail_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin, lin2);
 }
 
+// Rounds r0..r1-1 of the lockstep solve (round r reads list r & 1), with the
+// prologue first if `init` and the persistent tail last if `tail`.
 template <int NM, int NP2, bool G, int RS>
-void launch_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode, hipStream_t s) {
+void launch_all(const NewtonArgs &a, const LaunchPlan &lp, hipStream_t s) {
     const int nb = (a.ncells + kBlock - 1) / kBlock;
-    if (!a.skip_init) hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
-    int lin = 0;
-    for (int r = 0; r < rounds; ++r) {
+    if (lp.init) hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
+    int lin = lp.r0 & 1;
+    for (int r = lp.r0; r < lp.r1; ++r) {
         hipLaunchKernelGGL((sweep_kernel<NM, NP2, G, RS>), dim3(nb), dim3(kBlock), 0, s, a, lin);
         hipLaunchKernelGGL((logic_kernel<NM, NP2, G>), dim3(nb), dim3(kBlock), 0, s, a, lin, 2);
         lin ^= 1;
     }
-    launch_tail<NM, NP2, G, RS>(a, lin, 2, tail_blocks, tail_mode, s);   // list 2 is empty here
+    if (lp.tail) launch_tail<NM, NP2, G, RS>(a, lin, 2, lp.tail_blocks, lp.tail_mode, s);   // list 2 is empty here
 }
 
 template <int NM, int NP2, bool G, int RS>
-void launch_mixed_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail_mode, int mix_blocks,
-                      int defer_rounds, hipStream_t s) {
+void launch_mixed_all(const NewtonArgs &a, const LaunchPlan &lp, hipStream_t s) {
     constexpr int NNT = mix_cols(NM) / 16;
     const int nb = (a.ncells + kBlock - 1) / kBlock;
-    if (!a.skip_init) hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
+    const int mix_blocks = lp.mix_blocks, defer_rounds = lp.defer_rounds;
+    if (lp.init) hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
     // Mixed-precision cells on lists 0/1.  A cell whose step must be redone in
     // FP64 either waits on list 2 for the FP64 tail (defer_rounds 0) or moves
     // to the FP64 pair (lists 2/3), which the same rounds sweep with the FP64
     // lane kernels (defer_rounds 1); what is left after the rounds goes to the tail.
-    int lin = 0, flin = 2;
-    for (int r = 0; r < rounds; ++r) {
+    int lin = lp.r0 & 1, flin = 2 + (defer_rounds ? (lp.r0 & 1) : 0);
+    for (int r = lp.r0; r < lp.r1; ++r) {
         hipLaunchKernelGGL((sweep_mix_kernel<NM, NP2, NNT>), dim3(mix_blocks), dim3(64 * kMixWaves), 0, s, a, lin);
         if (defer_rounds) {
             hipLaunchKernelGGL((sweep_kernel<NM, NP2, G, RS>), dim3(nb), dim3(kBlock), 0, s, a, flin);
@@ -2004,7 +2080,7 @@ void launch_mixed_all(const NewtonArgs &a, int rounds, int tail_blocks, int tail
         }
         lin ^= 1;
     }
-    launch_tail<NM, NP2, G, RS>(a, lin, flin, tail_blocks, tail_mode, s);
+    if (lp.tail) launch_tail<NM, NP2, G, RS>(a, lin, flin, lp.tail_blocks, lp.tail_mode, s);
 }
 
 }  // namespace
@@ -2020,16 +2096,15 @@ void launch_ctl_reset(int *ctl, hipStream_t s) { hipLaunchKernelGGL(ctl_reset_ke
 void newton_store_fields(int nm, int np2, int *dfields, int *ifields) {
     const int tri = nm * (nm + 1) / 2;
     *dfields = 5 * nm + 9 + cmax(cmax(np2, tri), mix_cols(nm));
-    *ifields = 5;
+    *ifields = 6;
 }
 
-bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
-                        int tail_mode, hipStream_t s) {
+bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, const LaunchPlan &lp, hipStream_t s) {
 #define MNC_CASE(NMV, NP2D, NP2G, RSPLIT)                                                             \
     if (nm == NMV) {                                                                                   \
         static_assert(Store<NMV, NP2G>::dfields == 5 * NMV + 9 + Store<NMV, NP2G>::nacc, "store");     \
-        if (gaunt && np2 == NP2G) { launch_all<NMV, NP2G, true, NMV>(a, rounds, tail_blocks, tail_mode, s); return true; } \
-        if (!gaunt && np2 == NP2D) { launch_all<NMV, NP2D, false, RSPLIT>(a, rounds, tail_blocks, tail_mode, s); return true; } \
+        if (gaunt && np2 == NP2G) { launch_all<NMV, NP2G, true, NMV>(a, lp, s); return true; }             \
+        if (!gaunt && np2 == NP2D) { launch_all<NMV, NP2D, false, RSPLIT>(a, lp, s); return true; }        \
         return false;                                                                                  \
     }
     MNC_CASE(1, 1, 1, 1)      // M0
@@ -2041,13 +2116,12 @@ bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, int ro
     return false;
 }
 
-bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, int rounds, int tail_blocks,
-                         int tail_mode, int mix_blocks, int defer_rounds, hipStream_t s) {
+bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, const LaunchPlan &lp, hipStream_t s) {
 #define MNC_CASE(NMV, NP2D, NP2G, RSPLIT)                                                                \
     if (nm == NMV) {                                                                                      \
         if (a.mnt < 2 || a.mnt > kMixMaxTiles || (a.mnt & 1)) return false;                              \
-        if (gaunt && np2 == NP2G) { launch_mixed_all<NMV, NP2G, true, NMV>(a, rounds, tail_blocks, tail_mode, mix_blocks, defer_rounds, s); return true; } \
-        if (!gaunt && np2 == NP2D) { launch_mixed_all<NMV, NP2D, false, RSPLIT>(a, rounds, tail_blocks, tail_mode, mix_blocks, defer_rounds, s); return true; } \
+        if (gaunt && np2 == NP2G) { launch_mixed_all<NMV, NP2G, true, NMV>(a, lp, s); return true; }         \
+        if (!gaunt && np2 == NP2D) { launch_mixed_all<NMV, NP2D, false, RSPLIT>(a, lp, s); return true; }    \
         return false;                                                                                     \
     }
     MNC_CASE(1, 1, 1, 1)
@@ -2082,19 +2156,11 @@ bool launch_newton_g64_rounds(const NewtonArgs &a, int nm, int np2, int &lin, in
     return true;
 }
 
-bool launch_newton_init(const NewtonArgs &a, int nm, int np2, bool gaunt, hipStream_t s) {
-    const int nb = (a.ncells + kBlock - 1) / kBlock;
-    if (nm == 15 && !gaunt && np2 == 15) { hipLaunchKernelGGL((init_kernel<15, 15>), dim3(nb), dim3(kBlock), 0, s, a); return true; }
-    if (nm == 15 && gaunt && np2 == 45) { hipLaunchKernelGGL((init_kernel<15, 45>), dim3(nb), dim3(kBlock), 0, s, a); return true; }
-    return false;
-}
-
 bool launch_newton_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, hipStream_t s) {
     const int nb = (a.ncells + kBlock - 1) / kBlock;
     if (nm == 15 && !gaunt && np2 == 15) hipLaunchKernelGGL((judge_kernel<15, 15>), dim3(nb), dim3(kBlock), 0, s, a);
     else if (nm == 15 && gaunt && np2 == 45) hipLaunchKernelGGL((judge_kernel<15, 45>), dim3(nb), dim3(kBlock), 0, s, a);
     else return false;
-    hipLaunchKernelGGL(ctl_clear_kernel, dim3(1), dim3(64), 0, s, a.ctl, 4);
     return true;
 }
 

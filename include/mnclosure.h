@@ -180,6 +180,9 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *                  FP64 tail (default); 1: it joins FP64 lockstep rounds
  *                  that run beside the mixed-precision ones
  *   "mix_blocks"   workgroups of the mixed-precision sweep (1-65536)
+ *   "learned_gate_log10"  MNC_WARM_LEARNED: the network is consulted for a
+ *                  cell only when the gradient norm at its start exceeds
+ *                  10^value ||u|| (scaled problem; -16..4, default -2)
  *   "flux_tile"    0: ansatz written to HBM, then the flux; 1: ansatz fused
  *                  into the flux with 8x8-cell LDS tiles (default); 2: 16x16
  *                  tiles (n <= 15).  The results are identical.
@@ -196,8 +199,9 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value);
 /* Autotuner (no counterpart in the reference): times `steps` mnc_update steps
  * of dt from the current grid state for each candidate launch shape, one
  * coordinate-descent pass over "graph", "flux_tile", "rounds", "tail_mode",
- * "tail_blocks" and (mixed precision) "mix_cond_log10", "mix_defer" and
- * "mix_blocks", and keeps the fastest.  Every candidate converges to the
+ * "tail_blocks", (learned warm start) "learned_gate_log10" and (mixed
+ * precision) "mix_cond_log10", "mix_defer" and "mix_blocks", and keeps the
+ * fastest.  Every candidate converges to the
  * same tolerance; "mix_cond_log10" trades mixed-precision steps against
  * FP64 ones.  The grid state
  * and the statistics are restored after every trial, so the call leaves the
