@@ -122,6 +122,19 @@ def train_mlp(mnc, gi, d, pairs_target=2_000_000, train_steps=2000, seed=1):
     return params, info
 
 
+def stage_traffic(form):
+    """HBM bytes of one Newton stage of `form` from the committed counter
+    summary (profiles/newton_stage_traffic.md), or None."""
+    try:
+        for line in open(os.path.join(ROOT, "profiles", "newton_stage_traffic.md")):
+            w = line.split()
+            if len(w) == 3 and w[0] == "traffic_bytes_per_stage" and w[1] == form:
+                return float(w[2])
+    except OSError:
+        pass
+    return None
+
+
 def progress(msg):
     print("[bench %.1fs] %s" % (time.perf_counter() - T0, msg), file=sys.stderr, flush=True)
 
@@ -328,6 +341,48 @@ def cpu_baseline(d, seconds):
                 iters=st.iter_sum / max(st.num_dual_solves, 1))
 
 
+def stage_probe(mnc, gi, d, form, steps):
+    """One closure solve of every lattice cell (mnc_solve_grid) on the state
+    after `steps` plain steps, in the given form with the default launch shape:
+    the unit whose counters the HBM-traffic figure of the roofline sums."""
+    ctx, g = make_grid(mnc, gi, d)
+    dts = gi.time_steps(g.max_dt, T_FINAL)
+    plain = mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"], line_search=mnc.LS_ROBUST)
+    for k in range(steps):
+        ctx.update(dts[k], plain)
+    kw = {"plain-fp64": {}, "mixed": {"precision": mnc.PREC_MIXED},
+          "learned+mixed": {"precision": mnc.PREC_MIXED, "warm_start": mnc.WARM_LEARNED}}[form]
+    if form == "learned+mixed":
+        params, _ = train_mlp(mnc, gi, d, train_steps=300)
+        ctx.mlp_set_params(params)
+    ctx.ghost_fill()
+    ctx.sync()
+    progress("probe solve (%s) after %d steps" % (form, steps))
+    ctx.solve_grid(mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"],
+                               line_search=mnc.LS_ROBUST, **kw))
+    ctx.sync()
+    st = ctx.stats()
+    ctx.close()
+    progress("probe done: %d closures" % st.num_dual_solves)
+
+
+def copy_probe(mib):
+    """A device-to-device copy of `mib` MiB (the runtime's copy kernel): the
+    calibration of the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    progress("copied %d MiB device to device" % mib)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,6 +399,11 @@ def main():
     ap.add_argument("--m8-steps", type=int, default=20, help="steps of the M8 512^2 workload (0 skips it)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="force a launch/tuning setting after the autotuner (repeatable)")
+    ap.add_argument("--stage-probe", choices=["plain-fp64", "mixed", "learned+mixed"], default=None,
+                    help="only one closure solve of the lattice at --probe-step (for counter runs)")
+    ap.add_argument("--probe-step", type=int, default=284)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only a device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build's hot path runs on a single MI355X "
@@ -352,8 +412,14 @@ def main():
     for kv in args.tune:
         k, v = kv.split("=")
         FORCE_TUNE[k] = int(v)
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
+        return
     mnc, gi = _load("mnc"), _load("grid_init")
     d = dict(LATTICE)
+    if args.stage_probe:
+        stage_probe(mnc, gi, d, args.stage_probe, args.probe_step)
+        return
     n = (d["moment_order"] + 1) * (d["moment_order"] + 2) // 2
     nq = d["quad_order"] ** 2
     f_iter, f_trial = flops_per_iter(n, nq)
@@ -384,6 +450,7 @@ def main():
     value = best["closures"] / best["wall"]
     kn, kms = best["newton"]
     kst = best["kstats"]
+    stages = 2 * best["kernel_steps"]                 # two closure solves per Heun step
     alg_flops = kst.iter_sum * f_iter + kst.trial_sum * f_trial
     achieved = alg_flops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
     # drift of each form's final state from the plain form's (density rel-L2)
@@ -413,12 +480,14 @@ def main():
                    "final_density_rel_l2_vs_plain": drift,
                    "mlp": mlp_info},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": stage_traffic(form),
                      "kernel": "Newton solve per stage (init + lockstep rounds + tail), %s form, HIP events "
                                "over the first %d steps; flops are the reference algorithm's FP64 count "
-                               "(iterations x 73.0 kflop + trials x 8.2 kflop)" % (form, best["kernel_steps"]),
-                     "algorithmic_flops_per_launch": alg_flops / max(kn, 1),
-                     "avg_launch_ms": kms / max(kn, 1)},
+                               "(iterations x 73.0 kflop + trials x 8.2 kflop); traffic: HBM bytes per stage from "
+                               "calibrated FETCH_SIZE + WRITE_SIZE (profiles/newton_stage_traffic.md)"
+                               % (form, best["kernel_steps"]),
+                     "algorithmic_flops_per_launch": alg_flops / max(stages, 1),
+                     "avg_launch_ms": kms / max(stages, 1)},
     }
     if args.isolated > 0 and not args.plain_only and not args.only:
         out["config"]["isolated"] = bench_isolated(mnc, args.isolated, 12345, 20000, args.train_steps)
