@@ -47,13 +47,14 @@ struct mnc_ctx {
     int mix_cond_log10 = 5;          // mixed steps need cond(H) <= 10^this, else FP64
     int mix_defer = 0;               // 0: redone steps wait for the FP64 tail; 1: FP64 lockstep rounds
     int mix_blocks = 0;              // 0: one per CU
-    int flux_tile = 1;               // 0: ansatz to HBM + flux; 1: fused, 8x8 tiles; 2: fused, 16x16
+    int flux_tile = 3;               // 0: ansatz to HBM + flux; 1: fused, 8x8 tiles; 2: fused, 16x16;
+                                     // 3: fused, 16x16, directions split into kFluxGroups
     int learned_gate_log10 = -2;     // the network is consulted when ||g(best start)|| > 10^this ||u||
     // grid
     bool has_grid = false;
     mnc_grid_desc grid{};
     double *d_mom = nullptr, *d_alpha = nullptr, *d_old = nullptr, *d_st = nullptr, *d_ss = nullptr,
-           *d_flux = nullptr, *d_ansatz = nullptr;
+           *d_flux = nullptr, *d_ansatz = nullptr, *d_fpart = nullptr;
     int ansatz_chunk = 0;
     // batch staging for host entry points
     double *d_bu = nullptr, *d_ba = nullptr;
@@ -395,6 +396,7 @@ mnc::GridArgs grid_args(mnc_ctx *ctx) {
     g.sigma_s = ctx->d_ss;
     g.flux = ctx->d_flux;
     g.ansatz = ctx->d_ansatz;
+    g.fpart = ctx->d_fpart;
     return g;
 }
 
@@ -502,7 +504,7 @@ void mnc_destroy(mnc_ctx *ctx) {
                     ctx->d_mz, ctx->d_mg, ctx->d_mh, ctx->d_mgptr, ctx->d_mgk, ctx->d_mgv, ctx->d_mpv, ctx->d_gz, ctx->d_gg, ctx->d_gh,
                     ctx->d_mlp, ctx->d_mlp_scratch, ctx->d_mlp_loss, ctx->d_mlp_io, ctx->d_us, ctx->d_guess,
                     ctx->d_ctl, ctx->d_stats, ctx->d_std, ctx->d_sti, ctx->d_lists, ctx->d_mom, ctx->d_alpha, ctx->d_old,
-                    ctx->d_st, ctx->d_ss, ctx->d_flux, ctx->d_ansatz, ctx->d_bu, ctx->d_ba, ctx->d_bo};
+                    ctx->d_st, ctx->d_ss, ctx->d_flux, ctx->d_ansatz, ctx->d_fpart, ctx->d_bu, ctx->d_ba, ctx->d_bo};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -628,7 +630,9 @@ int mnc_grid_create(mnc_ctx *ctx, const mnc_grid_desc *grid, const double *momen
     int st;
     if ((st = dalloc(ctx, &ctx->d_mom, n)) || (st = dalloc(ctx, &ctx->d_alpha, n)) ||
         (st = dalloc(ctx, &ctx->d_old, n)) || (st = dalloc(ctx, &ctx->d_flux, n)) ||
-        (st = dalloc(ctx, &ctx->d_st, nc)) || (st = dalloc(ctx, &ctx->d_ss, nc)))
+        (st = dalloc(ctx, &ctx->d_st, nc)) || (st = dalloc(ctx, &ctx->d_ss, nc)) ||
+        (st = dalloc(ctx, &ctx->d_fpart, size_t(mnc::kFluxGroups) * grid->num_cells_x * grid->num_cells_y *
+                                             ctx->tab.nm)))
         return st;
     // ansatz buffer: all q at once unless that exceeds 8 GiB
     const size_t max_elems = (size_t(8) << 30) / sizeof(double);
@@ -888,7 +892,7 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         return MNC_OK;
     }
     if (std::strcmp(key, "flux_tile") == 0) {
-        if (value < 0 || value > 2) return ctx_fail(ctx, MNC_ERR_ARG, "flux_tile must be 0, 1 or 2");
+        if (value < 0 || value > 3) return ctx_fail(ctx, MNC_ERR_ARG, "flux_tile must be 0, 1, 2 or 3");
         ctx->flux_tile = value;
         return MNC_OK;
     }
@@ -967,7 +971,7 @@ int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, cha
     const int cu = ctx->num_cu;
     std::vector<Knob> knobs = {
         {"graph", {0, 1}, &ctx->use_graph},
-        {"flux_tile", {0, 1, 2}, &ctx->flux_tile},
+        {"flux_tile", {0, 1, 2, 3}, &ctx->flux_tile},
         {"rounds", {2, 4, 6, 8, 12}, &ctx->rounds},
         {"tail_mode", {0, 1, 2}, &ctx->tail_mode},
         {"tail_blocks", {cu, 2 * cu, 4 * cu}, &ctx->tail_blocks},

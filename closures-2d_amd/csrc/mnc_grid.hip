@@ -24,6 +24,37 @@ __device__ __forceinline__ double slopefit(double l, double c, double r, double 
     return minmod(theta * (r - c), minmod(0.5 * (r - l), theta * (c - l)));
 }
 
+// The Euler stage of interior cell (i, j) = c with flux F (:316-336 / :344-364),
+// the lattice source (:328-334) and, on the second stage, the Heun average
+// (:368-377); stage 0 stores the flux only.
+template <int NM>
+__device__ __forceinline__ void euler_epilogue(const GridArgs &g, int i, int j, size_t c, const double (&F)[NM],
+                                               int stage, double dt) {
+    if (stage == 0) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) g.flux[c * NM + k] = F[k];
+        return;
+    }
+    double *u = g.moments + c * NM;
+    const double st = g.sigma_t[c], ss = g.sigma_s[c];
+    double v[NM];
+    v[0] = u[0] * (1.0 + dt * ss - dt * st) - dt * F[0];
+#pragma unroll
+    for (int k = 1; k < NM; ++k) v[k] = u[k] * (1.0 - dt * st) - dt * F[k];
+    if (g.init_cond == 1) {
+        const double x_i = g.init_x + (i - 2) * g.dx;
+        const double y_j = g.init_y + (j - 2) * g.dy;
+        if (fabs(x_i) < 0.5 && fabs(y_j) < 0.5) v[0] = v[0] + dt * 2.0 * sqrt(M_PI);
+    }
+    if (stage == 2) {
+        const double *uo = g.moments_old + c * NM;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) v[k] = 0.5 * (v[k] + uo[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) u[k] = v[k];
+}
+
 // E[q][cell] = exp(alpha_cell . m_q) for q in [q0, q0+nqc)  (computeAnsatz, :38-53)
 template <int NM>
 __global__ __launch_bounds__(kBlock) void ansatz_kernel(GridArgs g, int q0, int nqc) {
@@ -257,8 +288,126 @@ void launch_flux_t(const GridArgs &g, int q0, int nqc, int stage, double dt, boo
                        stage, dt, int(fc), int(lc));
 }
 
+// The fused ansatz + flux with the directions split into gridDim.z groups of
+// consecutive q: block (tile, z) sums the flux of its TX x TY cells over group
+// z's directions into fpart[z] (interior cells, [cell][k]), and
+// flux_sum_kernel adds the groups in order and applies the Euler stage.  With
+// 80,656 cells a launch of one group per tile is about one wave per SIMD and
+// waits on latency; the groups multiply the waves in flight.  alpha of the
+// halo cells a thread evaluates stays in its registers, so LDS holds only E.
+// Per cell and direction the arithmetic is flux_fused_kernel's except that
+// (f1 + f2) / dx + (f3 + f4) / dy multiplies by reciprocals, and the q sum is
+// split into the groups: the result moves at rounding level.
+template <int NM, int TX, int TY, int QC>
+__global__ __launch_bounds__(TX * TY) void flux_split_kernel(GridArgs g) {
+    constexpr int HX = TX + 4, HY = TY + 4, NH = HX * HY, NT = TX * TY;
+    constexpr int HPT = (NH + NT - 1) / NT;                  // halo cells evaluated per thread
+    __shared__ double E_s[QC][NH];
+    const int tid = threadIdx.x;
+    const int tx0 = blockIdx.x * TX, ty0 = blockIdx.y * TY;
+    const int NX = g.NX, NY = g.NY;
+    double al[HPT][NM];
+#pragma unroll
+    for (int hh = 0; hh < HPT; ++hh) {
+        const int h = tid + hh * NT;
+        const int gi = tx0 + h / HY, gj = ty0 + h % HY;
+        const bool in = h < NH && gi < NX && gj < NY;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) al[hh][k] = in ? g.alpha[(size_t(gi) * NY + gj) * NM + k] : 0.0;
+    }
+    const int li = tid / TY, lj = tid % TY;
+    const int i = 2 + tx0 + li, j = 2 + ty0 + lj;
+    const bool active = (tx0 + li < g.nx) && (ty0 + lj < g.ny);
+    const int hc = (li + 2) * HY + (lj + 2);
+    const double theta = g.theta, rdx = 1.0 / g.dx, rdy = 1.0 / g.dy;
+    const int per = (g.nq + gridDim.z - 1) / gridDim.z;
+    const int qa = blockIdx.z * per, qb = min(g.nq, qa + per);
+    double F[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) F[k] = 0.0;
+    for (int q0 = qa; q0 < qb; q0 += QC) {
+        const int nqc = min(QC, qb - q0);
+        __syncthreads();
+        for (int qq = 0; qq < nqc; ++qq) {
+            const double *row = g.p + size_t(q0 + qq) * g.np2;
+#pragma unroll
+            for (int hh = 0; hh < HPT; ++hh) {
+                const int h = tid + hh * NT;
+                if (h < NH) {
+                    double sa = 0.0;
+#pragma unroll
+                    for (int k = 0; k < NM; ++k) sa = fma(al[hh][k], row[k], sa);
+                    E_s[qq][h] = exp(sa);
+                }
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
+        for (int qq = 0; qq < nqc; ++qq) {
+            const int q = q0 + qq;
+            const double *E = E_s[qq];
+            const double xi = g.xi[q], eta = g.eta[q], w = g.w[q];
+            double f1 = 0.0, f2 = 0.0, f3 = 0.0, f4 = 0.0;
+            if (xi > 0) {
+                const double k1 = E[hc - 2 * HY], k2 = E[hc - HY], k3 = E[hc], k4 = E[hc + HY];
+                f1 = k3 + 0.5 * slopefit(k2, k3, k4, theta) - k2 - 0.5 * slopefit(k1, k2, k3, theta);
+                f1 = f1 * xi * w;
+            }
+            if (xi < 0) {
+                const double k1 = E[hc - HY], k2 = E[hc], k3 = E[hc + HY], k4 = E[hc + 2 * HY];
+                f2 = k3 - 0.5 * slopefit(k2, k3, k4, theta) - k2 + 0.5 * slopefit(k1, k2, k3, theta);
+                f2 = f2 * xi * w;
+            }
+            if (eta > 0) {
+                const double k1 = E[hc - 2], k2 = E[hc - 1], k3 = E[hc], k4 = E[hc + 1];
+                f3 = k3 + 0.5 * slopefit(k2, k3, k4, theta) - k2 - 0.5 * slopefit(k1, k2, k3, theta);
+                f3 = f3 * eta * w;
+            }
+            if (eta < 0) {
+                const double k1 = E[hc - 1], k2 = E[hc], k3 = E[hc + 1], k4 = E[hc + 2];
+                f4 = k3 - 0.5 * slopefit(k2, k3, k4, theta) - k2 + 0.5 * slopefit(k1, k2, k3, theta);
+                f4 = f4 * eta * w;
+            }
+            const double comb = (f1 + f2) * rdx + (f3 + f4) * rdy;
+            const double *row = g.p + size_t(q) * g.np2;
+#pragma unroll
+            for (int k = 0; k < NM; ++k) F[k] = fma(row[k], comb, F[k]);
+        }
+    }
+    if (!active) return;
+    double *o = g.fpart + (size_t(blockIdx.z) * g.nx * g.ny + size_t(i - 2) * g.ny + (j - 2)) * NM;
+#pragma unroll
+    for (int k = 0; k < NM; ++k) o[k] = F[k];
+}
+
+// F = fpart[0] + fpart[1] + ... (in group order), then the Euler stage.
+template <int NM>
+__global__ __launch_bounds__(kBlock) void flux_sum_kernel(GridArgs g, int groups, int stage, double dt) {
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= g.nx * g.ny) return;
+    const int i = 2 + idx / g.ny, j = 2 + idx % g.ny;
+    double F[NM];
+#pragma unroll
+    for (int k = 0; k < NM; ++k) F[k] = g.fpart[size_t(idx) * NM + k];
+    for (int z = 1; z < groups; ++z) {
+        const double *p = g.fpart + (size_t(z) * g.nx * g.ny + idx) * NM;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) F[k] += p[k];
+    }
+    euler_epilogue<NM>(g, i, j, size_t(i) * g.NY + j, F, stage, dt);
+}
+
 template <int NM>
 void launch_fused_t(const GridArgs &g, int tile, int stage, double dt, hipStream_t s) {
+    if constexpr (NM <= 15) {   // alpha of two halo cells per thread in registers
+        if (tile == 3) {
+            hipLaunchKernelGGL((flux_split_kernel<NM, 16, 16, 8>),
+                               dim3((g.nx + 15) / 16, (g.ny + 15) / 16, kFluxGroups), dim3(256), 0, s, g);
+            hipLaunchKernelGGL((flux_sum_kernel<NM>), dim3((g.nx * g.ny + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                               g, kFluxGroups, stage, dt);
+            return;
+        }
+    }
     if constexpr (NM <= 15) {   // alpha of a 20 x 20 halo'd tile fits LDS up to n = 15
         if (tile == 2) {
             hipLaunchKernelGGL((flux_fused_kernel<NM, 16, 16, 8>), dim3((g.nx + 15) / 16, (g.ny + 15) / 16), dim3(256),

@@ -184,8 +184,11 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *                  cell only when the gradient norm at its start exceeds
  *                  10^value ||u|| (scaled problem; -16..4, default -2)
  *   "flux_tile"    0: ansatz written to HBM, then the flux; 1: ansatz fused
- *                  into the flux with 8x8-cell LDS tiles (default); 2: 16x16
- *                  tiles (n <= 15).  The results are identical.
+ *                  into the flux with 8x8-cell LDS tiles; 2: 16x16 tiles
+ *                  (n <= 15); these three give identical results.  3 (default;
+ *                  n <= 15, else as 1): 16x16 tiles with the directions split
+ *                  into four groups summed in order and reciprocals of dx, dy
+ *                  (rounding-level differences from 0-2).
  *   "graph"        1: mnc_update steps are captured into hipGraphs (one per
  *                  distinct dt and options, captured on the second step with
  *                  that key) and replayed; 0: eager launches (default).  While
