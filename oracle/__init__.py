@@ -21,10 +21,14 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 
+MOMENT_MN, MOMENT_PPN = 0, 1
+
+
 class Options(C.Structure):
-    """OPTIONS (opt.h:11-22), the fields the Mn path reads."""
+    """OPTIONS (opt.h:11-22), the fields the opt path reads."""
     _fields_ = [("max_iter", C.c_int), ("tol_rel", C.c_double),
-                ("cond_h_max", C.c_double), ("use_gaunt", C.c_int), ("line_search", C.c_int)]
+                ("cond_h_max", C.c_double), ("use_gaunt", C.c_int), ("line_search", C.c_int),
+                ("moment_type", C.c_int), ("delta_ppn", C.c_double)]
 
 
 class Outs(C.Structure):
@@ -41,7 +45,8 @@ class Deck(C.Structure):
                 ("moment_order", C.c_int), ("quad_order", C.c_int),
                 ("cfl_factor", C.c_double), ("tol", C.c_double),
                 ("cond_h_max", C.c_double), ("max_iter", C.c_int),
-                ("use_gaunt", C.c_int), ("theta", C.c_double), ("line_search", C.c_int)]
+                ("use_gaunt", C.c_int), ("theta", C.c_double), ("line_search", C.c_int),
+                ("moment_type", C.c_int), ("delta_ppn", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -72,6 +77,8 @@ def lib():
         L.orc_tables_get.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_tables_gaunt_get.argtypes = [C.c_void_p, _ip, _ip, _dp]
         L.orc_fobj.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int]
+        L.orc_fobj_ppn.argtypes = [C.c_void_p, _dp, _dp, C.c_double, _dp, _dp, C.c_int]
+        L.orc_fobj_ppn.restype = C.c_double
         L.orc_fobj.restype = C.c_double
         L.orc_cholesky.argtypes = [C.c_int, _dp]
         L.orc_rcond.argtypes = [C.c_int, _dp, C.c_double]
@@ -157,23 +164,34 @@ class Tables:
                            _p(h) if h is not None else None, int(ug))
         return f, g, h
 
-    def opt(self, u, alpha, max_iter=100, tol=1e-8, cond_h_max=1e10, use_gaunt=None, line_search=0):
+    def fobj_ppn(self, alpha, u, delta, want_g=True, want_h=True, use_gaunt=None):
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        g = np.zeros(self.nm) if want_g else None
+        h = np.zeros((self.nm, self.nm)) if want_h else None
+        ug = self.use_gaunt if use_gaunt is None else bool(use_gaunt)
+        f = lib().orc_fobj_ppn(self.h, _p(alpha), _p(u), delta, _p(g) if g is not None else None,
+                               _p(h) if h is not None else None, int(ug))
+        return f, g, h
+
+    def opt(self, u, alpha, max_iter=100, tol=1e-8, cond_h_max=1e10, use_gaunt=None, line_search=0,
+            moment_type=MOMENT_MN, delta_ppn=0.0):
         """One opt() call (opt.cpp:82-148); returns (status, u_out, alpha_out, outs)."""
         u = np.array(u, dtype=np.float64)
         alpha = np.array(alpha, dtype=np.float64)
         o = Options(max_iter, tol, cond_h_max, int(self.use_gaunt if use_gaunt is None else use_gaunt),
-                    line_search)
+                    line_search, moment_type, delta_ppn)
         outs = Outs()
         st = lib().orc_opt(self.h, _p(u), _p(alpha), C.byref(o), C.byref(outs))
         return st, u, alpha, outs
 
     def opt_batch(self, u, alpha, max_iter=100, tol=1e-8, cond_h_max=1e10, use_gaunt=None,
-                  nthreads=0, line_search=0):
+                  nthreads=0, line_search=0, moment_type=MOMENT_MN, delta_ppn=0.0):
         u = np.array(u, dtype=np.float64, order="C")
         alpha = np.array(alpha, dtype=np.float64, order="C")
         n = u.shape[0]
         o = Options(max_iter, tol, cond_h_max, int(self.use_gaunt if use_gaunt is None else use_gaunt),
-                    line_search)
+                    line_search, moment_type, delta_ppn)
         outs = (Outs * n)()
         st = lib().orc_opt_batch(self.h, n, _p(u), _p(alpha), C.byref(o), outs, nthreads)
         it = np.array([o_.iter for o_ in outs], dtype=np.int32)
@@ -266,7 +284,8 @@ class Solver:
 DECK_DEFAULTS = dict(num_cells_x=50, num_cells_y=50, a_x=-1.5, b_x=1.5, a_y=-1.5, b_y=1.5,
                      gaussian_sigma=0.4, floor_value=1e-4, sigma=1.0, init_cond=0,
                      moment_order=3, quad_order=30, cfl_factor=0.9, tol=1e-4,
-                     cond_h_max=1e10, max_iter=100, use_gaunt=1, theta=2.0, line_search=0)
+                     cond_h_max=1e10, max_iter=100, use_gaunt=1, theta=2.0, line_search=0,
+                     moment_type=MOMENT_MN, delta_ppn=1e-10)
 
 
 def deck_from_dict(d):

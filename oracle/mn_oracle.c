@@ -415,6 +415,98 @@ double orc_fobj(const orc_tables *t, const double *alpha, const double *u,
 }
 
 /* ------------------------------------------------------------------ */
+/* fobjPPn / solveHPPn (fobj.cpp:137-236,355-428).                       */
+/* ------------------------------------------------------------------ */
+/* The PPn ansatz at t = (alpha.m)/2: G = t + sqrt(t^2 + delta), taken as
+ * -delta / (t - sqrt(t^2 + delta)) for t < 0 (no cancellation); s is the
+ * square root. */
+static double ppn_ansatz(double t, double delta, double *s)
+{
+    *s = sqrt(t * t + delta);
+    double G = t + *s;
+    if (t < 0) G = -delta / (t - sqrt(t * t + delta));
+    return G;
+}
+
+double orc_fobj_ppn(const orc_tables *t, const double *alpha, const double *u, double delta,
+                    double *g, double *h, int use_gaunt)
+{
+    int nm = t->nm, np2 = t->np2, nq = t->nq;
+    const double *w = t->w, *p = t->p;
+    double f = 0;
+    if (g != NULL)
+        for (int i = 0; i < nm; i++) g[i] = 0.0;
+    for (int q = 0; q < nq; q++) {                         /* :370-396 */
+        double tq = 0;
+        for (int i = 0; i < nm; i++)
+            tq = tq + alpha[i] * p[q * np2 + i];
+        tq = 0.5 * tq;
+        double sq;
+        double G = ppn_ansatz(tq, delta, &sq);
+        f = f + w[q] * (tq * G - 0.5 * delta + delta * log(G));
+        if (g != NULL) {
+            double wg = w[q] * G;
+            for (int i = 0; i < nm; i++)
+                g[i] += p[q * np2 + i] * wg;
+        }
+    }
+    if (h != NULL) {                                       /* solveHPPn, :137-236 */
+        double gext_stack[256];
+        double *gext = (np2 <= 256) ? gext_stack : (double *)malloc(sizeof(double) * np2);
+        for (int k = 0; k < np2; k++) gext[k] = 0.0;
+        for (int i = 0; i < nm * nm; i++) h[i] = 0.0;
+        for (int q = 0; q < nq; q++) {
+            double tq = 0;
+            for (int i = 0; i < nm; i++)
+                tq = tq + alpha[i] * p[q * np2 + i];
+            tq = 0.5 * tq;
+            double sq;
+            double G = ppn_ansatz(tq, delta, &sq);
+            double wh = 0.5 * w[q] * (G / sq);
+            if (use_gaunt) {
+                for (int k = 0; k < np2; k++)
+                    gext[k] += p[q * np2 + k] * wh;
+            } else {
+                for (int i = 0; i < nm; i++) {
+                    double pi = p[q * np2 + i];
+                    for (int j = 0; j <= i; j++)
+                        h[i * nm + j] += pi * p[q * np2 + j] * wh;
+                }
+            }
+        }
+        if (use_gaunt) {
+            for (int i = 0; i < nm; i++)
+                for (int j = i; j < nm; j++) {
+                    double hij = 0.0;
+                    int idx = i * nm + j;
+                    for (int c = t->gptr[idx]; c < t->gptr[idx + 1]; c++)
+                        hij += t->gv[c] * gext[t->gk[c]];
+                    h[i * nm + j] = h[j * nm + i] = hij;
+                }
+        } else {
+            for (int i = 0; i < nm; i++)
+                for (int j = i + 1; j < nm; j++)
+                    h[i * nm + j] = h[j * nm + i];
+        }
+        if (gext != gext_stack) free(gext);
+    }
+    for (int i = 0; i < nm; i++) {                         /* :414-421 */
+        f = f - alpha[i] * u[i];
+        if (g != NULL) g[i] = g[i] - u[i];
+    }
+    return f;
+}
+
+/* f (and g, H) of either moment type */
+static double fobj_any(const orc_tables *t, const double *alpha, const double *u, double *g, double *h,
+                       const orc_options *o)
+{
+    if (o->moment_type == ORC_MOMENT_PPN)
+        return orc_fobj_ppn(t, alpha, u, o->delta_ppn, g, h, o->use_gaunt);
+    return orc_fobj(t, alpha, u, g, h, o->use_gaunt);
+}
+
+/* ------------------------------------------------------------------ */
 /* Dense kernels in the roles of dpotrf / dpocon / dpotrs / dnrm2.       */
 /* Matrices are row-major n x n; the factor L is stored lower (a[i*n+j], */
 /* i >= j).  H is exactly symmetric, so the row/column-major view of the */
@@ -596,15 +688,20 @@ static double nrm2(int n, const double *x)
 /* ------------------------------------------------------------------ */
 /* isotropicAlpha / linesearch / optScaled / opt                        */
 /* ------------------------------------------------------------------ */
-static void isotropic_alpha(int n, double *alpha)          /* opt.cpp:27-46 */
+static void isotropic_alpha(int n, double *alpha, const orc_options *o)   /* opt.cpp:27-46 */
 {
-    alpha[0] = 2.0 * sqrt(M_PI) * log(0.5 / sqrt(M_PI));
+    if (o->moment_type == ORC_MOMENT_PPN)
+        alpha[0] = 1.0 - 4.0 * o->delta_ppn * M_PI;
+    else
+        alpha[0] = 2.0 * sqrt(M_PI) * log(0.5 / sqrt(M_PI));
     for (int i = 1; i < n; i++) alpha[i] = 0.0;
 }
 
 static double linesearch(const orc_tables *t, double *alpha, const double *d, double f,
-                         const double *g, const double *u, double *alpha_old, int robust, double fscale)
+                         const double *g, const double *u, double *alpha_old, const orc_options *o,
+                         double fscale)
 {                                                          /* linesearch.cpp:21-99 */
+    const int robust = o->line_search;
     int n1 = t->nm;
     double tt = 1.0;
     for (int i = 0; i < n1; i++) alpha_old[i] = alpha[i];
@@ -618,7 +715,8 @@ static double linesearch(const orc_tables *t, double *alpha, const double *d, do
     while (tt * maxFactor > DBL_EPSILON / 2) {
         for (int i = 0; i < n1; i++)
             alpha[i] = alpha_old[i] + tt * d[i];
-        double fNew = orc_fobj(t, alpha, u, NULL, NULL, 0);
+        double fNew = (o->moment_type == ORC_MOMENT_PPN) ? orc_fobj_ppn(t, alpha, u, o->delta_ppn, NULL, NULL, 0)
+                                                          : orc_fobj(t, alpha, u, NULL, NULL, 0);
         double temp = 0.0;
         for (int i = 0; i < n1; i++)
             temp += g[i] * d[i];
@@ -658,9 +756,15 @@ static int opt_scaled(const orc_tables *t, double *u, double *alpha, const orc_o
     for (;;) {
         iter = iter + 1;
         totalIter = totalIter + 1;
-        f = orc_fobj(t, alpha, u, g, h, o->use_gaunt);
-        double fscale = fabs(h[0] / (t->p[0] * t->p[0]));
-        for (int i = 0; i < nm; i++) fscale += fabs(alpha[i] * u[i]);
+        f = fobj_any(t, alpha, u, g, h, o);
+        /* the rounding scale of f: |its integral term| + sum |alpha_i u_i| */
+        double fscale = 0.0, au = 0.0;
+        for (int i = 0; i < nm; i++) {
+            fscale += fabs(alpha[i] * u[i]);
+            au += alpha[i] * u[i];
+        }
+        if (o->moment_type == ORC_MOMENT_PPN) fscale += fabs(f + au);
+        else fscale += fabs(h[0] / (t->p[0] * t->p[0]));
         memcpy(chol, h, sizeof(double) * nm * nm);
         int error = orc_cholesky(nm, chol);
         double norm = orc_norm1(nm, h);
@@ -680,7 +784,7 @@ static int opt_scaled(const orc_tables *t, double *u, double *alpha, const orc_o
                 status = ORC_ERR_MAXREG;
                 break;
             }
-            isotropic_alpha(nm, alpha0);
+            isotropic_alpha(nm, alpha0, o);
             rtol = nrm2(nm, u) * tolRel;
             memcpy(alpha, alpha0, sizeof(double) * nm);
             iter = 0;
@@ -695,15 +799,17 @@ static int opt_scaled(const orc_tables *t, double *u, double *alpha, const orc_o
         for (int i = 0; i < nm; i++) d[i] = -d[i];
         err = nrm2(nm, g);
         if (err <= rtol) {
-            double u0 = u[0] + g[0];
-            for (int i = 0; i < nm; i++) {
-                u[i] += g[i];
-                u[i] /= u0;
+            if (o->moment_type == ORC_MOMENT_MN) {             /* :296-303 */
+                double u0 = u[0] + g[0];
+                for (int i = 0; i < nm; i++) {
+                    u[i] += g[i];
+                    u[i] /= u0;
+                }
+                alpha[0] -= log(u0) * 2.0 * sqrt(M_PI);
             }
-            alpha[0] -= log(u0) * 2.0 * sqrt(M_PI);
             break;
         }
-        tt = linesearch(t, alpha, d, f, g, u, alphaOld, o->line_search, fscale);
+        tt = linesearch(t, alpha, d, f, g, u, alphaOld, o, fscale);
     }
     if (outs != NULL) {
         outs->iter = totalIter;
@@ -717,6 +823,23 @@ static int opt_scaled(const orc_tables *t, double *u, double *alpha, const orc_o
 int orc_opt(const orc_tables *t, double *u, double *alpha, const orc_options *o, orc_outs *outs)
 {                                                          /* opt.cpp:82-112 */
     int nm = t->nm;
+    if (o->moment_type == ORC_MOMENT_PPN) {                /* :113-141 */
+        double u0 = u[0];
+        for (int i = 0; i < nm; i++) {
+            alpha[i] = alpha[i] / u0;
+            u[i] = u[i] / u0;
+        }
+        orc_options os = *o;
+        os.delta_ppn = o->delta_ppn / (u0 * u0);
+        if (u0 <= 0)
+            return ORC_ERR_U0;
+        int st = opt_scaled(t, u, alpha, &os, outs);
+        for (int i = 0; i < nm; i++) {
+            alpha[i] = alpha[i] * u0;
+            u[i] = u[i] * u0;
+        }
+        return st;
+    }
     double u0 = u[0];
     double alphaScale = log(u0) * 2.0 * sqrt(M_PI);
     alpha[0] = alpha[0] - alphaScale;
@@ -878,8 +1001,11 @@ int orc_solver_create(const orc_deck *deck, int nthreads, orc_solver **out)
             s->moments[IDX3(s, i, j, 0)] = init[IDX2(s, i, j)] * 2.0 * sqrt(M_PI);
             for (int k = 1; k < s->nm; k++) s->moments[IDX3(s, i, j, k)] = 0.0;
             for (int k = 1; k < s->nm; k++) s->alpha[IDX3(s, i, j, k)] = 0.0;
-            s->alpha[IDX3(s, i, j, 0)] =
-                2.0 * sqrt(M_PI) * log(s->moments[IDX3(s, i, j, 0)] / (2.0 * sqrt(M_PI)));
+            const double m0 = s->moments[IDX3(s, i, j, 0)];
+            if (deck->moment_type == ORC_MOMENT_PPN)           /* :138-141 */
+                s->alpha[IDX3(s, i, j, 0)] = m0 - 4.0 * deck->delta_ppn * M_PI / m0;
+            else
+                s->alpha[IDX3(s, i, j, 0)] = 2.0 * sqrt(M_PI) * log(m0 / (2.0 * sqrt(M_PI)));
         }
     }
     free(init);
@@ -940,6 +1066,8 @@ int orc_solver_solve_all(orc_solver *s)
     o.cond_h_max = s->deck.cond_h_max;
     o.use_gaunt = s->deck.use_gaunt;
     o.line_search = s->deck.line_search;
+    o.moment_type = s->deck.moment_type;
+    o.delta_ppn = s->deck.delta_ppn;
     int ncell = s->NX * s->NY, nm = s->nm;
     int status = ORC_OK;
     int maxit = s->stats.max_iter;
@@ -1013,10 +1141,20 @@ void orc_solver_flux(orc_solver *s, double *flux_out)
             const double *a = &s->alpha[(size_t)c * nm];
             for (int qq = 0; qq < qn; qq++) {
                 int q = q0 + qq;
-                double kinetic = 0.0;                           /* computeAnsatz :44-53 */
+                double kinetic = 0.0;                           /* computeAnsatz :44-58 */
                 for (int k = 0; k < nm; k++)
                     kinetic += a[k] * p[q * np2 + k];
-                s->ansatz[(size_t)qq * ncell + c] = ORC_EXP(kinetic);
+                double E;
+                if (s->deck.moment_type == ORC_MOMENT_PPN) {
+                    double dl = s->deck.delta_ppn;
+                    if (kinetic > 0)
+                        E = 0.5 * kinetic + 0.5 * sqrt(kinetic * kinetic + 4 * dl);
+                    else
+                        E = -dl / (0.5 * kinetic - 0.5 * sqrt(kinetic * kinetic + 4 * dl));
+                } else {
+                    E = ORC_EXP(kinetic);
+                }
+                s->ansatz[(size_t)qq * ncell + c] = E;
             }
         }
 #ifdef _OPENMP

@@ -26,14 +26,18 @@ enum {
     ORC_ERR_ALLOC = 4
 };
 
-/* Mirrors OPTIONS (opt.h:11-22) for the fields the Mn/opt path reads. */
+/* Mirrors OPTIONS (opt.h:11-22) for the fields the opt path reads. */
 typedef struct {
     int max_iter;
     double tol_rel;
     double cond_h_max;
     int use_gaunt;
     int line_search;   /* 0: linesearch.cpp exactly; 1: robust (see orc_opt) */
+    int moment_type;   /* ORC_MOMENT_MN or ORC_MOMENT_PPN (opt.h:35-37) */
+    double delta_ppn;  /* PPn regularization delta (opt.h:21) */
 } orc_options;
+
+enum { ORC_MOMENT_MN = 0, ORC_MOMENT_PPN = 1 };
 
 /* Mirrors OUTS (opt.h:25-32). */
 typedef struct {
@@ -79,7 +83,15 @@ double orc_rcond(int n, const double *l, double anorm);
 /* max absolute row sum (utils.cpp:36-52) */
 double orc_norm1(int n, const double *a);
 
-/* One closure solve: opt() for MOMENT_TYPE_MN (opt.cpp:82-112,154-343).
+/* f, g, H of the PPn dual objective (fobj.cpp:137-236,355-428): the ansatz
+ * G(a) = a/2 + sqrt(a^2/4 + delta) of a = alpha.m, f = sum w (t G - delta/2 +
+ * delta log G) - alpha.u with t = a/2, g = sum w G m - u, H = sum w G/(2 s) m m^T
+ * with s = sqrt(t^2 + delta).  h may be NULL; g may be NULL. */
+double orc_fobj_ppn(const orc_tables *t, const double *alpha, const double *u, double delta,
+                    double *g, double *h, int use_gaunt);
+
+/* One closure solve: opt() for MOMENT_TYPE_MN and MOMENT_TYPE_PPN
+ * (opt.cpp:82-148,154-343).
  * u and alpha are in/out exactly as in the reference.  With
  * opt->line_search == 1 the Armijo loop additionally accepts the full Newton
  * step when its predicted decrease -g.d is at most 1e3 * DBL_EPSILON * fscale,
@@ -107,6 +119,8 @@ typedef struct {
     int max_iter, use_gaunt;
     double theta;
     int line_search;
+    int moment_type;    /* MOMOPT_MOMENT_TYPE (momopt_init.cpp:67) */
+    double delta_ppn;   /* MOMOPT_DELTA_PPN (momopt_init.cpp:66) */
 } orc_deck;
 
 /* Mirrors OPTIMIZATION_STATS (momopt_solver.h:33-41). */
