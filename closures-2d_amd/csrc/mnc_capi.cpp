@@ -30,6 +30,7 @@ struct mnc_ctx {
     // mixed precision operand tables (absent when the order/quadrature has none)
     mnc::MixTables mix;
     bool has_mix = false;
+    bool ppn = false;                // moment type PPn (setup->moment_type == 1)
     double *d_mz = nullptr, *d_mg = nullptr, *d_mgv = nullptr;
     float *d_mpv = nullptr;          // Gaunt map in compile-time pattern order (mixed logic)
     unsigned short *d_mh = nullptr;
@@ -174,6 +175,12 @@ int check_opts(mnc_ctx *ctx, const mnc_options *o) {
         return ctx_fail(ctx, MNC_ERR_ARG, "unknown warm_start");
     if (o->line_search != MNC_LS_REFERENCE && o->line_search != MNC_LS_ROBUST)
         return ctx_fail(ctx, MNC_ERR_ARG, "unknown line_search");
+    if (ctx->ppn) {
+        if (!(o->delta_ppn > 0.0)) return ctx_fail(ctx, MNC_ERR_ARG, "PPn needs delta_ppn > 0");
+        if (o->precision != MNC_PREC_FP64 || o->warm_start == MNC_WARM_LEARNED)
+            return ctx_fail(ctx, MNC_ERR_ARG, "PPn runs the FP64 form from the previous or isotropic alpha");
+        if (ctx->tab.nm > 15) return ctx_fail(ctx, MNC_ERR_ARG, "PPn has kernels for moment orders up to 4");
+    }
     return MNC_OK;
 }
 
@@ -269,6 +276,8 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     a.gptr = ctx->d_gptr;
     a.gk = ctx->d_gk;
     a.gv = ctx->d_gv;
+    a.ppn = ctx->ppn ? 1 : 0;
+    a.delta = o->delta_ppn;
     a.max_iter = o->max_iter;
     a.tol = o->tol_rel;
     a.cond_h_max = o->cond_h_max;
@@ -397,6 +406,8 @@ mnc::GridArgs grid_args(mnc_ctx *ctx) {
     g.flux = ctx->d_flux;
     g.ansatz = ctx->d_ansatz;
     g.fpart = ctx->d_fpart;
+    g.ppn = ctx->ppn ? 1 : 0;
+    g.delta = ctx->grid.delta_ppn;
     return g;
 }
 
@@ -435,9 +446,10 @@ int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
     if (!out) return MNC_ERR_ARG;
     *out = nullptr;
     if (!setup) return MNC_ERR_ARG;
-    if (setup->moment_type != 0) return MNC_ERR_ARG;   // only Mn is built
+    if (setup->moment_type != 0 && setup->moment_type != 1) return MNC_ERR_ARG;   // Mn or PPn
     mnc_ctx *ctx = new mnc_ctx();
     ctx->device = setup->device;
+    ctx->ppn = setup->moment_type == 1;
     int st = mnc::build_tables(setup->moment_order, setup->quad_order, setup->use_gaunt, ctx->tab);
     if (st != MNC_OK) { delete ctx; return st; }
     if (ctx->device < 0) {   // host-only context: tables, no device state
@@ -470,7 +482,7 @@ int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
         mnc_destroy(ctx);
         return MNC_ERR_HIP;
     }
-    if (t.nm == 45 && mnc::build_g64_tables(setup->moment_order, setup->quad_order, ctx->g64) == MNC_OK &&
+    if (!ctx->ppn && t.nm == 45 && mnc::build_g64_tables(setup->moment_order, setup->quad_order, ctx->g64) == MNC_OK &&
         ctx->g64.kz == 12 && ctx->g64.ngt == 3 && ctx->g64.nht == 10) {   // the instantiated tile counts
         if (up(&ctx->d_gz, ctx->g64.z) || up(&ctx->d_gg, ctx->g64.g) || up(&ctx->d_gh, ctx->g64.h) ||
             up(&ctx->d_mgptr, ctx->g64.gptr) || up(&ctx->d_mgk, ctx->g64.gk) || up(&ctx->d_mgv, ctx->g64.gv)) {
@@ -479,7 +491,7 @@ int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
         }
         ctx->has_g64 = true;
     }
-    if (mnc::build_mix_tables(setup->moment_order, setup->quad_order, ctx->mix) == MNC_OK &&
+    if (!ctx->ppn && mnc::build_mix_tables(setup->moment_order, setup->quad_order, ctx->mix) == MNC_OK &&
         ctx->mix.nnt == mnc::mix_tiles(t.nm)) {
         if (up(&ctx->d_mz, ctx->mix.z) || up(&ctx->d_mg, ctx->mix.g) || up(&ctx->d_mh, ctx->mix.h) ||
             up(&ctx->d_mgptr, ctx->mix.gptr) || up(&ctx->d_mgk, ctx->mix.gk) || up(&ctx->d_mgv, ctx->mix.gv) ||
@@ -623,6 +635,7 @@ int mnc_grid_create(mnc_ctx *ctx, const mnc_grid_desc *grid, const double *momen
     if (grid->num_cells_x < 2 || grid->num_cells_y < 2)
         return ctx_fail(ctx, MNC_ERR_ARG, "grid needs at least 2x2 interior cells");
     if (!(grid->dx > 0.0) || !(grid->dy > 0.0)) return ctx_fail(ctx, MNC_ERR_ARG, "dx, dy must be > 0");
+    if (ctx->ppn && !(grid->delta_ppn > 0.0)) return ctx_fail(ctx, MNC_ERR_ARG, "PPn grid needs delta_ppn > 0");
     MNC_HIP_CHECK(hipSetDevice(ctx->device));
     ctx->grid = *grid;
     const size_t nc = size_t(grid->num_cells_x + 4) * (grid->num_cells_y + 4);
@@ -733,7 +746,8 @@ int enqueue_step(mnc_ctx *ctx, double dt, const mnc_options *opt) {
 
 bool same_opts(const mnc_options &a, const mnc_options &b) {
     return a.max_iter == b.max_iter && a.tol_rel == b.tol_rel && a.cond_h_max == b.cond_h_max &&
-           a.precision == b.precision && a.warm_start == b.warm_start && a.line_search == b.line_search;
+           a.precision == b.precision && a.warm_start == b.warm_start && a.line_search == b.line_search &&
+           a.delta_ppn == b.delta_ppn;
 }
 
 // Captures enqueue_step into g.exec.  On any capture error the context falls
@@ -764,6 +778,8 @@ int mnc_update(mnc_ctx *ctx, double dt, const mnc_options *opt) {
     if (!ctx->has_grid) return ctx_fail(ctx, MNC_ERR_STATE, "no grid");
     int st = check_opts(ctx, opt);
     if (st) return st;
+    if (ctx->ppn && opt->delta_ppn != ctx->grid.delta_ppn)   // one deck value in the reference
+        return ctx_fail(ctx, MNC_ERR_ARG, "options.delta_ppn differs from the grid's delta_ppn");
     MNC_HIP_CHECK(hipSetDevice(ctx->device));
     if (ctx->use_graph && !ctx->timing && !ctx->has_g64) {   // timing needs eager launches; M8 syncs per batch
         mnc_ctx::StepGraph *g = nullptr;

@@ -55,7 +55,18 @@ __device__ __forceinline__ void euler_epilogue(const GridArgs &g, int i, int j, 
     for (int k = 0; k < NM; ++k) u[k] = v[k];
 }
 
-// E[q][cell] = exp(alpha_cell . m_q) for q in [q0, q0+nqc)  (computeAnsatz, :38-53)
+// The ansatz at kinetic = alpha.m_q (computeAnsatz, :38-58): exp for Mn; for
+// PPn kinetic/2 + sqrt(kinetic^2 + 4 delta)/2, written for kinetic <= 0 in
+// the form without cancellation.
+__device__ __forceinline__ double ansatz_value(const GridArgs &g, double kinetic) {
+    if (g.ppn) {
+        if (kinetic > 0) return 0.5 * kinetic + 0.5 * sqrt(kinetic * kinetic + 4 * g.delta);
+        return -g.delta / (0.5 * kinetic - 0.5 * sqrt(kinetic * kinetic + 4 * g.delta));
+    }
+    return exp(kinetic);
+}
+
+// E[q][cell] = ansatz(alpha_cell . m_q) for q in [q0, q0+nqc)  (computeAnsatz, :38-58)
 template <int NM>
 __global__ __launch_bounds__(kBlock) void ansatz_kernel(GridArgs g, int q0, int nqc) {
     const int ncell = g.NX * g.NY;
@@ -69,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void ansatz_kernel(GridArgs g, int q0, int 
         double s = 0.0;
 #pragma unroll
         for (int k = 0; k < NM; ++k) s = fma(a[k], row[k], s);
-        g.ansatz[size_t(qq) * ncell + c] = exp(s);
+        g.ansatz[size_t(qq) * ncell + c] = ansatz_value(g, s);
     }
 }
 
@@ -156,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void flux_euler_kernel(GridArgs g, int q0, 
 
 // The ansatz fused into the flux: a block owns a TX x TY tile of interior cells,
 // keeps alpha of the tile and its 2-cell halo in LDS, and for each chunk of QC
-// directions evaluates E = exp(alpha.m_q) of the halo'd tile into LDS (the halo
+// directions evaluates the ansatz E of the halo'd tile into LDS (the halo
 // is evaluated by both neighbouring tiles) before the stencil reads it.  E never
 // goes to HBM.  The arithmetic of every cell is that of ansatz_kernel +
 // flux_euler_kernel (same summation order over k and q), so the results are
@@ -191,7 +202,7 @@ __global__ __launch_bounds__(TX * TY) void flux_fused_kernel(GridArgs g, int sta
                 double sa = 0.0;
 #pragma unroll
                 for (int k = 0; k < NM; ++k) sa = fma(al_s[h][k], row[k], sa);
-                E_s[qq][h] = exp(sa);
+                E_s[qq][h] = ansatz_value(g, sa);
             }
         }
         __syncthreads();
@@ -337,7 +348,7 @@ __global__ __launch_bounds__(TX * TY) void flux_split_kernel(GridArgs g) {
                     double sa = 0.0;
 #pragma unroll
                     for (int k = 0; k < NM; ++k) sa = fma(al[hh][k], row[k], sa);
-                    E_s[qq][h] = exp(sa);
+                    E_s[qq][h] = ansatz_value(g, sa);
                 }
             }
         }

@@ -51,6 +51,8 @@ struct NewtonArgs {
     double gate;          // learned warm start: the network is consulted when the best of isotropic
                           // and previous alpha has ||g|| > gate * ||u|| (scaled problem)
     int divert;           // learned warm start: this launch's logic holds the gate (first round)
+    int ppn;              // MOMENT_TYPE_PPN: the PPn closure (opt.cpp:113-141, fobj.cpp:137-236,355-428)
+    double delta;         // its delta (MOMOPT_DELTA_PPN), unscaled
     // per-cell solver state and the active lists
     double *st_d;         // [dfields][cap]
     int *st_i;            // [ifields][cap]
@@ -185,6 +187,8 @@ struct GridArgs {
     const double *w, *p, *xi, *eta;
     double *moments, *alpha, *moments_old, *sigma_t, *sigma_s, *flux, *ansatz;
     double *fpart;        // [kFluxGroups][nx*ny][nm]: per-group partial fluxes (flux_tile 3)
+    int ppn;              // the PPn ansatz (computeAnsatz, momopt_update.cpp:54-57)
+    double delta;         // its delta
 };
 constexpr int kFluxGroups = 4;
 

@@ -28,6 +28,7 @@ namespace {
 
 constexpr int kBlock = 256;
 #define kSqrtPi sqrt(M_PI)
+#define kPi M_PI
 __constant__ double kReg[5] = {0.0, 1e-3, 1e-2, 1e-1, 5e-1};   // opt.h:39-40
 
 template <int N> struct Tri {
@@ -297,6 +298,55 @@ __device__ __forceinline__ double sweep(const double *Pg, const double *Wg, int 
 // The same sweep shared by a whole wave for one cell: lane l takes the
 // quadrature points q = l, l+64, ..., and the partial sums are reduced across
 // the wave (butterfly), so every lane ends with the full sums.
+// The PPn ansatz at t = (alpha.m)/2 (fobj.cpp:166-172): G = t + s with
+// s = sqrt(t^2 + delta), taken as -delta / (t - s) for t < 0.
+__device__ __forceinline__ double ppn_g(double t, double delta, double &s) {
+    s = sqrt(t * t + delta);
+    return t < 0.0 ? -delta / (t - s) : t + s;
+}
+
+// The PPn sweep (fobj.cpp:137-236,355-428): f = sum w (t G - delta/2 + delta log G)
+// is returned, the Hessian sums use the weight w G / (2 s) and, with NG = NM,
+// gs collects the gradient sums sum w G m_k (G's weight differs from H's, so g
+// is not H's first column as it is for Mn).
+template <int NM, int NP2, bool GAUNT, int R0, int R1, int NA, int NG>
+__device__ __forceinline__ double sweep_ppn(const double *Pg, const double *Wg, int nq, const double (&al)[NM],
+                                            double delta, double (&acc)[NA], double (&gs)[NG > 0 ? NG : 1]) {
+    cdouble *P = (cdouble *)Pg;
+    cdouble *W = (cdouble *)Wg;
+    double sf = 0.0;
+#pragma unroll
+    for (int k = 0; k < NA; ++k) acc[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NG; ++k) gs[k] = 0.0;
+    for (int q = 0; q < nq; ++q) {
+        cdouble *row = P + size_t(q) * NP2;
+        double am = 0.0;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) am = fma(al[k], row[k], am);
+        const double t = 0.5 * am;
+        double sq;
+        const double G = ppn_g(t, delta, sq);
+        sf = sf + W[q] * (t * G - 0.5 * delta + delta * log(G));
+        const double wg = W[q] * G, wh = 0.5 * W[q] * (G / sq);
+#pragma unroll
+        for (int k = 0; k < NG; ++k) gs[k] = fma(row[k], wg, gs[k]);
+        if constexpr (GAUNT) {
+#pragma unroll
+            for (int k = 0; k < NP2; ++k) acc[k] = fma(row[k], wh, acc[k]);
+        } else {
+            constexpr int off = Tri<NM>::at(R0, 0);
+#pragma unroll
+            for (int i = R0; i < R1; ++i) {
+                const double v = row[i] * wh;
+#pragma unroll
+                for (int j = 0; j <= i; ++j) acc[Tri<NM>::at(i, j) - off] = fma(v, row[j], acc[Tri<NM>::at(i, j) - off]);
+            }
+        }
+    }
+    return sf;
+}
+
 template <int NM, int NP2, bool GAUNT, int R0, int R1, int NA>
 __device__ __forceinline__ double sweep_wave(const double *P, const double *W, int nq, const double (&al)[NM],
                                              double (&acc)[NA]) {
@@ -347,7 +397,7 @@ struct Store {
     static constexpr int nacc = cmax(cmax(NP2, NM * (NM + 1) / 2), mix_cols(NM));
     enum : int {
         F_AL = 0, F_U = NM, F_AOLD = 2 * NM, F_D = 3 * NM, F_T = 4 * NM, F_RTOL, F_U0, F_ASCALE, F_FCUR,
-        F_GD, F_MAXF, F_FSCALE, F_SF, F_GS, F_ACC = F_GS + NM
+        F_GD, F_MAXF, F_FSCALE, F_DELTA, F_SF, F_GS, F_ACC = F_GS + NM
     };
     static constexpr int dfields = F_ACC + nacc;
     enum : int { I_ITER = 0, I_TOTAL, I_REG, I_MAXIT, I_MODE, I_WARM, ifields };
@@ -484,10 +534,15 @@ __device__ int cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, La
         iso = !finite;
         warm = finite ? 2 : 1;
     }
-    if (iso) {
-        al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
+    // PPn (opt.cpp:113-141): alpha / u0, u / u0 and delta / u0^2, no shift
+    const double delta = A.ppn ? A.delta / (u0 * u0) : 0.0;
+    if (iso) {                                                     // isotropicAlpha (opt.cpp:27-46)
+        al[0] = A.ppn ? 1.0 - 4.0 * delta * kPi : 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
 #pragma unroll
         for (int k = 1; k < NM; ++k) al[k] = 0.0;
+    } else if (A.ppn) {
+#pragma unroll
+        for (int k = 0; k < NM; ++k) al[k] = ain[k] / u0;
     } else {
 #pragma unroll
         for (int k = 0; k < NM; ++k) al[k] = ain[k];
@@ -502,6 +557,7 @@ __device__ int cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, La
     }
     S.d(St::F_U0, c) = u0;
     S.d(St::F_ASCALE, c) = ascale;
+    S.d(St::F_DELTA, c) = delta;
     S.d(St::F_RTOL, c) = sqrt(ss) * A.tol;
     S.d(St::F_T, c) = 1.0;
     S.i(St::I_ITER, c) = 0;
@@ -527,7 +583,7 @@ __device__ int cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, La
 // above min(cmix, cond_h_max), an exhausted line search, the iteration limit),
 // the iteration is not taken: the cell returns LOGIC_DEFER at the point where
 // that Newton iteration starts, to be redone by an FP64 sweep.
-template <int NM, int NP2, bool GAUNT, bool MIX = false>
+template <int NM, int NP2, bool GAUNT, bool MIX = false, bool PPN = false>
 __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, LaneStats &st,
                           float *gstage = nullptr) {
     using St = Store<NM, NP2>;
@@ -635,12 +691,12 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
         // g goes to the gradient slot of the state (F_D doubles as it until the
         // direction is formed), so only the factor stays live below.
         {
-            double fcur = MIX ? S.d(St::F_SF, c) : h[0] / (p0 * p0);
+            double fcur = (MIX || PPN) ? S.d(St::F_SF, c) : h[0] / (p0 * p0);
             double fscale = fabs(fcur), gg = 0.0;
 #pragma unroll
             for (int i = 0; i < NM; ++i) {
                 const double ui = S.d(St::F_U + i, c);
-                const double gi = (MIX ? S.d(St::F_GS + i, c) : h[Tri<NM>::at(i, 0)] / p0) - ui;
+                const double gi = ((MIX || PPN) ? S.d(St::F_GS + i, c) : h[Tri<NM>::at(i, 0)] / p0) - ui;
                 S.d(St::F_D + i, c) = gi;
                 gg = fma(gi, gi, gg);
                 fcur = fcur - al[i] * ui;
@@ -690,10 +746,10 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
         }
         if (!diverted) {
             double anorm = 0.0, gersh = DBL_MAX;
-    #pragma unroll
+#pragma unroll
             for (int i = 0; i < NM; ++i) {
                 double s = 0.0;
-    #pragma unroll
+#pragma unroll
                 for (int j = 0; j < NM; ++j) s += fabs(h[i >= j ? Tri<NM>::at(i, j) : Tri<NM>::at(j, i)]);
                 anorm = s > anorm ? s : anorm;
                 gersh = fmin(gersh, 2.0 * double(h[Tri<NM>::at(i, i)]) - s);
@@ -714,20 +770,20 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
                 double d[NM];
                 {
                     HT dh[NM];
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < NM; ++i) dh[i] = HT(S.d(St::F_D + i, c));
                     apply_inv<NM, HT>(h, dh);                          // dpotrs (:278-282)
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < NM; ++i) d[i] = double(dh[i]);
                 }
                 // Phase 3: stop test, or the line search set-up (linesearch.cpp:28-43)
                 if (err <= S.d(St::F_RTOL, c)) {
                     converged = true;                                  // :293-306
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < NM; ++i) g0[i] = S.d(St::F_D + i, c);
                 } else {
                     double maxf = 0.0, gd = 0.0;
-    #pragma unroll
+#pragma unroll
                     for (int i = 0; i < NM; ++i) {
                         const double ai = S.d(St::F_AL + i, c);
                         const double gi = S.d(St::F_D + i, c);
@@ -744,7 +800,7 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
                     S.d(St::F_GD, c) = gd;
                     t = 1.0;
                     if (t * maxf > DBL_EPSILON / 2) {
-    #pragma unroll
+#pragma unroll
                         for (int i = 0; i < NM; ++i) al[i] = al[i] + t * d[i];
                         mode = ST_TRIAL;
                     } else if constexpr (MIX) {
@@ -760,7 +816,7 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
                 }
             }
             if (!converged && mode != ST_TRIAL) {
-    #pragma unroll
+#pragma unroll
                 for (int i = 0; i < NM; ++i) al[i] = S.d(St::F_AL + i, c);
             }
         }
@@ -785,7 +841,8 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
                 S.d(St::F_U + k, c) = uk;
                 ss = fma(uk, uk, ss);
             }
-            al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
+            // isotropicAlpha (opt.cpp:27-46)
+            al[0] = PPN ? 1.0 - 4.0 * S.d(St::F_DELTA, c) * kPi : 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
 #pragma unroll
             for (int k = 1; k < NM; ++k) al[k] = 0.0;
             S.d(St::F_RTOL, c) = sqrt(ss) * A.tol;
@@ -802,15 +859,23 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
         const double u0 = S.d(St::F_U0, c);
 #pragma unroll
         for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
-        const double u0p = S.d(St::F_U + 0, c) + g0[0];
         double *uo = A.u + size_t(c) * NM;
         double *ao = A.alpha + size_t(c) * NM;
+        if constexpr (PPN) {                                   // no reconstruction; opt.cpp:131-136
 #pragma unroll
-        for (int k = 0; k < NM; ++k) uo[k] = ((S.d(St::F_U + k, c) + g0[k]) / u0p) * u0;   // :298-301, opt.cpp:107-108
-        al[0] -= log(u0p) * 2.0 * kSqrtPi;                                               // :302
-        al[0] = al[0] + S.d(St::F_ASCALE, c);                                            // opt.cpp:109
+            for (int k = 0; k < NM; ++k) {
+                ao[k] = al[k] * u0;
+                uo[k] = S.d(St::F_U + k, c) * u0;
+            }
+        } else {
+            const double u0p = S.d(St::F_U + 0, c) + g0[0];
 #pragma unroll
-        for (int k = 0; k < NM; ++k) ao[k] = al[k];
+            for (int k = 0; k < NM; ++k) uo[k] = ((S.d(St::F_U + k, c) + g0[k]) / u0p) * u0;   // :298-301, opt.cpp:107-108
+            al[0] -= log(u0p) * 2.0 * kSqrtPi;                                               // :302
+            al[0] = al[0] + S.d(St::F_ASCALE, c);                                            // opt.cpp:109
+#pragma unroll
+            for (int k = 0; k < NM; ++k) ao[k] = al[k];
+        }
         if (A.outs) { A.outs[c].iter = total; A.outs[c].r = kReg[reg]; A.outs[c].norm_g = err; }
         st.max_iter = total > st.max_iter ? total : st.max_iter;
         st.sum += total;
@@ -831,8 +896,9 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
     return result;
 }
 
-// One evaluation at the cell's current point: S.F_SF and S.F_ACC.
-template <int NM, int NP2, bool GAUNT, int RS>
+// One evaluation at the cell's current point: S.F_SF and S.F_ACC (and, for
+// PPn, the gradient sums S.F_GS).
+template <int NM, int NP2, bool GAUNT, int RS, bool PPN = false>
 __device__ __forceinline__ void cell_sweep(const NewtonArgs &A, const Store<NM, NP2> &S, int c) {
     using St = Store<NM, NP2>;
     constexpr int NA = GAUNT ? NP2 : Tri<NM>::at(RS, 0);
@@ -841,6 +907,26 @@ __device__ __forceinline__ void cell_sweep(const NewtonArgs &A, const Store<NM, 
     double al[NM];
 #pragma unroll
     for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
+    if constexpr (PPN) {
+        const double delta = S.d(St::F_DELTA, c);
+        {
+            double acc[NA], gs[NM];
+            const double sf = sweep_ppn<NM, NP2, GAUNT, 0, (GAUNT ? NM : RS), NA, NM>(A.p, A.w, A.nq, al, delta,
+                                                                                       acc, gs);
+            S.d(St::F_SF, c) = sf;
+#pragma unroll
+            for (int k = 0; k < NM; ++k) S.d(St::F_GS + k, c) = gs[k];
+#pragma unroll
+            for (int k = 0; k < NA; ++k) S.d(St::F_ACC + k, c) = acc[k];
+        }
+        if constexpr (!GAUNT && RS < NM) {
+            double accb[NB], none[1];
+            sweep_ppn<NM, NP2, false, RS, NM, NB, 0>(A.p, A.w, A.nq, al, delta, accb, none);
+#pragma unroll
+            for (int k = 0; k < NB; ++k) S.d(St::F_ACC + NA + k, c) = accb[k];
+        }
+        return;
+    }
     {
         double acc[NA];
         const double sf = sweep<NM, NP2, GAUNT, 0, (GAUNT ? NM : RS), NA>(A.p, A.w, A.nq, al, acc);
@@ -907,21 +993,21 @@ __global__ __launch_bounds__(kBlock) void judge_kernel(NewtonArgs A) {
 
 // One sweep for every cell on list `lin`; also clears the count of the list
 // the following logic kernel fills.
-template <int NM, int NP2, bool GAUNT, int RS>
+template <int NM, int NP2, bool GAUNT, int RS, bool PPN = false>
 __global__ __launch_bounds__(kBlock) void sweep_kernel(NewtonArgs A, int lin) {
     if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl[cw(lin ^ 1)] = 0;
     const int slot = blockIdx.x * kBlock + threadIdx.x;
     if (slot >= A.ctl[cw(lin)]) return;
     const int c = A.lists[size_t(lin) * A.cap + slot];
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
-    cell_sweep<NM, NP2, GAUNT, RS>(A, S, c);
+    cell_sweep<NM, NP2, GAUNT, RS, PPN>(A, S, c);
 }
 
 // Newton / Armijo logic for every cell on list `lin`; cells needing another
 // sweep are compacted onto list lin^1.
 // MIX: after a mixed-precision sweep; deferred cells go to list defer_to
 // (the FP64 pair's next list).
-template <int NM, int NP2, bool GAUNT, bool MIX = false>
+template <int NM, int NP2, bool GAUNT, bool MIX = false, bool PPN = false>
 __global__ __launch_bounds__(kBlock) void logic_kernel(NewtonArgs A, int lin, int defer_to) {
     const int slot = blockIdx.x * kBlock + threadIdx.x;
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
@@ -931,7 +1017,7 @@ __global__ __launch_bounds__(kBlock) void logic_kernel(NewtonArgs A, int lin, in
     int c = -1;
     if (slot < A.ctl[cw(lin)]) {
         c = A.lists[size_t(lin) * A.cap + slot];
-        res = cell_logic<NM, NP2, GAUNT, MIX>(A, S, c, st, &gstage[0][threadIdx.x]);
+        res = cell_logic<NM, NP2, GAUNT, MIX, PPN>(A, S, c, st, &gstage[0][threadIdx.x]);
         if (res == LOGIC_DEFER) ++st.deferred;
     }
     list_append(A, lin ^ 1, cw(lin ^ 1), res == LOGIC_CONTINUE, c);
@@ -1179,16 +1265,45 @@ __device__ __forceinline__ double reg_rcond(const double (&row)[NM], const doubl
     return (1.0 / est) / anorm;
 }
 
-template <int NM, int NP2, bool GAUNT, int RS>
+// An upper bound of ||A^{-1}||_1 for A = L L^T from the factor (rows in
+// registers as after reg_cholesky, col[k] = L[k][lane]): with M the comparison
+// matrix of L (|diagonal|, -|off-diagonal|), |L^{-1}| <= M^{-1} entrywise, so
+// ||A^{-1}||_1 <= ||L^{-1}||_inf ||L^{-1}||_1 <= ||M^{-1} e||_inf ||M^{-T} e||_inf:
+// two triangular solves with non-negative data.  Higham's estimate never
+// exceeds ||A^{-1}||_1, so where anorm times this bound is within the limit
+// (with a factor 2 of margin) the estimate cannot fail the condition test.
+template <int NM>
+__device__ __forceinline__ double reg_inv_norm_bound(const double (&row)[NM], const double (&col)[NM],
+                                                     const double (&rd)[NM], int lane) {
+    double y = lane < NM ? 1.0 : 0.0;
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        const double yj = readlane_d(y, j) * rd[j];
+        if (lane == j) y = yj;
+        else if (lane > j && lane < NM) y = fma(fabs(row[j]), yj, y);
+    }
+    double z = lane < NM ? 1.0 : 0.0;
+#pragma unroll
+    for (int j = NM - 1; j >= 0; --j) {
+        const double zj = readlane_d(z, j) * rd[j];
+        if (lane == j) z = zj;
+        else if (lane < j) z = fma(fabs(col[j]), zj, z);
+    }
+    return wave_max(lane < NM ? y : 0.0) * wave_max(lane < NM ? z : 0.0);
+}
+
+template <int NM, int NP2, bool GAUNT, int RS, bool PPN = false>
 struct CoopShared {
     static constexpr int NA = GAUNT ? NP2 : Tri<NM>::at(RS, 0);
     static constexpr int NB = (GAUNT || RS >= NM) ? 1 : Tri<NM>::size - Tri<NM>::at(RS, 0);
-    static constexpr int NPART = NA > NB ? NA : NB;
+    static constexpr int NG = PPN ? NM : 0;          // PPn: the gradient sums ride with the first pass
+    static constexpr int NPART = (NA + NG) > NB ? (NA + NG) : NB;
     static constexpr int NACC = Store<NM, NP2>::nacc;
     double part[NPART][65];   // padded: lane-contiguous writes, column sums conflict-free
     double accs[NACC];        // reduced sweep sums
     double L[NM][NM];         // H, then its Cholesky factor (lower)
     double x[NM], gvec[NM];
+    double gsum[NM];          // PPn: sum w G m
     double red[64];
 };
 
@@ -1298,11 +1413,11 @@ __device__ __forceinline__ double coop_rcond(double (*L)[NM], double *x, double 
     return (1.0 / est) / anorm;
 }
 
-template <int NM, int NP2, bool GAUNT, int RS>
+template <int NM, int NP2, bool GAUNT, int RS, bool PPN = false>
 __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, int lin2) {
     using St = Store<NM, NP2>;
-    using CS = CoopShared<NM, NP2, GAUNT, RS>;
-    constexpr int NA = CS::NA, NB = CS::NB;
+    using CS = CoopShared<NM, NP2, GAUNT, RS, PPN>;
+    constexpr int NA = CS::NA, NB = CS::NB, NG = CS::NG;
     __shared__ CS sh;
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
     const int lane = threadIdx.x & 63;
@@ -1316,17 +1431,30 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
         // wave-uniform: state loads become scalar loads
         const int c = __builtin_amdgcn_readfirstlane(tail_cell(A, lin, count, lin2, dcount, s));
         if (c < 0) break;
-        for (;;) {
-            double al[NM];
+        // The cell's state comes into registers once: the tail runs it to the
+        // end, so nothing is stored back until the outputs.  Vectors are
+        // lane-distributed (element k on lane k), scalars wave-uniform.
+        double al[NM];
 #pragma unroll
-            for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
+        for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
+        int mode = S.i(St::I_MODE, c), iter = S.i(St::I_ITER, c), total = S.i(St::I_TOTAL, c);
+        int reg = S.i(St::I_REG, c), maxit = S.i(St::I_MAXIT, c);
+        double t = S.d(St::F_T, c), rtol = S.d(St::F_RTOL, c);
+        const double u0 = S.d(St::F_U0, c), ascale = S.d(St::F_ASCALE, c);
+        double fcur_s = S.d(St::F_FCUR, c), gd_s = S.d(St::F_GD, c), fscale_s = S.d(St::F_FSCALE, c);
+        double maxf_s = S.d(St::F_MAXF, c);
+        double ul = lane < NM ? S.d(St::F_U + lane, c) : 0.0;
+        const double delta = PPN ? S.d(St::F_DELTA, c) : 0.0;
+        double aold_l = lane < NM ? S.d(St::F_AOLD + lane, c) : 0.0;
+        double d_l = lane < NM ? S.d(St::F_D + lane, c) : 0.0;
+        for (;;) {
             // ---- sweep(s): partial sums per lane, reduced through LDS ----
             double sf = 0.0;
             double wes[kTailPts];
 #pragma unroll
             for (int pass = 0; pass < ((!GAUNT && RS < NM) ? 2 : 1); ++pass) {
                 const int R0 = pass == 0 ? 0 : RS, R1 = pass == 0 ? (GAUNT ? NM : RS) : NM;
-                const int nacc = pass == 0 ? NA : NB;
+                const int nacc = pass == 0 ? NA + NG : NB;
                 const int off = Tri<NM>::at(R0, 0);
                 double acc[CS::NPART];
 #pragma unroll
@@ -1344,17 +1472,30 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                         double row[GAUNT ? NP2 : NM];
 #pragma unroll
                         for (int k = 0; k < (GAUNT ? NP2 : NM); ++k) row[k] = A.pT[size_t(k) * A.nq + q];
-                        double we;
+                        double we;                    // the Hessian's weight
                         if (pass > 0 && cached) {
                             we = wes[qi];
                         } else {
                             double am = 0.0;
 #pragma unroll
                             for (int k = 0; k < NM; ++k) am = fma(al[k], row[k], am);
-                            we = A.w[q] * exp(am);
+                            if constexpr (PPN) {
+                                const double tq = 0.5 * am;
+                                double sq;
+                                const double G = ppn_g(tq, delta, sq);
+                                we = 0.5 * A.w[q] * (G / sq);
+                                if (pass == 0) {
+                                    sf = sf + A.w[q] * (tq * G - 0.5 * delta + delta * log(G));
+                                    const double wg = A.w[q] * G;
+#pragma unroll
+                                    for (int k = 0; k < NG; ++k) acc[NA + k] = fma(row[k], wg, acc[NA + k]);
+                                }
+                            } else {
+                                we = A.w[q] * exp(am);
+                            }
                             wes[qi] = we;
                         }
-                        if (pass == 0) sf += we;
+                        if (!PPN && pass == 0) sf += we;
                         if constexpr (GAUNT) {
 #pragma unroll
                             for (int k = 0; k < NP2; ++k) acc[k] = fma(we, row[k], acc[k]);
@@ -1383,7 +1524,9 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
 #pragma unroll
                         for (int u = 0; u < 4; ++u) t4[u] += sh.part[k][l2 + u];
                     }
-                    sh.accs[off + k] = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+                    const double tot = (t4[0] + t4[1]) + (t4[2] + t4[3]);
+                    if (PPN && pass == 0 && k >= NA) sh.gsum[k - NA] = tot;
+                    else sh.accs[off + k] = tot;
                 }
                 __builtin_amdgcn_wave_barrier();
             }
@@ -1391,20 +1534,16 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
             for (int o = 32; o > 0; o >>= 1) sf += __shfl_xor(sf, o);
             __builtin_amdgcn_wave_barrier();
             // ---- per-cell logic (cell_logic's steps, cooperative) ----
-            int mode = S.i(St::I_MODE, c), iter = S.i(St::I_ITER, c), total = S.i(St::I_TOTAL, c);
-            int reg = S.i(St::I_REG, c), maxit = S.i(St::I_MAXIT, c);
-            double t = S.d(St::F_T, c);
             bool newton = (mode == ST_NEWTON), do_reg = false, converged = false;
             double err = 0.0;
             if (mode == ST_TRIAL) {
                 if (lane == 0) ++st.trials;
                 double fnew = sf;
 #pragma unroll
-                for (int k = 0; k < NM; ++k) fnew = fnew - al[k] * S.d(St::F_U + k, c);
-                const double fcur = S.d(St::F_FCUR, c), gd = S.d(St::F_GD, c);
+                for (int k = 0; k < NM; ++k) fnew = fnew - al[k] * readlane_d(ul, k);
+                const double fcur = fcur_s, gd = gd_s;
                 bool accept = fnew <= fcur + 1e-3 * t * gd;
-                if (!accept && A.ls_mode == MNC_LS_ROBUST && t == 1.0 &&
-                    -gd <= 1e3 * DBL_EPSILON * S.d(St::F_FSCALE, c))
+                if (!accept && A.ls_mode == MNC_LS_ROBUST && t == 1.0 && -gd <= 1e3 * DBL_EPSILON * fscale_s)
                     accept = true;
                 if (accept) {
                     newton = true;
@@ -1415,10 +1554,10 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                     // sweep (f only, the same sums in the same order as a
                     // trial sweep) and the first accepted one is taken: the
                     // reference's decisions without one pass per halving.
-                    const double maxf = S.d(St::F_MAXF, c);
+                    const double maxf = maxf_s;
                     if (lane < NM) {
-                        sh.x[lane] = S.d(St::F_AOLD + lane, c);
-                        sh.gvec[lane] = S.d(St::F_D + lane, c);
+                        sh.x[lane] = aold_l;
+                        sh.gvec[lane] = d_l;
                     }
                     __builtin_amdgcn_wave_barrier();
                     int state = 0;                 // 0 searching, 1 accepted, 2 exhausted
@@ -1442,7 +1581,14 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                                 const double tj = ldexp(t, -(jj + 1));
                                 double am = 0.0;
                                 for (int k = 0; k < NM; ++k) am = fma(fma(tj, sh.gvec[k], sh.x[k]), row[k], am);
-                                fj[jj] += wq * exp(am);
+                                if constexpr (PPN) {
+                                    const double tq = 0.5 * am;
+                                    double sq;
+                                    const double G = ppn_g(tq, delta, sq);
+                                    fj[jj] += wq * (tq * G - 0.5 * delta + delta * log(G));
+                                } else {
+                                    fj[jj] += wq * exp(am);
+                                }
                             }
                         }
                         int acc = -1;
@@ -1453,7 +1599,7 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                             for (int o = 32; o > 0; o >>= 1) fv += __shfl_xor(fv, o);
                             const double tj = ldexp(t, -(jj + 1));
                             for (int k = 0; k < NM; ++k)
-                                fv = fv - fma(tj, sh.gvec[k], sh.x[k]) * S.d(St::F_U + k, c);
+                                fv = fv - fma(tj, sh.gvec[k], sh.x[k]) * readlane_d(ul, k);
                             if (acc < 0 && fv <= fcur + 1e-3 * tj * gd) acc = jj;
                         }
                         if (lane == 0) st.trials += (acc >= 0) ? acc + 1 : nval;
@@ -1467,11 +1613,11 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                     }
                     if (state == 1) {                  // a full sweep at the accepted point next
 #pragma unroll
-                        for (int k = 0; k < NM; ++k) al[k] = fma(t, S.d(St::F_D + k, c), S.d(St::F_AOLD + k, c));
+                        for (int k = 0; k < NM; ++k) al[k] = fma(t, readlane_d(d_l, k), readlane_d(aold_l, k));
                         mode = ST_NEWTON;
                     } else {
 #pragma unroll
-                        for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AOLD + k, c);
+                        for (int k = 0; k < NM; ++k) al[k] = readlane_d(aold_l, k);
                         t = 0.0;
                         ++iter; ++total;
                         do_reg = true;
@@ -1499,13 +1645,12 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                     row[jj] = hij;
                 }
                 // g_i = H_i0 / m0 - u_i on lane i; f, its scale and ||g|| in the serial order
-                const double ul = lane < NM ? S.d(St::F_U + lane, c) : 0.0;
-                const double gv = lane < NM ? row[0] / p0 - ul : 0.0;
+                const double gv = lane < NM ? (PPN ? sh.gsum[lane] : row[0] / p0) - ul : 0.0;
                 if (lane < NM) sh.gvec[lane] = gv;
-                double fcur = readlane_d(row[0], 0) / (p0 * p0), fscale = fabs(fcur), gg = 0.0;
+                double fcur = PPN ? sf : readlane_d(row[0], 0) / (p0 * p0), fscale = fabs(fcur), gg = 0.0;
 #pragma unroll
                 for (int i = 0; i < NM; ++i) {
-                    const double ui = S.d(St::F_U + i, c);
+                    const double ui = readlane_d(ul, i);
                     const double gi = readlane_d(gv, i);
                     gg = fma(gi, gi, gg);
                     fcur = fcur - al[i] * ui;
@@ -1532,14 +1677,19 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                 for (int k = 0; k < NM; ++k) col[k] = (lane < NM && k > lane) ? sh.L[k][lane] : 0.0;
                 __builtin_amdgcn_wave_barrier();
                 double condest = 0.0;
-                if (chol_ok && !cond_proven_below(anorm, gersh, NM, A.cond_h_max))
-                    condest = 1.0 / reg_rcond<NM>(row, col, anorm, lane);
+                if (chol_ok && !cond_proven_below(anorm, gersh, NM, A.cond_h_max)) {
+                    double rd[NM];
+#pragma unroll
+                    for (int k = 0; k < NM; ++k) rd[k] = 1.0 / readlane_d(row[k], k);
+                    if (!(2.0 * anorm * reg_inv_norm_bound<NM>(row, col, rd, lane) <= A.cond_h_max))
+                        condest = 1.0 / reg_rcond<NM>(row, col, anorm, lane);
+                }
                 if (iter >= maxit || !chol_ok || t == 0.0 || condest > A.cond_h_max) {
                     do_reg = true;
                 } else {
                     double xv = gv;
                     reg_apply_inv<NM>(row, col, xv, lane);
-                    if (err <= S.d(St::F_RTOL, c)) {
+                    if (err <= rtol) {
                         converged = true;
                     } else {
                         const double di = -xv;
@@ -1550,14 +1700,12 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                         double gd = 0.0;
 #pragma unroll
                         for (int i = 0; i < NM; ++i) gd = fma(readlane_d(gv, i), readlane_d(di, i), gd);
-                        if (lane < NM) {
-                            S.d(St::F_AOLD + lane, c) = al_l;
-                            S.d(St::F_D + lane, c) = di;
-                        }
-                        S.d(St::F_MAXF, c) = maxf;
-                        S.d(St::F_GD, c) = gd;
-                        S.d(St::F_FCUR, c) = fcur;
-                        S.d(St::F_FSCALE, c) = fscale;
+                        aold_l = lane < NM ? al_l : 0.0;
+                        d_l = lane < NM ? di : 0.0;
+                        maxf_s = maxf;
+                        gd_s = gd;
+                        fcur_s = fcur;
+                        fscale_s = fscale;
                         t = 1.0;
                         if (t * maxf > DBL_EPSILON / 2) {
 #pragma unroll
@@ -1578,16 +1726,14 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                     ++reg;
                     const double r = kReg[reg];
                     const double *uin = A.u + size_t(c) * NM;
-                    const double u0 = S.d(St::F_U0, c);
+                    ul = lane == 0 ? 1.0 : (lane < NM ? (uin[lane] / u0) * (1.0 - r) : 0.0);
                     double ss = 1.0;
                     for (int k = 1; k < NM; ++k) {
-                        const double uk = (uin[k] / u0) * (1.0 - r);
+                        const double uk = readlane_d(ul, k);
                         ss = fma(uk, uk, ss);
-                        S.d(St::F_U + k, c) = uk;
                     }
-                    S.d(St::F_U, c) = 1.0;
-                    S.d(St::F_RTOL, c) = sqrt(ss) * A.tol;
-                    al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
+                    rtol = sqrt(ss) * A.tol;
+                    al[0] = PPN ? 1.0 - 4.0 * delta * kPi : 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
 #pragma unroll
                     for (int k = 1; k < NM; ++k) al[k] = 0.0;
                     iter = 0;
@@ -1602,17 +1748,26 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                     done = true;
                 }
             } else if (converged) {
-                const double u0 = S.d(St::F_U0, c);
-                const double u0p = S.d(St::F_U, c) + sh.gvec[0];
-                double a0 = al[0];
-                a0 -= log(u0p) * 2.0 * kSqrtPi;
-                a0 = a0 + S.d(St::F_ASCALE, c);
-                if (lane < NM) {
-                    A.u[size_t(c) * NM + lane] = ((S.d(St::F_U + lane, c) + sh.gvec[lane]) / u0p) * u0;
-                    double ak = 0.0;
+                if (PPN) {                                    // no reconstruction; opt.cpp:131-136
+                    if (lane < NM) {
+                        double ak = 0.0;
 #pragma unroll
-                    for (int kk = 0; kk < NM; ++kk) ak = (kk == lane) ? al[kk] : ak;
-                    A.alpha[size_t(c) * NM + lane] = lane == 0 ? a0 : ak;
+                        for (int kk = 0; kk < NM; ++kk) ak = (kk == lane) ? al[kk] : ak;
+                        A.u[size_t(c) * NM + lane] = ul * u0;
+                        A.alpha[size_t(c) * NM + lane] = ak * u0;
+                    }
+                } else {
+                    const double u0p = readlane_d(ul, 0) + sh.gvec[0];
+                    double a0 = al[0];
+                    a0 -= log(u0p) * 2.0 * kSqrtPi;
+                    a0 = a0 + ascale;
+                    if (lane < NM) {
+                        A.u[size_t(c) * NM + lane] = ((ul + sh.gvec[lane]) / u0p) * u0;
+                        double ak = 0.0;
+#pragma unroll
+                        for (int kk = 0; kk < NM; ++kk) ak = (kk == lane) ? al[kk] : ak;
+                        A.alpha[size_t(c) * NM + lane] = lane == 0 ? a0 : ak;
+                    }
                 }
                 if (lane == 0) {
                     if (A.outs) { A.outs[c].iter = total; A.outs[c].r = kReg[reg]; A.outs[c].norm_g = err; }
@@ -1622,16 +1777,6 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                     ++st.hist[reg];
                 }
                 done = true;
-            }
-            if (!done) {
-#pragma unroll
-                for (int k = 0; k < NM; ++k) S.d(St::F_AL + k, c) = al[k];
-                S.d(St::F_T, c) = t;
-                S.i(St::I_ITER, c) = iter;
-                S.i(St::I_TOTAL, c) = total;
-                S.i(St::I_REG, c) = reg;
-                S.i(St::I_MAXIT, c) = maxit;
-                S.i(St::I_MODE, c) = mode;
             }
             __builtin_amdgcn_wave_barrier();
             if (done) break;
@@ -2032,10 +2177,11 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
     }
 }
 
-template <int NM, int NP2, bool G, int RS>
+template <int NM, int NP2, bool G, int RS, bool PPN = false>
 void launch_tail(const NewtonArgs &a, int lin, int lin2, int tail_blocks, int tail_mode, hipStream_t s) {
-    if (tail_mode == 0)
-        hipLaunchKernelGGL((tail_coop_kernel<NM, NP2, G, RS>), dim3(4 * tail_blocks), dim3(64), 0, s, a, lin, lin2);
+    if (tail_mode == 0 || PPN)   // PPn: the cooperative tail only
+        hipLaunchKernelGGL((tail_coop_kernel<NM, NP2, G, RS, PPN>), dim3(4 * tail_blocks), dim3(64), 0, s, a, lin,
+                           lin2);
     else if (tail_mode == 2)
         hipLaunchKernelGGL((tail_wave_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin, lin2);
     else
@@ -2044,17 +2190,17 @@ void launch_tail(const NewtonArgs &a, int lin, int lin2, int tail_blocks, int ta
 
 // Rounds r0..r1-1 of the lockstep solve (round r reads list r & 1), with the
 // prologue first if `init` and the persistent tail last if `tail`.
-template <int NM, int NP2, bool G, int RS>
+template <int NM, int NP2, bool G, int RS, bool PPN = false>
 void launch_all(const NewtonArgs &a, const LaunchPlan &lp, hipStream_t s) {
     const int nb = (a.ncells + kBlock - 1) / kBlock;
     if (lp.init) hipLaunchKernelGGL((init_kernel<NM, NP2>), dim3(nb), dim3(kBlock), 0, s, a);
     int lin = lp.r0 & 1;
     for (int r = lp.r0; r < lp.r1; ++r) {
-        hipLaunchKernelGGL((sweep_kernel<NM, NP2, G, RS>), dim3(nb), dim3(kBlock), 0, s, a, lin);
-        hipLaunchKernelGGL((logic_kernel<NM, NP2, G>), dim3(nb), dim3(kBlock), 0, s, a, lin, 2);
+        hipLaunchKernelGGL((sweep_kernel<NM, NP2, G, RS, PPN>), dim3(nb), dim3(kBlock), 0, s, a, lin);
+        hipLaunchKernelGGL((logic_kernel<NM, NP2, G, false, PPN>), dim3(nb), dim3(kBlock), 0, s, a, lin, 2);
         lin ^= 1;
     }
-    if (lp.tail) launch_tail<NM, NP2, G, RS>(a, lin, 2, lp.tail_blocks, lp.tail_mode, s);   // list 2 is empty here
+    if (lp.tail) launch_tail<NM, NP2, G, RS, PPN>(a, lin, 2, lp.tail_blocks, lp.tail_mode, s);   // list 2 empty
 }
 
 template <int NM, int NP2, bool G, int RS>
@@ -2095,16 +2241,24 @@ void launch_ctl_reset(int *ctl, hipStream_t s) { hipLaunchKernelGGL(ctl_reset_ke
 
 void newton_store_fields(int nm, int np2, int *dfields, int *ifields) {
     const int tri = nm * (nm + 1) / 2;
-    *dfields = 5 * nm + 9 + cmax(cmax(np2, tri), mix_cols(nm));
+    *dfields = 5 * nm + 10 + cmax(cmax(np2, tri), mix_cols(nm));
     *ifields = 6;
 }
 
 bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, const LaunchPlan &lp, hipStream_t s) {
 #define MNC_CASE(NMV, NP2D, NP2G, RSPLIT)                                                             \
     if (nm == NMV) {                                                                                   \
-        static_assert(Store<NMV, NP2G>::dfields == 5 * NMV + 9 + Store<NMV, NP2G>::nacc, "store");     \
-        if (gaunt && np2 == NP2G) { launch_all<NMV, NP2G, true, NMV>(a, lp, s); return true; }             \
-        if (!gaunt && np2 == NP2D) { launch_all<NMV, NP2D, false, RSPLIT>(a, lp, s); return true; }        \
+        static_assert(Store<NMV, NP2G>::dfields == 5 * NMV + 10 + Store<NMV, NP2G>::nacc, "store");    \
+        if (gaunt && np2 == NP2G) {                                                                    \
+            if (a.ppn) launch_all<NMV, NP2G, true, NMV, true>(a, lp, s);                               \
+            else launch_all<NMV, NP2G, true, NMV>(a, lp, s);                                           \
+            return true;                                                                               \
+        }                                                                                              \
+        if (!gaunt && np2 == NP2D) {                                                                   \
+            if (a.ppn) launch_all<NMV, NP2D, false, RSPLIT, true>(a, lp, s);                           \
+            else launch_all<NMV, NP2D, false, RSPLIT>(a, lp, s);                                       \
+            return true;                                                                               \
+        }                                                                                              \
         return false;                                                                                  \
     }
     MNC_CASE(1, 1, 1, 1)      // M0

@@ -19,7 +19,7 @@ class GridState:
 
 
 def initial_state(nx, ny, a_x, b_x, a_y, b_y, init_cond, gaussian_sigma, floor_value, sigma, nm,
-                  cfl_factor=0.9, theta=2.0):
+                  cfl_factor=0.9, theta=2.0, moment_type=0, delta_ppn=1e-10):
     g = GridState()
     g.nx, g.ny = nx, ny
     g.dx = (b_x - a_x) / nx
@@ -58,7 +58,10 @@ def initial_state(nx, ny, a_x, b_x, a_y, b_y, init_cond, gaussian_sigma, floor_v
     moments = np.zeros((NX, NY, nm))
     alpha = np.zeros((NX, NY, nm))
     moments[:, :, 0] = init * 2.0 * math.sqrt(math.pi)
-    alpha[:, :, 0] = 2.0 * math.sqrt(math.pi) * np.log(moments[:, :, 0] / (2.0 * math.sqrt(math.pi)))
+    if moment_type == 1:      # PPn (momopt_init.cpp:138-141)
+        alpha[:, :, 0] = moments[:, :, 0] - 4.0 * delta_ppn * math.pi / moments[:, :, 0]
+    else:
+        alpha[:, :, 0] = 2.0 * math.sqrt(math.pi) * np.log(moments[:, :, 0] / (2.0 * math.sqrt(math.pi)))
     g.moments = moments
     g.alpha = alpha
     g.max_dt = cfl_factor * (2.0 / (2.0 + theta)) * (g.dx * g.dy) / (g.dx + g.dy)

@@ -145,7 +145,7 @@ int main(int argc, char **argv) {
         return 1;
     }
     int order, qorder, max_iter, use_gaunt, mtype, otype;
-    double cfl, tol, cond_h_max, theta;
+    double cfl, tol, cond_h_max, theta, delta_ppn;
     if (!deck.get("MOMOPT_MOMENT_ORDER", order)) fail("momopt_init.cpp", 56);
     if (!deck.get("MOMOPT_QUAD_ORDER", qorder)) fail("momopt_init.cpp", 57);
     if (!deck.get("MOMOPT_CFL_FACTOR", cfl)) fail("momopt_init.cpp", 58);
@@ -154,10 +154,11 @@ int main(int argc, char **argv) {
     if (!deck.get("MOMOPT_MAX_ITER", max_iter)) fail("momopt_init.cpp", 62);
     if (!deck.get("MOMOPT_USE_GAUNT", use_gaunt)) fail("momopt_init.cpp", 64);
     if (!deck.get("MOMOPT_THETA", theta)) fail("momopt_init.cpp", 65);
+    if (!deck.get("MOMOPT_DELTA_PPN", delta_ppn)) fail("momopt_init.cpp", 66);
     if (!deck.get("MOMOPT_MOMENT_TYPE", mtype)) fail("momopt_init.cpp", 67);
     if (!deck.get("MOMOPT_OPTIMIZATION_TYPE", otype)) fail("momopt_init.cpp", 68);
-    if (mtype != 0 || otype != 0) {
-        std::printf("only MOMOPT_MOMENT_TYPE 0 (Mn) with MOMOPT_OPTIMIZATION_TYPE 0 is built\n");
+    if ((mtype != 0 && mtype != 1) || otype != 0) {
+        std::printf("only MOMOPT_MOMENT_TYPE 0 (Mn) or 1 (PPn) with MOMOPT_OPTIMIZATION_TYPE 0 is built\n");
         return 1;
     }
     int prec = MNC_PREC_FP64, warm = MNC_WARM_PREVIOUS, device = 0, ls = MNC_LS_REFERENCE;
@@ -185,14 +186,17 @@ int main(int argc, char **argv) {
     std::vector<double> mom(nc * nm, 0.0), alpha(nc * nm, 0.0);
     for (size_t c = 0; c < nc; ++c) {                // momopt_init.cpp:120-143
         mom[c * nm] = g.init[c] * 2.0 * std::sqrt(M_PI);
-        alpha[c * nm] = 2.0 * std::sqrt(M_PI) * std::log(mom[c * nm] / (2.0 * std::sqrt(M_PI)));
+        if (mtype == 1)                              // PPn (:138-141)
+            alpha[c * nm] = mom[c * nm] - 4.0 * delta_ppn * M_PI / mom[c * nm];
+        else
+            alpha[c * nm] = 2.0 * std::sqrt(M_PI) * std::log(mom[c * nm] / (2.0 * std::sqrt(M_PI)));
     }
-    mnc_grid_desc gd{g.nx, g.ny, g.dx, g.dy, g.init_x, g.init_y, init_cond, theta};
+    mnc_grid_desc gd{g.nx, g.ny, g.dx, g.dy, g.init_x, g.init_y, init_cond, theta, delta_ppn};
     st = mnc_grid_create(ctx, &gd, mom.data(), alpha.data(), g.sigma_t.data(), g.sigma_s.data());
     if (st != MNC_OK) { std::printf("mnc_grid_create: %s\n", mnc_last_error(ctx)); return 1; }
     const double max_dt = cfl * (2.0 / (2.0 + theta)) * (g.dx * g.dy) / (g.dx + g.dy);   // :72
     std::printf("--- Init done ---\n");
-    mnc_options opt{max_iter, tol, cond_h_max, prec, warm, ls};
+    mnc_options opt{max_iter, tol, cond_h_max, prec, warm, ls, delta_ppn};
 
     double t = 0, dt = max_dt;                        // main.cpp:204-230
     write_output(g, ctx, nm, t);

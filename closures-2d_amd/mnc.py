@@ -18,6 +18,7 @@ OK, ERR_ARG, ERR_U0, ERR_MAXREG, ERR_HIP, ERR_STATE = 0, 1, 2, 3, 4, 5
 PREC_FP64, PREC_MIXED = 0, 1
 LS_REFERENCE, LS_ROBUST = 0, 1
 WARM_PREVIOUS, WARM_ISOTROPIC, WARM_LEARNED = 0, 1, 2
+MOMENT_MN, MOMENT_PPN = 0, 1
 KCLASS_NEWTON, KCLASS_FLUX, KCLASS_OTHER, KCLASS_MLP = 0, 1, 2, 3
 
 _dp = C.POINTER(C.c_double)
@@ -31,7 +32,8 @@ class Setup(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("tol_rel", C.c_double), ("cond_h_max", C.c_double),
-                ("precision", C.c_int), ("warm_start", C.c_int), ("line_search", C.c_int)]
+                ("precision", C.c_int), ("warm_start", C.c_int), ("line_search", C.c_int),
+                ("delta_ppn", C.c_double)]
 
 
 class Outs(C.Structure):
@@ -41,7 +43,7 @@ class Outs(C.Structure):
 class GridDesc(C.Structure):
     _fields_ = [("num_cells_x", C.c_int), ("num_cells_y", C.c_int), ("dx", C.c_double),
                 ("dy", C.c_double), ("init_x", C.c_double), ("init_y", C.c_double),
-                ("init_cond", C.c_int), ("theta", C.c_double)]
+                ("init_cond", C.c_int), ("theta", C.c_double), ("delta_ppn", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -120,16 +122,16 @@ def _p(a):
 
 
 def options(max_iter=100, tol=1e-8, cond_h_max=1e10, precision=PREC_FP64, warm_start=WARM_PREVIOUS,
-            line_search=LS_REFERENCE):
-    return Options(max_iter, tol, cond_h_max, precision, warm_start, line_search)
+            line_search=LS_REFERENCE, delta_ppn=0.0):
+    return Options(max_iter, tol, cond_h_max, precision, warm_start, line_search, delta_ppn)
 
 
 class Context:
     """One mnc_ctx: tables, a HIP stream and (optionally) one grid."""
 
-    def __init__(self, moment_order, quad_order, use_gaunt=False, device=0):
+    def __init__(self, moment_order, quad_order, use_gaunt=False, device=0, moment_type=MOMENT_MN):
         h = C.c_void_p()
-        st = lib().mnc_create(C.byref(Setup(moment_order, quad_order, int(bool(use_gaunt)), 0, device)),
+        st = lib().mnc_create(C.byref(Setup(moment_order, quad_order, int(bool(use_gaunt)), moment_type, device)),
                               C.byref(h))
         if st != OK:
             raise MncError(st, "mnc_create failed")
@@ -139,6 +141,7 @@ class Context:
         self.nq = L.mnc_num_quad(h)
         self.np2 = L.mnc_table_stride(h)
         self.use_gaunt = bool(use_gaunt)
+        self.moment_type = moment_type
         self.grid = None
 
     def close(self):
@@ -209,8 +212,9 @@ class Context:
         return st, u, alpha, it, r, ng
 
     # ---- grid ----
-    def grid_create(self, nx, ny, dx, dy, init_x, init_y, init_cond, theta, moments, alpha, sigma_t, sigma_s):
-        gd = GridDesc(nx, ny, dx, dy, init_x, init_y, init_cond, theta)
+    def grid_create(self, nx, ny, dx, dy, init_x, init_y, init_cond, theta, moments, alpha, sigma_t, sigma_s,
+                    delta_ppn=0.0):
+        gd = GridDesc(nx, ny, dx, dy, init_x, init_y, init_cond, theta, delta_ppn)
         m = np.ascontiguousarray(moments, dtype=np.float64).ravel()
         a = np.ascontiguousarray(alpha, dtype=np.float64).ravel()
         st_ = np.ascontiguousarray(sigma_t, dtype=np.float64).ravel()

@@ -61,7 +61,7 @@ typedef struct {
     int moment_order;          /* MOMOPT_MOMENT_ORDER  N (n = (N+1)(N+2)/2)    */
     int quad_order;            /* MOMOPT_QUAD_ORDER    Q (even; nq = Q*Q)      */
     int use_gaunt;             /* MOMOPT_USE_GAUNT     0/1                     */
-    int moment_type;           /* MOMOPT_MOMENT_TYPE   0 = Mn (only one built)  */
+    int moment_type;           /* MOMOPT_MOMENT_TYPE: 0 = Mn, 1 = PPn           */
     int device;                /* HIP device ordinal; < 0 = host-only tables    */
 } mnc_setup;
 
@@ -73,6 +73,7 @@ typedef struct {
     int precision;             /* MNC_PREC_*                                    */
     int warm_start;            /* MNC_WARM_*                                    */
     int line_search;           /* MNC_LS_*                                      */
+    double delta_ppn;          /* MOMOPT_DELTA_PPN (contexts of moment type 1)  */
 } mnc_options;
 
 /* OUTS (src/momopt/opt/opt.h:25-32). */
@@ -91,6 +92,7 @@ typedef struct {
     double init_x, init_y;     /* c_initX, c_initY (lower-left domain corner)   */
     int init_cond;             /* 1 = lattice source term, 2 = periodic ghosts  */
     double theta;              /* MOMOPT_THETA (minmod slope factor)            */
+    double delta_ppn;          /* MOMOPT_DELTA_PPN: the PPn ansatz of the flux  */
 } mnc_grid_desc;
 
 /* Per-solve statistics (OPTIMIZATION_STATS, src/momopt/momopt_solver.h:33-41). */

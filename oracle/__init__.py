@@ -13,6 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("MN_ORACLE_LIB") or os.path.join(HERE, "libmnoracle.so")
 XP_LIB_PATH = os.path.join(HERE, "libmnoracle_xp.so")   # exp in long double (sensitivity probe)
+FMA_LIB_PATH = os.path.join(HERE, "libmnoracle_fma.so")  # FMA contraction (sensitivity probe)
 
 OK, ERR_ARG, ERR_U0, ERR_MAXREG, ERR_ALLOC = 0, 1, 2, 3, 4
 REGULARIZATIONS = (0.0, 1e-3, 1e-2, 1e-1, 5e-1)  # opt.h:39-40

@@ -98,18 +98,19 @@ def deck_text(d, t_final, extra=None):
 
 def gpu_grid_from_deck(mnc, gi, d, device=0):
     """A mnc.Context holding the grid of deck d, initialised by grid_init."""
-    ctx = mnc.Context(d["moment_order"], d["quad_order"], use_gaunt=d["use_gaunt"], device=device)
+    mt = d.get("moment_type", 0)
+    ctx = mnc.Context(d["moment_order"], d["quad_order"], use_gaunt=d["use_gaunt"], device=device, moment_type=mt)
     g = gi.initial_state(d["num_cells_x"], d["num_cells_y"], d["a_x"], d["b_x"], d["a_y"], d["b_y"],
                          d["init_cond"], d["gaussian_sigma"], d["floor_value"], d["sigma"], ctx.nm,
-                         d["cfl_factor"], d["theta"])
+                         d["cfl_factor"], d["theta"], moment_type=mt, delta_ppn=d.get("delta_ppn", 1e-10))
     ctx.grid_create(g.nx, g.ny, g.dx, g.dy, g.init_x, g.init_y, d["init_cond"], d["theta"],
-                    g.moments, g.alpha, g.sigma_t, g.sigma_s)
+                    g.moments, g.alpha, g.sigma_t, g.sigma_s, delta_ppn=d.get("delta_ppn", 0.0))
     return ctx, g
 
 
 def mn_options(mnc, d, **over):
     return mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"],
-                       line_search=d.get("line_search", 0), **over)
+                       line_search=d.get("line_search", 0), delta_ppn=d.get("delta_ppn", 0.0), **over)
 
 
 def config2_batch(w, p, xi, eta, nm, n, rng, near_boundary_frac=0.2, kappa=400.0, scale_k=3.0):

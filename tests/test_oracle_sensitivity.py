@@ -1,5 +1,7 @@
 """How far the reference algorithm's own output moves under a last-bit change
-of exp: the oracle against its ORC_EXP_LONG build (exp in long double).
+of exp: the oracle against its ORC_EXP_LONG build (exp in long double); and,
+for the PPn lattice regression deck, under FMA contraction (the oracle built
+with -ffp-contract=fast -mfma), which is what the GPU's arithmetic differs by.
 
 This bounds what any non-bit-identical implementation (the GPU path included)
 can reproduce.  On the M2 Gaussian deck (50x50, sigma 0.1, Q=12) at MOMOPT_TOL
@@ -50,3 +52,39 @@ def test_reference_rounding_sensitivity(tmp_path, tol, lo, hi):
     rel = np.abs(a - b).max() / np.abs(a).max()
     print("tol %.0e: state moved by %.3e; regularization histograms %s vs %s" % (tol, rel, ha, hb))
     assert lo <= rel <= hi
+
+
+PPN_SCRIPT = r'''
+import sys
+import numpy as np
+sys.path.insert(0, %(root)r)
+sys.path.insert(0, %(tests)r)
+import oracle as o
+from helpers import FIXTURE_T_FINAL, fixture_deck
+s = o.Solver(fixture_deck("lattice", moment_type=1, delta_ppn=1e-10), nthreads=8)
+s.run(FIXTURE_T_FINAL)
+np.save(%(out)r, s.moments())
+'''
+
+
+def test_ppn_lattice_fma_sensitivity(tmp_path):
+    """The PPn lattice deck (M3 Q=30, delta 1e-10, to t = 1.03) moves by ~1e-8
+    in density between the no-FMA and the FMA builds of the same oracle, while
+    the other three decks stay at rounding level: the GPU path (FMA) is held to
+    the oracle at that measured level on this deck."""
+    import oracle
+    from helpers import density, rel_l2
+    out = {}
+    for tag, lib in (("base", None), ("fma", oracle.FMA_LIB_PATH)):
+        path = str(tmp_path / ("%s.npy" % tag))
+        env = dict(os.environ)
+        if lib:
+            env["MN_ORACLE_LIB"] = lib
+        r = subprocess.run([sys.executable, "-c", PPN_SCRIPT % dict(root=ROOT, tests=os.path.join(ROOT, "tests"),
+                                                                     out=path)],
+                           env=env, capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stderr
+        out[tag] = np.load(path)
+    rel = rel_l2(density(out["fma"]), density(out["base"]))
+    print("PPn lattice: no-FMA vs FMA oracle density rel L2 %.3e" % rel)
+    assert 1e-10 <= rel <= 5e-8
