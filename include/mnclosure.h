@@ -1,5 +1,5 @@
 /*
- * mnclosure.h -- C-ABI of the MI355X Mn entropy-closure hot path.
+ * mnclosure.h -- C-ABI of the MI355X Mn (and PPn) closure hot path.
  *
  * The reference (ckrisgarrett/closures-2d) runs the Mn closure as one host
  * call per cell, opt() (src/momopt/opt/opt.h:43-44), from an OpenMP loop in
@@ -7,6 +7,14 @@
  * consumes the ansatz in MomOptSolver::solveFlux / update
  * (src/momopt/momopt_update.cpp:96-202,283-382).  This header is the thin
  * boundary those calls cross into hand-written gfx950 HIP kernels.
+ *
+ * Moment types (mnc_setup.moment_type, MOMOPT_MOMENT_TYPE): 0 = Mn, the
+ * entropy closure with the exp ansatz; 1 = PPn, the positive Pn closure with
+ * the ansatz a/2 + sqrt(a^2/4 + delta) of a = alpha.m (fobj.cpp:137-236,
+ * 355-428; opt.cpp:113-141; momopt_update.cpp:54-57).  PPn runs the FP64 form
+ * for moment orders 1-4 (options.delta_ppn > 0; the grid description carries
+ * the same delta for the flux); mixed precision, the learned warm start and
+ * M8 are Mn only and return MNC_ERR_ARG for PPn.
  *
  * Conventions
  *   - Plain C types only; every array is row-major double.
