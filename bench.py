@@ -25,7 +25,9 @@ Three forms run on the same steps:
                 plain-form lattice steps and an M4 Gaussian-pulse run (not timed,
                 reported in config.mlp)
 `value` is the learned+mixed form's rate (the metric's headline form); the
-other forms are reported beside it.
+other forms are reported beside it.  config.isolated (the 10^6-vector M4
+batch), config.m8 (M8 512^2) and config.ppn (the PPn closure on the same
+lattice) are reported with their own rates.
 
 Prints one JSON line.  Single GPU only.
 """
@@ -58,6 +60,16 @@ def _load(name):
     return mod
 
 
+def flops_per_iter_ppn(n, nq):
+    """The same count for PPn: alpha.m (2n nq), H sums (n(n+1) nq), g sums
+    (2n nq), the ansatz terms per point (~12 nq: G, its square root, log and
+    the weights), Cholesky, solve and condition estimate; a trial is alpha.m
+    and the f terms."""
+    f_iter = 2 * n * nq + n * (n + 1) * nq + 2 * n * nq + 12 * nq + n ** 3 / 3.0 + 2 * n * n + 8 * n * n
+    f_trial = 2 * n * nq + 8 * nq
+    return f_iter, f_trial
+
+
 def flops_per_iter(n, nq):
     """Algorithmic FP64 flops of one Newton iteration (full f, g, H) and of one
     Armijo trial (f only): alpha.m (2n nq), lower-triangle M^T D M (n(n+1) nq),
@@ -68,12 +80,13 @@ def flops_per_iter(n, nq):
 
 
 def make_grid(mnc, gi, d):
-    ctx = mnc.Context(d["moment_order"], d["quad_order"], use_gaunt=d["use_gaunt"])
+    mt = d.get("moment_type", 0)
+    ctx = mnc.Context(d["moment_order"], d["quad_order"], use_gaunt=d["use_gaunt"], moment_type=mt)
     g = gi.initial_state(d["num_cells_x"], d["num_cells_y"], d["a_x"], d["b_x"], d["a_y"], d["b_y"],
                          d["init_cond"], d["gaussian_sigma"], d["floor_value"], d["sigma"], ctx.nm,
-                         d["cfl_factor"], d["theta"])
+                         d["cfl_factor"], d["theta"], moment_type=mt, delta_ppn=d.get("delta_ppn", 1e-10))
     ctx.grid_create(g.nx, g.ny, g.dx, g.dy, g.init_x, g.init_y, d["init_cond"], d["theta"],
-                    g.moments, g.alpha, g.sigma_t, g.sigma_s)
+                    g.moments, g.alpha, g.sigma_t, g.sigma_s, delta_ppn=d.get("delta_ppn", 0.0))
     return ctx, g
 
 
@@ -319,6 +332,37 @@ def bench_m8(mnc, gi, steps, n=512, sample=1000):
             "oracle_same_iterations": float(np.mean(itg == ito))}
 
 
+def bench_ppn(mnc, gi, d, steps, warmup, cpu_seconds):
+    """The PPn form of the same lattice workload (MOMOPT_MOMENT_TYPE 1,
+    MOMOPT_DELTA_PPN 1e-10; FP64 lane kernels and cooperative tail, robust line
+    search): converged closures/s and s/step over K steps from t = 0, the
+    Newton roofline as for the Mn forms, and the CPU oracle on the first steps
+    of the same run (cpu_seconds of wall clock, 0 skips it)."""
+    dp = dict(d, moment_type=1, delta_ppn=1e-10)
+    n = (dp["moment_order"] + 1) * (dp["moment_order"] + 2) // 2
+    f_iter, f_trial = flops_per_iter_ppn(n, dp["quad_order"] ** 2)
+    opt = mnc.options(max_iter=dp["max_iter"], tol=dp["tol"], cond_h_max=dp["cond_h_max"],
+                      line_search=mnc.LS_ROBUST, delta_ppn=dp["delta_ppn"])
+    r = run_form(mnc, gi, dp, opt, steps, warmup)
+    st, kst = r["stats"], r["kstats"]
+    kn, kms = r["newton"]
+    achieved = (kst.iter_sum * f_iter + kst.trial_sum * f_trial) / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    out = {"closures_per_s": r["closures"] / r["wall"], "s_per_timestep": r["wall"] / len(r["dts"]),
+           "steps": len(r["dts"]), "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
+           "trials_per_closure": st.trial_sum / st.num_dual_solves, "max_iter": st.max_iter,
+           "reg_hist": list(st.hist_reg), "tuned": r["tuned"],
+           "kernel_ms": {"newton": kms, "flux": r["flux"][1]},
+           "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_PEAK_TFLOPS,
+                        "flops_per_iteration": f_iter, "flops_per_trial": f_trial}}
+    if cpu_seconds > 0:
+        cb = cpu_baseline(dp, cpu_seconds)
+        out["cpu_baseline"] = {"value": cb["value"], "unit": "closures/s", "cores": cb["cores"], "kind": "port",
+                               "sample": "oracle PPn on the first %d steps of the same run (%.1f s)"
+                                         % (cb["steps"], cb["wall"])}
+    return out
+
+
 def cpu_baseline(d, seconds):
     """The CPU oracle (FP64 C, OpenMP schedule(dynamic,1) over cells, as the
     reference's solveOptimization) on the first steps of the same run, for
@@ -397,6 +441,8 @@ def main():
     ap.add_argument("--isolated", type=int, default=1_000_000,
                     help="vectors of the isolated closure batch (0 skips it)")
     ap.add_argument("--m8-steps", type=int, default=20, help="steps of the M8 512^2 workload (0 skips it)")
+    ap.add_argument("--ppn-steps", type=int, default=569, help="steps of the PPn lattice form (0 skips it)")
+    ap.add_argument("--ppn-only", action="store_true", help="only the PPn lattice form (for profiling)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="force a launch/tuning setting after the autotuner (repeatable)")
     ap.add_argument("--stage-probe", choices=["plain-fp64", "mixed", "learned+mixed"], default=None,
@@ -419,6 +465,12 @@ def main():
     d = dict(LATTICE)
     if args.stage_probe:
         stage_probe(mnc, gi, d, args.stage_probe, args.probe_step)
+        return
+    if args.ppn_only:
+        r = bench_ppn(mnc, gi, d, args.ppn_steps, args.warmup, 0.0 if args.no_cpu else min(args.cpu_seconds, 10.0))
+        print(json.dumps({"metric": "converged PPn closures/s (M4 lattice 280^2, TOL 1e-8)",
+                          "value": r["closures_per_s"], "unit": "closures/s", "n_gpus": 1,
+                          "steps": r["steps"], "higher_is_better": True, "config": {"ppn": r}}))
         return
     n = (d["moment_order"] + 1) * (d["moment_order"] + 2) // 2
     nq = d["quad_order"] ** 2
@@ -495,6 +547,10 @@ def main():
     if args.m8_steps > 0 and not args.plain_only and not args.only:
         out["config"]["m8"] = bench_m8(mnc, gi, args.m8_steps)
         progress("M8 done")
+    if args.ppn_steps > 0 and not args.plain_only and not args.only:
+        out["config"]["ppn"] = bench_ppn(mnc, gi, d, args.ppn_steps, args.warmup,
+                                         0.0 if args.no_cpu else min(args.cpu_seconds, 10.0))
+        progress("PPn done")
     if not args.no_cpu:
         progress("CPU baseline")
         cb = cpu_baseline(d, args.cpu_seconds)
