@@ -167,13 +167,14 @@ T0 = time.perf_counter()
 FORCE_TUNE = {}
 
 
-def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=60, autotune=True, tune_step=None):
+def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=None, autotune=True, tune_step=None):
     """W warm-up steps, then (untimed) on to step `tune_step` (default: the
     middle of the run) and the library's autotuner on that state, then (from
     t = 0 again) K steps timed by wall clock with the tuned launch shape
     (hipGraph replay if the tuner picked it; per-launch timing off), then the
-    first min(K, kernel_steps) steps again, eagerly, with HIP-event timing of
-    every launch for the per-kernel-class split and the roofline."""
+    same K steps (or the first kernel_steps of them) again from t = 0, eagerly,
+    with HIP-event timing of every launch for the per-kernel-class split and
+    the roofline."""
     ctx, g = make_grid(mnc, gi, d)
     if mlp_params is not None:
         ctx.mlp_set_params(mlp_params)
@@ -188,7 +189,7 @@ def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=60, a
     for k, v in FORCE_TUNE.items():
         ctx.tune(k, v)
     tuned = {k: ctx.get_tune(k) for k in ("graph", "rounds", "tail_mode", "tail_blocks", "mix_blocks", "flux_tile",
-                                          "mix_cond_log10", "mix_defer", "learned_gate_log10")}
+                                          "mix_cond_log10", "mix_defer", "mix_split", "learned_gate_log10")}
     ctx.grid_set(g.moments, g.alpha)        # back to t = 0
     ctx.reset_stats()
     ctx.sync()
@@ -201,7 +202,7 @@ def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=60, a
     m, a = ctx.grid_get()
     progress("timed %d steps in %.2f s" % (steps, wall))
     # eager pass with per-launch timing
-    ks = min(steps, kernel_steps)
+    ks = steps if kernel_steps is None else min(steps, kernel_steps)
     ctx.grid_set(g.moments, g.alpha)
     ctx.reset_stats()
     ctx.timing_enable(True)
@@ -543,7 +544,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": stage_traffic(form),
                      "kernel": "Newton solve per stage (init + lockstep rounds + tail), %s form, HIP events "
-                               "over the first %d steps; flops are the reference algorithm's FP64 count "
+                               "over the %d timed steps run again eagerly; flops are the reference algorithm's FP64 count "
                                "(iterations x 73.0 kflop + trials x 8.2 kflop); traffic: HBM bytes per stage from "
                                "calibrated FETCH_SIZE + WRITE_SIZE (profiles/newton_stage_traffic.md)"
                                % (form, best["kernel_steps"]),

@@ -45,8 +45,9 @@ struct mnc_ctx {
     int rounds = 6;
     int tail_blocks = 0;             // 0: 2 per CU
     int tail_mode = 0;               // 0: wave per cell, 1: lane per cell, 2: wave sweep + serial logic
-    int mix_cond_log10 = 5;          // mixed steps need cond(H) <= 10^this, else FP64
+    int mix_cond_log10 = 9;          // mixed steps need cond(H) <= 10^this, else FP64
     int mix_defer = 0;               // 0: redone steps wait for the FP64 tail; 1: FP64 lockstep rounds
+    int mix_split = 3;               // bf16 parts of e and of the table in the mixed Hessian products
     int mix_blocks = 0;              // 0: one per CU
     int flux_tile = 3;               // 0: ansatz to HBM + flux; 1: fused, 8x8 tiles; 2: fused, 16x16;
                                      // 3: fused, 16x16, directions split into kFluxGroups
@@ -312,6 +313,7 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     lp.tail_mode = ctx->tail_mode;
     lp.mix_blocks = ctx->mix_blocks > 0 ? ctx->mix_blocks : ctx->num_cu;
     lp.defer_rounds = ctx->mix_defer;
+    lp.mix_split = ctx->mix_split;
     const bool gaunt = t.use_gaunt != 0;
     auto launch = [&](const mnc::NewtonArgs &aa, const mnc::LaunchPlan &p) -> int {
         if (mixed) {
@@ -902,6 +904,11 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         ctx->mix_defer = value;
         return MNC_OK;
     }
+    if (std::strcmp(key, "mix_split") == 0) {
+        if (value < 2 || value > 3) return ctx_fail(ctx, MNC_ERR_ARG, "mix_split is 2 or 3");
+        ctx->mix_split = value;
+        return MNC_OK;
+    }
     if (std::strcmp(key, "mix_blocks") == 0) {
         if (value < 1 || value > 65536) return ctx_fail(ctx, MNC_ERR_ARG, "mix_blocks out of range");
         ctx->mix_blocks = value;
@@ -926,6 +933,7 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
     if (std::strcmp(key, "tail_mode") == 0) { *value = ctx->tail_mode; return MNC_OK; }
     if (std::strcmp(key, "mix_cond_log10") == 0) { *value = ctx->mix_cond_log10; return MNC_OK; }
     if (std::strcmp(key, "mix_defer") == 0) { *value = ctx->mix_defer; return MNC_OK; }
+    if (std::strcmp(key, "mix_split") == 0) { *value = ctx->mix_split; return MNC_OK; }
     if (std::strcmp(key, "mix_blocks") == 0) {
         *value = ctx->mix_blocks > 0 ? ctx->mix_blocks : ctx->num_cu;
         return MNC_OK;
@@ -985,20 +993,23 @@ int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, cha
     if ((st = save())) { (void)hipFree(bk); ctx->timing = timing; return st; }
     struct Knob { const char *key; std::vector<int> vals; int *slot; };
     const int cu = ctx->num_cu;
-    std::vector<Knob> knobs = {
-        {"graph", {0, 1}, &ctx->use_graph},
-        {"flux_tile", {0, 1, 2, 3}, &ctx->flux_tile},
-        {"rounds", {2, 4, 6, 8, 12}, &ctx->rounds},
-        {"tail_mode", {0, 1, 2}, &ctx->tail_mode},
-        {"tail_blocks", {cu, 2 * cu, 4 * cu}, &ctx->tail_blocks},
-    };
+    // Knobs that change which work is done (the network gate, which steps run
+    // in mixed precision) come first, so the launch shapes are tuned for the
+    // work that is left.
+    std::vector<Knob> knobs;
     if (opt->warm_start == MNC_WARM_LEARNED)
         knobs.push_back({"learned_gate_log10", {-3, -2, -1, 0, 1}, &ctx->learned_gate_log10});
     if (opt->precision == MNC_PREC_MIXED) {
-        knobs.push_back({"mix_cond_log10", {4, 5, 6}, &ctx->mix_cond_log10});
+        knobs.push_back({"mix_cond_log10", {5, 6, 7, 8, 9, 10}, &ctx->mix_cond_log10});
+        knobs.push_back({"mix_split", {2, 3}, &ctx->mix_split});
         knobs.push_back({"mix_defer", {0, 1}, &ctx->mix_defer});
-        knobs.push_back({"mix_blocks", {cu / 2, cu, 2 * cu}, &ctx->mix_blocks});
     }
+    knobs.push_back({"graph", {0, 1}, &ctx->use_graph});
+    knobs.push_back({"flux_tile", {0, 1, 2, 3}, &ctx->flux_tile});
+    knobs.push_back({"rounds", {2, 4, 6, 8, 12}, &ctx->rounds});
+    knobs.push_back({"tail_mode", {0, 1, 2}, &ctx->tail_mode});
+    knobs.push_back({"tail_blocks", {cu, 2 * cu, 4 * cu}, &ctx->tail_blocks});
+    if (opt->precision == MNC_PREC_MIXED) knobs.push_back({"mix_blocks", {cu / 2, cu, 2 * cu}, &ctx->mix_blocks});
     double best = 0.0;
     if ((st = trial(&best)) == MNC_OK) {
         char line[160];

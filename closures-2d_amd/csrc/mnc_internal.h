@@ -131,6 +131,7 @@ struct LaunchPlan {
     bool init = true, tail = true;
     int tail_blocks = 0, tail_mode = 0;
     int mix_blocks = 0, defer_rounds = 0;   // mixed precision only
+    int mix_split = 3;                       // bf16 parts in the mixed sweep's Hessian products (2 or 3)
 };
 bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, const LaunchPlan &lp, hipStream_t s);
 // The same with the lockstep rounds in mixed precision (MFMA sweeps); the tail
@@ -146,6 +147,7 @@ bool launch_newton_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, hipSt
 bool launch_newton_g64_init(const NewtonArgs &a, int nm, int np2, hipStream_t s);
 bool launch_newton_g64_rounds(const NewtonArgs &a, int nm, int np2, int &lin, int rounds, int blocks, hipStream_t s);
 constexpr int kMixMaxTiles = 16;   // quadrature tiles of 16 points held in LDS (nq <= 256)
+constexpr int kMixParts = 3;       // bf16 parts per table value in the mixed sweep's Hessian table
 int mix_tiles(int nm);             // 16-column tiles of the order-2N moments the mixed kernel expects
 int newton_block();
 // Zeroes the solve-control words (a kernel, so the step can be captured).

@@ -1802,9 +1802,13 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
 //   z = alpha.m       v_mfma_f64_16x16x4f64, (16 points x 16 moments) x (16 moments x 16 cells)
 //   e = exp(z)        FP64, per lane
 //   G += e (w m, w)   v_mfma_f64_16x16x4f64: FP64 sums of w e m_i and w e (columns 0..14, 15)
-//   Hm += e (w m2N)   v_mfma_f32_16x16x32_bf16 on hi/lo bf16 pairs of e and of the
-//                     table (three products, FP32 sums): the order-2N moments from
-//                     which the logic forms H through the Gaunt map.
+//   Hm += e (w m2N)   v_mfma_f32_16x16x32_bf16 on bf16 parts of e and of the
+//                     table: the order-2N moments from which the logic forms H
+//                     through the Gaunt map.  NS = 2: hi/lo pairs, three products
+//                     (hi hi, hi lo, lo hi; relative error ~2^-16), FP32 sums over
+//                     the whole quadrature.  NS = 3: hi/mid/lo triples, the six
+//                     products down to order 2^-16 (error ~2^-24), FP32 sums per
+//                     32 points added in FP64.
 // The operand tables (built by build_mix_tables) sit in LDS for the whole
 // launch; blocks loop over tiles.
 typedef double mf64x4 __attribute__((ext_vector_type(4)));
@@ -1814,11 +1818,11 @@ typedef unsigned short mu16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kMixWaves = 16;
 
-template <int NNT>
+template <int NNT, int NS>
 struct MixShared {
     double z[kMixMaxTiles][4][64];
     double g[kMixMaxTiles][4][64];
-    mu16x8 h[kMixMaxTiles / 2][NNT][2][64];
+    mu16x8 h[kMixMaxTiles / 2][NNT][NS][64];
 };
 
 // x = hi + lo up to 2^-17 |x|, both rounded to nearest even (non-finite x
@@ -1831,12 +1835,29 @@ __device__ __forceinline__ void split_bf16(float x, unsigned short &hi, unsigned
     lo = (unsigned short)((v + 0x7fffu + ((v >> 16) & 1u)) >> 16);
 }
 
-template <int NM, int NP2, int NNT>
+// x = hi + mid + lo up to 2^-25 |x| (x - hi and x - hi - mid are exact in FP32)
+__device__ __forceinline__ unsigned short bf16_rne_bits(float x) {
+    const unsigned u = __float_as_uint(x);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ float bf16_bits_val(unsigned short h) { return __uint_as_float(unsigned(h) << 16); }
+
+__device__ __forceinline__ void split3_bf16(float x, unsigned short &hi, unsigned short &mid, unsigned short &lo) {
+    hi = bf16_rne_bits(x);
+    const float r = x - bf16_bits_val(hi);
+    mid = bf16_rne_bits(r);
+    lo = bf16_rne_bits(r - bf16_bits_val(mid));
+}
+
+template <int NM, int NP2, int NNT, int NS>
 __global__ __launch_bounds__(64 * kMixWaves) void sweep_mix_kernel(NewtonArgs A, int lin) {
     using St = Store<NM, NP2>;
     static_assert(NM <= 15, "moment 15 of the sum tile holds sum w e");
     static_assert(16 * NNT <= St::nacc, "Hessian moments exceed the store");
-    __shared__ MixShared<NNT> sh;
+    static_assert(NS == 2 || NS == 3, "two or three bf16 parts");
+    static_assert(NS <= kMixParts, "the table holds kMixParts parts");
+    __shared__ MixShared<NNT, NS> sh;
     if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl[lin ^ 1] = 0;
     const int count = A.ctl[lin];
     const int ntiles = (count + 15) >> 4;
@@ -1848,9 +1869,13 @@ __global__ __launch_bounds__(64 * kMixWaves) void sweep_mix_kernel(NewtonArgs A,
             sz[i] = A.mz[i];
             sg[i] = A.mg[i];
         }
+        // table blocks (kb, n) hold kMixParts parts of 64 lanes; the first NS are used
         const mu16x8 *gh = reinterpret_cast<const mu16x8 *>(A.mh);
         mu16x8 *hh = &sh.h[0][0][0][0];
-        for (int i = threadIdx.x; i < nkb * NNT * 128; i += blockDim.x) hh[i] = gh[i];
+        for (int i = threadIdx.x; i < nkb * NNT * NS * 64; i += blockDim.x) {
+            const int l = i & 63, rest = i >> 6;
+            hh[i] = gh[((rest / NS) * kMixParts + rest % NS) * 64 + l];
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1867,10 +1892,13 @@ __global__ __launch_bounds__(64 * kMixWaves) void sweep_mix_kernel(NewtonArgs A,
         }
         mf64x4 gacc = {0.0, 0.0, 0.0, 0.0};
         mf32x4 hacc[NNT];
+        mf64x4 hsum[NS == 3 ? NNT : 1];
 #pragma unroll
         for (int n = 0; n < NNT; ++n) hacc[n] = mf32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int n = 0; n < (NS == 3 ? NNT : 1); ++n) hsum[n] = mf64x4{0.0, 0.0, 0.0, 0.0};
         for (int kb = 0; kb < nkb; ++kb) {
-            mu16x8 ahi, alo;
+            mu16x8 ahi, alo, amid;
 #pragma unroll
             for (int hh = 0; hh < 2; ++hh) {
                 const int T = 2 * kb + hh;
@@ -1882,20 +1910,44 @@ __global__ __launch_bounds__(64 * kMixWaves) void sweep_mix_kernel(NewtonArgs A,
                 for (int r = 0; r < 4; ++r) {
                     const double e = exp(z[r]);
                     gacc = __builtin_amdgcn_mfma_f64_16x16x4f64(e, sh.g[T][r][lane], gacc, 0, 0, 0);
-                    unsigned short hi, lo;
-                    split_bf16((float)e, hi, lo);
+                    unsigned short hi, mid, lo;
+                    if constexpr (NS == 3) {
+                        split3_bf16((float)e, hi, mid, lo);
+                        amid[4 * hh + r] = mid;
+                    } else {
+                        split_bf16((float)e, hi, lo);
+                    }
                     ahi[4 * hh + r] = hi;
                     alo[4 * hh + r] = lo;
                 }
             }
             const mbf16x8 Ahi = __builtin_bit_cast(mbf16x8, ahi), Alo = __builtin_bit_cast(mbf16x8, alo);
+            if constexpr (NS == 3) {
+                const mbf16x8 Amid = __builtin_bit_cast(mbf16x8, amid);
 #pragma unroll
-            for (int n = 0; n < NNT; ++n) {
-                const mbf16x8 Bhi = __builtin_bit_cast(mbf16x8, sh.h[kb][n][0][lane]);
-                const mbf16x8 Blo = __builtin_bit_cast(mbf16x8, sh.h[kb][n][1][lane]);
-                hacc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi, Bhi, hacc[n], 0, 0, 0);
-                hacc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi, Blo, hacc[n], 0, 0, 0);
-                hacc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Alo, Bhi, hacc[n], 0, 0, 0);
+                for (int n = 0; n < NNT; ++n) {
+                    const mbf16x8 Bhi = __builtin_bit_cast(mbf16x8, sh.h[kb][n][0][lane]);
+                    const mbf16x8 Bmid = __builtin_bit_cast(mbf16x8, sh.h[kb][n][1][lane]);
+                    const mbf16x8 Blo = __builtin_bit_cast(mbf16x8, sh.h[kb][n][NS - 1][lane]);
+                    // smallest terms first
+                    mf32x4 t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi, Blo, mf32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Amid, Bmid, t, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Alo, Bhi, t, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi, Bmid, t, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Amid, Bhi, t, 0, 0, 0);
+                    t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi, Bhi, t, 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) hsum[n][r] += (double)t[r];
+                }
+            } else {
+#pragma unroll
+                for (int n = 0; n < NNT; ++n) {
+                    const mbf16x8 Bhi = __builtin_bit_cast(mbf16x8, sh.h[kb][n][0][lane]);
+                    const mbf16x8 Blo = __builtin_bit_cast(mbf16x8, sh.h[kb][n][1][lane]);
+                    hacc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi, Bhi, hacc[n], 0, 0, 0);
+                    hacc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ahi, Blo, hacc[n], 0, 0, 0);
+                    hacc[n] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Alo, Bhi, hacc[n], 0, 0, 0);
+                }
             }
         }
         // Result rows are cells: the cell of slot 16 tile + i is held by lane i.
@@ -1912,7 +1964,8 @@ __global__ __launch_bounds__(64 * kMixWaves) void sweep_mix_kernel(NewtonArgs A,
             }
             if (ch >= 0) {
 #pragma unroll
-                for (int n = 0; n < NNT; ++n) S.d(St::F_ACC + 16 * n + j, ch) = (double)hacc[n][r];
+                for (int n = 0; n < NNT; ++n)
+                    S.d(St::F_ACC + 16 * n + j, ch) = NS == 3 ? hsum[n < (NS == 3 ? NNT : 1) ? n : 0][r] : (double)hacc[n][r];
             }
         }
     }
@@ -2215,7 +2268,10 @@ void launch_mixed_all(const NewtonArgs &a, const LaunchPlan &lp, hipStream_t s) 
     // lane kernels (defer_rounds 1); what is left after the rounds goes to the tail.
     int lin = lp.r0 & 1, flin = 2 + (defer_rounds ? (lp.r0 & 1) : 0);
     for (int r = lp.r0; r < lp.r1; ++r) {
-        hipLaunchKernelGGL((sweep_mix_kernel<NM, NP2, NNT>), dim3(mix_blocks), dim3(64 * kMixWaves), 0, s, a, lin);
+        if (lp.mix_split == 3)
+            hipLaunchKernelGGL((sweep_mix_kernel<NM, NP2, NNT, 3>), dim3(mix_blocks), dim3(64 * kMixWaves), 0, s, a, lin);
+        else
+            hipLaunchKernelGGL((sweep_mix_kernel<NM, NP2, NNT, 2>), dim3(mix_blocks), dim3(64 * kMixWaves), 0, s, a, lin);
         if (defer_rounds) {
             hipLaunchKernelGGL((sweep_kernel<NM, NP2, G, RS>), dim3(nb), dim3(kBlock), 0, s, a, flin);
             hipLaunchKernelGGL((logic_kernel<NM, NP2, G, true>), dim3(nb), dim3(kBlock), 0, s, a, lin, flin ^ 1);

@@ -242,8 +242,10 @@ int build_mix_tables(int moment_order, int quad_order, MixTables &m) {
                 const int q = 16 * T + mfma_f64_row(k4, l / 16), j = l % 16;
                 m.g[(size_t(T) * 4 + k4) * 64 + l] = j < nm ? W(q) * P(q, j) : (j == 15 ? W(q) : 0.0);
             }
+    // three bf16 parts per table value: hi, then the rounded remainders mid
+    // and lo (the two-part kernels read hi and mid)
     const int nkb = m.nt / 2;
-    m.h.assign(size_t(nkb) * m.nnt * 2 * 64 * 8, 0);
+    m.h.assign(size_t(nkb) * m.nnt * kMixParts * 64 * 8, 0);
     for (int kb = 0; kb < nkb; ++kb)
         for (int nt = 0; nt < m.nnt; ++nt)
             for (int l = 0; l < 64; ++l)
@@ -251,10 +253,13 @@ int build_mix_tables(int moment_order, int quad_order, MixTables &m) {
                     const int q = 16 * (2 * kb + tt / 4) + mfma_f64_row(tt % 4, l / 16), col = 16 * nt + l % 16;
                     const float v = (float)(col < t.nmm ? W(q) * P(q, col) : 0.0);
                     const unsigned short hi = bf16_rne(v);
-                    const unsigned short lo = bf16_rne(v - bf16_val(hi));
-                    const size_t base = ((size_t(kb) * m.nnt + nt) * 2) * 64 * 8;
+                    const float r1 = v - bf16_val(hi);
+                    const unsigned short mid = bf16_rne(r1);
+                    const unsigned short lo = bf16_rne(r1 - bf16_val(mid));
+                    const size_t base = ((size_t(kb) * m.nnt + nt) * kMixParts) * 64 * 8;
                     m.h[base + size_t(l) * 8 + tt] = hi;
-                    m.h[base + 64 * 8 + size_t(l) * 8 + tt] = lo;
+                    m.h[base + 64 * 8 + size_t(l) * 8 + tt] = mid;
+                    m.h[base + 2 * 64 * 8 + size_t(l) * 8 + tt] = lo;
                 }
     m.gptr = t.gptr;
     m.gk = t.gk;

@@ -185,10 +185,14 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *                  1: one lane per cell; 2: one wave per cell, algebra per lane
  *   "mix_cond_log10"  MNC_PREC_MIXED: a Newton step uses the mixed-precision
  *                  Hessian only while its condition estimate is <= 10^value
- *                  (0-10, default 5); otherwise the step is redone in FP64
+ *                  (0-10, default 9); otherwise the step is redone in FP64
  *   "mix_defer"    MNC_PREC_MIXED: 0: a step to redo in FP64 waits for the
  *                  FP64 tail (default); 1: it joins FP64 lockstep rounds
  *                  that run beside the mixed-precision ones
+ *   "mix_split"    MNC_PREC_MIXED: bf16 parts of the quadrature values and of
+ *                  the table in the Hessian products: 2 (three products,
+ *                  FP32 sums, ~1e-5 relative) or 3 (default; six products,
+ *                  FP64 sums per 32 points, ~1e-7 relative)
  *   "mix_blocks"   workgroups of the mixed-precision sweep (1-65536)
  *   "learned_gate_log10"  MNC_WARM_LEARNED: the network is consulted for a
  *                  cell only when the gradient norm at its start exceeds
@@ -210,13 +214,14 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value);
 int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value);
 
 /* Autotuner (no counterpart in the reference): times `steps` mnc_update steps
- * of dt from the current grid state for each candidate launch shape, one
- * coordinate-descent pass over "graph", "flux_tile", "rounds", "tail_mode",
- * "tail_blocks", (learned warm start) "learned_gate_log10" and (mixed
- * precision) "mix_cond_log10", "mix_defer" and "mix_blocks", and keeps the
- * fastest.  Every candidate converges to the
- * same tolerance; "mix_cond_log10" trades mixed-precision steps against
- * FP64 ones.  The grid state
+ * of dt from the current grid state for each candidate setting, one
+ * coordinate-descent pass over, in this order, the knobs that change the work
+ * -- (learned warm start) "learned_gate_log10", (mixed precision)
+ * "mix_cond_log10", "mix_split", "mix_defer" -- then the launch shapes
+ * "graph", "flux_tile", "rounds", "tail_mode", "tail_blocks" and (mixed
+ * precision) "mix_blocks", and keeps the fastest.  Every candidate converges
+ * to the same tolerance; "mix_cond_log10" trades mixed-precision steps
+ * against FP64 ones.  The grid state
  * and the statistics are restored after every trial, so the call leaves the
  * solution where it was.  report (may be NULL) receives one "key=value ms/step"
  * line per trial.  The tail kernels sum in different orders, so the choice can

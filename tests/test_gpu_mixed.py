@@ -2,7 +2,8 @@
 
 The mixed path keeps f, g, the stop test, the Armijo test and the
 reconstruction in FP64 and takes only the Newton direction from a Hessian built
-on the matrix cores (bf16 hi/lo pairs, FP32 sums, ~1e-5 relative).  Its answers
+on the matrix cores (bf16 triples with FP64 sums per 32 points, ~1e-7
+relative; or, with mix_split 2, bf16 hi/lo pairs with FP32 sums, ~1e-5).  Its answers
 are therefore solutions of the same problem to the same tolerance as the
 reference's (||g|| <= tol ||u||), not bitwise equal to them: the checks are the
 tolerance-level ones the FP64 path also meets against the oracle when rounding
@@ -72,6 +73,17 @@ def test_mixed_batch_matches_oracle(order, Q, gaunt):
     R = _run(order, Q, gaunt, n=3000, tol=tol, seed=4321 + order)
     _check_solutions(R, tol)
     # the approximate Hessian costs at most a few extra iterations on average
+    assert R["itg"].mean() <= 1.5 * R["ito"].mean() + 0.5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,Q,gaunt", [(2, 12, False), (4, 16, False), (4, 16, True)])
+def test_mixed_two_part_split_matches_oracle(order, Q, gaunt):
+    """The two-part bf16 split (the default is three parts) with the cond bound
+    at 10^5: the same solutions as the oracle, in about as many iterations."""
+    tol = 1e-6 if gaunt else 1e-8
+    R = _run(order, Q, gaunt, n=3000, tol=tol, seed=991 + order, tune={"mix_split": 2, "mix_cond_log10": 5})
+    _check_solutions(R, tol)
     assert R["itg"].mean() <= 1.5 * R["ito"].mean() + 0.5
 
 
@@ -146,4 +158,8 @@ def test_mixed_lattice_steps_match_fp64():
     e_fo = rel_l2(density(runs[mnc.PREC_FP64]), density(mo))
     print("5 lattice steps, density rel-L2: mixed-fp64 %.3e mixed-oracle %.3e fp64-oracle %.3e" % (e_mf, e_mo, e_fo))
     assert e_fo <= 1e-12
-    assert e_mf <= 1e-8 and e_mo <= 1e-8
+    # each of the ten closure solves returns ||g|| <= tol ||u||: the FP64 path
+    # usually ends far below tol (quadratic convergence), the mixed one just
+    # below it (linear convergence with the approximate Hessian), so the runs
+    # differ by a small multiple of tol -- the same 4 tol as the per-cell checks
+    assert e_mf <= 4e-8 and e_mo <= 4e-8
