@@ -141,3 +141,41 @@ def test_cholesky_and_rcond_kernels():
         assert true_rc * (1 - 1e-12) <= rc <= 3 * true_rc
     B = np.diag([1.0, -1.0])
     assert o.lib().orc_cholesky(2, o._p(B)) == 2
+
+
+def test_rcond_estimate_within_exact_norm_when_ill_conditioned():
+    """The tail skips the condition estimate when 2 ||A||_1 ||A^-1||_1 is
+    within the limit, with ||A^-1||_1 computed from the factor: that relies on
+    the estimate (1 / (rcond ||A||_1)) never exceeding ||A^-1||_1 by more than
+    rounding, up to the regularization threshold kappa = 1e10.  Checked on
+    SPD matrices of order 15 with spectra spread over 1e2 .. 1e10 and a
+    spectrum with one isolated small eigenvalue, against the exact norm in
+    long double from the eigendecomposition."""
+    o = oracle()
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    n = 15
+    for trial in range(300):
+        kappa = 10.0 ** rng.uniform(2, 10)
+        Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+        if trial % 3 == 0:
+            lam = np.ones(n)
+            lam[rng.integers(n)] = 1.0 / kappa
+        else:
+            lam = np.exp(rng.uniform(0.0, np.log(kappa), n))
+            lam[0], lam[1] = 1.0, kappa
+        lam = lam * 10.0 ** rng.uniform(-3, 3)
+        A = (Q * lam) @ Q.T
+        A = 0.5 * (A + A.T)
+        L = A.copy()
+        if o.lib().orc_cholesky(n, o._p(L)) != 0:
+            continue
+        anorm = o.lib().orc_norm1(n, o._p(A))
+        rc = o.lib().orc_rcond(n, o._p(L), anorm)
+        Ql = Q.astype(np.longdouble)
+        inv = (Ql / lam.astype(np.longdouble)) @ Ql.T
+        exact = float(np.abs(inv).sum(axis=0).max())
+        est = 1.0 / (rc * anorm)
+        worst = max(worst, est / exact)
+    print("largest estimate / exact ||A^-1||_1: %.6f" % worst)
+    assert 0.0 < worst <= 1.0 + 1e-4
