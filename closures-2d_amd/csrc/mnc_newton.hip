@@ -252,6 +252,8 @@ __device__ __forceinline__ double wave_min(double v) {
 enum : int { ST_NEWTON = 0, ST_TRIAL = 1 };
 constexpr int kHalvings = 8;   // line-search halvings evaluated per wave sweep in the tail
 constexpr int kTailPts = 4;    // quadrature points per lane whose w e the tail keeps between passes
+constexpr int kTailWaves = 4;  // waves (cells in flight) per block of the cooperative tail
+constexpr int kFirstHalvings = 2;   // candidates in the first batch of halvings after a failed trial
 enum : int { LOGIC_CONTINUE = 0, LOGIC_DONE = 1, LOGIC_DEFER = 2, LOGIC_MLP = 3 };
 constexpr int kMlpList = kMlpListIndex;   // cells waiting for the learned warm start's network
 static_assert(kMlpListWord == kMlpListIndex + 2, "list word of cw()");
@@ -1296,20 +1298,50 @@ __device__ __forceinline__ double reg_inv_norm1(const double (&row)[NM], const d
     return wave_max(lane < NM ? c : 0.0);
 }
 
+// Per-wave state of the cooperative tail.  The direct Hessian's TRI sums are
+// accumulated in NPASS passes over the quadrature (a flat split of the packed
+// triangle: KH sums, then the rest), so that a pass's partial sums fit the
+// registers; the first pass also carries the PPn gradient sums.
 template <int NM, int NP2, bool GAUNT, int RS, bool PPN = false>
 struct CoopShared {
-    static constexpr int NA = GAUNT ? NP2 : Tri<NM>::at(RS, 0);
-    static constexpr int NB = (GAUNT || RS >= NM) ? 1 : Tri<NM>::size - Tri<NM>::at(RS, 0);
-    static constexpr int NG = PPN ? NM : 0;          // PPn: the gradient sums ride with the first pass
-    static constexpr int NPART = (NA + NG) > NB ? (NA + NG) : NB;
+    static constexpr int NPASS = (!GAUNT && RS < NM) ? 2 : 1;
+    static constexpr int TRI = Tri<NM>::size;
+    static constexpr int KH = GAUNT ? NP2 : (NPASS == 2 ? (TRI + 1) / 2 : TRI);
+    static constexpr int NG = PPN ? NM : 0;
+    static constexpr int NV0 = KH + NG, NV1 = NPASS == 2 ? TRI - KH : 0;
+    static constexpr int NV = NV0 > NV1 ? NV0 : NV1;      // partial sums per lane and pass
+    static constexpr int H1 = (NV + 1) / 2, H2 = (H1 + 1) / 2;   // values per lane after the two exchanges
     static constexpr int NACC = Store<NM, NP2>::nacc;
-    double part[NPART][65];   // padded: lane-contiguous writes, column sums conflict-free
+    double part[H2][65];      // the lane quads' partials, padded
     double accs[NACC];        // reduced sweep sums
     double L[NM][NM];         // H, then its Cholesky factor (lower)
     double x[NM], gvec[NM];
     double gsum[NM];          // PPn: sum w G m
-    double red[64];
 };
+
+// The quadrature table (transposed, pT[k][q]) and weights in LDS, shared by the
+// block's waves, when nq <= 64 kTailPts and the rows are the moments (direct
+// Hessian): LDS reads instead of L2 round trips in every sweep.
+template <int NM, bool ON>
+struct TailTable {
+    double t[ON ? NM : 1][ON ? 64 * kTailPts : 1];
+    double w[ON ? 64 * kTailPts : 1];
+};
+
+// v from lane l ^ 1 and from lane l ^ 2 (DPP quad permutations)
+__device__ __forceinline__ double dpp_xor1(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ double dpp_xor2(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x4E, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x4E, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 
 // x <- L^{-1} x then L^{-T} x, lane i owning x_i (i < NM); division as dtrsv.
 template <int NM>
@@ -1418,13 +1450,23 @@ __device__ __forceinline__ double coop_rcond(double (*L)[NM], double *x, double 
 }
 
 template <int NM, int NP2, bool GAUNT, int RS, bool PPN = false>
-__global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, int lin2) {
+__global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A, int lin, int lin2) {
     using St = Store<NM, NP2>;
     using CS = CoopShared<NM, NP2, GAUNT, RS, PPN>;
-    constexpr int NA = CS::NA, NB = CS::NB, NG = CS::NG;
-    __shared__ CS sh;
+    constexpr int NG = CS::NG;
+    __shared__ CS shw[kTailWaves];
+    __shared__ TailTable<NM, !GAUNT> tab;
+    CS &sh = shw[threadIdx.x >> 6];
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
     const int lane = threadIdx.x & 63;
+    const bool tabled = !GAUNT && A.nq <= 64 * kTailPts;
+    if constexpr (!GAUNT) {
+        if (tabled) {
+            for (int i = threadIdx.x; i < NM * A.nq; i += blockDim.x) tab.t[i / A.nq][i % A.nq] = A.pT[i];
+            for (int i = threadIdx.x; i < A.nq; i += blockDim.x) tab.w[i] = A.w[i];
+        }
+    }
+    __syncthreads();      // the only block-wide barrier: from here on the waves are independent
     const int count = A.ctl[cw(lin)], dcount = A.ctl[cw(lin2)];
     const double p0 = A.p[0];
     LaneStats st;
@@ -1452,30 +1494,52 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
         double aold_l = lane < NM ? S.d(St::F_AOLD + lane, c) : 0.0;
         double d_l = lane < NM ? S.d(St::F_D + lane, c) : 0.0;
         for (;;) {
-            // ---- sweep(s): partial sums per lane, reduced through LDS ----
+            // ---- sweep(s): partial sums per lane, reduced through DPP and LDS ----
             double sf = 0.0;
             double wes[kTailPts];
 #pragma unroll
-            for (int pass = 0; pass < ((!GAUNT && RS < NM) ? 2 : 1); ++pass) {
-                const int R0 = pass == 0 ? 0 : RS, R1 = pass == 0 ? (GAUNT ? NM : RS) : NM;
-                const int nacc = pass == 0 ? NA + NG : NB;
-                const int off = Tri<NM>::at(R0, 0);
-                double acc[CS::NPART];
+            for (int pass = 0; pass < CS::NPASS; ++pass) {
+                // flat range [K0, K1) of the sums of this pass (GAUNT: the
+                // order-2N moments), then the PPn gradient sums
+                const int K0 = pass == 0 ? 0 : CS::KH, K1 = pass == 0 ? CS::KH : CS::TRI;
+                const int nv = pass == 0 ? CS::NV0 : CS::NV1;
+                double acc[CS::NV];
 #pragma unroll
-                for (int k = 0; k < CS::NPART; ++k) acc[k] = 0.0;
-                // The moment values come from the transposed table (pT[k][q]):
-                // for each k the wave reads 64 consecutive doubles.  With at
-                // most kTailPts points per lane, w e of the first pass is kept
-                // for the second.
+                for (int k = 0; k < CS::NV; ++k) acc[k] = 0.0;
+                // Lane l takes the points q = l, l + 64, ...; the moment values
+                // come from the LDS table or from the transposed table in global
+                // memory (for each k the wave reads 64 consecutive doubles).
+                // With at most kTailPts points per lane, w e of the first pass
+                // is kept for the second.
                 const bool cached = A.nq <= 64 * kTailPts;
                 for (int q0 = 0; q0 < A.nq; q0 += 64 * kTailPts) {
 #pragma unroll
                     for (int qi = 0; qi < kTailPts; ++qi) {
+                        // points past nq read point 0 and add exact zeros, so
+                        // the kTailPts points' chains can interleave (the
+                        // order-2N rows of the Gaunt form are too many registers
+                        // for that: there the points stay sequential)
                         const int q = q0 + 64 * qi + lane;
-                        if (q >= A.nq) break;
+                        if (GAUNT && q >= A.nq) break;
+                        const bool valid = q < A.nq;
+                        const int qq = valid ? q : 0;
                         double row[GAUNT ? NP2 : NM];
+                        double wq;
+                        if constexpr (!GAUNT) {
+                            if (tabled) {
 #pragma unroll
-                        for (int k = 0; k < (GAUNT ? NP2 : NM); ++k) row[k] = A.pT[size_t(k) * A.nq + q];
+                                for (int k = 0; k < NM; ++k) row[k] = tab.t[k][qq];
+                                wq = tab.w[qq];
+                            } else {
+#pragma unroll
+                                for (int k = 0; k < NM; ++k) row[k] = A.pT[size_t(k) * A.nq + qq];
+                                wq = A.w[qq];
+                            }
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < NP2; ++k) row[k] = A.pT[size_t(k) * A.nq + qq];
+                            wq = A.w[qq];
+                        }
                         double we;                    // the Hessian's weight
                         if (pass > 0 && cached) {
                             we = wes[qi];
@@ -1487,15 +1551,15 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                                 const double tq = 0.5 * am;
                                 double sq;
                                 const double G = ppn_g(tq, delta, sq);
-                                we = 0.5 * A.w[q] * (G / sq);
+                                we = valid ? 0.5 * wq * (G / sq) : 0.0;
                                 if (pass == 0) {
-                                    sf = sf + A.w[q] * (tq * G - 0.5 * delta + delta * log(G));
-                                    const double wg = A.w[q] * G;
+                                    sf = sf + (valid ? wq * (tq * G - 0.5 * delta + delta * log(G)) : 0.0);
+                                    const double wg = valid ? wq * G : 0.0;
 #pragma unroll
-                                    for (int k = 0; k < NG; ++k) acc[NA + k] = fma(row[k], wg, acc[NA + k]);
+                                    for (int k = 0; k < NG; ++k) acc[CS::KH + k] = fma(row[k], wg, acc[CS::KH + k]);
                                 }
                             } else {
-                                we = A.w[q] * exp(am);
+                                we = valid ? wq * exp(am) : 0.0;
                             }
                             wes[qi] = we;
                         }
@@ -1506,31 +1570,50 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                         } else {
 #pragma unroll
                             for (int i = 0; i < NM; ++i) {
-                                if (i < R0 || i >= R1) continue;
+                                if (Tri<NM>::at(i, i) < K0 || Tri<NM>::at(i, 0) >= K1) continue;
                                 const double v = we * row[i];
 #pragma unroll
                                 for (int jj = 0; jj <= i; ++jj) {
-                                    const int k = Tri<NM>::at(i, jj) - off;
-                                    if (k >= 0 && k < CS::NPART) acc[k] = fma(v, row[jj], acc[k]);
+                                    const int k = Tri<NM>::at(i, jj) - K0;
+                                    if (k >= 0 && k < K1 - K0) acc[k] = fma(v, row[jj], acc[k]);
                                 }
                             }
                         }
                     }
                 }
+                // Two exchanges between lanes (l ^ 1, then l ^ 2): each halves
+                // the values a lane holds and sums them over the pair, so that
+                // lane l ends with H2 values summed over its quad -- value
+                // (l & 1 ? H1 : 0) + (l & 2 ? H2 : 0) + r in slot r -- and the
+                // sixteen quads' partials of each value are then added through
+                // LDS by one lane, four chains in quad order.
+                double a1[CS::H1];
 #pragma unroll
-                for (int k = 0; k < CS::NPART; ++k)
-                    if (k < nacc) sh.part[k][lane] = acc[k];
+                for (int r = 0; r < CS::H1; ++r) {
+                    const double lo = acc[r], hi = (CS::H1 + r < CS::NV) ? acc[CS::H1 + r] : 0.0;
+                    const bool b = lane & 1;
+                    a1[r] = (b ? hi : lo) + dpp_xor1(b ? lo : hi);
+                }
+#pragma unroll
+                for (int r = 0; r < CS::H2; ++r) {
+                    const double lo = a1[r], hi = (CS::H2 + r < CS::H1) ? a1[CS::H2 + r] : 0.0;
+                    const bool b = lane & 2;
+                    sh.part[r][lane] = (b ? hi : lo) + dpp_xor2(b ? lo : hi);
+                }
                 __builtin_amdgcn_wave_barrier();
-                for (int k = lane; k < nacc; k += 64) {       // four partial sums per column
+                for (int k = lane; k < nv; k += 64) {
+                    const int hb = k >= CS::H1 ? 1 : 0, rel = k - hb * CS::H1;
+                    const int qb = rel >= CS::H2 ? 1 : 0, r = rel - qb * CS::H2;
+                    const int cls = hb | (qb << 1);
                     double t4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                    for (int l2 = 0; l2 < 64; l2 += 4) {
+                    for (int m = 0; m < 16; m += 4) {
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) t4[u] += sh.part[k][l2 + u];
+                        for (int u = 0; u < 4; ++u) t4[u] += sh.part[r][4 * (m + u) + cls];
                     }
                     const double tot = (t4[0] + t4[1]) + (t4[2] + t4[3]);
-                    if (PPN && pass == 0 && k >= NA) sh.gsum[k - NA] = tot;
-                    else sh.accs[off + k] = tot;
+                    if (PPN && pass == 0 && k >= CS::KH) sh.gsum[k - CS::KH] = tot;
+                    else sh.accs[K0 + k] = tot;
                 }
                 __builtin_amdgcn_wave_barrier();
             }
@@ -1554,10 +1637,12 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                 } else {
                     // The reference halves t and evaluates f again until
                     // Armijo holds or t * maxf <= eps/2 (linesearch.cpp:45-98).
-                    // The next kHalvings candidates are evaluated in one wave
-                    // sweep (f only, the same sums in the same order as a
-                    // trial sweep) and the first accepted one is taken: the
-                    // reference's decisions without one pass per halving.
+                    // The next candidates (kFirstHalvings in the first batch,
+                    // where most searches end, then kHalvings per batch) are
+                    // evaluated in one wave sweep (f only, the same sums in the
+                    // same order as a trial sweep) and the first accepted one is
+                    // taken: the reference's decisions without one pass per
+                    // halving.
                     const double maxf = maxf_s;
                     if (lane < NM) {
                         sh.x[lane] = aold_l;
@@ -1565,33 +1650,60 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                     }
                     __builtin_amdgcn_wave_barrier();
                     int state = 0;                 // 0 searching, 1 accepted, 2 exhausted
+                    int cap = kFirstHalvings;      // candidates in this batch
                     while (state == 0) {
                         int nval = 0;
 #pragma unroll
                         for (int jj = 0; jj < kHalvings; ++jj)
-                            if (nval == jj && ldexp(t, -(jj + 1)) * maxf > DBL_EPSILON / 2) nval = jj + 1;
+                            if (jj < cap && nval == jj && ldexp(t, -(jj + 1)) * maxf > DBL_EPSILON / 2) nval = jj + 1;
                         if (nval == 0) { state = 2; break; }
                         double fj[kHalvings];
 #pragma unroll
                         for (int jj = 0; jj < kHalvings; ++jj) fj[jj] = 0.0;
-                        for (int q = lane; q < A.nq; q += 64) {
-                            double row[NM];
-#pragma unroll
-                            for (int k = 0; k < NM; ++k) row[k] = A.pT[size_t(k) * A.nq + q];
-                            const double wq = A.w[q];
+                        // candidates outside, the lane's points inside (their
+                        // chains interleave); each fj sums the lane's points in
+                        // order, as the trial sweep's f does
+                        for (int q0 = 0; q0 < A.nq; q0 += 64 * kTailPts) {
 #pragma unroll
                             for (int jj = 0; jj < kHalvings; ++jj) {
-                                if (jj >= nval) break;
+                                if (jj >= nval) break;                  // wave-uniform
                                 const double tj = ldexp(t, -(jj + 1));
-                                double am = 0.0;
-                                for (int k = 0; k < NM; ++k) am = fma(fma(tj, sh.gvec[k], sh.x[k]), row[k], am);
-                                if constexpr (PPN) {
-                                    const double tq = 0.5 * am;
-                                    double sq;
-                                    const double G = ppn_g(tq, delta, sq);
-                                    fj[jj] += wq * (tq * G - 0.5 * delta + delta * log(G));
-                                } else {
-                                    fj[jj] += wq * exp(am);
+                                double aj[NM];
+#pragma unroll
+                                for (int k = 0; k < NM; ++k) aj[k] = fma(tj, sh.gvec[k], sh.x[k]);
+#pragma unroll
+                                for (int qi = 0; qi < kTailPts; ++qi) {
+                                    const int q = q0 + 64 * qi + lane;
+                                    if (GAUNT && q >= A.nq) break;      // Gaunt form: sequential points
+                                    const bool valid = q < A.nq;
+                                    const int qq = valid ? q : 0;
+                                    double row[NM], wq;
+                                    if constexpr (!GAUNT) {
+                                        if (tabled) {
+#pragma unroll
+                                            for (int k = 0; k < NM; ++k) row[k] = tab.t[k][qq];
+                                            wq = tab.w[qq];
+                                        } else {
+#pragma unroll
+                                            for (int k = 0; k < NM; ++k) row[k] = A.pT[size_t(k) * A.nq + qq];
+                                            wq = A.w[qq];
+                                        }
+                                    } else {
+#pragma unroll
+                                        for (int k = 0; k < NM; ++k) row[k] = A.pT[size_t(k) * A.nq + qq];
+                                        wq = A.w[qq];
+                                    }
+                                    double am = 0.0;
+#pragma unroll
+                                    for (int k = 0; k < NM; ++k) am = fma(aj[k], row[k], am);
+                                    if constexpr (PPN) {
+                                        const double tq = 0.5 * am;
+                                        double sq;
+                                        const double G = ppn_g(tq, delta, sq);
+                                        fj[jj] += valid ? wq * (tq * G - 0.5 * delta + delta * log(G)) : 0.0;
+                                    } else {
+                                        fj[jj] += valid ? wq * exp(am) : 0.0;
+                                    }
                                 }
                             }
                         }
@@ -1612,8 +1724,9 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
                             state = 1;
                         } else {
                             t = ldexp(t, -nval);
-                            if (nval < kHalvings) state = 2;     // the next halving fails the bound
+                            if (nval < cap) state = 2;           // the next halving fails the bound
                         }
+                        cap = kHalvings;
                     }
                     if (state == 1) {                  // a full sweep at the accepted point next
 #pragma unroll
@@ -1786,7 +1899,7 @@ __global__ __launch_bounds__(64) void tail_coop_kernel(NewtonArgs A, int lin, in
             if (done) break;
         }
     }
-    // statistics: only lane 0 counted; one wave per block
+    // statistics: only lane 0 of each wave counted
     if (lane == 0 && (st.n || st.trials || st.err_bits)) {
         atomicMax((unsigned long long *)&A.stats[0], (unsigned long long)st.max_iter);
         atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)st.sum);
@@ -2237,8 +2350,8 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
 template <int NM, int NP2, bool G, int RS, bool PPN = false>
 void launch_tail(const NewtonArgs &a, int lin, int lin2, int tail_blocks, int tail_mode, hipStream_t s) {
     if (tail_mode == 0 || PPN)   // PPn: the cooperative tail only
-        hipLaunchKernelGGL((tail_coop_kernel<NM, NP2, G, RS, PPN>), dim3(4 * tail_blocks), dim3(64), 0, s, a, lin,
-                           lin2);
+        hipLaunchKernelGGL((tail_coop_kernel<NM, NP2, G, RS, PPN>), dim3(tail_blocks * 4 / kTailWaves),
+                           dim3(64 * kTailWaves), 0, s, a, lin, lin2);
     else if (tail_mode == 2)
         hipLaunchKernelGGL((tail_wave_kernel<NM, NP2, G, RS>), dim3(tail_blocks), dim3(kBlock), 0, s, a, lin, lin2);
     else
