@@ -165,6 +165,7 @@ T0 = time.perf_counter()
 
 
 FORCE_TUNE = {}
+TUNE_STEPS = 10           # steps per autotuning trial
 
 
 def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=None, autotune=True, tune_step=None):
@@ -184,7 +185,7 @@ def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=None,
     for k in range(min(max(warmup, tune_step if autotune else 0), len(dts))):
         ctx.update(dts[k], opt)
     ctx.sync()
-    report = ctx.autotune(dts[0], opt, steps=5) if autotune else []
+    report = ctx.autotune(dts[0], opt, steps=TUNE_STEPS) if autotune else []
     progress("autotuned: %s" % "; ".join(report[-3:]))
     for k, v in FORCE_TUNE.items():
         ctx.tune(k, v)
@@ -231,6 +232,7 @@ def form_summary(r, f_iter, f_trial):
             "reg_hist": list(st.hist_reg), "warm_pick_mlp_iso_prev": list(st.warm_pick),
             "mixed_steps_redone_fp64": st.mixed_deferred,
             "tuned": r["tuned"],
+            "autotune_trials": r["tune_report"],
             "kernel_timed_steps": r["kernel_steps"],
             "kernel_ms": {"newton": kms, "flux": r["flux"][1], "other": r["other"][1], "mlp": r["mlp"][1]},
             "newton_tflops_fp64_equiv": (kst.iter_sum * f_iter + kst.trial_sum * f_trial) / (kms * 1e-3) / 1e12

@@ -51,7 +51,8 @@ struct mnc_ctx {
     int mix_blocks = 0;              // 0: one per CU
     int flux_tile = 3;               // 0: ansatz to HBM + flux; 1: fused, 8x8 tiles; 2: fused, 16x16;
                                      // 3: fused, 16x16, directions split into kFluxGroups
-    int learned_gate_log10 = -2;     // the network is consulted when ||g(best start)|| > 10^this ||u||
+    int learned_gate_log10 = -2;     // the network is consulted when ||g(best start)|| > 10^this ||u||;
+                                     // kGateOff: never (no network launches)
     // grid
     bool has_grid = false;
     mnc_grid_desc grid{};
@@ -244,6 +245,9 @@ int mlp_predict_dev(mnc_ctx *ctx, int n, const double *us, double *alpha) {
 
 // Enqueues the batched Newton on device arrays (stats accumulate in d_stats).
 // With check, synchronises and turns the kernels' error bits into a status.
+// learned_gate_log10 at this value switches the network off
+constexpr int kGateOff = 5;
+
 int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_options *o, mnc_outs *outs,
                const double *alpha_guess, bool check) {
     if (ncells <= 0) return MNC_OK;
@@ -256,7 +260,7 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     int st = ensure_state(ctx, size_t(ncells));
     if (st) return st;
     const bool learned = o->warm_start == MNC_WARM_LEARNED;
-    if (learned) {                               // buffers of the network's guesses
+    if (learned && ctx->learned_gate_log10 < kGateOff) {   // buffers of the network's guesses
         if ((st = mlp_check(ctx, true))) return st;
         if (size_t(ncells) > ctx->guess_cap) {
             if ((st = dalloc(ctx, &ctx->d_guess, size_t(ncells) * 15))) return st;
@@ -324,7 +328,15 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
         }
         return MNC_OK;
     };
-    if (learned) {
+    if (learned && ctx->learned_gate_log10 >= kGateOff) {
+        // network off: every cell starts from the alpha passed in (isotropicAlpha
+        // if it is not finite), as the gate would decide at any threshold above
+        // the gradient norms, without the network's or the judge's launches
+        if (t.nm != 15) return ctx_fail(ctx, MNC_ERR_ARG, "the learned warm start is for moment order 4");
+        Timed tm(ctx, MNC_KCLASS_NEWTON);
+        mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
+        if ((st = launch(a, lp))) return st;
+    } else if (learned) {
         // Round 1 evaluates every cell at its start and parks those failing the
         // gate on the network's list; the network runs on those rows only (the
         // count is read on the device), the judge moves them to round 2's list.
@@ -920,7 +932,7 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         return MNC_OK;
     }
     if (std::strcmp(key, "learned_gate_log10") == 0) {
-        if (value < -16 || value > 4) return ctx_fail(ctx, MNC_ERR_ARG, "learned_gate_log10 out of range");
+        if (value < -16 || value > kGateOff) return ctx_fail(ctx, MNC_ERR_ARG, "learned_gate_log10 out of range");
         ctx->learned_gate_log10 = value;
         return MNC_OK;
     }
@@ -998,7 +1010,7 @@ int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, cha
     // work that is left.
     std::vector<Knob> knobs;
     if (opt->warm_start == MNC_WARM_LEARNED)
-        knobs.push_back({"learned_gate_log10", {-3, -2, -1, 0, 1}, &ctx->learned_gate_log10});
+        knobs.push_back({"learned_gate_log10", {-3, -2, -1, 0, 1, kGateOff}, &ctx->learned_gate_log10});
     if (opt->precision == MNC_PREC_MIXED) {
         knobs.push_back({"mix_cond_log10", {5, 6, 7, 8, 9, 10}, &ctx->mix_cond_log10});
         knobs.push_back({"mix_split", {2, 3}, &ctx->mix_split});

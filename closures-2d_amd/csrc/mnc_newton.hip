@@ -511,7 +511,8 @@ enum : int { INIT_INVALID = 0, INIT_READY = 1 };
 // Returns INIT_INVALID if u[0] <= 0 (the reference aborts: opt.cpp:97-101).
 // MNC_WARM_LEARNED starts from the alpha passed in (isotropicAlpha if it is not
 // finite; I_WARM records which); the first round's logic then decides whether
-// the network is consulted (cell_logic, divert).
+// the network is consulted (cell_logic, divert).  With the network switched off
+// (no divert at the prologue) the start is counted here.
 template <int NM, int NP2>
 __device__ int cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, LaneStats &st) {
     using St = Store<NM, NP2>;
@@ -568,6 +569,7 @@ __device__ int cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, La
     S.i(St::I_MAXIT, c) = A.max_iter;
     S.i(St::I_MODE, c) = ST_NEWTON;
     S.i(St::I_WARM, c) = warm;
+    if (A.warm == MNC_WARM_LEARNED && !A.divert) ++st.warm[warm];
     return INIT_READY;
 }
 

@@ -196,7 +196,10 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *   "mix_blocks"   workgroups of the mixed-precision sweep (1-65536)
  *   "learned_gate_log10"  MNC_WARM_LEARNED: the network is consulted for a
  *                  cell only when the gradient norm at its start exceeds
- *                  10^value ||u|| (scaled problem; -16..4, default -2)
+ *                  10^value ||u|| (scaled problem; -16..4, default -2);
+ *                  5 switches the network off: every cell starts from the
+ *                  alpha passed in (isotropicAlpha if it is not finite) and
+ *                  neither the network nor the judge is launched
  *   "flux_tile"    0: ansatz written to HBM, then the flux; 1: ansatz fused
  *                  into the flux with 8x8-cell LDS tiles; 2: 16x16 tiles
  *                  (n <= 15); these three give identical results.  3 (default;
@@ -216,7 +219,8 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value);
 /* Autotuner (no counterpart in the reference): times `steps` mnc_update steps
  * of dt from the current grid state for each candidate setting, one
  * coordinate-descent pass over, in this order, the knobs that change the work
- * -- (learned warm start) "learned_gate_log10", (mixed precision)
+ * -- (learned warm start) "learned_gate_log10" (5, the network off, among
+ * the candidates), (mixed precision)
  * "mix_cond_log10", "mix_split", "mix_defer" -- then the launch shapes
  * "graph", "flux_tile", "rounds", "tail_mode", "tail_blocks" and (mixed
  * precision) "mix_blocks", and keeps the fastest.  Every candidate converges

@@ -177,3 +177,42 @@ def test_learned_start_gate_keeps_a_good_previous_alpha():
     ok = r1 == 0
     assert s.warm_pick[2] >= ok.sum()
     assert it2[ok].max() <= 2
+
+
+@pytest.mark.gpu
+def test_learned_start_with_network_off_is_the_previous_start():
+    """Gate value 5 switches the network off: without any network parameters
+    the learned form then solves exactly as the previous-alpha start does (the
+    same kernels, bit for bit), counts every finite start as 'previous', and a
+    cell whose alpha is not finite starts from isotropicAlpha."""
+    print("case learned gate off", flush=True)
+    mnc = load_pkg("mnc")
+    ctx = mnc.Context(4, 16, use_gaunt=False)
+    tab = oracle().Tables(4, 16, False)
+    rng = np.random.default_rng(23)
+    u, astar, lobe = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, 2000, rng, near_boundary_frac=0.2)
+    a0 = isotropic_start(u) + rng.normal(0, 0.05, u.shape)
+    ctx.tune("learned_gate_log10", 5)
+    assert ctx.get_tune("learned_gate_log10") == 5
+    out = {}
+    for prec in (mnc.PREC_FP64, mnc.PREC_MIXED):
+        for warm in (mnc.WARM_PREVIOUS, mnc.WARM_LEARNED):
+            ctx.reset_stats()
+            opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, precision=prec, warm_start=warm)
+            out[prec, warm] = ctx.opt_batch_host(u, a0, opt) + (list(ctx.stats().warm_pick),)
+    for prec in (mnc.PREC_FP64, mnc.PREC_MIXED):
+        p, l = out[prec, mnc.WARM_PREVIOUS], out[prec, mnc.WARM_LEARNED]
+        assert p[0] == 0 and l[0] == 0
+        for k in (1, 2, 3, 4):
+            assert np.array_equal(p[k], l[k]), (prec, k)
+        assert l[6] == [0, 0, len(u)]
+    bad = a0.copy()
+    bad[:5, 3] = np.nan
+    ctx.reset_stats()
+    opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, warm_start=mnc.WARM_LEARNED)
+    st, ug, ag, itg, rg, _ = ctx.opt_batch_host(u, bad, opt)
+    picks = list(ctx.stats().warm_pick)
+    ctx.close()
+    print("non-finite starts: picks %s" % picks)
+    assert st == 0 and picks == [0, 5, len(u) - 5]
+    assert np.all(np.isfinite(ag))
