@@ -50,7 +50,8 @@ struct mnc_ctx {
     int mix_split = 3;               // bf16 parts of e and of the table in the mixed Hessian products
     int mix_blocks = 0;              // 0: one per CU
     int flux_tile = 3;               // 0: ansatz to HBM + flux; 1: fused, 8x8 tiles; 2: fused, 16x16;
-                                     // 3: fused, 16x16, directions split into kFluxGroups
+                                     // 3: fused, 16x16, directions split into flux_groups
+    int flux_groups = 6;             // direction groups of flux_tile 3
     int learned_gate_log10 = -2;     // the network is consulted when ||g(best start)|| > 10^this ||u||;
                                      // kGateOff: never (no network launches)
     // grid
@@ -420,6 +421,7 @@ mnc::GridArgs grid_args(mnc_ctx *ctx) {
     g.flux = ctx->d_flux;
     g.ansatz = ctx->d_ansatz;
     g.fpart = ctx->d_fpart;
+    g.groups = ctx->flux_groups;
     g.ppn = ctx->ppn ? 1 : 0;
     g.delta = ctx->grid.delta_ppn;
     return g;
@@ -658,7 +660,7 @@ int mnc_grid_create(mnc_ctx *ctx, const mnc_grid_desc *grid, const double *momen
     if ((st = dalloc(ctx, &ctx->d_mom, n)) || (st = dalloc(ctx, &ctx->d_alpha, n)) ||
         (st = dalloc(ctx, &ctx->d_old, n)) || (st = dalloc(ctx, &ctx->d_flux, n)) ||
         (st = dalloc(ctx, &ctx->d_st, nc)) || (st = dalloc(ctx, &ctx->d_ss, nc)) ||
-        (st = dalloc(ctx, &ctx->d_fpart, size_t(mnc::kFluxGroups) * grid->num_cells_x * grid->num_cells_y *
+        (st = dalloc(ctx, &ctx->d_fpart, size_t(mnc::kMaxFluxGroups) * grid->num_cells_x * grid->num_cells_y *
                                              ctx->tab.nm)))
         return st;
     // ansatz buffer: all q at once unless that exceeds 8 GiB
@@ -931,6 +933,11 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         ctx->flux_tile = value;
         return MNC_OK;
     }
+    if (std::strcmp(key, "flux_groups") == 0) {
+        if (value < 1 || value > mnc::kMaxFluxGroups) return ctx_fail(ctx, MNC_ERR_ARG, "flux_groups must be 1 to 8");
+        ctx->flux_groups = value;
+        return MNC_OK;
+    }
     if (std::strcmp(key, "learned_gate_log10") == 0) {
         if (value < -16 || value > kGateOff) return ctx_fail(ctx, MNC_ERR_ARG, "learned_gate_log10 out of range");
         ctx->learned_gate_log10 = value;
@@ -953,6 +960,7 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
     if (std::strcmp(key, "has_mixed") == 0) { *value = ctx->has_mix ? 1 : 0; return MNC_OK; }
     if (std::strcmp(key, "graph") == 0) { *value = ctx->use_graph; return MNC_OK; }
     if (std::strcmp(key, "flux_tile") == 0) { *value = ctx->flux_tile; return MNC_OK; }
+    if (std::strcmp(key, "flux_groups") == 0) { *value = ctx->flux_groups; return MNC_OK; }
     if (std::strcmp(key, "learned_gate_log10") == 0) { *value = ctx->learned_gate_log10; return MNC_OK; }
     if (std::strcmp(key, "tail_blocks") == 0) {
         *value = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
@@ -1018,6 +1026,7 @@ int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, cha
     }
     knobs.push_back({"graph", {0, 1}, &ctx->use_graph});
     knobs.push_back({"flux_tile", {0, 1, 2, 3}, &ctx->flux_tile});
+    knobs.push_back({"flux_groups", {2, 3, 4, 6, 8}, &ctx->flux_groups});
     knobs.push_back({"rounds", {2, 4, 6, 8, 12}, &ctx->rounds});
     knobs.push_back({"tail_mode", {0, 1, 2}, &ctx->tail_mode});
     knobs.push_back({"tail_blocks", {cu, 2 * cu, 4 * cu}, &ctx->tail_blocks});

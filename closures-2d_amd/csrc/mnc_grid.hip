@@ -412,10 +412,10 @@ template <int NM>
 void launch_fused_t(const GridArgs &g, int tile, int stage, double dt, hipStream_t s) {
     if constexpr (NM <= 15) {   // alpha of two halo cells per thread in registers
         if (tile == 3) {
-            hipLaunchKernelGGL((flux_split_kernel<NM, 16, 16, 8>),
-                               dim3((g.nx + 15) / 16, (g.ny + 15) / 16, kFluxGroups), dim3(256), 0, s, g);
+            const dim3 grid((g.nx + 15) / 16, (g.ny + 15) / 16, g.groups);
+            hipLaunchKernelGGL((flux_split_kernel<NM, 16, 16, 8>), grid, dim3(256), 0, s, g);
             hipLaunchKernelGGL((flux_sum_kernel<NM>), dim3((g.nx * g.ny + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                               g, kFluxGroups, stage, dt);
+                               g, g.groups, stage, dt);
             return;
         }
     }

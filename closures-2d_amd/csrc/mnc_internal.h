@@ -188,11 +188,12 @@ struct GridArgs {
     int init_cond;
     const double *w, *p, *xi, *eta;
     double *moments, *alpha, *moments_old, *sigma_t, *sigma_s, *flux, *ansatz;
-    double *fpart;        // [kFluxGroups][nx*ny][nm]: per-group partial fluxes (flux_tile 3)
+    double *fpart;        // [groups][nx*ny][nm]: per-group partial fluxes (flux_tile 3)
+    int groups;           // direction groups of flux_tile 3 (1..kMaxFluxGroups)
     int ppn;              // the PPn ansatz (computeAnsatz, momopt_update.cpp:54-57)
     double delta;         // its delta
 };
-constexpr int kFluxGroups = 4;
+constexpr int kMaxFluxGroups = 8;
 
 void launch_ghost_fill(const GridArgs &g, hipStream_t s);
 void launch_ansatz(const GridArgs &g, int q0, int nqc, hipStream_t s);
@@ -201,7 +202,7 @@ bool launch_flux_euler(const GridArgs &g, int q0, int nqc, int stage, double dt,
                        bool last_chunk, hipStream_t s);
 // The ansatz fused into the flux/Euler stage (E kept in LDS per tile):
 // tile 1 = 8 x 8 cells, 2 = 16 x 16, 3 = 16 x 16 with the directions split
-// into kFluxGroups launches' worth of blocks (needs fpart).
+// into g.groups launches' worth of blocks (needs fpart).
 bool launch_flux_fused(const GridArgs &g, int tile, int stage, double dt, hipStream_t s);
 
 }  // namespace mnc

@@ -204,8 +204,9 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *                  into the flux with 8x8-cell LDS tiles; 2: 16x16 tiles
  *                  (n <= 15); these three give identical results.  3 (default;
  *                  n <= 15, else as 1): 16x16 tiles with the directions split
- *                  into four groups summed in order and reciprocals of dx, dy
- *                  (rounding-level differences from 0-2).
+ *                  into "flux_groups" groups summed in order and reciprocals
+ *                  of dx, dy (rounding-level differences from 0-2)
+ *   "flux_groups"  direction groups of flux_tile 3 (1-8, default 6)
  *   "graph"        1: mnc_update steps are captured into hipGraphs (one per
  *                  distinct dt and options, captured on the second step with
  *                  that key) and replayed; 0: eager launches (default).  While
@@ -222,7 +223,7 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value);
  * -- (learned warm start) "learned_gate_log10" (5, the network off, among
  * the candidates), (mixed precision)
  * "mix_cond_log10", "mix_split", "mix_defer" -- then the launch shapes
- * "graph", "flux_tile", "rounds", "tail_mode", "tail_blocks" and (mixed
+ * "graph", "flux_tile", "flux_groups", "rounds", "tail_mode", "tail_blocks" and (mixed
  * precision) "mix_blocks", and keeps the fastest.  Every candidate converges
  * to the same tolerance; "mix_cond_log10" trades mixed-precision steps
  * against FP64 ones.  The grid state

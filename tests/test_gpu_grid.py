@@ -225,3 +225,26 @@ def test_split_flux_matches_fused(ic):
     print("flux rel %.3e, moments after 3 steps rel %.3e" % (ferr, merr))
     assert ferr <= 1e-13
     assert merr <= 1e-10
+
+
+@pytest.mark.gpu
+def test_split_flux_variants():
+    """flux_tile 3 with other direction-group counts than the default six:
+    only the order of the group sums changes (1e-13 relative)."""
+    mnc, gi = load_pkg("mnc"), load_pkg("grid_init")
+    print("case split flux variants", flush=True)
+    d = fixture_deck("lattice")
+    res = {}
+    for tile, groups in ((3, 6), (3, 1), (3, 3), (3, 8)):
+        ctx, g = gpu_grid_from_deck(mnc, gi, d)
+        ctx.tune("flux_tile", tile)
+        ctx.tune("flux_groups", groups)
+        assert ctx.get_tune("flux_groups") == groups
+        ctx.update(g.max_dt, mn_options(mnc, d))
+        res[tile, groups] = ctx.flux()
+        ctx.close()
+    ref = res[3, 6]
+    for key in ((3, 1), (3, 3), (3, 8)):
+        err = np.abs(res[key] - ref).max() / np.abs(ref).max()
+        print("tile %d groups %d: flux rel %.3e" % (key[0], key[1], err))
+        assert err <= 1e-13
