@@ -1171,25 +1171,9 @@ __device__ __forceinline__ bool reg_cholesky(double (&row)[NM], int lane) {
     return ok;
 }
 
-// x <- (L L^T)^{-1} x, x_i on lane i
-template <int NM>
-__device__ __forceinline__ void reg_apply_inv(const double (&row)[NM], const double (&col)[NM], double &x, int lane) {
-#pragma unroll
-    for (int j = 0; j < NM; ++j) {
-        const double xj = readlane_d(x, j) / readlane_d(row[j], j);
-        if (lane == j) x = xj;
-        else if (lane > j && lane < NM) x = fma(-xj, row[j], x);
-    }
-#pragma unroll
-    for (int j = NM - 1; j >= 0; --j) {
-        const double xj = readlane_d(x, j) / readlane_d(row[j], j);
-        if (lane == j) x = xj;
-        else if (lane < j) x = fma(-col[j], xj, x);
-    }
-}
-
-// reg_apply_inv with the reciprocals of the diagonal (wave-uniform rd[j]):
-// the condition estimate's repeated solves (see apply_inv_rcp).
+// x <- (L L^T)^{-1} x, x_i on lane i, with the reciprocals of the diagonal
+// (wave-uniform rd[j]) in place of divisions: the tail's Newton direction and
+// the condition estimate's repeated solves.
 template <int NM>
 __device__ __forceinline__ void reg_apply_inv_rcp(const double (&row)[NM], const double (&col)[NM],
                                                   const double (&rd)[NM], double &x, int lane) {
@@ -1795,11 +1779,14 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
 #pragma unroll
                 for (int k = 0; k < NM; ++k) col[k] = (lane < NM && k > lane) ? sh.L[k][lane] : 0.0;
                 __builtin_amdgcn_wave_barrier();
+                // reciprocals of the factor's diagonal, independent of each
+                // other: the solves below multiply by them (as blocked trsm
+                // kernels do) instead of dividing on their dependent chains
+                double rd[NM];
+#pragma unroll
+                for (int k = 0; k < NM; ++k) rd[k] = 1.0 / readlane_d(row[k], k);
                 double condest = 0.0;
                 if (chol_ok && !cond_proven_below(anorm, gersh, NM, A.cond_h_max)) {
-                    double rd[NM];
-#pragma unroll
-                    for (int k = 0; k < NM; ++k) rd[k] = 1.0 / readlane_d(row[k], k);
                     if (!(2.0 * anorm * reg_inv_norm1<NM>(row, rd, lane) <= A.cond_h_max))
                         condest = 1.0 / reg_rcond<NM>(row, col, anorm, lane);
                 }
@@ -1807,7 +1794,7 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
                     do_reg = true;
                 } else {
                     double xv = gv;
-                    reg_apply_inv<NM>(row, col, xv, lane);
+                    reg_apply_inv_rcp<NM>(row, col, rd, xv, lane);
                     if (err <= rtol) {
                         converged = true;
                     } else {
