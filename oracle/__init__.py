@@ -89,6 +89,8 @@ def lib():
         L.orc_opt.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(Options), C.POINTER(Outs)]
         L.orc_opt_batch.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(Options),
                                     C.POINTER(Outs), C.c_int]
+        L.orc_optcb_batch.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(Options),
+                                      C.POINTER(Outs), C.c_int]
         L.orc_solver_create.argtypes = [C.POINTER(Deck), C.c_int, C.POINTER(C.c_void_p)]
         L.orc_solver_destroy.argtypes = [C.c_void_p]
         L.orc_solver_max_dt.argtypes = [C.c_void_p]
@@ -199,6 +201,26 @@ class Tables:
         r = np.array([o_.r for o_ in outs])
         ng = np.array([o_.norm_g for o_ in outs])
         return st, u, alpha, it, r, ng
+
+
+    def optcb_batch(self, u, P2M, alphaP, max_iter=100, tol=1e-8, cond_h_max=1e10, nthreads=0,
+                    line_search=0, moment_type=MOMENT_MN, delta_ppn=0.0):
+        """The change-of-basis optimiser (optcb.cpp:90-415) per cell.  P2M is
+        [n][n*n] (each cell's basis, column-major), alphaP [n][nm]; both are
+        updated in place as the reference's per-cell state.  Returns (status,
+        u, P2M, alphaP, alphaM, iterations, r, |g|)."""
+        u = np.array(u, dtype=np.float64, order="C")
+        n = u.shape[0]
+        P2M = np.array(P2M, dtype=np.float64, order="C").reshape(n, self.nm * self.nm)
+        alphaP = np.array(alphaP, dtype=np.float64, order="C")
+        alphaM = np.zeros_like(alphaP)
+        o = Options(max_iter, tol, cond_h_max, 0, line_search, moment_type, delta_ppn)
+        outs = (Outs * n)()
+        st = lib().orc_optcb_batch(self.h, n, _p(u), _p(P2M), _p(alphaP), _p(alphaM), C.byref(o), outs, nthreads)
+        it = np.array([o_.iter for o_ in outs], dtype=np.int32)
+        r = np.array([o_.r for o_ in outs])
+        ng = np.array([o_.norm_g for o_ in outs])
+        return st, u, P2M, alphaP, alphaM, it, r, ng
 
 
 def isotropic_alpha(nm):

@@ -877,6 +877,273 @@ int orc_opt_batch(const orc_tables *t, int ncells, double *u, double *alpha,
 }
 
 /* ------------------------------------------------------------------ */
+/* Change-of-basis optimiser optcb (optcb.cpp:90-415).                  */
+/* P2M and the factor are column-major (element (i,j) at a[i + j*n]),   */
+/* the layout of the reference's BLAS calls; the triangular kernels     */
+/* follow the Netlib reference BLAS loops.                              */
+/* ------------------------------------------------------------------ */
+/* dtrsv('L','N','N'): x <- A^{-1} x */
+static void cb_trsv_ln(int n, const double *a, double *x)
+{
+    for (int j = 0; j < n; j++) {
+        if (x[j] != 0.0) {
+            x[j] = x[j] / a[j + j * n];
+            double temp = x[j];
+            for (int i = j + 1; i < n; i++)
+                x[i] = x[i] - temp * a[i + j * n];
+        }
+    }
+}
+
+/* dtrsv('L','T','N'): x <- A^{-T} x */
+static void cb_trsv_lt(int n, const double *a, double *x)
+{
+    for (int j = n - 1; j >= 0; j--) {
+        double temp = x[j];
+        for (int i = n - 1; i > j; i--)
+            temp = temp - a[i + j * n] * x[i];
+        temp = temp / a[j + j * n];
+        x[j] = temp;
+    }
+}
+
+/* dtrmv('L','N','N'): x <- A x */
+static void cb_trmv_ln(int n, const double *a, double *x)
+{
+    for (int j = n - 1; j >= 0; j--) {
+        if (x[j] != 0.0) {
+            double temp = x[j];
+            for (int i = n - 1; i > j; i--)
+                x[i] = x[i] + temp * a[i + j * n];
+            x[j] = x[j] * a[j + j * n];
+        }
+    }
+}
+
+/* dtrmv('L','T','N'): x <- A^T x */
+static void cb_trmv_lt(int n, const double *a, double *x)
+{
+    for (int j = 0; j < n; j++) {
+        double temp = x[j];
+        temp = temp * a[j + j * n];
+        for (int i = j + 1; i < n; i++)
+            temp = temp + a[i + j * n] * x[i];
+        x[j] = temp;
+    }
+}
+
+/* dtrsm('L','L','N','N', n, nq, 1, A, B): every column b_q <- A^{-1} b_q;
+ * B is the table [nq][n], column q at b[q*n] */
+static void cb_trsm_lln(int n, int nq, const double *a, double *b)
+{
+    for (int q = 0; q < nq; q++) cb_trsv_ln(n, a, b + (size_t)q * n);
+}
+
+/* dtrmm('R','L','N','N', n, n, 1, A, B): B <- B A */
+static void cb_trmm_rln(int n, const double *a, double *b)
+{
+    for (int j = 0; j < n; j++) {
+        double temp = a[j + j * n];
+        for (int i = 0; i < n; i++)
+            b[i + j * n] = temp * b[i + j * n];
+        for (int k = j + 1; k < n; k++) {
+            if (a[k + j * n] != 0.0) {
+                temp = a[k + j * n];
+                for (int i = 0; i < n; i++)
+                    b[i + j * n] = b[i + j * n] + temp * b[i + k * n];
+            }
+        }
+    }
+}
+
+static void cb_identity(int n, double *a)
+{
+    for (int i = 0; i < n * n; i++) a[i] = 0.0;
+    for (int i = 0; i < n; i++) a[i + i * n] = 1.0;
+}
+
+static int optcb_scaled(const orc_tables *t, double *u, double *P2M, double *alphaP, double *alphaM,
+                        const orc_options *o, orc_outs *outs)
+{                                                          /* optcb.cpp:159-415 */
+    const int nm = t->nm, nq = t->nq;
+    int status = ORC_OK;
+    double *mem = (double *)malloc(sizeof(double) * ((size_t)8 * nm + 4 * (size_t)nm * nm + (size_t)nq * nm));
+    if (!mem) return ORC_ERR_ALLOC;
+    double *alphaP0 = mem, *dP = alphaP0 + nm, *dM = dP + nm, *gP = dM + nm, *uP = gP + nm, *gM = uP + nm,
+           *alphaOld = gM + nm;
+    double *chol_rm = alphaOld + 2 * nm, *chol = chol_rm + nm * nm, *hP = chol + nm * nm, *P2Min = hP + nm * nm;
+    double *pP = P2Min + nm * nm;
+    /* the tables seen through the current basis: p -> pP (stride nm) */
+    orc_tables tv = *t;
+    tv.p = pP;
+    tv.np2 = nm;
+    tv.use_gaunt = 0;
+    orc_options od = *o;
+    od.use_gaunt = 0;
+    double f = 0.0;
+    memcpy(alphaP0, alphaP, sizeof(double) * nm);
+    memcpy(P2Min, P2M, sizeof(double) * nm * nm);
+    for (int q = 0; q < nq; q++)                            /* pP <- P2M \ p (:185-187) */
+        memcpy(pP + (size_t)q * nm, t->p + (size_t)q * t->np2, sizeof(double) * nm);
+    cb_trsm_lln(nm, nq, P2Min, pP);
+    int maxIter = o->max_iter;
+    const double tolRel = o->tol_rel, condHMax = o->cond_h_max;
+    double r = 0.0;
+    int regIndex = 0;
+    memcpy(uP, u, sizeof(double) * nm);                    /* :198-200 */
+    cb_trsv_ln(nm, P2M, uP);
+    double rtol = nrm2(nm, u) * tolRel;
+    int iter = 0, totalIter = 0;
+    double tt = 1;
+    double err = 0.0;
+    int hitMaxRegularization = 0;
+    for (;;) {
+        iter = iter + 1;
+        totalIter = totalIter + 1;
+        f = fobj_any(&tv, alphaP, uP, gP, hP, &od);         /* :234-242 */
+        double fscale = 0.0, au = 0.0;                      /* the robust mode's resolution of f */
+        for (int i = 0; i < nm; i++) {
+            fscale += fabs(alphaP[i] * uP[i]);
+            au += alphaP[i] * uP[i];
+        }
+        if (o->moment_type == ORC_MOMENT_PPN) fscale += fabs(f + au);
+        else fscale += fabs(hP[0] / (pP[0] * pP[0]));
+        memcpy(chol_rm, hP, sizeof(double) * nm * nm);      /* :247-259 */
+        int error = orc_cholesky(nm, chol_rm);
+        double norm = orc_norm1(nm, hP);
+        double condest = 0.0;
+        if (error == 0) condest = 1.0 / orc_rcond(nm, chol_rm, norm);
+        for (int i = 0; i < nm; i++)
+            for (int j = 0; j < nm; j++)
+                chol[i + j * nm] = (i >= j) ? chol_rm[i * nm + j] : 0.0;
+        if (totalIter == 1 && (error != 0 || condest > condHMax)) {   /* :264-281 */
+            isotropic_alpha(nm, alphaP0, o);
+            cb_identity(nm, P2M);
+            cb_identity(nm, P2Min);
+            memcpy(alphaP, alphaP0, sizeof(double) * nm);
+            memcpy(uP, u, sizeof(double) * nm);
+            for (int q = 0; q < nq; q++)
+                memcpy(pP + (size_t)q * nm, t->p + (size_t)q * t->np2, sizeof(double) * nm);
+            rtol = nrm2(nm, u) * tolRel;
+            iter = 0;
+            tt = 1;
+            continue;
+        }
+        if (iter >= maxIter || error != 0 || tt == 0 || condest > condHMax) {   /* :285-329 */
+            if (regIndex < ORC_NUM_REG - 1) {
+                regIndex++;
+                r = ORC_REGULARIZATIONS[regIndex];
+                uP[0] = 1.0;
+                for (int i = 1; i < nm; i++)
+                    uP[i] = u[i] * (1.0 - r);
+            } else {
+                if (hitMaxRegularization) {                 /* "Max regularization achieved." */
+                    status = ORC_ERR_MAXREG;
+                    break;
+                }
+                isotropic_alpha(nm, alphaP0, o);
+                cb_identity(nm, P2Min);
+                memcpy(uP, u, sizeof(double) * nm);
+                regIndex = 0;
+                hitMaxRegularization = 1;
+                maxIter = o->max_iter;
+            }
+            rtol = nrm2(nm, uP) * tolRel;
+            memcpy(alphaP, alphaP0, sizeof(double) * nm);
+            memcpy(P2M, P2Min, sizeof(double) * nm * nm);
+            cb_trsv_ln(nm, P2M, uP);
+            for (int q = 0; q < nq; q++)
+                memcpy(pP + (size_t)q * nm, t->p + (size_t)q * t->np2, sizeof(double) * nm);
+            cb_trsm_lln(nm, nq, P2Min, pP);
+            iter = 0;
+            tt = 1;
+            if (regIndex == ORC_NUM_REG - 1)
+                maxIter = 10 * maxIter;
+            continue;
+        }
+        cb_trsv_ln(nm, chol, gP);                           /* :333-343: the basis moves by L */
+        cb_trsm_lln(nm, nq, chol, pP);
+        cb_trmm_rln(nm, chol, P2M);
+        cb_trmv_lt(nm, chol, alphaP);
+        cb_trsv_ln(nm, chol, uP);
+        for (int i = 0; i < nm; i++) dP[i] = -gP[i];        /* :347-348 */
+        memcpy(gM, gP, sizeof(double) * nm);                /* :352-354 */
+        cb_trmv_ln(nm, P2M, gM);
+        err = nrm2(nm, gM);
+        if (err <= rtol) {                                  /* :358-375 */
+            memcpy(dM, dP, sizeof(double) * nm);
+            cb_trsv_lt(nm, P2M, dM);
+            memcpy(alphaM, alphaP, sizeof(double) * nm);
+            cb_trsv_lt(nm, P2M, alphaM);
+            u[0] = 1.0;
+            for (int i = 1; i < nm; i++)
+                u[i] = u[i] * (1.0 - r);
+            break;
+        }
+        tt = linesearch(&tv, alphaP, dP, f, gP, uP, alphaOld, &od, fscale);   /* :387-388 */
+    }
+    if (outs != NULL) {
+        outs->iter = totalIter;
+        outs->norm_g = err;
+        outs->r = r;
+    }
+    free(mem);
+    return status;
+}
+
+int orc_optcb(const orc_tables *t, double *u, double *P2M, double *alphaP, double *alphaM,
+              const orc_options *o, orc_outs *outs)
+{                                                          /* optcb.cpp:90-153 */
+    const int nm = t->nm;
+    if (o->use_gaunt || t->np2 != nm) return ORC_ERR_ARG;  /* pP is copied with stride nm (:183) */
+    const double u0 = u[0];
+    if (o->moment_type == ORC_MOMENT_PPN) {
+        orc_options os = *o;
+        os.delta_ppn = o->delta_ppn / (u0 * u0);
+        for (int i = 0; i < nm; i++) u[i] = u[i] / u0;
+        if (u0 <= 0) return ORC_ERR_U0;
+        int st = optcb_scaled(t, u, P2M, alphaP, alphaM, &os, outs);
+        for (int i = 0; i < nm; i++) {
+            alphaM[i] = alphaM[i] * u0;
+            u[i] = u[i] * u0;
+        }
+        return st;
+    }
+    const double alphaScale = log(u0) * 2.0 * sqrt(M_PI);
+    for (int i = 0; i < nm; i++) u[i] = u[i] / u0;
+    if (u0 <= 0) return ORC_ERR_U0;
+    int st = optcb_scaled(t, u, P2M, alphaP, alphaM, o, outs);
+    for (int i = 0; i < nm; i++) u[i] = u[i] * u0;
+    alphaM[0] = alphaM[0] + alphaScale;
+    return st;
+}
+
+int orc_optcb_batch(const orc_tables *t, int ncells, double *u, double *P2M, double *alphaP, double *alphaM,
+                    const orc_options *o, orc_outs *outs, int nthreads)
+{
+    const int nm = t->nm;
+    int status = ORC_OK;
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+#endif
+    for (int c = 0; c < ncells; c++) {
+        orc_outs tmp;
+        int st = orc_optcb(t, &u[(size_t)c * nm], &P2M[(size_t)c * nm * nm], &alphaP[(size_t)c * nm],
+                           &alphaM[(size_t)c * nm], o, &tmp);
+        if (outs) outs[c] = tmp;
+        if (st != ORC_OK) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+            { if (status == ORC_OK) status = st; }
+        }
+    }
+    (void)nthreads;
+    return status;
+}
+
+/* ------------------------------------------------------------------ */
 /* Grid solver                                                          */
 /* ------------------------------------------------------------------ */
 struct orc_solver {

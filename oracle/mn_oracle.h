@@ -106,6 +106,23 @@ int orc_opt(const orc_tables *t, double *u, double *alpha,
 int orc_opt_batch(const orc_tables *t, int ncells, double *u, double *alpha,
                   const orc_options *opt, orc_outs *outs, int nthreads);
 
+/* One closure solve by the change-of-basis optimiser optcb (optcb.cpp:90-415,
+ * MOMOPT_OPTIMIZATION_TYPE 1).  Per cell it carries the basis P2M (n x n,
+ * lower triangular, column-major) and alphaP, the multipliers in that basis;
+ * both are in/out state for the next call.  Each Newton step factors the
+ * Hessian in the current basis, H_P = L L^T, and moves the basis by L
+ * (P2M <- P2M L, alphaP <- L^T alphaP, g_P, u_P and the table <- L^{-1} .), so
+ * the direction is -g_P; the stop test measures g in the moment basis.  On
+ * success alphaM = P2M^{-T} alphaP (rescaled as opt does) and u is the
+ * (regularized) input, not a reconstruction.  Only the direct Hessian: the
+ * reference copies the table with stride n (optcb.cpp:183), so use_gaunt is
+ * ORC_ERR_ARG here. */
+int orc_optcb(const orc_tables *t, double *u, double *P2M, double *alphaP, double *alphaM,
+              const orc_options *opt, orc_outs *outs);
+/* Batched optcb, [ncells][n], [ncells][n*n] layouts, OpenMP dynamic,1. */
+int orc_optcb_batch(const orc_tables *t, int ncells, double *u, double *P2M, double *alphaP, double *alphaM,
+                    const orc_options *opt, orc_outs *outs, int nthreads);
+
 /* ---------------- grid solver (main.cpp, momopt_*.cpp) ---------------- */
 
 /* Deck values the momopt path reads (main.cpp:69-83, momopt_init.cpp:56-68). */
