@@ -601,6 +601,78 @@ int mnc_opt_batch_host(mnc_ctx *ctx, int ncells, double *u, double *alpha, const
     return st;
 }
 
+int mnc_optcb_batch_host(mnc_ctx *ctx, int ncells, double *u, double *P2M, double *alphaP, double *alphaM,
+                         const mnc_options *opt, mnc_outs *outs) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (ncells < 0 || (ncells > 0 && (!u || !P2M || !alphaP || !alphaM)))
+        return ctx_fail(ctx, MNC_ERR_ARG, "bad optcb batch arguments");
+    int st = check_opts(ctx, opt);
+    if (st) return st;
+    if (ctx->ppn) return ctx_fail(ctx, MNC_ERR_ARG, "the change-of-basis optimiser runs Mn contexts only");
+    if (ctx->tab.use_gaunt || ctx->tab.np2 != ctx->tab.nm)
+        return ctx_fail(ctx, MNC_ERR_ARG, "the change-of-basis optimiser needs the direct Hessian (use_gaunt 0)");
+    if (opt->precision != MNC_PREC_FP64 || opt->warm_start != MNC_WARM_PREVIOUS)
+        return ctx_fail(ctx, MNC_ERR_ARG, "the change-of-basis optimiser is FP64 from the cell's own alphaP");
+    if (ncells == 0) return MNC_OK;
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    const int nm = ctx->tab.nm, nq = ctx->tab.nq;
+    const int blocks = std::min(ncells, 16 * ctx->num_cu);
+    const size_t nv = size_t(ncells) * nm, nb = size_t(ncells) * nm * nm, nws = size_t(blocks) * nq * nm;
+    double *d = nullptr;
+    mnc_outs *d_outs = nullptr;
+    MNC_HIP_CHECK(hipMalloc((void **)&d, (3 * nv + nb + nws) * sizeof(double)));
+    if (hipMalloc((void **)&d_outs, size_t(ncells) * sizeof(mnc_outs)) != hipSuccess) {
+        (void)hipFree(d);
+        return ctx_fail(ctx, MNC_ERR_HIP, "hipMalloc failed");
+    }
+    double *d_u = d, *d_aP = d + nv, *d_aM = d + 2 * nv, *d_B = d + 3 * nv, *d_ws = d_B + nb;
+    mnc::CbArgs a{};
+    a.ncells = ncells;
+    a.nm = nm;
+    a.nq = nq;
+    a.w = ctx->d_w;
+    a.p = ctx->d_p;
+    a.u = d_u;
+    a.P2M = d_B;
+    a.alphaP = d_aP;
+    a.alphaM = d_aM;
+    a.outs = d_outs;
+    a.ws = d_ws;
+    a.max_iter = opt->max_iter;
+    a.ls_mode = opt->line_search;
+    a.tol = opt->tol_rel;
+    a.cond_h_max = opt->cond_h_max;
+    a.err = ctx->d_ctl + 3;
+    auto run = [&]() -> int {
+        MNC_HIP_CHECK(hipMemcpyAsync(d_u, u, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        MNC_HIP_CHECK(hipMemcpyAsync(d_aP, alphaP, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        MNC_HIP_CHECK(hipMemcpyAsync(d_aM, alphaM, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        MNC_HIP_CHECK(hipMemcpyAsync(d_B, P2M, nb * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
+        {
+            Timed tm(ctx, MNC_KCLASS_NEWTON);
+            if (!mnc::launch_optcb(a, blocks, ctx->stream))
+                return ctx_fail(ctx, MNC_ERR_ARG, "no change-of-basis kernel instance for this moment order");
+        }
+        MNC_HIP_CHECK(hipGetLastError());
+        int s2 = check_errors(ctx);
+        if (s2 != MNC_OK && s2 != MNC_ERR_U0 && s2 != MNC_ERR_MAXREG) return s2;
+        MNC_HIP_CHECK(hipMemcpyAsync(u, d_u, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        MNC_HIP_CHECK(hipMemcpyAsync(alphaP, d_aP, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        MNC_HIP_CHECK(hipMemcpyAsync(alphaM, d_aM, nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        MNC_HIP_CHECK(hipMemcpyAsync(P2M, d_B, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        if (outs)
+            MNC_HIP_CHECK(hipMemcpyAsync(outs, d_outs, size_t(ncells) * sizeof(mnc_outs), hipMemcpyDeviceToHost,
+                                         ctx->stream));
+        MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        return s2;
+    };
+    st = run();
+    (void)hipFree(d);
+    (void)hipFree(d_outs);
+    return st;
+}
+
 int mnc_batch_set(mnc_ctx *ctx, int ncells, const double *u, const double *alpha) {
     if (!ctx) return MNC_ERR_ARG;
     if (ctx->device < 0) return ctx_fail(ctx, MNC_ERR_STATE, "host-only context (device < 0)");

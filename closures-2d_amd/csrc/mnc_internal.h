@@ -195,6 +195,20 @@ struct GridArgs {
 };
 constexpr int kMaxFluxGroups = 8;
 
+// The change-of-basis optimiser (optcb.cpp:90-415) on a batch: one block per
+// cell at a time, ws holds each block's table seen through its cell's basis.
+struct CbArgs {
+    int ncells, nm, nq;
+    const double *w, *p;              // quadrature weights, moment table [nq][nm]
+    double *u, *P2M, *alphaP, *alphaM;  // [ncells][nm], [ncells][nm*nm] (column-major), [ncells][nm] x2
+    mnc_outs *outs;                   // may be null
+    double *ws;                       // [blocks][nq][nm]
+    int max_iter, ls_mode;
+    double tol, cond_h_max;
+    int *err;                         // error bits (1 << MNC_ERR_*)
+};
+bool launch_optcb(const CbArgs &a, int blocks, hipStream_t s);
+
 void launch_ghost_fill(const GridArgs &g, hipStream_t s);
 void launch_ansatz(const GridArgs &g, int q0, int nqc, hipStream_t s);
 // stage: 0 = flux only (writes g.flux), 1 = Euler stage, 2 = Euler + Heun average

@@ -85,6 +85,7 @@ def lib():
         L.mnc_get_gaunt.argtypes = [vp, _ip, _ip, _dp]
         L.mnc_opt_batch.argtypes = [vp, C.c_int, vp, vp, C.POINTER(Options), vp]
         L.mnc_opt_batch_host.argtypes = [vp, C.c_int, _dp, _dp, C.POINTER(Options), C.POINTER(Outs)]
+        L.mnc_optcb_batch_host.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(Options), C.POINTER(Outs)]
         L.mnc_batch_set.argtypes = [vp, C.c_int, _dp, _dp]
         L.mnc_batch_solve.argtypes = [vp, C.POINTER(Options)]
         L.mnc_batch_get.argtypes = [vp, _dp, _dp, C.POINTER(Outs)]
@@ -210,6 +211,26 @@ class Context:
         r = np.array([o.r for o in outs[:n]])
         ng = np.array([o.norm_g for o in outs[:n]])
         return st, u, alpha, it, r, ng
+
+    def optcb_batch_host(self, u, P2M, alphaP, opt=None, allow_status=(OK,)):
+        """The change-of-basis optimiser (optcb.cpp:90-415) on host arrays: P2M
+        [n][nm*nm] (column-major bases) and alphaP [n][nm] are the per-cell
+        state, updated; returns (status, u, P2M, alphaP, alphaM, iter, r,
+        normG)."""
+        opt = opt or options()
+        u = np.array(u, dtype=np.float64, order="C").reshape(-1, self.nm)
+        n = u.shape[0]
+        P2M = np.array(P2M, dtype=np.float64, order="C").reshape(n, self.nm * self.nm)
+        alphaP = np.array(alphaP, dtype=np.float64, order="C").reshape(n, self.nm)
+        alphaM = np.zeros_like(alphaP)
+        outs = (Outs * max(n, 1))()
+        st = lib().mnc_optcb_batch_host(self.h, n, _p(u), _p(P2M), _p(alphaP), _p(alphaM), C.byref(opt), outs)
+        if st not in allow_status:
+            self._check(st, "mnc_optcb_batch_host")
+        it = np.array([o.iter for o in outs[:n]], dtype=np.int32)
+        r = np.array([o.r for o in outs[:n]])
+        ng = np.array([o.norm_g for o in outs[:n]])
+        return st, u, P2M, alphaP, alphaM, it, r, ng
 
     # ---- grid ----
     def grid_create(self, nx, ny, dx, dy, init_x, init_y, init_cond, theta, moments, alpha, sigma_t, sigma_s,

@@ -144,6 +144,21 @@ int mnc_opt_batch(mnc_ctx *ctx, int ncells, double *u, double *alpha,
  * reference's c_moments / c_alpha host buffers. */
 int mnc_opt_batch_host(mnc_ctx *ctx, int ncells, double *u, double *alpha,
                        const mnc_options *opt, mnc_outs *outs);
+/* The change-of-basis optimiser on a batch, HOST arrays (copies in and out):
+ * replaces the per-cell optcb() loop that solveOptimization runs for
+ * MOMOPT_OPTIMIZATION_TYPE 1 (momopt_update.cpp:240-243 -> optcb.cpp:90-415).
+ * Per cell: u[nm] (in; out: the input, blended by the regularization used),
+ * P2M[nm*nm] (the basis, lower triangular, column-major; in/out state),
+ * alphaP[nm] (the multipliers in that basis; in/out state) and alphaM[nm]
+ * (out: the multipliers in the moment basis, as c_alpha).  The reference's
+ * grid initialisation (ones in the first column of P2M, isotropic alphaP;
+ * momopt_init.cpp:144-158) is a valid first input: its singular basis takes
+ * the first-evaluation reset to the identity (optcb.cpp:264-281).  Mn
+ * contexts with the direct Hessian only (the reference copies the table with
+ * stride nm, optcb.cpp:183), FP64, warm_start MNC_WARM_PREVIOUS; otherwise
+ * MNC_ERR_ARG.  Errors as mnc_opt_batch. */
+int mnc_optcb_batch_host(mnc_ctx *ctx, int ncells, double *u, double *P2M, double *alphaP, double *alphaM,
+                         const mnc_options *opt, mnc_outs *outs);
 
 /* A batch kept in device memory across calls (the staging arrays of
  * mnc_opt_batch_host): upload it, solve it in place any number of times (each
