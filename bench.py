@@ -295,6 +295,53 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
             "forms": res}
 
 
+def bench_optcb(mnc, n, seed, oracle_sample):
+    """The change-of-basis optimiser (optcb.cpp:90-415) on the isolated M4
+    batch of bench_isolated (n vectors, Q = 16, 20% near-boundary lobes, TOL
+    1e-8, reference line search) from the identity basis and isotropic alphaP:
+    converged closures/s from the kernel's HIP-event time (the entry point
+    takes host arrays; the wall time with the PCIe copies is reported beside
+    it), and the FP64 CPU oracle's optcb on the first `oracle_sample` vectors."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    wl = _load("workloads")
+    ctx = mnc.Context(4, 16, use_gaunt=False)
+    w, p, xi, eta = ctx.tables()
+    u, astar, lobe = wl.isolated_batch(w, p, xi, eta, 15, n, seed)
+    nm = 15
+    P2M = np.tile(np.eye(nm).reshape(1, -1), (n, 1))
+    aP = np.tile(oracle.isotropic_alpha(nm), (n, 1))
+    opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10)
+    best = None
+    for rep_ in range(2):
+        ctx.timing_enable(True)
+        t1 = time.perf_counter()
+        st, ug, Bg, aPg, aMg, it, r, ng = ctx.optcb_batch_host(u, P2M, aP, opt, allow_status=(mnc.OK, mnc.ERR_MAXREG))
+        wall = time.perf_counter() - t1
+        kn, kms = ctx.timing_get(mnc.KCLASS_NEWTON)
+        ctx.timing_enable(False)
+        if best is None or kms < best[0]:
+            best = (kms, wall)
+    ctx.close()
+    progress("optcb batch: kernel %.3f s, with copies %.3f s" % (best[0] * 1e-3, best[1]))
+    tab = oracle.Tables(4, 16, False)
+    m = oracle_sample
+    threads = len(os.sched_getaffinity(0))
+    t1 = time.perf_counter()
+    sto, uo, Bo, aPo, aMo, ito, ro, _ = tab.optcb_batch(u[:m], P2M[:m], aP[:m], nthreads=threads)
+    cpu_s = time.perf_counter() - t1
+    same = (it[:m] == ito) & (r[:m] == ro)
+    return {"workload": "isolated M4 batch, %d vectors (20%% near-boundary lobes), Q=16, identity basis, "
+                        "isotropic alphaP, TOL 1e-8, direct Hessian, reference line search" % n,
+            "status": int(st), "closures_per_s": n / (best[0] * 1e-3), "kernel_s": best[0] * 1e-3,
+            "closures_per_s_with_pcie": n / best[1],
+            "newton_iters_per_closure": float(it.mean()), "max_iter": int(it.max()),
+            "reg_hist": [int((r == v).sum()) for v in (0.0, 1e-3, 1e-2, 1e-1, 0.5)],
+            "cpu_baseline": {"value": m / cpu_s, "unit": "closures/s", "cores": threads, "kind": "port",
+                             "sample": "oracle optcb on the first %d vectors (%.1f s)" % (m, cpu_s)},
+            "oracle_sample_identical_path_frac": float(same.mean())}
+
+
 def bench_m8(mnc, gi, steps, n=512, sample=1000):
     """The M8 workload: 45 moments, Q = 32 (1024 points), n x n periodic cells on
     [-1,1]^2, sigma_s = 1, isotropic density 1 + 0.5 sin(pi x) sin(pi y), TOL
@@ -453,6 +500,8 @@ def main():
     ap.add_argument("--isolated", type=int, default=1_000_000,
                     help="vectors of the isolated closure batch (0 skips it)")
     ap.add_argument("--m8-steps", type=int, default=20, help="steps of the M8 512^2 workload (0 skips it)")
+    ap.add_argument("--optcb", type=int, default=200_000,
+                    help="vectors of the change-of-basis optimiser's isolated batch (0 skips it)")
     ap.add_argument("--ppn-steps", type=int, default=569, help="steps of the PPn lattice form (0 skips it)")
     ap.add_argument("--ppn-only", action="store_true", help="only the PPn lattice form (for profiling)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -556,6 +605,8 @@ def main():
     if args.isolated > 0 and not args.plain_only and not args.only:
         out["config"]["isolated"] = bench_isolated(mnc, args.isolated, 12345, 20000, args.train_steps)
         progress("isolated batch done")
+        out["config"]["optcb"] = bench_optcb(mnc, args.optcb, 12345, 50000) if args.optcb > 0 else None
+        progress("optcb batch done")
     if args.m8_steps > 0 and not args.plain_only and not args.only:
         out["config"]["m8"] = bench_m8(mnc, gi, args.m8_steps)
         progress("M8 done")
