@@ -327,10 +327,12 @@ def bench_optcb(mnc, n, seed, oracle_sample):
     tab = oracle.Tables(4, 16, False)
     m = oracle_sample
     threads = len(os.sched_getaffinity(0))
-    t1 = time.perf_counter()
-    sto, uo, Bo, aPo, aMo, ito, ro, _ = tab.optcb_batch(u[:m], P2M[:m], aP[:m], nthreads=threads)
-    cpu_s = time.perf_counter() - t1
-    same = (it[:m] == ito) & (r[:m] == ro)
+    cpu_s, same = 0.0, np.zeros(0, dtype=bool)
+    if m > 0:
+        t1 = time.perf_counter()
+        sto, uo, Bo, aPo, aMo, ito, ro, _ = tab.optcb_batch(u[:m], P2M[:m], aP[:m], nthreads=threads)
+        cpu_s = time.perf_counter() - t1
+        same = (it[:m] == ito) & (r[:m] == ro)
     return {"workload": "isolated M4 batch, %d vectors (20%% near-boundary lobes), Q=16, identity basis, "
                         "isotropic alphaP, TOL 1e-8, direct Hessian, reference line search" % n,
             "status": int(st), "closures_per_s": n / (best[0] * 1e-3), "kernel_s": best[0] * 1e-3,
@@ -338,8 +340,8 @@ def bench_optcb(mnc, n, seed, oracle_sample):
             "newton_iters_per_closure": float(it.mean()), "max_iter": int(it.max()),
             "reg_hist": [int((r == v).sum()) for v in (0.0, 1e-3, 1e-2, 1e-1, 0.5)],
             "cpu_baseline": {"value": m / cpu_s, "unit": "closures/s", "cores": threads, "kind": "port",
-                             "sample": "oracle optcb on the first %d vectors (%.1f s)" % (m, cpu_s)},
-            "oracle_sample_identical_path_frac": float(same.mean())}
+                             "sample": "oracle optcb on the first %d vectors (%.1f s)" % (m, cpu_s)} if m > 0 else None,
+            "oracle_sample_identical_path_frac": float(same.mean()) if m > 0 else None}
 
 
 def bench_m8(mnc, gi, steps, n=512, sample=1000):
@@ -504,6 +506,8 @@ def main():
                     help="vectors of the change-of-basis optimiser's isolated batch (0 skips it)")
     ap.add_argument("--ppn-steps", type=int, default=569, help="steps of the PPn lattice form (0 skips it)")
     ap.add_argument("--ppn-only", action="store_true", help="only the PPn lattice form (for profiling)")
+    ap.add_argument("--optcb-only", action="store_true",
+                    help="only the change-of-basis optimiser's isolated batch (for profiling)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="force a launch/tuning setting after the autotuner (repeatable)")
     ap.add_argument("--stage-probe", choices=["plain-fp64", "mixed", "learned+mixed"], default=None,
@@ -526,6 +530,12 @@ def main():
     d = dict(LATTICE)
     if args.stage_probe:
         stage_probe(mnc, gi, d, args.stage_probe, args.probe_step)
+        return
+    if args.optcb_only:
+        r = bench_optcb(mnc, args.optcb, 12345, 0 if args.no_cpu else 50000)
+        print(json.dumps({"metric": "converged optcb closures/s (isolated M4 batch, TOL 1e-8)",
+                          "value": r["closures_per_s"], "unit": "closures/s", "n_gpus": 1,
+                          "higher_is_better": True, "config": {"optcb": r}}))
         return
     if args.ppn_only:
         r = bench_ppn(mnc, gi, d, args.ppn_steps, args.warmup, 0.0 if args.no_cpu else min(args.cpu_seconds, 10.0))
