@@ -113,6 +113,51 @@ void write_output(const Grid &g, mnc_ctx *ctx, int nm, double time) {
     std::fprintf(f, "Regularizations Histogram\n");
     for (int r = 0; r < 5; ++r) std::fprintf(f, "r = %.3e   %lld\n", reg[r], st.hist_reg[r]);
     std::fclose(f);
+    // Restart side file (no counterpart in the reference, whose .pn holds the
+    // moments only): the multipliers of every cell, ghosts included, since they
+    // are the next closure's warm start.  int64 nx, ny, nm; double time; then
+    // alpha[(nx+4)*(ny+4)][nm].
+    std::vector<double> alpha(size_t(g.NX) * g.NY * nm);
+    if (mnc_grid_get(ctx, nullptr, alpha.data()) != MNC_OK) {
+        std::printf("mnc_grid_get: %s\n", mnc_last_error(ctx));
+        std::exit(1);
+    }
+    std::snprintf(name, sizeof name, "out_%.3f_%d.alpha", time, 0);
+    f = std::fopen(name, "wb");
+    if (!f) { std::printf("cannot write %s\n", name); std::exit(1); }
+    int64_t hdr[3] = {g.nx, g.ny, nm};
+    std::fwrite(hdr, sizeof(int64_t), 3, f);
+    std::fwrite(&time, sizeof(double), 1, f);
+    std::fwrite(alpha.data(), sizeof(double), alpha.size(), f);
+    std::fclose(f);
+}
+
+// Reads a restart: <prefix>.pn (the moments of the interior cells) and
+// <prefix>.alpha (time and the multipliers of every cell).  Returns false with
+// a message if the files do not match the deck's grid.
+bool read_restart(const std::string &prefix, const Grid &g, int nm, std::vector<double> &mom,
+                  std::vector<double> &alpha, double &time) {
+    FILE *f = std::fopen((prefix + ".pn").c_str(), "rb");
+    if (!f) { std::printf("cannot read %s.pn\n", prefix.c_str()); return false; }
+    int64_t nx = 0, ny = 0, n = 0;
+    double box[4];
+    bool ok = std::fread(&nx, sizeof nx, 1, f) == 1 && std::fread(&ny, sizeof ny, 1, f) == 1 &&
+              std::fread(box, sizeof(double), 4, f) == 4 && std::fread(&n, sizeof n, 1, f) == 1;
+    ok = ok && nx == g.nx && ny == g.ny && n == nm;
+    for (int i = 2; ok && i < 2 + g.nx; ++i)
+        for (int j = 2; ok && j < 2 + g.ny; ++j)
+            ok = std::fread(&mom[(size_t(i) * g.NY + j) * nm], sizeof(double), nm, f) == size_t(nm);
+    std::fclose(f);
+    if (!ok) { std::printf("%s.pn does not match the deck's grid\n", prefix.c_str()); return false; }
+    f = std::fopen((prefix + ".alpha").c_str(), "rb");
+    if (!f) { std::printf("cannot read %s.alpha\n", prefix.c_str()); return false; }
+    int64_t hdr[3] = {0, 0, 0};
+    ok = std::fread(hdr, sizeof(int64_t), 3, f) == 3 && std::fread(&time, sizeof(double), 1, f) == 1;
+    ok = ok && hdr[0] == g.nx && hdr[1] == g.ny && hdr[2] == nm;
+    ok = ok && std::fread(alpha.data(), sizeof(double), alpha.size(), f) == alpha.size();
+    std::fclose(f);
+    if (!ok) { std::printf("%s.alpha does not match the deck's grid\n", prefix.c_str()); return false; }
+    return true;
 }
 
 }  // namespace
@@ -191,15 +236,25 @@ int main(int argc, char **argv) {
         else
             alpha[c * nm] = 2.0 * std::sqrt(M_PI) * std::log(mom[c * nm] / (2.0 * std::sqrt(M_PI)));
     }
+    // MOMOPT_GPU_RESTART <prefix>: start from <prefix>.pn and <prefix>.alpha
+    // (written at every output time) instead of the initial condition
+    std::string restart;
+    double t0 = 0.0;
+    const bool restarting = deck.get("MOMOPT_GPU_RESTART", restart);
+    if (restarting && !read_restart(restart, g, nm, mom, alpha, t0)) return 1;
     mnc_grid_desc gd{g.nx, g.ny, g.dx, g.dy, g.init_x, g.init_y, init_cond, theta, delta_ppn};
     st = mnc_grid_create(ctx, &gd, mom.data(), alpha.data(), g.sigma_t.data(), g.sigma_s.data());
     if (st != MNC_OK) { std::printf("mnc_grid_create: %s\n", mnc_last_error(ctx)); return 1; }
+    if (restarting && (st = mnc_ghost_fill(ctx)) != MNC_OK) {        // the ghost moments of the .pn's interior
+        std::printf("mnc_ghost_fill: %s\n", mnc_last_error(ctx));
+        return 1;
+    }
     const double max_dt = cfl * (2.0 / (2.0 + theta)) * (g.dx * g.dy) / (g.dx + g.dy);   // :72
     std::printf("--- Init done ---\n");
     mnc_options opt{max_iter, tol, cond_h_max, prec, warm, ls, delta_ppn};
 
-    double t = 0, dt = max_dt;                        // main.cpp:204-230
-    write_output(g, ctx, nm, t);
+    double t = t0, dt = max_dt;                       // main.cpp:204-230
+    if (!restarting) write_output(g, ctx, nm, t);
     for (double t_out = std::fmin(t + out_dt, t_final); t_out <= t_final; t_out = std::fmin(t_out + out_dt, t_final)) {
         dt = max_dt;
         while (t < t_out) {

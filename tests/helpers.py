@@ -76,13 +76,13 @@ def fixture_path(ic):
     return os.path.join(GOLDEN, "momopt", ic, "mn.pn")
 
 
-def deck_text(d, t_final, extra=None):
+def deck_text(d, t_final, extra=None, out_dt=1000.1):
     """An input.deck for the drivers (util/decks.py:26-104 key set, %.17f values)."""
     lines = ["SOLVER momopt",
              "NUM_CELLS_X %d" % d["num_cells_x"], "NUM_CELLS_Y %d" % d["num_cells_y"],
              "NUM_MPI_PARTITIONS_X 1", "NUM_MPI_PARTITIONS_Y 1",
              "A_X %.17f" % d["a_x"], "B_X %.17f" % d["b_x"], "A_Y %.17f" % d["a_y"], "B_Y %.17f" % d["b_y"],
-             "T_FINAL %.17f" % t_final, "OUT_DELTA_T %.17f" % 1000.1,
+             "T_FINAL %.17f" % t_final, "OUT_DELTA_T %.17f" % out_dt,
              "GAUSSIAN_SIGMA %.17f" % d["gaussian_sigma"], "FLOOR %.17f" % d["floor_value"],
              "INIT_COND %d" % d["init_cond"], "SIGMA %.17f" % d["sigma"],
              "MOMOPT_MOMENT_ORDER %d" % d["moment_order"], "MOMOPT_QUAD_ORDER %d" % d["quad_order"],

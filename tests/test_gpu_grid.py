@@ -177,6 +177,37 @@ def test_driver_reproduces_fixture(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_restart_is_bitwise(tmp_path):
+    """The driver writes the multipliers beside each .pn (out_<t>_0.alpha, the
+    closures' warm start); a run restarted from the t = 0.5 files
+    (MOMOPT_GPU_RESTART) ends bit for bit where the uninterrupted run does."""
+    d = fixture_deck("gaussian")
+    exe = os.path.join(PKG, "momopt_gpu")
+    full, part = tmp_path / "full", tmp_path / "restart"
+    full.mkdir()
+    part.mkdir()
+    (full / "input.deck").write_text(deck_text(d, 1.0, out_dt=0.5))
+    r = subprocess.run([exe], cwd=full, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:]
+    for ext in ("pn", "alpha"):
+        (part / ("mid." + ext)).write_bytes((full / ("out_0.500_0." + ext)).read_bytes())
+    (part / "input.deck").write_text(deck_text(d, 1.0, {"MOMOPT_GPU_RESTART": "mid"}, out_dt=0.5))
+    r = subprocess.run([exe], cwd=part, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert not (part / "out_0.000_0.pn").exists()
+    for ext in ("pn", "alpha"):
+        a = (full / ("out_1.000_0." + ext)).read_bytes()
+        b = (part / ("out_1.000_0." + ext)).read_bytes()
+        assert a == b, ext
+    h, m = read_pn(str(part / "out_1.000_0.pn"))
+    assert h[:2] == (36, 74)
+    # a restart whose files do not match the deck is refused
+    (part / "input.deck").write_text(deck_text(dict(d, num_cells_x=20), 1.0, {"MOMOPT_GPU_RESTART": "mid"}))
+    r = subprocess.run([exe], cwd=part, capture_output=True, text=True, timeout=600)
+    assert r.returncode != 0 and "does not match" in r.stdout
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("ic", ["lattice", "smooth"])
 def test_fused_flux_tiles_equal_unfused(ic):
     """The ansatz fused into the flux (8x8 and 16x16 LDS tiles) gives the same
