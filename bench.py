@@ -1,6 +1,6 @@
 """Benchmark: converged Mn closures/s and s/timestep on the M4 lattice.
 
-Workload (the north-star lattice case): M4 closure (15 moments), Q = 16
+Workload (the lattice case): M4 closure (15 moments), Q = 16
 (256 hemisphere quadrature points), 280 x 280 cells on [-3.5, 3.5]^2 with the
 7 x 7 block checkerboard of absorbers (sigma_a = 10) and a unit isotropic
 source in the centre block, floor 1e-10, MOMOPT_TOL 1e-8, direct Hessian, the

@@ -332,7 +332,7 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     if (learned && ctx->learned_gate_log10 >= kGateOff) {
         // network off: every cell starts from the alpha passed in (isotropicAlpha
         // if it is not finite), as the gate would decide at any threshold above
-        // the gradient norms, without the network's or the judge's launches
+        // the gradient norms, without the network's or judge_kernel's launches
         if (t.nm != 15) return ctx_fail(ctx, MNC_ERR_ARG, "the learned warm start is for moment order 4");
         Timed tm(ctx, MNC_KCLASS_NEWTON);
         mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
@@ -340,7 +340,7 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     } else if (learned) {
         // Round 1 evaluates every cell at its start and parks those failing the
         // gate on the network's list; the network runs on those rows only (the
-        // count is read on the device), the judge moves them to round 2's list.
+        // count is read on the device), judge_kernel moves them to round 2's list.
         if (t.nm != 15) return ctx_fail(ctx, MNC_ERR_ARG, "the learned warm start is for moment order 4");
         mnc::LaunchPlan first = lp, rest = lp;
         first.r1 = 1;

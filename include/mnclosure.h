@@ -214,7 +214,8 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *                  10^value ||u|| (scaled problem; -16..4, default -2);
  *                  5 switches the network off: every cell starts from the
  *                  alpha passed in (isotropicAlpha if it is not finite) and
- *                  neither the network nor the judge is launched
+ *                  neither the network nor the kernel that compares its guess
+ *                  with the start is launched
  *   "flux_tile"    0: ansatz written to HBM, then the flux; 1: ansatz fused
  *                  into the flux with 8x8-cell LDS tiles; 2: 16x16 tiles
  *                  (n <= 15); these three give identical results.  3 (default;

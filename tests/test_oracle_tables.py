@@ -98,7 +98,7 @@ def test_sph_plm_special_values():
 
 @pytest.mark.parametrize("N,Q", [(2, 12), (3, 30), (4, 16)])
 def test_gaunt_map_invariant(N, Q):
-    """sum_k G_ijk m_k(q) = m_i(q) m_j(q) at every node; nnz counts as surveyed
+    """sum_k G_ijk m_k(q) = m_i(q) m_j(q) at every node; nnz counts of the reference's map
     (upper triangle M2 45, M3 167, M4 432)."""
     o = oracle()
     t = o.Tables(N, Q, True)
