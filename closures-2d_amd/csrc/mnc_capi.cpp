@@ -24,6 +24,11 @@ struct mnc_ctx {
     // solver state / stats
     int *d_ctl = nullptr;            // [0],[1] list counts, [2] tail queue, [3] error bits, [4] deferred
     long long *d_stats = nullptr;    // [kStatWords]
+    // change-of-basis grid state: per-cell basis (column-major) and multipliers
+    // in that basis, and the blocks' tables seen through their cells' bases
+    double *d_cbB = nullptr, *d_cbaP = nullptr, *d_cbws = nullptr;
+    size_t cb_cells = 0;
+    int cb_blocks = 0;
     double *d_std = nullptr;         // per-cell solver state (doubles)
     int *d_sti = nullptr;            // per-cell solver state (ints)
     int *d_lists = nullptr;          // mnc::kNumLists lists of cell indices (see mnc_internal.h)
@@ -184,6 +189,13 @@ int check_opts(mnc_ctx *ctx, const mnc_options *o) {
             return ctx_fail(ctx, MNC_ERR_ARG, "PPn runs the FP64 form from the previous or isotropic alpha");
         if (ctx->tab.nm > 15) return ctx_fail(ctx, MNC_ERR_ARG, "PPn has kernels for moment orders up to 4");
     }
+    if (o->opt_type != MNC_OPT_NEWTON && o->opt_type != MNC_OPT_CHANGE_BASIS)
+        return ctx_fail(ctx, MNC_ERR_ARG, "unknown opt_type");
+    if (o->opt_type == MNC_OPT_CHANGE_BASIS &&
+        (ctx->ppn || ctx->tab.use_gaunt || ctx->tab.np2 != ctx->tab.nm || ctx->tab.nm > 15 ||
+         o->precision != MNC_PREC_FP64 || o->warm_start != MNC_WARM_PREVIOUS))
+        return ctx_fail(ctx, MNC_ERR_ARG, "the change-of-basis optimiser runs Mn, direct Hessian, FP64, "
+                                          "from the cell's own state");
     return MNC_OK;
 }
 
@@ -252,6 +264,9 @@ constexpr int kGateOff = 5;
 int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_options *o, mnc_outs *outs,
                const double *alpha_guess, bool check) {
     if (ncells <= 0) return MNC_OK;
+    if (o->opt_type != MNC_OPT_NEWTON)
+        return ctx_fail(ctx, MNC_ERR_ARG, "batch solves run the Newton optimiser; the change-of-basis batch is "
+                                          "mnc_optcb_batch_host");
     const auto &t = ctx->tab;
     const bool mixed = o->precision == MNC_PREC_MIXED;
     if (mixed && !ctx->has_mix)
@@ -387,6 +402,65 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
         } else if ((st = launch(a, lp))) {
             return st;
         }
+    }
+    MNC_HIP_CHECK(hipGetLastError());
+    if (!check) return MNC_OK;
+    return check_errors(ctx);
+}
+
+// The change-of-basis state as MomOptSolver::init leaves it
+// (momopt_init.cpp:144-158): ones in the first nm entries of each cell's
+// P2M (its first column) and isotropicAlpha as alphaP.
+int cb_state_init(mnc_ctx *ctx) {
+    const int nm = ctx->tab.nm;
+    const size_t nc = ctx->cb_cells;
+    std::vector<double> B(nc * nm * nm, 0.0), aP(nc * nm, 0.0);
+    const double a0 = 2.0 * std::sqrt(M_PI) * std::log(0.5 / std::sqrt(M_PI));
+    for (size_t c = 0; c < nc; ++c) {
+        for (int k = 0; k < nm; ++k) B[c * nm * nm + k] = 1.0;
+        aP[c * nm] = a0;
+    }
+    MNC_HIP_CHECK(hipMemcpy(ctx->d_cbB, B.data(), B.size() * sizeof(double), hipMemcpyHostToDevice));
+    MNC_HIP_CHECK(hipMemcpy(ctx->d_cbaP, aP.data(), aP.size() * sizeof(double), hipMemcpyHostToDevice));
+    return MNC_OK;
+}
+
+// optcb over every cell of the grid (solveOptimization with
+// OPT_TYPE_CHANGE_BASIS, momopt_update.cpp:240-243): u = the moments, alphaM =
+// the grid's alpha, the basis state kept on the device between solves.
+int run_optcb_grid(mnc_ctx *ctx, int nc, const mnc_options *o, bool check) {
+    const int nm = ctx->tab.nm, nq = ctx->tab.nq;
+    int st;
+    if (size_t(nc) > ctx->cb_cells) {   // first use: allocate and initialise (never inside a capture)
+        ctx->cb_cells = size_t(nc);
+        ctx->cb_blocks = std::min(nc, 16 * ctx->num_cu);
+        if ((st = dalloc(ctx, &ctx->d_cbB, size_t(nc) * nm * nm)) || (st = dalloc(ctx, &ctx->d_cbaP, size_t(nc) * nm)) ||
+            (st = dalloc(ctx, &ctx->d_cbws, size_t(ctx->cb_blocks) * nq * nm)) || (st = cb_state_init(ctx)))
+            return st;
+    }
+    mnc::CbArgs a{};
+    a.ncells = nc;
+    a.nm = nm;
+    a.nq = nq;
+    a.w = ctx->d_w;
+    a.p = ctx->d_p;
+    a.u = ctx->d_mom;
+    a.P2M = ctx->d_cbB;
+    a.alphaP = ctx->d_cbaP;
+    a.alphaM = ctx->d_alpha;
+    a.outs = nullptr;
+    a.ws = ctx->d_cbws;
+    a.max_iter = o->max_iter;
+    a.ls_mode = o->line_search;
+    a.tol = o->tol_rel;
+    a.cond_h_max = o->cond_h_max;
+    a.err = ctx->d_ctl + 3;
+    a.stats = ctx->d_stats;
+    {
+        Timed tm(ctx, MNC_KCLASS_NEWTON);
+        mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
+        if (!mnc::launch_optcb(a, ctx->cb_blocks, ctx->stream))
+            return ctx_fail(ctx, MNC_ERR_ARG, "no change-of-basis kernel instance for this moment order");
     }
     MNC_HIP_CHECK(hipGetLastError());
     if (!check) return MNC_OK;
@@ -532,7 +606,8 @@ void mnc_destroy(mnc_ctx *ctx) {
                     ctx->d_mz, ctx->d_mg, ctx->d_mh, ctx->d_mgptr, ctx->d_mgk, ctx->d_mgv, ctx->d_mpv, ctx->d_gz, ctx->d_gg, ctx->d_gh,
                     ctx->d_mlp, ctx->d_mlp_scratch, ctx->d_mlp_loss, ctx->d_mlp_io, ctx->d_us, ctx->d_guess,
                     ctx->d_ctl, ctx->d_stats, ctx->d_std, ctx->d_sti, ctx->d_lists, ctx->d_mom, ctx->d_alpha, ctx->d_old,
-                    ctx->d_st, ctx->d_ss, ctx->d_flux, ctx->d_ansatz, ctx->d_fpart, ctx->d_bu, ctx->d_ba, ctx->d_bo};
+                    ctx->d_st, ctx->d_ss, ctx->d_flux, ctx->d_ansatz, ctx->d_fpart, ctx->d_bu, ctx->d_ba, ctx->d_bo,
+                    ctx->d_cbB, ctx->d_cbaP, ctx->d_cbws};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -638,6 +713,7 @@ int mnc_optcb_batch_host(mnc_ctx *ctx, int ncells, double *u, double *P2M, doubl
     a.alphaM = d_aM;
     a.outs = d_outs;
     a.ws = d_ws;
+    a.stats = ctx->d_stats;
     a.max_iter = opt->max_iter;
     a.ls_mode = opt->line_search;
     a.tol = opt->tol_rel;
@@ -770,6 +846,7 @@ int mnc_grid_set(mnc_ctx *ctx, const double *moments, const double *alpha) {
     if (moments) MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_mom, moments, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     if (alpha) MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_alpha, alpha, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    if (ctx->d_cbB) return cb_state_init(ctx);        // a new state starts the change-of-basis state afresh
     return MNC_OK;
 }
 
@@ -792,6 +869,7 @@ int mnc_solve_grid(mnc_ctx *ctx, const mnc_options *opt) {
     if (st) return st;
     MNC_HIP_CHECK(hipSetDevice(ctx->device));
     const int nc = (ctx->grid.num_cells_x + 4) * (ctx->grid.num_cells_y + 4);
+    if (opt->opt_type == MNC_OPT_CHANGE_BASIS) return run_optcb_grid(ctx, nc, opt, true);
     return run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr, true);
 }
 
@@ -824,7 +902,11 @@ int enqueue_step(mnc_ctx *ctx, double dt, const mnc_options *opt) {
     MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_old, ctx->d_mom, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     for (int stage = 1; stage <= 2; ++stage) {
         if ((st = mnc_ghost_fill(ctx))) return st;
-        if ((st = run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr, false))) return st;
+        if (opt->opt_type == MNC_OPT_CHANGE_BASIS) {
+            if ((st = run_optcb_grid(ctx, nc, opt, false))) return st;
+        } else if ((st = run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr, false))) {
+            return st;
+        }
         // errors of the first stage are seen at the end of the step (the
         // kernels flag them in device memory; the state is left as computed)
         if ((st = flux_stage(ctx, stage, dt))) return st;
@@ -835,7 +917,7 @@ int enqueue_step(mnc_ctx *ctx, double dt, const mnc_options *opt) {
 bool same_opts(const mnc_options &a, const mnc_options &b) {
     return a.max_iter == b.max_iter && a.tol_rel == b.tol_rel && a.cond_h_max == b.cond_h_max &&
            a.precision == b.precision && a.warm_start == b.warm_start && a.line_search == b.line_search &&
-           a.delta_ppn == b.delta_ppn;
+           a.delta_ppn == b.delta_ppn && a.opt_type == b.opt_type;
 }
 
 // Captures enqueue_step into g.exec.  On any capture error the context falls

@@ -206,6 +206,7 @@ struct CbArgs {
     int max_iter, ls_mode;
     double tol, cond_h_max;
     int *err;                         // error bits (1 << MNC_ERR_*)
+    long long *stats;                 // [kStatWords] as the Newton kernels, or null
 };
 bool launch_optcb(const CbArgs &a, int blocks, hipStream_t s);
 

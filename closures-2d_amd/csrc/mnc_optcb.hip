@@ -483,6 +483,12 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
                 for (int k = 0; k < N; k++) aPg[k] = sh.aP[k];
                 for (int k = 0; k < N * N; k++) Bg[k] = sh.P2M[k];
                 if (A.outs) { A.outs[c].iter = total; A.outs[c].r = r; A.outs[c].norm_g = sh.err; }
+                if (A.stats) {                              // c_optStats (momopt_update.cpp:254-273)
+                    atomicMax((unsigned long long *)&A.stats[0], (unsigned long long)total);
+                    atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)total);
+                    atomicAdd((unsigned long long *)&A.stats[2], 1ull);
+                    atomicAdd((unsigned long long *)&A.stats[3 + reg], 1ull);
+                }
             }
         }
         __syncthreads();

@@ -18,6 +18,7 @@ OK, ERR_ARG, ERR_U0, ERR_MAXREG, ERR_HIP, ERR_STATE = 0, 1, 2, 3, 4, 5
 PREC_FP64, PREC_MIXED = 0, 1
 LS_REFERENCE, LS_ROBUST = 0, 1
 WARM_PREVIOUS, WARM_ISOTROPIC, WARM_LEARNED = 0, 1, 2
+OPT_NEWTON, OPT_CHANGE_BASIS = 0, 1
 MOMENT_MN, MOMENT_PPN = 0, 1
 KCLASS_NEWTON, KCLASS_FLUX, KCLASS_OTHER, KCLASS_MLP = 0, 1, 2, 3
 
@@ -33,7 +34,7 @@ class Setup(C.Structure):
 class Options(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("tol_rel", C.c_double), ("cond_h_max", C.c_double),
                 ("precision", C.c_int), ("warm_start", C.c_int), ("line_search", C.c_int),
-                ("delta_ppn", C.c_double)]
+                ("delta_ppn", C.c_double), ("opt_type", C.c_int)]
 
 
 class Outs(C.Structure):
@@ -123,8 +124,8 @@ def _p(a):
 
 
 def options(max_iter=100, tol=1e-8, cond_h_max=1e10, precision=PREC_FP64, warm_start=WARM_PREVIOUS,
-            line_search=LS_REFERENCE, delta_ppn=0.0):
-    return Options(max_iter, tol, cond_h_max, precision, warm_start, line_search, delta_ppn)
+            line_search=LS_REFERENCE, delta_ppn=0.0, opt_type=0):
+    return Options(max_iter, tol, cond_h_max, precision, warm_start, line_search, delta_ppn, opt_type)
 
 
 class Context:

@@ -82,7 +82,19 @@ typedef struct {
     int warm_start;            /* MNC_WARM_*                                    */
     int line_search;           /* MNC_LS_*                                      */
     double delta_ppn;          /* MOMOPT_DELTA_PPN (contexts of moment type 1)  */
+    int opt_type;              /* MOMOPT_OPTIMIZATION_TYPE: MNC_OPT_* (0 when the
+                                  struct is zero-initialised past delta_ppn)    */
 } mnc_options;
+
+/* Optimisers (opt.h:42-46).  MNC_OPT_CHANGE_BASIS in the grid solves
+ * (mnc_solve_grid, mnc_update) runs optcb (optcb.cpp:90-415) with the
+ * per-cell basis and multipliers kept on the device, initialised as
+ * momopt_init.cpp:144-158 does at mnc_grid_create and mnc_grid_set: Mn,
+ * direct Hessian, FP64, warm start MNC_WARM_PREVIOUS.  The batch entry points
+ * take MNC_OPT_NEWTON only (the change-of-basis batch is
+ * mnc_optcb_batch_host). */
+#define MNC_OPT_NEWTON 0
+#define MNC_OPT_CHANGE_BASIS 1
 
 /* OUTS (src/momopt/opt/opt.h:25-32). */
 typedef struct {

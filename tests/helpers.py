@@ -76,7 +76,7 @@ def fixture_path(ic):
     return os.path.join(GOLDEN, "momopt", ic, "mn.pn")
 
 
-def deck_text(d, t_final, extra=None, out_dt=1000.1):
+def deck_text(d, t_final, extra=None, out_dt=1000.1, opt_type=0):
     """An input.deck for the drivers (util/decks.py:26-104 key set, %.17f values)."""
     lines = ["SOLVER momopt",
              "NUM_CELLS_X %d" % d["num_cells_x"], "NUM_CELLS_Y %d" % d["num_cells_y"],
@@ -90,7 +90,7 @@ def deck_text(d, t_final, extra=None, out_dt=1000.1):
              "MOMOPT_COND_H_MAX %.17f" % d["cond_h_max"], "MOMOPT_COND_H_MAX_BFGS 20",
              "MOMOPT_MAX_ITER %d" % d["max_iter"], "MOMOPT_MAX_BFGS_ITER 5",
              "MOMOPT_USE_GAUNT %d" % d["use_gaunt"], "MOMOPT_THETA %.17f" % d["theta"],
-             "MOMOPT_DELTA_PPN %.17f" % 1e-10, "MOMOPT_MOMENT_TYPE 0", "MOMOPT_OPTIMIZATION_TYPE 0"]
+             "MOMOPT_DELTA_PPN %.17f" % 1e-10, "MOMOPT_MOMENT_TYPE 0", "MOMOPT_OPTIMIZATION_TYPE %d" % opt_type]
     for k, v in (extra or {}).items():
         lines.append("%s %s" % (k, v))
     return "\n".join(lines) + "\n"

@@ -9,7 +9,11 @@ and bases to rounding on those cells, and every cell solved to the tolerance."""
 import numpy as np
 import pytest
 
-from helpers import config2_batch, load_pkg, oracle
+import os
+import subprocess
+
+from helpers import (PKG, config2_batch, deck_text, density, fixture_deck, fixture_path, gpu_grid_from_deck, load_pkg,
+                     oracle, read_pn, rel_l2)
 
 
 def _moments(tab, alpha):
@@ -122,3 +126,82 @@ def test_optcb_rejects_unsupported_forms():
                               allow_status=(mnc.OK, mnc.ERR_ARG))[0]
     assert st == mnc.ERR_ARG
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_grid_solve_with_change_of_basis_equals_batch():
+    """mnc_solve_grid with opt_type MNC_OPT_CHANGE_BASIS runs the same kernel
+    on the grid's moments, alpha and the device-held basis state, initialised
+    as momopt_init.cpp:144-158: bit for bit the batch entry point's result from
+    that initial state, and the state carries over to the next solve."""
+    mnc, gi = load_pkg("mnc"), load_pkg("grid_init")
+    print("case optcb grid solve", flush=True)
+    # the regression decks use the Gaunt form, which optcb cannot (it reads the
+    # table with stride n): the same deck with the direct Hessian
+    d = dict(fixture_deck("gaussian"), use_gaunt=0, tol=1e-8)
+    ctx, g = gpu_grid_from_deck(mnc, gi, d)
+    nm = ctx.nm
+    m0, a0 = ctx.grid_get()
+    opt = mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"], opt_type=mnc.OPT_CHANGE_BASIS)
+    ctx.reset_stats()
+    ctx.solve_grid(opt)
+    st = ctx.stats()
+    mg, ag = ctx.grid_get()
+    n = m0.reshape(-1, nm).shape[0]
+    assert st.num_dual_solves == n
+    B = np.zeros((n, nm * nm))
+    B[:, :nm] = 1.0
+    aP = np.tile(oracle().isotropic_alpha(nm), (n, 1))
+    stb, ub, Bb, aPb, aMb, itb, rb, _ = ctx.optcb_batch_host(m0.reshape(-1, nm), B, aP, opt)
+    assert stb == 0
+    assert np.array_equal(ag.reshape(-1, nm), aMb)
+    assert np.array_equal(mg.reshape(-1, nm), ub)
+    # the same moments again: the kept basis solves them at once
+    ctx.reset_stats()
+    ctx.solve_grid(opt)
+    st2 = ctx.stats()
+    print("first solve %.2f iterations per closure, second %.2f" % (st.iter_sum / n, st2.iter_sum / n))
+    assert st2.iter_sum / n <= 1.2
+    # a new grid state restarts the basis state
+    ctx.grid_set(m0, a0)
+    ctx.solve_grid(opt)
+    assert np.array_equal(ctx.grid_get()[1].reshape(-1, nm), aMb)
+    ctx.close()
+    # the batch entry points refuse the change-of-basis optimiser
+    ctx = mnc.Context(2, 12, use_gaunt=False)
+    u = np.zeros((1, 6))
+    u[0, 0] = 1.0
+    st = ctx.opt_batch_host(u, np.zeros((1, 6)), opt, allow_status=(mnc.OK, mnc.ERR_ARG))[0]
+    ctx.close()
+    assert st == mnc.ERR_ARG
+
+
+@pytest.mark.gpu
+def test_driver_change_of_basis_time_stepping(tmp_path):
+    """MOMOPT_OPTIMIZATION_TYPE 1 through the deck driver: the smooth periodic
+    deck (36 x 74, M3, Q = 30) with the direct Hessian and TOL 1e-8 to t =
+    1.03, robust line search (at TOL 1e-8 the reference's Armijo test fails
+    on f's rounding for ~1% of these closures and regularizes them).  Neither
+    optimiser regularizes then (the .opt histograms), and both
+    solve every closure to the tolerance, so the change-of-basis run lands on
+    the Newton optimiser's run (itself checked against the oracle and the
+    fixtures) at tolerance level.  (Near vacuum the two part ways by design:
+    the Newton optimiser regularizes cells whose Hessian is ill-conditioned in
+    the moment basis, the change of basis solves them unblended.)"""
+    d = dict(fixture_deck("smooth"), use_gaunt=0, tol=1e-8)
+    exe = os.path.join(PKG, "momopt_gpu")
+    res = {}
+    for ot in (0, 1):
+        run = tmp_path / ("opt%d" % ot)
+        run.mkdir()
+        (run / "input.deck").write_text(deck_text(d, 1.03, {"MOMOPT_GPU_LINE_SEARCH": 1}, opt_type=ot))
+        r = subprocess.run([exe], cwd=run, capture_output=True, text=True, timeout=600)
+        assert r.returncode == 0, r.stdout[-800:]
+        res[ot] = read_pn(str(run / "out_1.030_0.pn"))[1]
+        hist = (run / "out_1.030_0.opt").read_text()
+        print("opt type %d:\n%s" % (ot, hist))
+        regs = [int(line.split()[-1]) for line in hist.splitlines() if line.startswith("r = ")]
+        assert len(regs) == 5 and sum(regs[1:]) == 0
+    err = rel_l2(density(res[1]), density(res[0]))
+    print("optcb vs Newton time stepping: density rel-L2 %.3e" % err)
+    assert err < 1e-7
