@@ -11,6 +11,7 @@
 // (Cholesky, condition estimate, the basis products and solves) runs on lane 0
 // in the reference's order, on LDS copies, between block barriers.
 #include "mnc_internal.h"
+#include "mnc_coop_la.h"
 
 #include <cfloat>
 
@@ -26,9 +27,9 @@ template <int N>
 struct CbShared {
     double P2M[N * N], P2Min[N * N];   // column-major (element (i,j) at [i + j*N])
     double H[N * N], L[N * N];         // row-major; L lower
+    double Lc[N * N];                  // the factor column-major, for lane 0's small solves
     double u[N], uP[N], aP[N], aP0[N], gP[N], dP[N], gM[N], aold[N], aM[N];
-    double x[N], v[N];
-    int isgn[N];
+    double x[N];
     double f, fscale, rtol, fnew, t, err;
     int action, done, converged;
 };
@@ -104,115 +105,6 @@ __device__ double cb_nrm2(int n, const double *x) {
     for (int i = 0; i < n; i++) s += x[i] * x[i];
     return sqrt(s);
 }
-// dpotf2 on a row-major copy (lower); 0 or the failing column + 1
-template <int N>
-__device__ int cb_cholesky(double *a) {
-    for (int j = 0; j < N; j++) {
-        double dot = 0.0;
-        for (int k = 0; k < j; k++) dot += a[j * N + k] * a[j * N + k];
-        double ajj = a[j * N + j] - dot;
-        if (ajj <= 0.0 || isnan(ajj)) {
-            a[j * N + j] = ajj;
-            return j + 1;
-        }
-        ajj = sqrt(ajj);
-        a[j * N + j] = ajj;
-        for (int i = j + 1; i < N; i++) {
-            double y = a[i * N + j];
-            for (int k = 0; k < j; k++) y += (-a[j * N + k]) * a[i * N + k];
-            a[i * N + j] = y;
-        }
-        const double r = 1.0 / ajj;
-        for (int i = j + 1; i < N; i++) a[i * N + j] *= r;
-    }
-    return 0;
-}
-// x <- (L L^T)^{-1} x with L row-major lower
-template <int N>
-__device__ void cb_apply_inv_rm(const double *l, double *x) {
-    for (int j = 0; j < N; j++) {
-        if (x[j] != 0.0) {
-            x[j] = x[j] / l[j * N + j];
-            const double temp = x[j];
-            for (int i = j + 1; i < N; i++) x[i] = x[i] - temp * l[i * N + j];
-        }
-    }
-    for (int j = N - 1; j >= 0; j--) {
-        double temp = x[j];
-        for (int i = N - 1; i > j; i--) temp = temp - l[i * N + j] * x[i];
-        x[j] = temp / l[j * N + j];
-    }
-}
-template <int N>
-__device__ double cb_dasum(const double *x) {
-    double s = 0.0;
-    for (int i = 0; i < N; i++) s += fabs(x[i]);
-    return s;
-}
-template <int N>
-__device__ int cb_idamax(const double *x) {
-    int idx = 0;
-    double m = fabs(x[0]);
-    for (int i = 1; i < N; i++)
-        if (fabs(x[i]) > m) { m = fabs(x[i]); idx = i; }
-    return idx;
-}
-// dpocon's role: Higham's estimate of ||A^{-1}||_1 (ACM TOMS 14(4), 1988), then
-// the reciprocal condition number
-template <int N>
-__device__ double cb_rcond(const double *l, double anorm, double *x, double *v, int *isgn) {
-    if (anorm == 0.0) return 0.0;
-    double est;
-    for (int i = 0; i < N; i++) x[i] = 1.0 / N;
-    cb_apply_inv_rm<N>(l, x);
-    if (N == 1) {
-        est = fabs(x[0]);
-    } else {
-        est = cb_dasum<N>(x);
-        for (int i = 0; i < N; i++) {
-            x[i] = (x[i] >= 0.0) ? 1.0 : -1.0;
-            isgn[i] = (int)x[i];
-        }
-        cb_apply_inv_rm<N>(l, x);
-        int j = cb_idamax<N>(x);
-        int iter = 2;
-        for (;;) {
-            for (int i = 0; i < N; i++) x[i] = 0.0;
-            x[j] = 1.0;
-            cb_apply_inv_rm<N>(l, x);
-            for (int i = 0; i < N; i++) v[i] = x[i];
-            const double estold = est;
-            est = cb_dasum<N>(v);
-            bool repeated = true;
-            for (int i = 0; i < N; i++)
-                if (((x[i] >= 0.0) ? 1 : -1) != isgn[i]) { repeated = false; break; }
-            if (repeated || est <= estold) break;
-            for (int i = 0; i < N; i++) {
-                x[i] = (x[i] >= 0.0) ? 1.0 : -1.0;
-                isgn[i] = (int)x[i];
-            }
-            cb_apply_inv_rm<N>(l, x);
-            const int jlast = j;
-            j = cb_idamax<N>(x);
-            if (x[jlast] != fabs(x[j]) && iter < 5) {
-                ++iter;
-                continue;
-            }
-            break;
-        }
-        double altsgn = 1.0;
-        for (int i = 0; i < N; i++) {
-            x[i] = altsgn * (1.0 + (double)i / (double)(N - 1));
-            altsgn = -altsgn;
-        }
-        cb_apply_inv_rm<N>(l, x);
-        const double temp = 2.0 * (cb_dasum<N>(x) / (double)(3 * N));
-        if (temp > est) est = temp;
-    }
-    if (!isfinite(est) || est == 0.0) return 0.0;
-    return (1.0 / est) / anorm;
-}
-
 // ---- the table seen through a basis (all lanes, their points) ----
 // pP_q <- p_q, then (if basis) pP_q <- B^{-1} pP_q with B column-major
 template <int N>
@@ -345,19 +237,21 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
             ++total;
             cb_fobj_full<N>(A, pP, sh, lane);               // :234-242
             __syncthreads();
-            if (lane == 0) {                                // :247-259
-                for (int k = 0; k < N * N; k++) sh.L[k] = sh.H[k];
-                const int error = cb_cholesky<N>(sh.L);
-                double norm = 0.0;                          // utils_norm1
-                for (int i = 0; i < N; i++) {
-                    double s = 0.0;
-                    for (int j = 0; j < N; j++) s += fabs(sh.H[i * N + j]);
-                    if (s > norm) norm = s;
-                }
-                double condest = 0.0;
-                if (error == 0) condest = 1.0 / cb_rcond<N>(sh.L, norm, sh.x, sh.v, sh.isgn);
-                if (total == 1 && (error != 0 || condest > A.cond_h_max)) sh.action = CB_RESET;
-                else if (iter >= maxit || error != 0 || t == 0.0 || condest > A.cond_h_max) sh.action = CB_REG;
+            // :247-259: the factor, the 1-norm and the condition estimate, the
+            // wave sharing each step's independent updates
+            for (int k = lane; k < N * N; k += 64) sh.L[k] = sh.H[k];
+            __syncthreads();
+            const bool chol_ok = coop_cholesky<N>(reinterpret_cast<double (*)[N]>(sh.L), lane);
+            double norm = 0.0;                              // utils_norm1: row sums in order, their max
+            if (lane < N)
+                for (int j = 0; j < N; j++) norm += fabs(sh.H[lane * N + j]);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) norm = fmax(norm, __shfl_xor(norm, o));
+            double condest = 0.0;
+            if (chol_ok) condest = 1.0 / coop_rcond<N>(reinterpret_cast<double (*)[N]>(sh.L), sh.x, norm, lane);
+            if (lane == 0) {
+                if (total == 1 && (!chol_ok || condest > A.cond_h_max)) sh.action = CB_RESET;
+                else if (iter >= maxit || !chol_ok || t == 0.0 || condest > A.cond_h_max) sh.action = CB_REG;
                 else sh.action = CB_STEP;
             }
             __syncthreads();
@@ -418,15 +312,30 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
                 if (reg == 4) maxit = 10 * maxit;
                 continue;
             }
-            // CB_STEP: the basis moves by L (:333-343)
+            // CB_STEP: the basis moves by L (:333-343).  P2M <- P2M L with
+            // dtrmm('R','L','N','N')'s order for each element, lane i forming
+            // row i from the old row; the factor copied column-major.
+            double brow[N];
+            if (lane < N) {
+                for (int j = 0; j < N; j++) {
+                    double b = sh.L[j * N + j] * sh.P2M[lane + j * N];
+                    for (int k = j + 1; k < N; k++)
+                        if (sh.L[k * N + j] != 0.0) b = b + sh.L[k * N + j] * sh.P2M[lane + k * N];
+                    brow[j] = b;
+                }
+            }
+            for (int k = lane; k < N * N; k += 64) {
+                const int i = k % N, j = k / N;
+                sh.Lc[k] = (i >= j) ? sh.L[i * N + j] : 0.0;
+            }
+            __syncthreads();
+            if (lane < N)
+                for (int j = 0; j < N; j++) sh.P2M[lane + j * N] = brow[j];
+            __syncthreads();
             if (lane == 0) {
-                double Lc[N * N];                           // column-major view of the factor
-                for (int i = 0; i < N; i++)
-                    for (int j = 0; j < N; j++) Lc[i + j * N] = (i >= j) ? sh.L[i * N + j] : 0.0;
-                cb_trsv_ln<N>(Lc, sh.gP);
-                cb_trmm_rln<N>(Lc, sh.P2M);
-                cb_trmv_lt<N>(Lc, sh.aP);
-                cb_trsv_ln<N>(Lc, sh.uP);
+                cb_trsv_ln<N>(sh.Lc, sh.gP);
+                cb_trmv_lt<N>(sh.Lc, sh.aP);
+                cb_trsv_ln<N>(sh.Lc, sh.uP);
                 for (int i = 0; i < N; i++) sh.dP[i] = -sh.gP[i];   // :347-348
                 for (int i = 0; i < N; i++) sh.gM[i] = sh.gP[i];    // :352-354
                 cb_trmv_ln<N>(sh.P2M, sh.gM);
