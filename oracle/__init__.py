@@ -29,7 +29,8 @@ class Options(C.Structure):
     """OPTIONS (opt.h:11-22), the fields the opt path reads."""
     _fields_ = [("max_iter", C.c_int), ("tol_rel", C.c_double),
                 ("cond_h_max", C.c_double), ("use_gaunt", C.c_int), ("line_search", C.c_int),
-                ("moment_type", C.c_int), ("delta_ppn", C.c_double)]
+                ("moment_type", C.c_int), ("delta_ppn", C.c_double),
+                ("max_bfgs_iter", C.c_int), ("cond_h_max_bfgs", C.c_double)]
 
 
 class Outs(C.Structure):
@@ -91,6 +92,9 @@ def lib():
                                     C.POINTER(Outs), C.c_int]
         L.orc_optcb_batch.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(Options),
                                       C.POINTER(Outs), C.c_int]
+        L.orc_optbfgs_batch.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(Options),
+                                        C.POINTER(Outs), C.c_int]
+        L.orc_bfgs_update.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
         L.orc_solver_create.argtypes = [C.POINTER(Deck), C.c_int, C.POINTER(C.c_void_p)]
         L.orc_solver_destroy.argtypes = [C.c_void_p]
         L.orc_solver_max_dt.argtypes = [C.c_void_p]
@@ -221,6 +225,35 @@ class Tables:
         r = np.array([o_.r for o_ in outs])
         ng = np.array([o_.norm_g for o_ in outs])
         return st, u, P2M, alphaP, alphaM, it, r, ng
+
+    def optbfgs_batch(self, u, P2M, alphaP, max_iter=100, tol=1e-8, cond_h_max=1e10, nthreads=0,
+                      line_search=0, moment_type=MOMENT_MN, delta_ppn=0.0, max_bfgs_iter=5, cond_h_max_bfgs=20.0):
+        """The BFGS variant (optbfgs.cpp:92-513) per cell, with optcb_batch's
+        arrays and returns; iterations count the outer (full-Hessian) ones."""
+        u = np.array(u, dtype=np.float64, order="C")
+        n = u.shape[0]
+        P2M = np.array(P2M, dtype=np.float64, order="C").reshape(n, self.nm * self.nm)
+        alphaP = np.array(alphaP, dtype=np.float64, order="C")
+        alphaM = np.zeros_like(alphaP)
+        o = Options(max_iter, tol, cond_h_max, 0, line_search, moment_type, delta_ppn, max_bfgs_iter,
+                    cond_h_max_bfgs)
+        outs = (Outs * n)()
+        st = lib().orc_optbfgs_batch(self.h, n, _p(u), _p(P2M), _p(alphaP), _p(alphaM), C.byref(o), outs, nthreads)
+        it = np.array([o_.iter for o_ in outs], dtype=np.int32)
+        r = np.array([o_.r for o_ in outs])
+        ng = np.array([o_.norm_g for o_ in outs])
+        return st, u, P2M, alphaP, alphaM, it, r, ng
+
+
+def bfgs_update(H, Hinv, s, y):
+    """One BFGS update of H and H^-1 (optbfgs.cpp:440-477); returns new copies."""
+    n = len(s)
+    Hc = np.array(H, dtype=np.float64).reshape(n, n).T.copy()      # column-major storage
+    Ic = np.array(Hinv, dtype=np.float64).reshape(n, n).T.copy()
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    lib().orc_bfgs_update(n, _p(Hc), _p(Ic), _p(s), _p(y))
+    return Hc.T.copy(), Ic.T.copy()
 
 
 def isotropic_alpha(nm):

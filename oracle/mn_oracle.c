@@ -962,17 +962,93 @@ static void cb_identity(int n, double *a)
     for (int i = 0; i < n; i++) a[i + i * n] = 1.0;
 }
 
+/* Netlib dgemv('N') with beta = 0: y <- alpha A x (A column-major) */
+static void bf_gemv_n(int n, double alpha, const double *a, const double *x, double *y)
+{
+    for (int i = 0; i < n; i++) y[i] = 0.0;
+    for (int j = 0; j < n; j++) {
+        const double temp = alpha * x[j];
+        for (int i = 0; i < n; i++) y[i] = y[i] + temp * a[i + j * n];
+    }
+}
+/* Netlib dger: A <- A + alpha x y^T (columns with y_j = 0 skipped) */
+static void bf_ger(int n, double alpha, const double *x, const double *y, double *a)
+{
+    for (int j = 0; j < n; j++) {
+        if (y[j] != 0.0) {
+            const double temp = alpha * y[j];
+            for (int i = 0; i < n; i++) a[i + j * n] = a[i + j * n] + x[i] * temp;
+        }
+    }
+}
+static double bf_dot(int n, const double *x, const double *y)
+{
+    double s = 0.0;
+    for (int i = 0; i < n; i++) s = s + x[i] * y[i];
+    return s;
+}
+/* cond (optbfgs.cpp:517-545): the largest |.| sum over h[i*n+j], j inner, of each */
+static double bf_cond(int n, const double *h, const double *hinv)
+{
+    double norm1 = 0.0, norm2 = 0.0;
+    for (int i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < n; j++) s += fabs(h[i * n + j]);
+        if (s > norm1) norm1 = s;
+    }
+    for (int i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int j = 0; j < n; j++) s += fabs(hinv[i * n + j]);
+        if (s > norm2) norm2 = s;
+    }
+    return norm1 * norm2;
+}
+
+void orc_bfgs_update(int n, double *H, double *Hinv, const double *s, const double *y)
+{                                                          /* optbfgs.cpp:440-477 */
+    double tmp[64];
+    double dot = bf_dot(n, s, y);                          /* H += y y^T / (s.y) */
+    double coef = 1.0 / dot;
+    bf_ger(n, coef, y, y, H);
+    bf_gemv_n(n, 1.0, H, s, tmp);                          /* H -= (H s)(H s)^T / (s.H s) */
+    dot = bf_dot(n, s, tmp);
+    coef = -1.0 / dot;
+    bf_ger(n, coef, tmp, tmp, H);
+    bf_gemv_n(n, 1.0, Hinv, y, tmp);                       /* H^-1 += (s.y + y.H^-1 y)/(s.y)^2 s s^T */
+    coef = bf_dot(n, y, tmp);
+    dot = bf_dot(n, s, y);
+    coef = (coef + dot) / (dot * dot);
+    bf_ger(n, coef, s, s, Hinv);
+    coef = -1.0 / dot;                                     /* H^-1 -= ((H^-1 y) s^T + s (H^-1 y)^T)/(s.y) */
+    bf_ger(n, coef, tmp, s, Hinv);
+    bf_ger(n, coef, s, tmp, Hinv);
+}
+
+/* f's resolution for the robust line search: |mass| + sum |alpha_i u_i|, with
+ * the mass f + alpha.u */
+static double bf_fscale(int nm, double f, const double *a, const double *u)
+{
+    double s = 0.0, au = 0.0;
+    for (int i = 0; i < nm; i++) {
+        s += fabs(a[i] * u[i]);
+        au += a[i] * u[i];
+    }
+    return s + fabs(f + au);
+}
+
 static int optcb_scaled(const orc_tables *t, double *u, double *P2M, double *alphaP, double *alphaM,
-                        const orc_options *o, orc_outs *outs)
-{                                                          /* optcb.cpp:159-415 */
+                        const orc_options *o, orc_outs *outs, int bfgs)
+{                                                          /* optcb.cpp:159-415, optbfgs.cpp:169-513 */
     const int nm = t->nm, nq = t->nq;
     int status = ORC_OK;
-    double *mem = (double *)malloc(sizeof(double) * ((size_t)8 * nm + 4 * (size_t)nm * nm + (size_t)nq * nm));
+    double *mem = (double *)malloc(sizeof(double) * ((size_t)12 * nm + 6 * (size_t)nm * nm + (size_t)nq * nm));
     if (!mem) return ORC_ERR_ALLOC;
     double *alphaP0 = mem, *dP = alphaP0 + nm, *dM = dP + nm, *gP = dM + nm, *uP = gP + nm, *gM = uP + nm,
            *alphaOld = gM + nm;
     double *chol_rm = alphaOld + 2 * nm, *chol = chol_rm + nm * nm, *hP = chol + nm * nm, *P2Min = hP + nm * nm;
-    double *pP = P2Min + nm * nm;
+    double *hB = P2Min + nm * nm, *hInv = hB + nm * nm;  /* BFGS's H and H^-1 (column-major) */
+    double *gOld = hInv + nm * nm, *aOld = gOld + nm, *sD = aOld + nm, *yD = sD + nm;
+    double *pP = yD + nm;
     /* the tables seen through the current basis: p -> pP (stride nm) */
     orc_tables tv = *t;
     tv.p = pP;
@@ -1066,21 +1142,53 @@ static int optcb_scaled(const orc_tables *t, double *u, double *P2M, double *alp
         cb_trmm_rln(nm, chol, P2M);
         cb_trmv_lt(nm, chol, alphaP);
         cb_trsv_ln(nm, chol, uP);
-        for (int i = 0; i < nm; i++) dP[i] = -gP[i];        /* :347-348 */
-        memcpy(gM, gP, sizeof(double) * nm);                /* :352-354 */
-        cb_trmv_ln(nm, P2M, gM);
-        err = nrm2(nm, gM);
-        if (err <= rtol) {                                  /* :358-375 */
-            memcpy(dM, dP, sizeof(double) * nm);
-            cb_trsv_lt(nm, P2M, dM);
-            memcpy(alphaM, alphaP, sizeof(double) * nm);
-            cb_trsv_lt(nm, P2M, alphaM);
-            u[0] = 1.0;
-            for (int i = 1; i < nm; i++)
-                u[i] = u[i] * (1.0 - r);
-            break;
+        if (!bfgs) {
+            for (int i = 0; i < nm; i++) dP[i] = -gP[i];    /* :347-348 */
+            memcpy(gM, gP, sizeof(double) * nm);            /* :352-354 */
+            cb_trmv_ln(nm, P2M, gM);
+            err = nrm2(nm, gM);
+            if (err <= rtol) {                              /* :358-375 */
+                memcpy(dM, dP, sizeof(double) * nm);
+                cb_trsv_lt(nm, P2M, dM);
+                memcpy(alphaM, alphaP, sizeof(double) * nm);
+                cb_trsv_lt(nm, P2M, alphaM);
+                u[0] = 1.0;
+                for (int i = 1; i < nm; i++)
+                    u[i] = u[i] * (1.0 - r);
+                break;
+            }
+            tt = linesearch(&tv, alphaP, dP, f, gP, uP, alphaOld, &od, fscale);   /* :387-388 */
+            continue;
         }
-        tt = linesearch(&tv, alphaP, dP, f, gP, uP, alphaOld, &od, fscale);   /* :387-388 */
+        cb_identity(nm, hB);                                /* optbfgs.cpp:371-373 */
+        cb_identity(nm, hInv);
+        int solved = 0;
+        for (int ib = 0; bf_cond(nm, hB, hInv) < o->cond_h_max_bfgs && ib < o->max_bfgs_iter; ib++) {
+            bf_gemv_n(nm, -1.0, hInv, gP, dP);              /* :383: d = -H^-1 g */
+            memcpy(gM, gP, sizeof(double) * nm);            /* :387-389 */
+            cb_trmv_ln(nm, P2M, gM);
+            err = nrm2(nm, gM);
+            if (err <= rtol) {                              /* :393-411 */
+                memcpy(alphaM, alphaP, sizeof(double) * nm);
+                cb_trsv_lt(nm, P2M, alphaM);
+                u[0] = 1.0;
+                for (int i = 1; i < nm; i++)
+                    u[i] = u[i] * (1.0 - r);
+                solved = 1;
+                break;
+            }
+            memcpy(aOld, alphaP, sizeof(double) * nm);      /* :423-425 */
+            memcpy(gOld, gP, sizeof(double) * nm);
+            tt = linesearch(&tv, alphaP, dP, f, gP, uP, alphaOld, &od, fscale);
+            f = fobj_any(&tv, alphaP, uP, gP, NULL, &od);   /* :429-436: f and g alone */
+            fscale = bf_fscale(nm, f, alphaP, uP);
+            for (int i = 0; i < nm; i++) {                  /* :440-445 */
+                yD[i] = gP[i] - gOld[i];
+                sD[i] = alphaP[i] - aOld[i];
+            }
+            orc_bfgs_update(nm, hB, hInv, sD, yD);
+        }
+        if (solved) break;
     }
     if (outs != NULL) {
         outs->iter = totalIter;
@@ -1091,18 +1199,19 @@ static int optcb_scaled(const orc_tables *t, double *u, double *P2M, double *alp
     return status;
 }
 
-int orc_optcb(const orc_tables *t, double *u, double *P2M, double *alphaP, double *alphaM,
-              const orc_options *o, orc_outs *outs)
-{                                                          /* optcb.cpp:90-153 */
+static int optcb_any(const orc_tables *t, double *u, double *P2M, double *alphaP, double *alphaM,
+                     const orc_options *o, orc_outs *outs, int bfgs)
+{                                                          /* optcb.cpp:90-153, optbfgs.cpp:92-163 */
     const int nm = t->nm;
     if (o->use_gaunt || t->np2 != nm) return ORC_ERR_ARG;  /* pP is copied with stride nm (:183) */
+    if (bfgs && (o->max_bfgs_iter < 1 || !(o->cond_h_max_bfgs > 1.0) || nm > 64)) return ORC_ERR_ARG;
     const double u0 = u[0];
     if (o->moment_type == ORC_MOMENT_PPN) {
         orc_options os = *o;
         os.delta_ppn = o->delta_ppn / (u0 * u0);
         for (int i = 0; i < nm; i++) u[i] = u[i] / u0;
         if (u0 <= 0) return ORC_ERR_U0;
-        int st = optcb_scaled(t, u, P2M, alphaP, alphaM, &os, outs);
+        int st = optcb_scaled(t, u, P2M, alphaP, alphaM, &os, outs, bfgs);
         for (int i = 0; i < nm; i++) {
             alphaM[i] = alphaM[i] * u0;
             u[i] = u[i] * u0;
@@ -1112,14 +1221,26 @@ int orc_optcb(const orc_tables *t, double *u, double *P2M, double *alphaP, doubl
     const double alphaScale = log(u0) * 2.0 * sqrt(M_PI);
     for (int i = 0; i < nm; i++) u[i] = u[i] / u0;
     if (u0 <= 0) return ORC_ERR_U0;
-    int st = optcb_scaled(t, u, P2M, alphaP, alphaM, o, outs);
+    int st = optcb_scaled(t, u, P2M, alphaP, alphaM, o, outs, bfgs);
     for (int i = 0; i < nm; i++) u[i] = u[i] * u0;
     alphaM[0] = alphaM[0] + alphaScale;
     return st;
 }
 
-int orc_optcb_batch(const orc_tables *t, int ncells, double *u, double *P2M, double *alphaP, double *alphaM,
-                    const orc_options *o, orc_outs *outs, int nthreads)
+int orc_optcb(const orc_tables *t, double *u, double *P2M, double *alphaP, double *alphaM,
+              const orc_options *o, orc_outs *outs)
+{
+    return optcb_any(t, u, P2M, alphaP, alphaM, o, outs, 0);
+}
+
+int orc_optbfgs(const orc_tables *t, double *u, double *P2M, double *alphaP, double *alphaM,
+                const orc_options *o, orc_outs *outs)
+{
+    return optcb_any(t, u, P2M, alphaP, alphaM, o, outs, 1);
+}
+
+static int optcb_batch_any(const orc_tables *t, int ncells, double *u, double *P2M, double *alphaP, double *alphaM,
+                           const orc_options *o, orc_outs *outs, int nthreads, int bfgs)
 {
     const int nm = t->nm;
     int status = ORC_OK;
@@ -1129,8 +1250,8 @@ int orc_optcb_batch(const orc_tables *t, int ncells, double *u, double *P2M, dou
 #endif
     for (int c = 0; c < ncells; c++) {
         orc_outs tmp;
-        int st = orc_optcb(t, &u[(size_t)c * nm], &P2M[(size_t)c * nm * nm], &alphaP[(size_t)c * nm],
-                           &alphaM[(size_t)c * nm], o, &tmp);
+        int st = optcb_any(t, &u[(size_t)c * nm], &P2M[(size_t)c * nm * nm], &alphaP[(size_t)c * nm],
+                           &alphaM[(size_t)c * nm], o, &tmp, bfgs);
         if (outs) outs[c] = tmp;
         if (st != ORC_OK) {
 #ifdef _OPENMP
@@ -1141,6 +1262,18 @@ int orc_optcb_batch(const orc_tables *t, int ncells, double *u, double *P2M, dou
     }
     (void)nthreads;
     return status;
+}
+
+int orc_optcb_batch(const orc_tables *t, int ncells, double *u, double *P2M, double *alphaP, double *alphaM,
+                    const orc_options *o, orc_outs *outs, int nthreads)
+{
+    return optcb_batch_any(t, ncells, u, P2M, alphaP, alphaM, o, outs, nthreads, 0);
+}
+
+int orc_optbfgs_batch(const orc_tables *t, int ncells, double *u, double *P2M, double *alphaP, double *alphaM,
+                      const orc_options *o, orc_outs *outs, int nthreads)
+{
+    return optcb_batch_any(t, ncells, u, P2M, alphaP, alphaM, o, outs, nthreads, 1);
 }
 
 /* ------------------------------------------------------------------ */

@@ -35,6 +35,8 @@ typedef struct {
     int line_search;   /* 0: linesearch.cpp exactly; 1: robust (see orc_opt) */
     int moment_type;   /* ORC_MOMENT_MN or ORC_MOMENT_PPN (opt.h:35-37) */
     double delta_ppn;  /* PPn regularization delta (opt.h:21) */
+    int max_bfgs_iter;       /* optbfgs only: inner iterations per basis (opt.h:15) */
+    double cond_h_max_bfgs;  /* optbfgs only: bound on cond(H) cond(H^-1) (opt.h:20) */
 } orc_options;
 
 enum { ORC_MOMENT_MN = 0, ORC_MOMENT_PPN = 1 };
@@ -122,6 +124,23 @@ int orc_optcb(const orc_tables *t, double *u, double *P2M, double *alphaP, doubl
 /* Batched optcb, [ncells][n], [ncells][n*n] layouts, OpenMP dynamic,1. */
 int orc_optcb_batch(const orc_tables *t, int ncells, double *u, double *P2M, double *alphaP, double *alphaM,
                     const orc_options *opt, orc_outs *outs, int nthreads);
+
+/* The BFGS variant optbfgs (optbfgs.cpp:92-513, MOMOPT_OPTIMIZATION_TYPE 2):
+ * optcb's outer loop (full Hessian, factor, basis move by L), then, from
+ * H = H^-1 = I in the new basis, quasi-Newton steps d = -H^-1 g with the
+ * gradient alone and rank-two updates of H and H^-1, while
+ * ||H||_inf ||H^-1||_inf < cond_h_max_bfgs and at most max_bfgs_iter of them
+ * (optbfgs.cpp:376-478).  outs.iter counts the outer iterations, as the
+ * reference's totalIter.  cond_h_max_bfgs <= 1 or max_bfgs_iter < 1 would
+ * take no step (the reference then loops to its max-regularization abort):
+ * ORC_ERR_ARG here. */
+int orc_optbfgs(const orc_tables *t, double *u, double *P2M, double *alphaP, double *alphaM,
+                const orc_options *opt, orc_outs *outs);
+int orc_optbfgs_batch(const orc_tables *t, int ncells, double *u, double *P2M, double *alphaP, double *alphaM,
+                      const orc_options *opt, orc_outs *outs, int nthreads);
+/* One BFGS update (optbfgs.cpp:440-477) of H and H^-1 (n x n, column-major)
+ * for the step s = alpha - alpha_old and y = g - g_old. */
+void orc_bfgs_update(int n, double *H, double *Hinv, const double *s, const double *y);
 
 /* ---------------- grid solver (main.cpp, momopt_*.cpp) ---------------- */
 
