@@ -189,9 +189,12 @@ int check_opts(mnc_ctx *ctx, const mnc_options *o) {
             return ctx_fail(ctx, MNC_ERR_ARG, "PPn runs the FP64 form from the previous or isotropic alpha");
         if (ctx->tab.nm > 15) return ctx_fail(ctx, MNC_ERR_ARG, "PPn has kernels for moment orders up to 4");
     }
-    if (o->opt_type != MNC_OPT_NEWTON && o->opt_type != MNC_OPT_CHANGE_BASIS)
+    if (o->opt_type != MNC_OPT_NEWTON && o->opt_type != MNC_OPT_CHANGE_BASIS && o->opt_type != MNC_OPT_BFGS)
         return ctx_fail(ctx, MNC_ERR_ARG, "unknown opt_type");
-    if (o->opt_type == MNC_OPT_CHANGE_BASIS &&
+    if (o->opt_type == MNC_OPT_BFGS && (o->max_bfgs_iter < 1 || !(o->cond_h_max_bfgs > 1.0)))
+        return ctx_fail(ctx, MNC_ERR_ARG, "the BFGS variant needs max_bfgs_iter >= 1 and cond_h_max_bfgs > 1 "
+                                          "(else it takes no step)");
+    if (o->opt_type != MNC_OPT_NEWTON &&
         (ctx->ppn || ctx->tab.use_gaunt || ctx->tab.np2 != ctx->tab.nm || ctx->tab.nm > 15 ||
          o->precision != MNC_PREC_FP64 || o->warm_start != MNC_WARM_PREVIOUS))
         return ctx_fail(ctx, MNC_ERR_ARG, "the change-of-basis optimiser runs Mn, direct Hessian, FP64, "
@@ -456,6 +459,9 @@ int run_optcb_grid(mnc_ctx *ctx, int nc, const mnc_options *o, bool check) {
     a.cond_h_max = o->cond_h_max;
     a.err = ctx->d_ctl + 3;
     a.stats = ctx->d_stats;
+    a.bfgs = o->opt_type == MNC_OPT_BFGS;
+    a.max_bfgs_iter = o->max_bfgs_iter;
+    a.cond_h_max_bfgs = o->cond_h_max_bfgs;
     {
         Timed tm(ctx, MNC_KCLASS_NEWTON);
         mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
@@ -719,6 +725,9 @@ int mnc_optcb_batch_host(mnc_ctx *ctx, int ncells, double *u, double *P2M, doubl
     a.tol = opt->tol_rel;
     a.cond_h_max = opt->cond_h_max;
     a.err = ctx->d_ctl + 3;
+    a.bfgs = opt->opt_type == MNC_OPT_BFGS;
+    a.max_bfgs_iter = opt->max_bfgs_iter;
+    a.cond_h_max_bfgs = opt->cond_h_max_bfgs;
     auto run = [&]() -> int {
         MNC_HIP_CHECK(hipMemcpyAsync(d_u, u, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         MNC_HIP_CHECK(hipMemcpyAsync(d_aP, alphaP, nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
@@ -869,7 +878,7 @@ int mnc_solve_grid(mnc_ctx *ctx, const mnc_options *opt) {
     if (st) return st;
     MNC_HIP_CHECK(hipSetDevice(ctx->device));
     const int nc = (ctx->grid.num_cells_x + 4) * (ctx->grid.num_cells_y + 4);
-    if (opt->opt_type == MNC_OPT_CHANGE_BASIS) return run_optcb_grid(ctx, nc, opt, true);
+    if (opt->opt_type != MNC_OPT_NEWTON) return run_optcb_grid(ctx, nc, opt, true);
     return run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr, true);
 }
 
@@ -902,7 +911,7 @@ int enqueue_step(mnc_ctx *ctx, double dt, const mnc_options *opt) {
     MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_old, ctx->d_mom, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
     for (int stage = 1; stage <= 2; ++stage) {
         if ((st = mnc_ghost_fill(ctx))) return st;
-        if (opt->opt_type == MNC_OPT_CHANGE_BASIS) {
+        if (opt->opt_type != MNC_OPT_NEWTON) {
             if ((st = run_optcb_grid(ctx, nc, opt, false))) return st;
         } else if ((st = run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr, false))) {
             return st;
@@ -917,7 +926,8 @@ int enqueue_step(mnc_ctx *ctx, double dt, const mnc_options *opt) {
 bool same_opts(const mnc_options &a, const mnc_options &b) {
     return a.max_iter == b.max_iter && a.tol_rel == b.tol_rel && a.cond_h_max == b.cond_h_max &&
            a.precision == b.precision && a.warm_start == b.warm_start && a.line_search == b.line_search &&
-           a.delta_ppn == b.delta_ppn && a.opt_type == b.opt_type;
+           a.delta_ppn == b.delta_ppn && a.opt_type == b.opt_type && a.max_bfgs_iter == b.max_bfgs_iter &&
+           a.cond_h_max_bfgs == b.cond_h_max_bfgs;
 }
 
 // Captures enqueue_step into g.exec.  On any capture error the context falls

@@ -207,6 +207,8 @@ struct CbArgs {
     double tol, cond_h_max;
     int *err;                         // error bits (1 << MNC_ERR_*)
     long long *stats;                 // [kStatWords] as the Newton kernels, or null
+    int bfgs, max_bfgs_iter;          // the BFGS variant (optbfgs.cpp:169-513) and its inner count
+    double cond_h_max_bfgs;           // its bound on cond(H) cond(H^-1)
 };
 bool launch_optcb(const CbArgs &a, int blocks, hipStream_t s);
 

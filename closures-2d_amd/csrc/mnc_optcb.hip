@@ -28,6 +28,8 @@ struct CbShared {
     double P2M[N * N], P2Min[N * N];   // column-major (element (i,j) at [i + j*N])
     double H[N * N], L[N * N];         // row-major; L lower
     double Lc[N * N];                  // the factor column-major, for lane 0's small solves
+    double hB[N * N], hI[N * N];       // BFGS's H and H^-1 in the current basis (column-major)
+    double gOld[N], sD[N], yD[N], tmp[N];
     double u[N], uP[N], aP[N], aP0[N], gP[N], dP[N], gM[N], aold[N], aM[N];
     double x[N];
     double f, fscale, rtol, fnew, t, err;
@@ -194,6 +196,111 @@ __device__ double cb_fobj_f(const CbArgs &A, const double *pP, const double *a, 
     return f;
 }
 
+// f and g alone at alphaP (fobj.cpp:291-321 on pP), and f's resolution for the
+// robust line search (the mass f + alpha.u plus sum |alpha_i u_i|); lane 0 stores
+template <int N>
+__device__ void cb_fobj_g(const CbArgs &A, const double *pP, CbShared<N> &sh, int lane) {
+    double part[N + 1];
+#pragma unroll
+    for (int k = 0; k <= N; k++) part[k] = 0.0;
+    for (int q = lane; q < A.nq; q += 64) {
+        double pq[N];
+#pragma unroll
+        for (int k = 0; k < N; k++) pq[k] = pP[size_t(q) * N + k];
+        double temp = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; i++) temp = temp + sh.aP[i] * pq[i];
+        temp = exp(temp);
+        const double wq = A.w[q];
+#pragma unroll
+        for (int i = 0; i < N; i++) part[i] += wq * pq[i] * temp;
+        part[N] += wq * temp;
+    }
+#pragma unroll
+    for (int k = 0; k <= N; k++) part[k] = wave_sum(part[k]);
+    if (lane == 0) {
+        double f = part[N], s = 0.0, au = 0.0;
+        for (int i = 0; i < N; i++) {
+            sh.gP[i] = part[i] - sh.uP[i];
+            f = f - sh.aP[i] * sh.uP[i];
+            s += fabs(sh.aP[i] * sh.uP[i]);
+            au += sh.aP[i] * sh.uP[i];
+        }
+        sh.f = f;
+        sh.fscale = s + fabs(f + au);
+    }
+}
+
+// cond (optbfgs.cpp:517-545): ||.|| over h[i*n+j] rows (j inner) of H and H^-1,
+// lane i a row each, the largest by a butterfly max (NaN sums drop out, as the
+// reference's '>' test); every lane ends with the product
+template <int N>
+__device__ double cb_bfgs_cond(const CbShared<N> &sh, int lane) {
+    double n1 = 0.0, n2 = 0.0;
+    if (lane < N) {
+        for (int j = 0; j < N; j++) n1 += fabs(sh.hB[lane * N + j]);
+        for (int j = 0; j < N; j++) n2 += fabs(sh.hI[lane * N + j]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        n1 = fmax(n1, __shfl_xor(n1, o));
+        n2 = fmax(n2, __shfl_xor(n2, o));
+    }
+    return n1 * n2;
+}
+
+// Netlib dger's element update for row i: a(i,j) += x_i (alpha y_j), y_j != 0
+template <int N>
+__device__ __forceinline__ void cb_ger_row(double *a, double alpha, double xi, const double *y, int i) {
+    for (int j = 0; j < N; j++)
+        if (y[j] != 0.0) a[i + j * N] = a[i + j * N] + xi * (alpha * y[j]);
+}
+// Netlib dgemv('N'), beta = 0, row i: sum_j (alpha x_j) a(i,j) in column order
+template <int N>
+__device__ __forceinline__ double cb_gemv_row(const double *a, double alpha, const double *x, int i) {
+    double y = 0.0;
+    for (int j = 0; j < N; j++) y = y + (alpha * x[j]) * a[i + j * N];
+    return y;
+}
+template <int N>
+__device__ __forceinline__ double cb_dot(const double *x, const double *y) {
+    double s = 0.0;
+    for (int i = 0; i < N; i++) s = s + x[i] * y[i];
+    return s;
+}
+
+// The rank-two updates of H and H^-1 (optbfgs.cpp:440-477), lane i owning row
+// i; each element takes the reference's sequence of dger updates.  sD and yD
+// hold the step and the gradient change.
+template <int N>
+__device__ void cb_bfgs_update(CbShared<N> &sh, int lane) {
+    const bool own = lane < N;
+    double dot = cb_dot<N>(sh.sD, sh.yD);
+    double coef = 1.0 / dot;
+    if (own) cb_ger_row<N>(sh.hB, coef, sh.yD[lane], sh.yD, lane);     // H += y y^T / (s.y)
+    __syncthreads();
+    if (own) sh.tmp[lane] = cb_gemv_row<N>(sh.hB, 1.0, sh.sD, lane);   // H s, from the updated H
+    __syncthreads();
+    dot = cb_dot<N>(sh.sD, sh.tmp);
+    coef = -1.0 / dot;
+    if (own) cb_ger_row<N>(sh.hB, coef, sh.tmp[lane], sh.tmp, lane);   // H -= (H s)(H s)^T / (s.H s)
+    double t2 = 0.0;
+    if (own) t2 = cb_gemv_row<N>(sh.hI, 1.0, sh.yD, lane);             // H^-1 y
+    __syncthreads();
+    if (own) sh.tmp[lane] = t2;
+    __syncthreads();
+    coef = cb_dot<N>(sh.yD, sh.tmp);
+    dot = cb_dot<N>(sh.sD, sh.yD);
+    coef = (coef + dot) / (dot * dot);
+    if (own) {
+        cb_ger_row<N>(sh.hI, coef, sh.sD[lane], sh.sD, lane);          // + coef s s^T
+        coef = -1.0 / dot;
+        cb_ger_row<N>(sh.hI, coef, sh.tmp[lane], sh.sD, lane);         // - (H^-1 y) s^T / (s.y)
+        cb_ger_row<N>(sh.hI, coef, sh.sD[lane], sh.tmp, lane);         // - s (H^-1 y)^T / (s.y)
+    }
+    __syncthreads();
+}
+
 template <int N>
 __device__ void cb_isotropic(double *a) {     // isotropicAlpha, Mn (opt.cpp:27-35)
     a[0] = 2.0 * kCbSqrtPi * log(0.5 / kCbSqrtPi);
@@ -336,52 +443,82 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
                 cb_trsv_ln<N>(sh.Lc, sh.gP);
                 cb_trmv_lt<N>(sh.Lc, sh.aP);
                 cb_trsv_ln<N>(sh.Lc, sh.uP);
-                for (int i = 0; i < N; i++) sh.dP[i] = -sh.gP[i];   // :347-348
-                for (int i = 0; i < N; i++) sh.gM[i] = sh.gP[i];    // :352-354
-                cb_trmv_ln<N>(sh.P2M, sh.gM);
-                sh.err = cb_nrm2(N, sh.gM);
-                sh.converged = sh.err <= sh.rtol;
-                if (sh.converged) {                         // :358-375
-                    for (int i = 0; i < N; i++) sh.aM[i] = sh.aP[i];
-                    cb_trsv_lt<N>(sh.P2M, sh.aM);
-                }
             }
             cb_table_solve<N>(A, pP, sh.L, lane);
+            if (A.bfgs)                                     // optbfgs.cpp:371-373: H = H^-1 = I
+                for (int k = lane; k < N * N; k += 64) sh.hB[k] = sh.hI[k] = (k % N == k / N) ? 1.0 : 0.0;
             __syncthreads();
+            // optcb takes one step along -g_P (:347-388); optbfgs takes steps
+            // along -H^-1 g_P while cond(H) cond(H^-1) and the count allow
+            // (optbfgs.cpp:376-478)
+            for (int ib = 0;; ++ib) {
+                if (A.bfgs) {
+                    const double cnd = cb_bfgs_cond<N>(sh, lane);
+                    if (!(cnd < A.cond_h_max_bfgs && ib < A.max_bfgs_iter)) break;
+                    double d = 0.0;
+                    if (lane < N) d = cb_gemv_row<N>(sh.hI, -1.0, sh.gP, lane);
+                    if (lane < N) sh.dP[lane] = d;
+                } else if (lane == 0) {
+                    for (int i = 0; i < N; i++) sh.dP[i] = -sh.gP[i];
+                }
+                __syncthreads();
+                if (lane == 0) {
+                    for (int i = 0; i < N; i++) sh.gM[i] = sh.gP[i];    // :352-354
+                    cb_trmv_ln<N>(sh.P2M, sh.gM);
+                    sh.err = cb_nrm2(N, sh.gM);
+                    sh.converged = sh.err <= sh.rtol;
+                    if (sh.converged) {                     // :358-375
+                        for (int i = 0; i < N; i++) sh.aM[i] = sh.aP[i];
+                        cb_trsv_lt<N>(sh.P2M, sh.aM);
+                    }
+                }
+                __syncthreads();
+                if (sh.converged) break;
+                if (A.bfgs && lane < N) sh.gOld[lane] = sh.gP[lane];
+                // line search in the current basis (linesearch.cpp:21-99 on pP)
+                double maxf = 0.0;
+                for (int i = 0; i < N; i++) {
+                    const double tmp = fabs(sh.dP[i]) / (fabs(sh.aP[i]) + 1e-100);
+                    if (tmp > maxf) maxf = tmp;
+                }
+                double gd = 0.0;
+                for (int i = 0; i < N; i++) gd += sh.gP[i] * sh.dP[i];
+                __syncthreads();
+                if (lane == 0)
+                    for (int i = 0; i < N; i++) sh.aold[i] = sh.aP[i];
+                __syncthreads();
+                double tt = 1.0;
+                bool accepted = false;
+                while (tt * maxf > DBL_EPSILON / 2) {
+                    if (lane == 0)
+                        for (int i = 0; i < N; i++) sh.aP[i] = sh.aold[i] + tt * sh.dP[i];
+                    __syncthreads();
+                    const double fnew = cb_fobj_f<N>(A, pP, sh.aP, sh.uP, lane);
+                    bool acc = fnew <= sh.f + 1e-3 * tt * gd;
+                    if (!acc && A.ls_mode == MNC_LS_ROBUST && tt == 1.0 && -gd <= 1e3 * DBL_EPSILON * sh.fscale)
+                        acc = true;
+                    __syncthreads();
+                    if (acc) { accepted = true; break; }
+                    tt = 0.5 * tt;
+                }
+                if (!accepted) {
+                    if (lane == 0)
+                        for (int i = 0; i < N; i++) sh.aP[i] = sh.aold[i];
+                    tt = 0.0;
+                }
+                t = tt;
+                __syncthreads();
+                if (!A.bfgs) break;
+                cb_fobj_g<N>(A, pP, sh, lane);              // optbfgs.cpp:429-436
+                __syncthreads();
+                if (lane < N) {                             // :440-445
+                    sh.yD[lane] = sh.gP[lane] - sh.gOld[lane];
+                    sh.sD[lane] = sh.aP[lane] - sh.aold[lane];
+                }
+                __syncthreads();
+                cb_bfgs_update<N>(sh, lane);
+            }
             if (sh.converged) break;
-            // line search in the current basis (linesearch.cpp:21-99 on pP)
-            double maxf = 0.0;
-            for (int i = 0; i < N; i++) {
-                const double tmp = fabs(sh.dP[i]) / (fabs(sh.aP[i]) + 1e-100);
-                if (tmp > maxf) maxf = tmp;
-            }
-            double gd = 0.0;
-            for (int i = 0; i < N; i++) gd += sh.gP[i] * sh.dP[i];
-            __syncthreads();
-            if (lane == 0)
-                for (int i = 0; i < N; i++) sh.aold[i] = sh.aP[i];
-            __syncthreads();
-            double tt = 1.0;
-            bool accepted = false;
-            while (tt * maxf > DBL_EPSILON / 2) {
-                if (lane == 0)
-                    for (int i = 0; i < N; i++) sh.aP[i] = sh.aold[i] + tt * sh.dP[i];
-                __syncthreads();
-                const double fnew = cb_fobj_f<N>(A, pP, sh.aP, sh.uP, lane);
-                bool acc = fnew <= sh.f + 1e-3 * tt * gd;
-                if (!acc && A.ls_mode == MNC_LS_ROBUST && tt == 1.0 && -gd <= 1e3 * DBL_EPSILON * sh.fscale)
-                    acc = true;
-                __syncthreads();
-                if (acc) { accepted = true; break; }
-                tt = 0.5 * tt;
-            }
-            if (!accepted) {
-                if (lane == 0)
-                    for (int i = 0; i < N; i++) sh.aP[i] = sh.aold[i];
-                tt = 0.0;
-            }
-            t = tt;
-            __syncthreads();
         }
         if (sh.converged) {
             if (lane == 0) {                                // outputs (:367-369, :111-118)

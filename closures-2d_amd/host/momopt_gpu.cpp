@@ -202,11 +202,15 @@ int main(int argc, char **argv) {
     if (!deck.get("MOMOPT_DELTA_PPN", delta_ppn)) fail("momopt_init.cpp", 66);
     if (!deck.get("MOMOPT_MOMENT_TYPE", mtype)) fail("momopt_init.cpp", 67);
     if (!deck.get("MOMOPT_OPTIMIZATION_TYPE", otype)) fail("momopt_init.cpp", 68);
-    if ((mtype != 0 && mtype != 1) || (otype != 0 && otype != 1) || (otype == 1 && (mtype != 0 || use_gaunt != 0))) {
+    if ((mtype != 0 && mtype != 1) || otype < 0 || otype > 2 || (otype != 0 && (mtype != 0 || use_gaunt != 0))) {
         std::printf("built: MOMOPT_MOMENT_TYPE 0 (Mn) or 1 (PPn) with MOMOPT_OPTIMIZATION_TYPE 0, and "
-                    "MOMOPT_OPTIMIZATION_TYPE 1 for Mn with MOMOPT_USE_GAUNT 0\n");
+                    "MOMOPT_OPTIMIZATION_TYPE 1 or 2 for Mn with MOMOPT_USE_GAUNT 0\n");
         return 1;
     }
+    int max_bfgs_iter = 0;
+    double cond_h_max_bfgs = 0.0;
+    if (!deck.get("MOMOPT_COND_H_MAX_BFGS", cond_h_max_bfgs)) fail("momopt_init.cpp", 61);
+    if (!deck.get("MOMOPT_MAX_BFGS_ITER", max_bfgs_iter)) fail("momopt_init.cpp", 63);
     int prec = MNC_PREC_FP64, warm = MNC_WARM_PREVIOUS, device = 0, ls = MNC_LS_REFERENCE;
     deck.get("MOMOPT_GPU_PRECISION", prec);
     deck.get("MOMOPT_GPU_WARM_START", warm);
@@ -252,7 +256,7 @@ int main(int argc, char **argv) {
     }
     const double max_dt = cfl * (2.0 / (2.0 + theta)) * (g.dx * g.dy) / (g.dx + g.dy);   // :72
     std::printf("--- Init done ---\n");
-    mnc_options opt{max_iter, tol, cond_h_max, prec, warm, ls, delta_ppn, otype};
+    mnc_options opt{max_iter, tol, cond_h_max, prec, warm, ls, delta_ppn, otype, max_bfgs_iter, cond_h_max_bfgs};
 
     double t = t0, dt = max_dt;                       // main.cpp:204-230
     if (!restarting) write_output(g, ctx, nm, t);

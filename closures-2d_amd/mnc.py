@@ -18,7 +18,7 @@ OK, ERR_ARG, ERR_U0, ERR_MAXREG, ERR_HIP, ERR_STATE = 0, 1, 2, 3, 4, 5
 PREC_FP64, PREC_MIXED = 0, 1
 LS_REFERENCE, LS_ROBUST = 0, 1
 WARM_PREVIOUS, WARM_ISOTROPIC, WARM_LEARNED = 0, 1, 2
-OPT_NEWTON, OPT_CHANGE_BASIS = 0, 1
+OPT_NEWTON, OPT_CHANGE_BASIS, OPT_BFGS = 0, 1, 2
 MOMENT_MN, MOMENT_PPN = 0, 1
 KCLASS_NEWTON, KCLASS_FLUX, KCLASS_OTHER, KCLASS_MLP = 0, 1, 2, 3
 
@@ -34,7 +34,8 @@ class Setup(C.Structure):
 class Options(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("tol_rel", C.c_double), ("cond_h_max", C.c_double),
                 ("precision", C.c_int), ("warm_start", C.c_int), ("line_search", C.c_int),
-                ("delta_ppn", C.c_double), ("opt_type", C.c_int)]
+                ("delta_ppn", C.c_double), ("opt_type", C.c_int), ("max_bfgs_iter", C.c_int),
+                ("cond_h_max_bfgs", C.c_double)]
 
 
 class Outs(C.Structure):
@@ -124,8 +125,9 @@ def _p(a):
 
 
 def options(max_iter=100, tol=1e-8, cond_h_max=1e10, precision=PREC_FP64, warm_start=WARM_PREVIOUS,
-            line_search=LS_REFERENCE, delta_ppn=0.0, opt_type=0):
-    return Options(max_iter, tol, cond_h_max, precision, warm_start, line_search, delta_ppn, opt_type)
+            line_search=LS_REFERENCE, delta_ppn=0.0, opt_type=0, max_bfgs_iter=5, cond_h_max_bfgs=20.0):
+    return Options(max_iter, tol, cond_h_max, precision, warm_start, line_search, delta_ppn, opt_type, max_bfgs_iter,
+                   cond_h_max_bfgs)
 
 
 class Context:

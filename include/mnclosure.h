@@ -84,6 +84,8 @@ typedef struct {
     double delta_ppn;          /* MOMOPT_DELTA_PPN (contexts of moment type 1)  */
     int opt_type;              /* MOMOPT_OPTIMIZATION_TYPE: MNC_OPT_* (0 when the
                                   struct is zero-initialised past delta_ppn)    */
+    int max_bfgs_iter;         /* MOMOPT_MAX_BFGS_ITER (MNC_OPT_BFGS only, >= 1) */
+    double cond_h_max_bfgs;    /* MOMOPT_COND_H_MAX_BFGS (MNC_OPT_BFGS only, > 1) */
 } mnc_options;
 
 /* Optimisers (opt.h:42-46).  MNC_OPT_CHANGE_BASIS in the grid solves
@@ -95,6 +97,13 @@ typedef struct {
  * mnc_optcb_batch_host). */
 #define MNC_OPT_NEWTON 0
 #define MNC_OPT_CHANGE_BASIS 1
+/* The BFGS variant (optbfgs.cpp:92-545): optcb's outer loop, then from
+ * H = H^-1 = I in the new basis up to max_bfgs_iter quasi-Newton steps with
+ * the gradient alone while ||H||_inf ||H^-1||_inf < cond_h_max_bfgs.  Same
+ * state, forms and entry points as MNC_OPT_CHANGE_BASIS (mnc_optcb_batch_host
+ * runs it when opt_type is MNC_OPT_BFGS); outs.iter counts the outer
+ * (full-Hessian) iterations. */
+#define MNC_OPT_BFGS 2
 
 /* OUTS (src/momopt/opt/opt.h:25-32). */
 typedef struct {

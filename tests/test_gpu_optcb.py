@@ -27,11 +27,12 @@ def _identity_start(o, nm, n):
     return P2M, aP
 
 
-def _run(order, Q, n, seed, nb_frac, P2M=None, aP=None, u=None, line_search=0):
+def _run(order, Q, n, seed, nb_frac, P2M=None, aP=None, u=None, line_search=0, bfgs=None):
+    """bfgs: None for optcb, else (max_bfgs_iter, cond_h_max_bfgs) for optbfgs."""
     mnc = load_pkg("mnc")
     o = oracle()
-    print("case optcb order=%d Q=%d n=%d seed=%d lobes=%.2f ls=%d" % (order, Q, n, seed, nb_frac, line_search),
-          flush=True)
+    print("case optcb order=%d Q=%d n=%d seed=%d lobes=%.2f ls=%d bfgs=%s" %
+          (order, Q, n, seed, nb_frac, line_search, bfgs), flush=True)
     tab = o.Tables(order, Q, False)
     if u is None:
         rng = np.random.default_rng(seed)
@@ -39,10 +40,18 @@ def _run(order, Q, n, seed, nb_frac, P2M=None, aP=None, u=None, line_search=0):
     if P2M is None:
         P2M, aP = _identity_start(o, tab.nm, len(u))
     ctx = mnc.Context(order, Q, use_gaunt=False)
-    opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, line_search=line_search)
+    if bfgs is None:
+        opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, line_search=line_search)
+    else:
+        opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, line_search=line_search,
+                          opt_type=mnc.OPT_BFGS, max_bfgs_iter=bfgs[0], cond_h_max_bfgs=bfgs[1])
     g = ctx.optcb_batch_host(u, P2M, aP, opt)
     ctx.close()
-    c = tab.optcb_batch(u, P2M, aP, tol=1e-8, nthreads=8, line_search=line_search)
+    if bfgs is None:
+        c = tab.optcb_batch(u, P2M, aP, tol=1e-8, nthreads=8, line_search=line_search)
+    else:
+        c = tab.optbfgs_batch(u, P2M, aP, tol=1e-8, nthreads=8, line_search=line_search, max_bfgs_iter=bfgs[0],
+                              cond_h_max_bfgs=bfgs[1])
     return tab, u, g, c
 
 
@@ -77,6 +86,42 @@ def _compare(tab, u, g, c, same_frac):
 def test_optcb_matches_oracle(order, Q):
     tab, u, g, c = _run(order, Q, 600, 100 + order, 0.1)
     _compare(tab, u, g, c, 0.95)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,Q,ls", [(2, 12, 0), (4, 16, 0), (4, 16, 1)])
+def test_optbfgs_matches_oracle(order, Q, ls):
+    """The BFGS variant (MNC_OPT_BFGS, the decks' 5 inner iterations and cond
+    bound 20) against the oracle's optbfgs: quasi-Newton paths amplify the
+    quadrature sums' rounding, so fewer cells keep the exact path."""
+    tab, u, g, c = _run(order, Q, 500, 200 + order, 0.1, line_search=ls, bfgs=(5, 20.0))
+    _compare(tab, u, g, c, 0.9)
+
+
+@pytest.mark.gpu
+def test_optbfgs_one_inner_iteration_is_optcb_bitwise():
+    """With one inner iteration and no cond bound the BFGS mode takes optcb's
+    steps (H^-1 = I gives d = -g exactly), so its outputs are optcb's, bit
+    for bit, on the GPU as in the oracle."""
+    mnc = load_pkg("mnc")
+    o = oracle()
+    tab = o.Tables(3, 12, False)
+    rng = np.random.default_rng(77)
+    u, astar, lobe = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, 400, rng, near_boundary_frac=0.2)
+    P2M, aP = _identity_start(o, tab.nm, len(u))
+    ctx = mnc.Context(3, 12, use_gaunt=False)
+    a = ctx.optcb_batch_host(u, P2M, aP, mnc.options())
+    b = ctx.optcb_batch_host(u, P2M, aP, mnc.options(opt_type=mnc.OPT_BFGS, max_bfgs_iter=1, cond_h_max_bfgs=1e300))
+    st = ctx.optcb_batch_host(u, P2M, aP, mnc.options(opt_type=mnc.OPT_BFGS, max_bfgs_iter=0),
+                              allow_status=(mnc.OK, mnc.ERR_ARG))[0]
+    assert st == mnc.ERR_ARG
+    st = ctx.optcb_batch_host(u, P2M, aP, mnc.options(opt_type=mnc.OPT_BFGS, cond_h_max_bfgs=1.0),
+                              allow_status=(mnc.OK, mnc.ERR_ARG))[0]
+    assert st == mnc.ERR_ARG
+    ctx.close()
+    assert a[0] == 0 and b[0] == 0
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x, y)
 
 
 @pytest.mark.gpu
@@ -129,6 +174,34 @@ def test_optcb_rejects_unsupported_forms():
 
 
 @pytest.mark.gpu
+def test_grid_solve_with_bfgs_equals_batch():
+    """mnc_solve_grid with MNC_OPT_BFGS: the batch entry point's result from
+    the reference's initial state, bit for bit."""
+    mnc, gi = load_pkg("mnc"), load_pkg("grid_init")
+    print("case optbfgs grid solve", flush=True)
+    d = dict(fixture_deck("gaussian"), use_gaunt=0, tol=1e-8)
+    ctx, g = gpu_grid_from_deck(mnc, gi, d)
+    nm = ctx.nm
+    m0, a0 = ctx.grid_get()
+    opt = mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"], opt_type=mnc.OPT_BFGS)
+    ctx.reset_stats()
+    ctx.solve_grid(opt)
+    st = ctx.stats()
+    mg, ag = ctx.grid_get()
+    n = m0.reshape(-1, nm).shape[0]
+    assert st.num_dual_solves == n
+    B = np.zeros((n, nm * nm))
+    B[:, :nm] = 1.0
+    aP = np.tile(oracle().isotropic_alpha(nm), (n, 1))
+    stb, ub, Bb, aPb, aMb, itb, rb, _ = ctx.optcb_batch_host(m0.reshape(-1, nm), B, aP, opt)
+    ctx.close()
+    assert stb == 0
+    print("outer iterations per closure %.2f" % (st.iter_sum / n))
+    assert np.array_equal(ag.reshape(-1, nm), aMb)
+    assert np.array_equal(mg.reshape(-1, nm), ub)
+
+
+@pytest.mark.gpu
 def test_grid_solve_with_change_of_basis_equals_batch():
     """mnc_solve_grid with opt_type MNC_OPT_CHANGE_BASIS runs the same kernel
     on the grid's moments, alpha and the device-held basis state, initialised
@@ -178,7 +251,7 @@ def test_grid_solve_with_change_of_basis_equals_batch():
 
 @pytest.mark.gpu
 def test_driver_change_of_basis_time_stepping(tmp_path):
-    """MOMOPT_OPTIMIZATION_TYPE 1 through the deck driver: the smooth periodic
+    """MOMOPT_OPTIMIZATION_TYPE 1 and 2 through the deck driver: the smooth periodic
     deck (36 x 74, M3, Q = 30) with the direct Hessian and TOL 1e-8 to t =
     1.03, robust line search (at TOL 1e-8 the reference's Armijo test fails
     on f's rounding for ~1% of these closures and regularizes them).  Neither
@@ -191,7 +264,7 @@ def test_driver_change_of_basis_time_stepping(tmp_path):
     d = dict(fixture_deck("smooth"), use_gaunt=0, tol=1e-8)
     exe = os.path.join(PKG, "momopt_gpu")
     res = {}
-    for ot in (0, 1):
+    for ot in (0, 1, 2):
         run = tmp_path / ("opt%d" % ot)
         run.mkdir()
         (run / "input.deck").write_text(deck_text(d, 1.03, {"MOMOPT_GPU_LINE_SEARCH": 1}, opt_type=ot))
@@ -204,4 +277,7 @@ def test_driver_change_of_basis_time_stepping(tmp_path):
         assert len(regs) == 5 and sum(regs[1:]) == 0
     err = rel_l2(density(res[1]), density(res[0]))
     print("optcb vs Newton time stepping: density rel-L2 %.3e" % err)
+    assert err < 1e-7
+    err = rel_l2(density(res[2]), density(res[0]))
+    print("optbfgs vs Newton time stepping: density rel-L2 %.3e" % err)
     assert err < 1e-7
