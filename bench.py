@@ -295,13 +295,15 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
             "forms": res}
 
 
-def bench_optcb(mnc, n, seed, oracle_sample):
-    """The change-of-basis optimiser (optcb.cpp:90-415) on the isolated M4
-    batch of bench_isolated (n vectors, Q = 16, 20% near-boundary lobes, TOL
-    1e-8, reference line search) from the identity basis and isotropic alphaP:
-    converged closures/s from the kernel's HIP-event time (the entry point
-    takes host arrays; the wall time with the PCIe copies is reported beside
-    it), and the FP64 CPU oracle's optcb on the first `oracle_sample` vectors."""
+def bench_optcb(mnc, n, seed, oracle_sample, bfgs=False):
+    """The change-of-basis optimiser (optcb.cpp:90-415), or with bfgs its BFGS
+    variant (optbfgs.cpp:92-545, 5 inner iterations, cond bound 20), on the
+    isolated M4 batch of bench_isolated (n vectors, Q = 16, 20% near-boundary
+    lobes, TOL 1e-8, reference line search) from the identity basis and
+    isotropic alphaP: converged closures/s from the kernel's HIP-event time
+    (the entry point takes host arrays; the wall time with the PCIe copies is
+    reported beside it), and the FP64 CPU oracle's form of the same optimiser
+    on the first `oracle_sample` vectors."""
     sys.path.insert(0, ROOT)
     import oracle
     wl = _load("workloads")
@@ -311,7 +313,12 @@ def bench_optcb(mnc, n, seed, oracle_sample):
     nm = 15
     P2M = np.tile(np.eye(nm).reshape(1, -1), (n, 1))
     aP = np.tile(oracle.isotropic_alpha(nm), (n, 1))
-    opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10)
+    if bfgs:
+        opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, opt_type=mnc.OPT_BFGS, max_bfgs_iter=5,
+                          cond_h_max_bfgs=20.0)
+    else:
+        opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10)
+    name = "optbfgs" if bfgs else "optcb"
     best = None
     for rep_ in range(2):
         ctx.timing_enable(True)
@@ -323,24 +330,31 @@ def bench_optcb(mnc, n, seed, oracle_sample):
         if best is None or kms < best[0]:
             best = (kms, wall)
     ctx.close()
-    progress("optcb batch: kernel %.3f s, with copies %.3f s" % (best[0] * 1e-3, best[1]))
+    progress("%s batch: kernel %.3f s, with copies %.3f s" % (name, best[0] * 1e-3, best[1]))
     tab = oracle.Tables(4, 16, False)
     m = oracle_sample
     threads = len(os.sched_getaffinity(0))
     cpu_s, same = 0.0, np.zeros(0, dtype=bool)
     if m > 0:
         t1 = time.perf_counter()
-        sto, uo, Bo, aPo, aMo, ito, ro, _ = tab.optcb_batch(u[:m], P2M[:m], aP[:m], nthreads=threads)
+        if bfgs:
+            sto, uo, Bo, aPo, aMo, ito, ro, _ = tab.optbfgs_batch(u[:m], P2M[:m], aP[:m], nthreads=threads,
+                                                                  max_bfgs_iter=5, cond_h_max_bfgs=20.0)
+        else:
+            sto, uo, Bo, aPo, aMo, ito, ro, _ = tab.optcb_batch(u[:m], P2M[:m], aP[:m], nthreads=threads)
         cpu_s = time.perf_counter() - t1
         same = (it[:m] == ito) & (r[:m] == ro)
     return {"workload": "isolated M4 batch, %d vectors (20%% near-boundary lobes), Q=16, identity basis, "
-                        "isotropic alphaP, TOL 1e-8, direct Hessian, reference line search" % n,
+                        "isotropic alphaP, TOL 1e-8, direct Hessian, reference line search%s" %
+                        (n, "; BFGS variant, 5 inner iterations, cond bound 20" if bfgs else ""),
             "status": int(st), "closures_per_s": n / (best[0] * 1e-3), "kernel_s": best[0] * 1e-3,
             "closures_per_s_with_pcie": n / best[1],
-            "newton_iters_per_closure": float(it.mean()), "max_iter": int(it.max()),
+            ("outer_iters_per_closure" if bfgs else "newton_iters_per_closure"): float(it.mean()),
+            "max_iter": int(it.max()),
             "reg_hist": [int((r == v).sum()) for v in (0.0, 1e-3, 1e-2, 1e-1, 0.5)],
             "cpu_baseline": {"value": m / cpu_s, "unit": "closures/s", "cores": threads, "kind": "port",
-                             "sample": "oracle optcb on the first %d vectors (%.1f s)" % (m, cpu_s)} if m > 0 else None,
+                             "sample": "oracle %s on the first %d vectors (%.1f s)" % (name, m, cpu_s)}
+            if m > 0 else None,
             "oracle_sample_identical_path_frac": float(same.mean()) if m > 0 else None}
 
 
@@ -533,9 +547,10 @@ def main():
         return
     if args.optcb_only:
         r = bench_optcb(mnc, args.optcb, 12345, 0 if args.no_cpu else 50000)
+        rb = bench_optcb(mnc, args.optcb, 12345, 0 if args.no_cpu else 50000, bfgs=True)
         print(json.dumps({"metric": "converged optcb closures/s (isolated M4 batch, TOL 1e-8)",
                           "value": r["closures_per_s"], "unit": "closures/s", "n_gpus": 1,
-                          "higher_is_better": True, "config": {"optcb": r}}))
+                          "higher_is_better": True, "config": {"optcb": r, "optbfgs": rb}}))
         return
     if args.ppn_only:
         r = bench_ppn(mnc, gi, d, args.ppn_steps, args.warmup, 0.0 if args.no_cpu else min(args.cpu_seconds, 10.0))
@@ -616,7 +631,8 @@ def main():
         out["config"]["isolated"] = bench_isolated(mnc, args.isolated, 12345, 20000, args.train_steps)
         progress("isolated batch done")
         out["config"]["optcb"] = bench_optcb(mnc, args.optcb, 12345, 50000) if args.optcb > 0 else None
-        progress("optcb batch done")
+        out["config"]["optbfgs"] = bench_optcb(mnc, args.optcb, 12345, 50000, bfgs=True) if args.optcb > 0 else None
+        progress("change-of-basis batches done")
     if args.m8_steps > 0 and not args.plain_only and not args.only:
         out["config"]["m8"] = bench_m8(mnc, gi, args.m8_steps)
         progress("M8 done")
