@@ -25,15 +25,15 @@ enum : int { CB_STEP = 0, CB_RESET = 1, CB_REG = 2, CB_FAIL = 3 };
 
 template <int N>
 struct CbShared {
-    double P2M[N * N], P2Min[N * N];   // column-major (element (i,j) at [i + j*N])
-    double H[N * N], L[N * N];         // row-major; L lower
-    double Lc[N * N];                  // the factor column-major, for lane 0's small solves
+    double P2M[N * N];                 // column-major (element (i,j) at [i + j*N])
+    double L[N * N];                   // row-major: H, then its factor in the lower triangle
     double hB[N * N], hI[N * N];       // BFGS's H and H^-1 in the current basis (column-major)
     double gOld[N], sD[N], yD[N], tmp[N];
     double u[N], uP[N], aP[N], aP0[N], gP[N], dP[N], gM[N], aold[N], aM[N];
     double x[N];
     double f, fscale, rtol, fnew, t, err;
     int action, done, converged;
+    int p2min_id;                      // the restart basis: 1 the identity, 0 the cell's input basis
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -43,14 +43,18 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // ---- n x n kernels (lane 0), the Netlib reference BLAS / LAPACK loop orders ----
-// dtrsv('L','N','N') on a column-major A: x <- A^{-1} x
-template <int N>
+// element (i,j) of an n x n matrix, column-major or (RM) row-major
+template <int N, bool RM>
+__device__ __forceinline__ int cb_ix(int i, int j) { return RM ? i * N + j : i + j * N; }
+// dtrsv('L','N','N') on a column-major A (RM: row-major, lower triangle read
+// only): x <- A^{-1} x
+template <int N, bool RM = false>
 __device__ void cb_trsv_ln(const double *a, double *x) {
     for (int j = 0; j < N; j++) {
         if (x[j] != 0.0) {
-            x[j] = x[j] / a[j + j * N];
+            x[j] = x[j] / a[cb_ix<N, RM>(j, j)];
             const double temp = x[j];
-            for (int i = j + 1; i < N; i++) x[i] = x[i] - temp * a[i + j * N];
+            for (int i = j + 1; i < N; i++) x[i] = x[i] - temp * a[cb_ix<N, RM>(i, j)];
         }
     }
 }
@@ -74,12 +78,12 @@ __device__ void cb_trmv_ln(const double *a, double *x) {
         }
     }
 }
-// dtrmv('L','T','N'): x <- A^T x
-template <int N>
+// dtrmv('L','T','N'): x <- A^T x (RM as cb_trsv_ln)
+template <int N, bool RM = false>
 __device__ void cb_trmv_lt(const double *a, double *x) {
     for (int j = 0; j < N; j++) {
-        double temp = x[j] * a[j + j * N];
-        for (int i = j + 1; i < N; i++) temp = temp + a[i + j * N] * x[i];
+        double temp = x[j] * a[cb_ix<N, RM>(j, j)];
+        for (int i = j + 1; i < N; i++) temp = temp + a[cb_ix<N, RM>(i, j)] * x[i];
         x[j] = temp;
     }
 }
@@ -139,40 +143,54 @@ __device__ void cb_table_solve(const CbArgs &A, double *pP, const double *l, int
     }
 }
 
-// f, g and H in the current basis (fobj.cpp:37-129,255-287 on pP): the lanes'
-// partial sums, then wave sums; lane 0 stores.
+typedef double cb_f64x4 __attribute__((ext_vector_type(4)));
+
+// f, g and H in the current basis (fobj.cpp:37-129,255-287 on pP).  H is an
+// FP64 matrix-core sum, 16 x 16 over 4 points per instruction: lane l supplies
+// point q0 + l/16, component l%16 (A = w e p_c, B = p_c), its exponent the
+// sequential dot over that point's row.  Result element r of lane l is row
+// 4r + l/16, column l%16; the lower triangle is stored and mirrored into L.
+// Lane 0 forms g and f from the first column.
 template <int N>
 __device__ void cb_fobj_full(const CbArgs &A, const double *pP, CbShared<N> &sh, int lane) {
-    constexpr int TRI = N * (N + 1) / 2;
-    double part[TRI];
+    static_assert(N <= 16, "one 16 x 16 tile");
+    cb_f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    const int c = lane & 15, k = lane >> 4;
+    for (int q0 = 0; q0 < A.nq; q0 += 8) {
 #pragma unroll
-    for (int k = 0; k < TRI; k++) part[k] = 0.0;
-    for (int q = lane; q < A.nq; q += 64) {
-        double pq[N];
+        for (int h = 0; h < 2; h++) {
+            const int q = q0 + 4 * h + k;
+            double av = 0.0, bv = 0.0;
+            if (q < A.nq) {
+                const double *row = pP + size_t(q) * N;
+                double temp = 0.0;
 #pragma unroll
-        for (int k = 0; k < N; k++) pq[k] = pP[size_t(q) * N + k];
-        double temp = 0.0;
-#pragma unroll
-        for (int i = 0; i < N; i++) temp = temp + sh.aP[i] * pq[i];
-        temp = exp(temp);
-        const double wq = A.w[q];
-#pragma unroll
-        for (int i = 0; i < N; i++)
-#pragma unroll
-            for (int j = 0; j <= i; j++) part[i * (i + 1) / 2 + j] += wq * pq[i] * pq[j] * temp;
+                for (int i = 0; i < N; i++) temp = temp + sh.aP[i] * row[i];
+                const double e = exp(temp);
+                if (c < N) {
+                    bv = row[c];
+                    av = A.w[q] * bv * e;
+                }
+            }
+            if (h == 0) acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc0, 0, 0, 0);
+            else acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc1, 0, 0, 0);
+        }
     }
 #pragma unroll
-    for (int i = 0; i < N; i++)
-#pragma unroll
-        for (int j = 0; j <= i; j++) {
-            const double s = wave_sum(part[i * (i + 1) / 2 + j]);
-            if (lane == 0) sh.H[i * N + j] = sh.H[j * N + i] = s;
+    for (int r = 0; r < 4; r++) {
+        const int i = 4 * r + k;
+        if (i < N && c <= i) {
+            const double v = acc0[r] + acc1[r];
+            sh.L[i * N + c] = v;
+            sh.L[c * N + i] = v;
         }
+    }
+    __syncthreads();
     if (lane == 0) {
         const double p0 = pP[0];
-        double f = sh.H[0] / (p0 * p0), fscale = fabs(f);
+        double f = sh.L[0] / (p0 * p0), fscale = fabs(f);
         for (int i = 0; i < N; i++) {
-            sh.gP[i] = sh.H[i] / p0 - sh.uP[i];
+            sh.gP[i] = sh.L[i * N] / p0 - sh.uP[i];
             f = f - sh.aP[i] * sh.uP[i];
             fscale += fabs(sh.aP[i] * sh.uP[i]);
         }
@@ -327,7 +345,8 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
         if (lane == 0) {
             for (int k = 0; k < N; k++) sh.u[k] = ug[k] / u0;
             for (int k = 0; k < N; k++) sh.aP[k] = sh.aP0[k] = aPg[k];
-            for (int k = 0; k < N * N; k++) sh.P2M[k] = sh.P2Min[k] = Bg[k];
+            for (int k = 0; k < N * N; k++) sh.P2M[k] = Bg[k];
+            sh.p2min_id = 0;
             for (int k = 0; k < N; k++) sh.uP[k] = sh.u[k];
             cb_trsv_ln<N>(sh.P2M, sh.uP);                   // :198-200
             sh.rtol = cb_nrm2(N, sh.u) * A.tol;
@@ -335,7 +354,7 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
             sh.converged = 0;
         }
         __syncthreads();
-        cb_table_reset<N>(A, pP, sh.P2Min, lane);           // :185-187
+        cb_table_reset<N>(A, pP, Bg, lane);                 // :185-187
         __syncthreads();
         int iter = 0, total = 0, maxit = A.max_iter, reg = 0, hitmax = 0;
         double t = 1.0, r = 0.0;
@@ -346,14 +365,13 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
             __syncthreads();
             // :247-259: the factor, the 1-norm and the condition estimate, the
             // wave sharing each step's independent updates
-            for (int k = lane; k < N * N; k += 64) sh.L[k] = sh.H[k];
-            __syncthreads();
-            const bool chol_ok = coop_cholesky<N>(reinterpret_cast<double (*)[N]>(sh.L), lane);
             double norm = 0.0;                              // utils_norm1: row sums in order, their max
             if (lane < N)
-                for (int j = 0; j < N; j++) norm += fabs(sh.H[lane * N + j]);
+                for (int j = 0; j < N; j++) norm += fabs(sh.L[lane * N + j]);
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) norm = fmax(norm, __shfl_xor(norm, o));
+            __syncthreads();
+            const bool chol_ok = coop_cholesky<N>(reinterpret_cast<double (*)[N]>(sh.L), lane);
             double condest = 0.0;
             if (chol_ok) condest = 1.0 / coop_rcond<N>(reinterpret_cast<double (*)[N]>(sh.L), sh.x, norm, lane);
             if (lane == 0) {
@@ -367,7 +385,7 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
                 if (lane == 0) {
                     cb_isotropic<N>(sh.aP0);
                     cb_identity<N>(sh.P2M);
-                    cb_identity<N>(sh.P2Min);
+                    sh.p2min_id = 1;
                     for (int k = 0; k < N; k++) sh.aP[k] = sh.aP0[k];
                     for (int k = 0; k < N; k++) sh.uP[k] = sh.u[k];
                     sh.rtol = cb_nrm2(N, sh.u) * A.tol;
@@ -398,7 +416,7 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
                     }
                     if (lane == 0) {
                         cb_isotropic<N>(sh.aP0);
-                        cb_identity<N>(sh.P2Min);
+                        sh.p2min_id = 1;
                         for (int k = 0; k < N; k++) sh.uP[k] = sh.u[k];
                     }
                     reg = 0;
@@ -408,11 +426,13 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
                 if (lane == 0) {
                     sh.rtol = cb_nrm2(N, sh.uP) * A.tol;
                     for (int k = 0; k < N; k++) sh.aP[k] = sh.aP0[k];
-                    for (int k = 0; k < N * N; k++) sh.P2M[k] = sh.P2Min[k];
+                    if (sh.p2min_id) cb_identity<N>(sh.P2M);
+                    else
+                        for (int k = 0; k < N * N; k++) sh.P2M[k] = Bg[k];
                     cb_trsv_ln<N>(sh.P2M, sh.uP);
                 }
                 __syncthreads();
-                cb_table_reset<N>(A, pP, sh.P2Min, lane);
+                cb_table_reset<N>(A, pP, sh.p2min_id ? nullptr : Bg, lane);
                 __syncthreads();
                 iter = 0;
                 t = 1.0;
@@ -421,7 +441,7 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
             }
             // CB_STEP: the basis moves by L (:333-343).  P2M <- P2M L with
             // dtrmm('R','L','N','N')'s order for each element, lane i forming
-            // row i from the old row; the factor copied column-major.
+            // row i from the old row.
             double brow[N];
             if (lane < N) {
                 for (int j = 0; j < N; j++) {
@@ -431,18 +451,14 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
                     brow[j] = b;
                 }
             }
-            for (int k = lane; k < N * N; k += 64) {
-                const int i = k % N, j = k / N;
-                sh.Lc[k] = (i >= j) ? sh.L[i * N + j] : 0.0;
-            }
             __syncthreads();
             if (lane < N)
                 for (int j = 0; j < N; j++) sh.P2M[lane + j * N] = brow[j];
             __syncthreads();
             if (lane == 0) {
-                cb_trsv_ln<N>(sh.Lc, sh.gP);
-                cb_trmv_lt<N>(sh.Lc, sh.aP);
-                cb_trsv_ln<N>(sh.Lc, sh.uP);
+                cb_trsv_ln<N, true>(sh.L, sh.gP);
+                cb_trmv_lt<N, true>(sh.L, sh.aP);
+                cb_trsv_ln<N, true>(sh.L, sh.uP);
             }
             cb_table_solve<N>(A, pP, sh.L, lane);
             if (A.bfgs)                                     // optbfgs.cpp:371-373: H = H^-1 = I
