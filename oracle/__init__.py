@@ -95,6 +95,8 @@ def lib():
         L.orc_optbfgs_batch.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(Options),
                                         C.POINTER(Outs), C.c_int]
         L.orc_bfgs_update.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_sn_step.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, C.c_double,
+                                  C.c_double, C.c_double, C.c_double, C.c_double, _dp]
         L.orc_solver_create.argtypes = [C.POINTER(Deck), C.c_int, C.POINTER(C.c_void_p)]
         L.orc_solver_destroy.argtypes = [C.c_void_p]
         L.orc_solver_max_dt.argtypes = [C.c_void_p]
@@ -254,6 +256,20 @@ def bfgs_update(H, Hinv, s, y):
     y = np.ascontiguousarray(y, dtype=np.float64)
     lib().orc_bfgs_update(n, _p(Hc), _p(Ic), _p(s), _p(y))
     return Hc.T.copy(), Ic.T.copy()
+
+
+def sn_steps(f, w, xi, eta, sigma_t, sigma_s, init_cond, init_x, init_y, dx, dy, dt, steps=1):
+    """`steps` Heun steps of the Sn solver (kinetic_update.cpp:120-235) on a
+    copy of f, shaped [nx+4][ny+4][nq]; sigma_t/sigma_s [nx+4][ny+4]."""
+    f = np.array(f, dtype=np.float64, order="C")
+    nx, ny, nq = f.shape[0] - 4, f.shape[1] - 4, f.shape[2]
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in (w, xi, eta, sigma_t, sigma_s)]
+    for _ in range(steps):
+        st = lib().orc_sn_step(nx, ny, nq, *[_p(a) for a in args], int(init_cond), float(init_x), float(init_y),
+                               float(dx), float(dy), float(dt), _p(f))
+        if st != 0:
+            raise RuntimeError("orc_sn_step status %d" % st)
+    return f
 
 
 def isotropic_alpha(nm):

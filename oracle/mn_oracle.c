@@ -1668,3 +1668,102 @@ int orc_solver_run(orc_solver *s, double t_final, double out_delta_t, int *steps
     if (steps) *steps = n;
     return ORC_OK;
 }
+
+/* ------------------------------------------------------------------ */
+/* Sn discrete ordinates (src/kinetic/), one rank: an independent truth */
+/* for the moment closures.  State f[((i*NY)+j)*nq + q], NX = nx + 4,   */
+/* NY = ny + 4, two ghost layers; sigma arrays [(i*NY)+j].              */
+/* ------------------------------------------------------------------ */
+static void sn_ghost_fill(int nx, int ny, int nq, double *f)
+{   /* kinetic_boundaries.cpp:190-223 */
+    const int NY = ny + 4;
+    const int x0 = 0, x1 = 2, x2 = nx, x3 = nx + 2, y0 = 0, y1 = 2, y2 = ny, y3 = ny + 2;
+    for (int i = 0; i < 2; i++)
+        for (int j = 2; j <= ny + 1; j++)
+            for (int q = 0; q < nq; q++) {
+                f[((size_t)(x0 + i) * NY + j) * nq + q] = f[((size_t)(x2 + i) * NY + j) * nq + q];
+                f[((size_t)(x3 + i) * NY + j) * nq + q] = f[((size_t)(x1 + i) * NY + j) * nq + q];
+            }
+    for (int i = 2; i <= nx + 1; i++)
+        for (int j = 0; j < 2; j++)
+            for (int q = 0; q < nq; q++) {
+                f[((size_t)i * NY + y0 + j) * nq + q] = f[((size_t)i * NY + y2 + j) * nq + q];
+                f[((size_t)i * NY + y3 + j) * nq + q] = f[((size_t)i * NY + y1 + j) * nq + q];
+            }
+}
+
+/* one forward-Euler stage in place: the flux of every interior cell from the
+ * stage's input, then scattering, the init_cond 1 source and the flux
+ * (kinetic_update.cpp:41-114 and :149-181) */
+static void sn_stage(int nx, int ny, int nq, const double *w, const double *xi, const double *eta,
+                     const double *sigma_t, const double *sigma_s, int init_cond, double init_x, double init_y,
+                     double dx, double dy, double dt, double *f, double *flux)
+{
+    const int NY = ny + 4;
+#define SNI(i, j, q) (((size_t)(i) * NY + (j)) * nq + (q))
+    if (init_cond == 2) sn_ghost_fill(nx, ny, nq, f);       /* comm.cpp:149-155 */
+    for (int i = 2; i <= nx + 1; i++)
+        for (int j = 2; j <= ny + 1; j++)
+            for (int q = 0; q < nq; q++) {
+                double nF, sF, eF, wF;
+                if (eta[q] > 0) {
+                    double s1 = f[SNI(i, j - 2, q)], s2 = f[SNI(i, j - 1, q)], s3 = f[SNI(i, j, q)], s4 = f[SNI(i, j + 1, q)];
+                    nF = s3 + 0.5 * slopefit(s2, s3, s4, 2.0);
+                    sF = s2 + 0.5 * slopefit(s1, s2, s3, 2.0);
+                } else {
+                    double s1 = f[SNI(i, j - 1, q)], s2 = f[SNI(i, j, q)], s3 = f[SNI(i, j + 1, q)], s4 = f[SNI(i, j + 2, q)];
+                    nF = s3 - 0.5 * slopefit(s2, s3, s4, 2.0);
+                    sF = s2 - 0.5 * slopefit(s1, s2, s3, 2.0);
+                }
+                if (xi[q] > 0) {
+                    double s1 = f[SNI(i - 2, j, q)], s2 = f[SNI(i - 1, j, q)], s3 = f[SNI(i, j, q)], s4 = f[SNI(i + 1, j, q)];
+                    eF = s3 + 0.5 * slopefit(s2, s3, s4, 2.0);
+                    wF = s2 + 0.5 * slopefit(s1, s2, s3, 2.0);
+                } else {
+                    double s1 = f[SNI(i - 1, j, q)], s2 = f[SNI(i, j, q)], s3 = f[SNI(i + 1, j, q)], s4 = f[SNI(i + 2, j, q)];
+                    eF = s3 - 0.5 * slopefit(s2, s3, s4, 2.0);
+                    wF = s2 - 0.5 * slopefit(s1, s2, s3, 2.0);
+                }
+                flux[SNI(i, j, q)] = (eF - wF) * xi[q] / dx + (nF - sF) * eta[q] / dy;
+            }
+    for (int i = 2; i <= nx + 1; i++)
+        for (int j = 2; j <= ny + 1; j++) {
+            const size_t c = (size_t)i * NY + j;
+            double integral = 0;
+            for (int q = 0; q < nq; q++) integral += w[q] * f[c * nq + q];
+            integral = integral * sigma_s[c] / (4 * M_PI);
+            for (int q = 0; q < nq; q++) {
+                double v = f[c * nq + q];
+                v = v - dt * sigma_t[c] * v + dt * integral;
+                if (init_cond == 1) {
+                    const double x_i = init_x + i * dx, y_j = init_y + j * dy;
+                    if (fabs(x_i) < 0.5 && fabs(y_j) < 0.5) v = v + dt;
+                }
+                f[c * nq + q] = v - dt * flux[c * nq + q];
+            }
+        }
+#undef SNI
+}
+
+int orc_sn_step(int nx, int ny, int nq, const double *w, const double *xi, const double *eta,
+                const double *sigma_t, const double *sigma_s, int init_cond, double init_x, double init_y,
+                double dx, double dy, double dt, double *f)
+{   /* KineticSolver::update, kinetic_update.cpp:120-235 */
+    if (nx < 2 || ny < 2 || nq < 1) return ORC_ERR_ARG;
+    const size_t n = (size_t)(nx + 4) * (ny + 4) * nq;
+    double *old = (double *)malloc(sizeof(double) * n), *flux = (double *)calloc(n, sizeof(double));
+    if (!old || !flux) { free(old); free(flux); return ORC_ERR_ALLOC; }
+    memcpy(old, f, sizeof(double) * n);
+    sn_stage(nx, ny, nq, w, xi, eta, sigma_t, sigma_s, init_cond, init_x, init_y, dx, dy, dt, f, flux);
+    sn_stage(nx, ny, nq, w, xi, eta, sigma_t, sigma_s, init_cond, init_x, init_y, dx, dy, dt, f, flux);
+    const int NY = ny + 4;
+    for (int i = 2; i <= nx + 1; i++)                        /* :221-230 */
+        for (int j = 2; j <= ny + 1; j++)
+            for (int q = 0; q < nq; q++) {
+                const size_t k = ((size_t)i * NY + j) * nq + q;
+                f[k] = 0.5 * (f[k] + old[k]);
+            }
+    free(old);
+    free(flux);
+    return ORC_OK;
+}

@@ -142,6 +142,16 @@ int orc_optbfgs_batch(const orc_tables *t, int ncells, double *u, double *P2M, d
  * for the step s = alpha - alpha_old and y = g - g_old. */
 void orc_bfgs_update(int n, double *H, double *Hinv, const double *s, const double *y);
 
+/* The Sn discrete-ordinates solver's Heun step (KineticSolver::update,
+ * kinetic/kinetic_update.cpp:120-235) on one rank: f is [(nx+4)(ny+4)][nq]
+ * with two ghost layers (I3D order), the ordinates (w, xi, eta) those of the
+ * moment tables for the same Q (kinetic_init.cpp:76-110 builds the same
+ * quadrature), sigma_t/sigma_s [(nx+4)(ny+4)].  Periodic ghosts for
+ * init_cond 2, the centre source for init_cond 1, as the reference. */
+int orc_sn_step(int nx, int ny, int nq, const double *w, const double *xi, const double *eta,
+                const double *sigma_t, const double *sigma_s, int init_cond, double init_x, double init_y,
+                double dx, double dy, double dt, double *f);
+
 /* ---------------- grid solver (main.cpp, momopt_*.cpp) ---------------- */
 
 /* Deck values the momopt path reads (main.cpp:69-83, momopt_init.cpp:56-68). */
