@@ -106,6 +106,8 @@ def lib():
         L.mnc_tune.argtypes = [vp, C.c_char_p, C.c_int]
         L.mnc_autotune.argtypes = [vp, C.c_double, C.POINTER(Options), C.c_int, C.c_char_p, C.c_int]
         L.mnc_get_tune.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int)]
+        L.mnc_sn_run.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                 _dp, _dp, C.c_double, C.c_int, _dp, C.c_int, C.POINTER(C.c_double)]
         L.mnc_last_error.argtypes = [vp]
         L.mnc_last_error.restype = C.c_char_p
         _fp = C.POINTER(C.c_float)
@@ -348,6 +350,24 @@ class Context:
         ms = C.c_double(0.0)
         self._check(lib().mnc_timing_get(self.h, kclass, C.byref(n), C.byref(ms)), "mnc_timing_get")
         return n.value, ms.value
+
+
+def sn_run(f, sigma_t, sigma_s, quad_order, dx, dy, init_x, init_y, init_cond, dt, steps, device=0):
+    """`steps` Heun steps of the Sn solver on the GPU (mnc_sn_run): f is
+    [nx+4][ny+4][Q*Q] with ghosts, sigma_t/sigma_s [nx+4][ny+4].  Returns
+    (f after the steps, kernel milliseconds)."""
+    f = np.array(f, dtype=np.float64, order="C")
+    st = np.ascontiguousarray(sigma_t, dtype=np.float64)
+    ss = np.ascontiguousarray(sigma_s, dtype=np.float64)
+    nx, ny = f.shape[0] - 4, f.shape[1] - 4
+    if f.shape[2] != quad_order * quad_order or st.shape != f.shape[:2] or ss.shape != f.shape[:2]:
+        raise ValueError("Sn state must be [nx+4][ny+4][Q*Q], sigma [nx+4][ny+4]")
+    ms = C.c_double(0.0)
+    rc = lib().mnc_sn_run(nx, ny, quad_order, dx, dy, init_x, init_y, init_cond, _p(st), _p(ss), dt, steps, _p(f),
+                          device, C.byref(ms))
+    if rc != OK:
+        raise MncError(rc, "mnc_sn_run failed")
+    return f, ms.value
 
 
 def scaled_pairs(u, alpha):

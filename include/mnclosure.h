@@ -181,6 +181,20 @@ int mnc_opt_batch_host(mnc_ctx *ctx, int ncells, double *u, double *alpha,
 int mnc_optcb_batch_host(mnc_ctx *ctx, int ncells, double *u, double *P2M, double *alphaP, double *alphaM,
                          const mnc_options *opt, mnc_outs *outs);
 
+/* The Sn discrete-ordinates solver (SOLVER kinetic), the independent truth
+ * for the closures: `steps` Heun steps of KineticSolver::update
+ * (src/kinetic/kinetic_update.cpp:120-235) on the GPU, the main loop's call
+ * repeated (src/main.cpp:204-230 with a fixed dt).  kinetic is the host state
+ * [(nx+4)*(ny+4)][Q*Q] with two ghost layers (I3D order), in and out;
+ * sigma_t/sigma_s [(nx+4)*(ny+4)].  The ordinates are the moment tables'
+ * (kinetic_init.cpp:76-110 builds the same quadrature).  Periodic ghosts
+ * for init_cond 2 (kinetic_boundaries.cpp:190-223), the centre source for
+ * init_cond 1.  kernel_ms (may be null) receives the device time of the
+ * steps.  MNC_ERR_ARG for bad sizes, MNC_ERR_HIP on a runtime error. */
+int mnc_sn_run(int nx, int ny, int quad_order, double dx, double dy, double init_x, double init_y, int init_cond,
+               const double *sigma_t, const double *sigma_s, double dt, int steps, double *kinetic, int device,
+               double *kernel_ms);
+
 /* A batch kept in device memory across calls (the staging arrays of
  * mnc_opt_batch_host): upload it, solve it in place any number of times (each
  * solve is mnc_opt_batch on those arrays, outs kept on the device), read it

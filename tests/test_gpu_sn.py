@@ -1,0 +1,80 @@
+"""The Sn discrete-ordinates solver on the GPU (mnc_sn_run) against the oracle's
+restatement (orc_sn_step, pinned in tests/test_oracle_sn.py).  The GPU sums
+the scattering integral in a different order than the oracle's sequential
+loop and contracts products into FMAs, so the states agree to rounding."""
+import numpy as np
+import pytest
+
+from helpers import load_pkg, oracle
+
+
+def _quad(Q):
+    t = oracle().Tables(1, Q, False)
+    return t.w, t.xi, t.eta
+
+
+def _case(nx, ny, Q, seed, init_cond, steps, dx, dy, init_x, init_y, dt):
+    mnc = load_pkg("mnc")
+    print("case sn nx=%d ny=%d Q=%d seed=%d ic=%d steps=%d" % (nx, ny, Q, seed, init_cond, steps), flush=True)
+    rng = np.random.default_rng(seed)
+    nq = Q * Q
+    f = 0.5 + rng.random((nx + 4, ny + 4, nq))
+    ss = rng.random((nx + 4, ny + 4))
+    st = ss + rng.random((nx + 4, ny + 4))
+    w, xi, eta = _quad(Q)
+    g, ms = mnc.sn_run(f, st, ss, Q, dx, dy, init_x, init_y, init_cond, dt, steps)
+    o = oracle().sn_steps(f, w, xi, eta, st, ss, init_cond, init_x, init_y, dx, dy, dt, steps=steps)
+    rel = np.abs(g - o).max() / np.abs(o).max()
+    rin = np.abs(g - o)[2:-2, 2:-2].max() / np.abs(o).max()
+    print("max |gpu - oracle| / max|f| = %.3e, interior %.3e (kernel %.3f ms)" % (rel, rin, ms))
+    return f, g, o, rel
+
+
+@pytest.mark.gpu
+def test_sn_periodic_matches_oracle():
+    f, g, o, rel = _case(20, 16, 8, 1, 2, 6, 0.1, 0.125, -1.0, -1.0, 0.02)
+    assert rel <= 1e-13
+    assert not np.allclose(g, f)
+
+
+@pytest.mark.gpu
+def test_sn_centre_source_fixed_ghosts_matches_oracle():
+    """init_cond 1: the ghosts keep their values and the cells within
+    |x|, |y| < 0.5 receive the unit source."""
+    f, g, o, rel = _case(16, 18, 6, 2, 1, 5, 0.2, 0.2, -1.7, -1.9, 0.03)
+    assert rel <= 1e-13
+    np.testing.assert_array_equal(g[:2], f[:2])
+    np.testing.assert_array_equal(g[:, -2:], f[:, -2:])
+
+
+@pytest.mark.gpu
+def test_sn_mass_conservation_at_size():
+    """128 x 128 cells, Q = 16 (256 ordinates), sigma_t = sigma_s per cell,
+    periodic: the total sum_q w_q f_q stays to rounding over 20 steps."""
+    mnc = load_pkg("mnc")
+    print("case sn mass 128x128 Q=16", flush=True)
+    nx = ny = 128
+    Q = 16
+    rng = np.random.default_rng(9)
+    f = 1.0 + rng.random((nx + 4, ny + 4, Q * Q))
+    sig = 0.5 + rng.random((nx + 4, ny + 4))
+    w = _quad(Q)[0]
+    m0 = (f[2:-2, 2:-2] * w).sum()
+    g, ms = mnc.sn_run(f, sig, sig, Q, 2.0 / nx, 2.0 / ny, -1.0, -1.0, 2, 0.004, 20)
+    m1 = (g[2:-2, 2:-2] * w).sum()
+    upd = 2.0 * 20 * nx * ny * Q * Q
+    print("mass drift %.3e; %.3f ms for 20 steps, %.3e cell-ordinate stage updates/s" %
+          (abs(m1 - m0) / m0, ms, upd / (ms * 1e-3)))
+    assert abs(m1 - m0) <= 1e-12 * m0
+
+
+@pytest.mark.gpu
+def test_sn_rejects_bad_sizes():
+    mnc = load_pkg("mnc")
+    f = np.ones((8, 8, 25))
+    s = np.ones((8, 8))
+    with pytest.raises(mnc.MncError) as e:
+        mnc.sn_run(f, s, s, 5, 0.1, 0.1, -1.0, -1.0, 2, 0.01, 1)     # odd Q
+    assert e.value.code == mnc.ERR_ARG
+    with pytest.raises(ValueError):
+        mnc.sn_run(np.ones((8, 8, 16)), s, s, 5, 0.1, 0.1, -1.0, -1.0, 2, 0.01, 1)
