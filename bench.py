@@ -358,6 +358,49 @@ def bench_optcb(mnc, n, seed, oracle_sample, bfgs=False):
             "oracle_sample_identical_path_frac": float(same.mean()) if m > 0 else None}
 
 
+def bench_sn(mnc, n=512, Q=16, steps=10, cpu_n=128, cpu_steps=2):
+    """The Sn discrete-ordinates solver (kinetic_update.cpp:120-235) on n x n
+    periodic cells of [-1,1]^2 with Q*Q ordinates, sigma_t = sigma_s = 1, an
+    isotropic density 1 + 0.5 sin(pi x) sin(pi y), dt at CFL 0.9
+    (kinetic_init.cpp:73): cell-ordinate updates/s from the device time of
+    `steps` Heun steps (the state already on the device), the algorithmic HBM
+    rate (a step streams the state 5 times: read and write per stage, the
+    step's start read once more) and the oracle's single-threaded step on a
+    cpu_n x cpu_n sample."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    nq = Q * Q
+    dx = dy = 2.0 / n
+    x = -1.0 + (np.arange(n + 4) - 2 + 0.5) * dx
+    rho = 1.0 + 0.5 * np.outer(np.sin(np.pi * x), np.sin(np.pi * x))
+    f = np.empty((n + 4, n + 4, nq))
+    f[:] = rho[:, :, None]
+    sig = np.ones((n + 4, n + 4))
+    dt = 0.9 * (2.0 / (2.0 + 2.0)) * (dx * dy) / (dx + dy)
+    mnc.sn_run(f[:8 + 4, :8 + 4].copy(), sig[:12, :12].copy(), sig[:12, :12].copy(), Q, dx, dy, -1.0, -1.0, 2, dt, 1)
+    out, ms = mnc.sn_run(f, sig, sig, Q, dx, dy, -1.0, -1.0, 2, dt, steps)
+    w = oracle.Tables(1, Q, False).w
+    drift = abs((out[2:-2, 2:-2] * w).sum() - (f[2:-2, 2:-2] * w).sum()) / (f[2:-2, 2:-2] * w).sum()
+    upd = float(n) * n * nq * steps
+    bytes_ = 5.0 * 8.0 * upd
+    progress("Sn %dx%d Q=%d: %d steps in %.3f ms" % (n, n, Q, steps, ms))
+    xi, eta = oracle.Tables(1, Q, False).xi, oracle.Tables(1, Q, False).eta
+    m = cpu_n
+    fc = np.ascontiguousarray(f[:m + 4, :m + 4])
+    sc = np.ascontiguousarray(sig[:m + 4, :m + 4])
+    t1 = time.perf_counter()
+    oracle.sn_steps(fc, w, xi, eta, sc, sc, 2, -1.0, -1.0, dx, dy, dt, steps=cpu_steps)
+    cpu_s = time.perf_counter() - t1
+    return {"workload": "Sn %d^2 periodic cells, Q=%d (%d ordinates), sigma_t = sigma_s = 1, isotropic "
+                        "1 + 0.5 sin(pi x) sin(pi y), CFL 0.9, %d Heun steps" % (n, Q, nq, steps),
+            "cell_ordinate_updates_per_s": upd / (ms * 1e-3), "ms_per_step": ms / steps,
+            "algorithmic_gb_per_s": bytes_ / (ms * 1e-3) / 1e9,
+            "hbm_frac": bytes_ / (ms * 1e-3) / 8.0e12, "mass_drift": drift,
+            "cpu_baseline": {"value": float(m) * m * nq * cpu_steps / cpu_s, "unit": "cell-ordinate updates/s",
+                             "cores": 1, "kind": "port",
+                             "sample": "oracle orc_sn_step, %d^2 cells, %d steps (%.1f s)" % (m, cpu_steps, cpu_s)}}
+
+
 def bench_m8(mnc, gi, steps, n=512, sample=1000):
     """The M8 workload: 45 moments, Q = 32 (1024 points), n x n periodic cells on
     [-1,1]^2, sigma_s = 1, isotropic density 1 + 0.5 sin(pi x) sin(pi y), TOL
@@ -520,6 +563,7 @@ def main():
                     help="vectors of the change-of-basis optimiser's isolated batch (0 skips it)")
     ap.add_argument("--ppn-steps", type=int, default=569, help="steps of the PPn lattice form (0 skips it)")
     ap.add_argument("--ppn-only", action="store_true", help="only the PPn lattice form (for profiling)")
+    ap.add_argument("--sn-only", action="store_true", help="only the Sn solver leg")
     ap.add_argument("--optcb-only", action="store_true",
                     help="only the change-of-basis optimiser's isolated batch (for profiling)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -551,6 +595,12 @@ def main():
         print(json.dumps({"metric": "converged optcb closures/s (isolated M4 batch, TOL 1e-8)",
                           "value": r["closures_per_s"], "unit": "closures/s", "n_gpus": 1,
                           "higher_is_better": True, "config": {"optcb": r, "optbfgs": rb}}))
+        return
+    if args.sn_only:
+        r = bench_sn(mnc)
+        print(json.dumps({"metric": "Sn cell-ordinate updates/s (512^2, Q=16)",
+                          "value": r["cell_ordinate_updates_per_s"], "unit": "updates/s", "n_gpus": 1,
+                          "higher_is_better": True, "config": {"sn": r}}))
         return
     if args.ppn_only:
         r = bench_ppn(mnc, gi, d, args.ppn_steps, args.warmup, 0.0 if args.no_cpu else min(args.cpu_seconds, 10.0))
@@ -633,6 +683,8 @@ def main():
         out["config"]["optcb"] = bench_optcb(mnc, args.optcb, 12345, 50000) if args.optcb > 0 else None
         out["config"]["optbfgs"] = bench_optcb(mnc, args.optcb, 12345, 50000, bfgs=True) if args.optcb > 0 else None
         progress("change-of-basis batches done")
+        out["config"]["sn"] = bench_sn(mnc)
+        progress("Sn done")
     if args.m8_steps > 0 and not args.plain_only and not args.only:
         out["config"]["m8"] = bench_m8(mnc, gi, args.m8_steps)
         progress("M8 done")

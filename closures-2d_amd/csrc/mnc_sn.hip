@@ -10,6 +10,7 @@
 // (kinetic_boundaries.cpp:190-223) are a separate small launch.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -67,13 +68,18 @@ __global__ void sn_ghost_kernel(double *f, double *f2, int nx, int ny, int nq) {
 
 // out <- Euler stage of in for every interior cell; with old, the Heun average
 // 0.5 (stage + old) (old may alias out: each element is read before it is written
-// by the same thread)
+// by the same thread).  One block per cell, XCD-aware: workgroups go to the 8
+// XCDs round robin, so block b takes cell (b % 8) * per + b / 8 and each XCD
+// sweeps one contiguous range of rows; the stencil's x neighbours (a row, ~NY
+// nq 8 bytes, apart) are then still in that XCD's L2 when they are reused.
 __global__ __launch_bounds__(kSnThreads) void sn_stage_kernel(SnArgs a, const double *__restrict__ in, double *out,
                                                               const double *old) {
     __shared__ double red[kSnThreads / 64];
     const int NY = a.ny + 4, nq = a.nq;
-    const int ncell = a.nx * a.ny;
-    for (int cid = blockIdx.x; cid < ncell; cid += gridDim.x) {
+    const int ncell = a.nx * a.ny, per = (ncell + 7) / 8;
+    {
+        const int cid = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+        if (cid >= ncell) return;   // uniform over the block
         const int i = 2 + cid / a.ny, j = 2 + cid % a.ny;
         const size_t c = size_t(i) * NY + j;
         double part = 0.0;
@@ -131,6 +137,7 @@ extern "C" int mnc_sn_run(int nx, int ny, int quad_order, double dx, double dy, 
         !(dx > 0.0) || !(dy > 0.0) || !std::isfinite(dt))
         return MNC_ERR_ARG;
     if (size_t(nx + 4) * (ny + 4) * size_t(quad_order) * quad_order > (size_t(1) << 33)) return MNC_ERR_ARG;
+    if (size_t(nx) * ny > size_t(INT_MAX) - 8) return MNC_ERR_ARG;   // one block per cell
     mnc::Tables t;
     if (mnc::build_tables(1, quad_order, 0, t) != MNC_OK) return MNC_ERR_ARG;   // the same ordinates
     const int nq = t.nq;
@@ -153,7 +160,7 @@ extern "C" int mnc_sn_run(int nx, int ny, int quad_order, double dx, double dy, 
         ok(hipMemcpyAsync(eta, t.eta.data(), nq * sizeof(double), hipMemcpyHostToDevice, s)) &&
         ok(hipEventRecord(e0, s))) {
         SnArgs a{nx, ny, nq, init_cond, dx, dy, dt, init_x, init_y, w, xi, eta, st, ss};
-        const int blocks = int(std::min<size_t>(size_t(nx) * ny, 16384));
+        const int blocks = 8 * ((nx * ny + 7) / 8);   // one per cell, in XCD order
         const long long gw = 2LL * (nx + ny) * nq;
         const int gblocks = int(std::min<long long>((gw + 255) / 256, 4096));
         for (int k = 0; k < steps && rc == MNC_OK; ++k) {
