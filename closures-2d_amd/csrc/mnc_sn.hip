@@ -1,9 +1,9 @@
 // mnc_sn.hip -- the Sn discrete-ordinates solver (src/kinetic/) on the GPU:
 // the independent truth for the moment closures.  One Heun step
 // (KineticSolver::update, kinetic_update.cpp:120-235) is two fused stage
-// launches: each block takes one interior cell, its threads the ordinates;
+// launches: each wave takes one interior cell, its lanes the ordinates;
 // the flux of every ordinate (kinetic_update.cpp:41-114, MUSCL with the
-// minmod slope fit, theta = 2), the isotropic scattering integral (a block
+// minmod slope fit, theta = 2), the isotropic scattering integral (a wave
 // sum), the init_cond 1 source and the Euler update are one pass from the
 // stage's input grid into its output grid.  The second stage folds in the
 // Heun average with the step's initial state.  Periodic ghosts
@@ -68,35 +68,32 @@ __global__ void sn_ghost_kernel(double *f, double *f2, int nx, int ny, int nq) {
 
 // out <- Euler stage of in for every interior cell; with old, the Heun average
 // 0.5 (stage + old) (old may alias out: each element is read before it is written
-// by the same thread).  One block per cell, XCD-aware: workgroups go to the 8
-// XCDs round robin, so block b takes cell (b % 8) * per + b / 8 and each XCD
-// sweeps one contiguous range of rows; the stencil's x neighbours (a row, ~NY
-// nq 8 bytes, apart) are then still in that XCD's L2 when they are reused.
+// by the same thread).  One wave per cell, four consecutive cells per block,
+// XCD-aware: workgroups go to the 8 XCDs round robin, so block b takes group
+// (b % 8) * per + b / 8 and each XCD sweeps one contiguous range of rows; the
+// stencil's x neighbours (a row, ~NY nq 8 bytes, apart) are then still in
+// that XCD's L2 when they are reused.
 __global__ __launch_bounds__(kSnThreads) void sn_stage_kernel(SnArgs a, const double *__restrict__ in, double *out,
                                                               const double *old) {
-    __shared__ double red[kSnThreads / 64];
+    constexpr int kWaves = kSnThreads / 64;
     const int NY = a.ny + 4, nq = a.nq;
-    const int ncell = a.nx * a.ny, per = (ncell + 7) / 8;
+    const int ncell = a.nx * a.ny, ngroup = (ncell + kWaves - 1) / kWaves, per = (ngroup + 7) / 8;
+    const int lane = threadIdx.x & 63;
     {
-        const int cid = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-        if (cid >= ncell) return;   // uniform over the block
+        const int grp = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+        const int cid = grp * kWaves + (threadIdx.x >> 6);
+        if (grp >= ngroup || cid >= ncell) return;   // uniform over the wave; no block barrier follows
         const int i = 2 + cid / a.ny, j = 2 + cid % a.ny;
         const size_t c = size_t(i) * NY + j;
         double part = 0.0;
-        for (int q = threadIdx.x; q < nq; q += kSnThreads) part += a.w[q] * in[c * nq + q];
+        for (int q = lane; q < nq; q += 64) part += a.w[q] * in[c * nq + q];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
-        __syncthreads();
-        double integral = 0.0;
-#pragma unroll
-        for (int k = 0; k < kSnThreads / 64; ++k) integral += red[k];
-        __syncthreads();
-        integral = integral * a.sigma_s[c] / (4 * M_PI);
+        double integral = part * a.sigma_s[c] / (4 * M_PI);
         const double st = a.sigma_t[c];
         const double x_i = a.init_x + i * a.dx, y_j = a.init_y + j * a.dy;
         const bool src = a.init_cond == 1 && fabs(x_i) < 0.5 && fabs(y_j) < 0.5;
-        for (int q = threadIdx.x; q < nq; q += kSnThreads) {
+        for (int q = lane; q < nq; q += 64) {
             auto F = [&](int di, int dj) { return in[(size_t(i + di) * NY + (j + dj)) * nq + q]; };
             double nF, sF, eF, wF;
             if (a.eta[q] > 0) {
@@ -160,7 +157,7 @@ extern "C" int mnc_sn_run(int nx, int ny, int quad_order, double dx, double dy, 
         ok(hipMemcpyAsync(eta, t.eta.data(), nq * sizeof(double), hipMemcpyHostToDevice, s)) &&
         ok(hipEventRecord(e0, s))) {
         SnArgs a{nx, ny, nq, init_cond, dx, dy, dt, init_x, init_y, w, xi, eta, st, ss};
-        const int blocks = 8 * ((nx * ny + 7) / 8);   // one per cell, in XCD order
+        const int blocks = 8 * (((nx * ny + 3) / 4 + 7) / 8);   // four cells each, in XCD order
         const long long gw = 2LL * (nx + ny) * nq;
         const int gblocks = int(std::min<long long>((gw + 255) / 256, 4096));
         for (int k = 0; k < steps && rc == MNC_OK; ++k) {
