@@ -134,15 +134,16 @@ struct Timed {
     mnc_ctx *ctx;
     int kc;
     hipEvent_t a = nullptr, b = nullptr;
-    Timed(mnc_ctx *c, int k) : ctx(c), kc(k) {
-        if (ctx->timing) {
+    bool on;
+    Timed(mnc_ctx *c, int k, bool enable = true) : ctx(c), kc(k), on(enable && c->timing) {
+        if (on) {
             (void)hipEventCreate(&a);
             (void)hipEventCreate(&b);
             (void)hipEventRecord(a, ctx->stream);
         }
     }
     ~Timed() {
-        if (ctx->timing) {
+        if (on) {
             (void)hipEventRecord(b, ctx->stream);
             ctx->events.push_back({kc, a, b});
         }
@@ -163,13 +164,20 @@ int ensure_state(mnc_ctx *ctx, size_t ncells) {
 }
 
 // Reads the kernels' error bits (synchronising the stream).
-int check_errors(mnc_ctx *ctx) {
+int check_errors(mnc_ctx *ctx, const int *d_ctl = nullptr) {
     int ctl[4] = {0, 0, 0, 0};
-    MNC_HIP_CHECK(hipMemcpyAsync(ctl, ctx->d_ctl, sizeof ctl, hipMemcpyDeviceToHost, ctx->stream));
+    MNC_HIP_CHECK(hipMemcpyAsync(ctl, d_ctl ? d_ctl : ctx->d_ctl, sizeof ctl, hipMemcpyDeviceToHost, ctx->stream));
     MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     if (ctl[3] & (1 << MNC_ERR_U0)) return ctx_fail(ctx, MNC_ERR_U0, "u[0] <= 0 in at least one cell");
     if (ctl[3] & (1 << MNC_ERR_MAXREG)) return ctx_fail(ctx, MNC_ERR_MAXREG, "Max regularization achieved");
     return MNC_OK;
+}
+
+// m_inf of the supplementary stop test (opt.cpp:61-62): the bound of the
+// real spherical harmonics up to order floor(sqrt(n)) + 1.
+double gamma_minf(int nm) {
+    const int pn_order = int(std::floor(std::sqrt(double(nm)))) + 1;
+    return std::sqrt((2 * pn_order + 1) / (2 * M_PI));
 }
 
 int check_opts(mnc_ctx *ctx, const mnc_options *o) {
@@ -177,6 +185,9 @@ int check_opts(mnc_ctx *ctx, const mnc_options *o) {
     if (!o) return ctx_fail(ctx, MNC_ERR_ARG, "options pointer is NULL");
     if (o->max_iter < 1) return ctx_fail(ctx, MNC_ERR_ARG, "max_iter must be >= 1");
     if (!(o->tol_rel >= 0.0)) return ctx_fail(ctx, MNC_ERR_ARG, "tol_rel must be >= 0");
+    if (std::isnan(o->tol_gamma)) return ctx_fail(ctx, MNC_ERR_ARG, "tol_gamma is NaN");
+    if (o->tol_gamma > 0.0 && o->opt_type != MNC_OPT_NEWTON)
+        return ctx_fail(ctx, MNC_ERR_ARG, "tol_gamma is a stop test of the Newton optimiser (opt_type 0)");
     if (o->precision != MNC_PREC_FP64 && o->precision != MNC_PREC_MIXED)
         return ctx_fail(ctx, MNC_ERR_ARG, "unknown precision");
     if (o->warm_start < MNC_WARM_PREVIOUS || o->warm_start > MNC_WARM_LEARNED)
@@ -276,9 +287,13 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
         return ctx_fail(ctx, MNC_ERR_ARG, "mixed precision needs moment order <= 4 and quad order <= 16");
     if (ctx->has_g64 && (mixed || o->warm_start == MNC_WARM_LEARNED))
         return ctx_fail(ctx, MNC_ERR_ARG, "moment order 8 runs the FP64 form only");
+    const bool learned = o->warm_start == MNC_WARM_LEARNED;
     int st = ensure_state(ctx, size_t(ncells));
     if (st) return st;
-    const bool learned = o->warm_start == MNC_WARM_LEARNED;
+    const bool timed = true;
+    double *st_d = ctx->d_std;
+    int *st_i = ctx->d_sti, *lists = ctx->d_lists, *ctl = ctx->d_ctl;
+    const size_t cap = ctx->state_cap;
     if (learned && ctx->learned_gate_log10 < kGateOff) {   // buffers of the network's guesses
         if ((st = mlp_check(ctx, true))) return st;
         if (size_t(ncells) > ctx->guess_cap) {
@@ -305,15 +320,17 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     a.max_iter = o->max_iter;
     a.tol = o->tol_rel;
     a.cond_h_max = o->cond_h_max;
+    a.tol_gamma = o->tol_gamma;
+    a.gamma_minf = gamma_minf(t.nm);
     a.warm = o->warm_start;
     a.ls_mode = o->line_search;
     a.stats = ctx->d_stats;
     a.alpha_guess = alpha_guess;
-    a.st_d = ctx->d_std;
-    a.st_i = ctx->d_sti;
-    a.lists = ctx->d_lists;
-    a.ctl = ctx->d_ctl;
-    a.cap = ctx->state_cap;
+    a.st_d = st_d;
+    a.st_i = st_i;
+    a.lists = lists;
+    a.ctl = ctl;
+    a.cap = cap;
     a.mz = ctx->d_mz;
     a.mg = ctx->d_mg;
     a.mh = ctx->d_mh;
@@ -352,8 +369,8 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
         // if it is not finite), as the gate would decide at any threshold above
         // the gradient norms, without the network's or judge_kernel's launches
         if (t.nm != 15) return ctx_fail(ctx, MNC_ERR_ARG, "the learned warm start is for moment order 4");
-        Timed tm(ctx, MNC_KCLASS_NEWTON);
-        mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
+        Timed tm(ctx, MNC_KCLASS_NEWTON, timed);
+        mnc::launch_ctl_reset(ctl, ctx->stream);
         if ((st = launch(a, lp))) return st;
     } else if (learned) {
         // Round 1 evaluates every cell at its start and parks those failing the
@@ -385,8 +402,8 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
             if ((st = launch(a, rest))) return st;
         }
     } else {
-        Timed tm(ctx, MNC_KCLASS_NEWTON);
-        mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
+        Timed tm(ctx, MNC_KCLASS_NEWTON, timed);
+        mnc::launch_ctl_reset(ctl, ctx->stream);
         if (ctx->has_g64) {
             // lockstep rounds in batches until the active list is empty (the
             // count is read back between batches, so this path is not captured)
@@ -398,7 +415,7 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
                 if (batch > 4000) return ctx_fail(ctx, MNC_ERR_STATE, "closure solve did not finish");
                 mnc::launch_newton_g64_rounds(a, t.nm, t.np2, lin, 4, blocks, ctx->stream);
                 int left = 0;
-                MNC_HIP_CHECK(hipMemcpyAsync(&left, ctx->d_ctl + lin, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+                MNC_HIP_CHECK(hipMemcpyAsync(&left, ctl + lin, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
                 MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
                 if (left == 0) break;
             }
@@ -408,7 +425,7 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     }
     MNC_HIP_CHECK(hipGetLastError());
     if (!check) return MNC_OK;
-    return check_errors(ctx);
+    return check_errors(ctx, ctl);
 }
 
 // The change-of-basis state as MomOptSolver::init leaves it
@@ -927,7 +944,7 @@ bool same_opts(const mnc_options &a, const mnc_options &b) {
     return a.max_iter == b.max_iter && a.tol_rel == b.tol_rel && a.cond_h_max == b.cond_h_max &&
            a.precision == b.precision && a.warm_start == b.warm_start && a.line_search == b.line_search &&
            a.delta_ppn == b.delta_ppn && a.opt_type == b.opt_type && a.max_bfgs_iter == b.max_bfgs_iter &&
-           a.cond_h_max_bfgs == b.cond_h_max_bfgs;
+           a.cond_h_max_bfgs == b.cond_h_max_bfgs && a.tol_gamma == b.tol_gamma;
 }
 
 // Captures enqueue_step into g.exec.  On any capture error the context falls

@@ -42,6 +42,8 @@ struct NewtonArgs {
     int max_iter;
     double tol;
     double cond_h_max;
+    double tol_gamma;     // the supplementary stop test's bound (opt.cpp:288-293); <= 0: off
+    double gamma_minf;    // its m_inf = sqrt((2 (floor(sqrt(n)) + 1) + 1) / (2 pi)) (opt.cpp:61-62)
     int warm;             // MNC_WARM_*
     long long *stats;     // [0] max iter, [1] iter sum, [2] solves, [3..7] hist, [8] trials,
                           // [9..11] warm starts taken: learned, isotropic, previous,

@@ -487,6 +487,17 @@ __device__ double grad_norm2(const NewtonArgs &A, const double (&al)[NM], const 
     return s;
 }
 
+// The supplementary stop test (opt.cpp:52-76,288-293): with tol_gamma > 0 a
+// cell whose gradient passes the tolerance stops only when
+// gamma = exp(2 |d|_1 m_inf) (Mn) or 1 + 2 |d|_1 m_inf (PPn) is <= tol_gamma,
+// d the Newton direction and m_inf = sqrt((2 P + 1) / (2 pi)) with
+// P = floor(sqrt(n)) + 1 (computed on the host).  da1 is |d|_1 summed in
+// index order.
+__device__ __forceinline__ bool gamma_ok(const NewtonArgs &A, double da1, bool ppn) {
+    const double x = 2.0 * da1 * A.gamma_minf;
+    return (ppn ? 1.0 + x : exp(x)) <= A.tol_gamma;
+}
+
 // Control word holding the count of list l: lists 0 and 1 (the lockstep
 // ping-pong pair) use words 0 and 1, lists 2 and 3 (the FP64 pair of a
 // mixed-precision solve) words 4 and 5, list 4 (the learned warm start's
@@ -782,7 +793,14 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
                     for (int i = 0; i < NM; ++i) d[i] = double(dh[i]);
                 }
                 // Phase 3: stop test, or the line search set-up (linesearch.cpp:28-43)
-                if (err <= S.d(St::F_RTOL, c)) {
+                bool stop = err <= S.d(St::F_RTOL, c);
+                if (stop && A.tol_gamma > 0.0) {
+                    double da1 = 0.0;
+#pragma unroll
+                    for (int i = 0; i < NM; ++i) da1 = da1 + fabs(d[i]);
+                    stop = gamma_ok(A, da1, PPN);
+                }
+                if (stop) {
                     converged = true;                                  // :293-306
 #pragma unroll
                     for (int i = 0; i < NM; ++i) g0[i] = S.d(St::F_D + i, c);
@@ -1691,7 +1709,14 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
                 } else {
                     double xv = gv;
                     reg_apply_inv_rcp<NM>(row, col, rd, xv, lane);
-                    if (err <= rtol) {
+                    bool stop = err <= rtol;
+                    if (stop && A.tol_gamma > 0.0) {
+                        double da1 = 0.0;
+#pragma unroll
+                        for (int i = 0; i < NM; ++i) da1 = da1 + fabs(readlane_d(xv, i));
+                        stop = gamma_ok(A, da1, PPN);
+                    }
+                    if (stop) {
                         converged = true;
                     } else {
                         const double di = -xv;
@@ -2135,7 +2160,13 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
                 if (lane < NM) sh.x[lane] = sh.gvec[lane];
                 __builtin_amdgcn_wave_barrier();
                 coop_apply_inv<NM>(sh.L, sh.x, lane);               // dpotrs (:278-282)
-                if (err <= S.d(St::F_RTOL, c)) {
+                bool stop = err <= S.d(St::F_RTOL, c);
+                if (stop && A.tol_gamma > 0.0) {
+                    double da1 = 0.0;
+                    for (int i = 0; i < NM; ++i) da1 = da1 + fabs(sh.x[i]);
+                    stop = gamma_ok(A, da1, false);
+                }
+                if (stop) {
                     converged = true;                               // :293-306
                 } else {                                            // linesearch.cpp:28-43
                     const double di = lane < NM ? -sh.x[lane] : 0.0;

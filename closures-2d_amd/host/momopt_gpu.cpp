@@ -6,7 +6,10 @@
 //
 // Usage: momopt_gpu [deck-path]      (default: ./input.deck, like the reference)
 // Extra deck keys (optional): MOMOPT_GPU_PRECISION 0|1, MOMOPT_GPU_WARM_START 0|1|2,
-// MOMOPT_GPU_LINE_SEARCH 0|1, MOMOPT_GPU_DEVICE n.
+// MOMOPT_GPU_LINE_SEARCH 0|1, MOMOPT_GPU_DEVICE n, and MOMOPT_TOL_GAMMA x: the
+// supplementary stop test gamma <= x of optScaled (opt.cpp:288-293; the
+// reference sets 1.1 at momopt_update.cpp:226 behind a comment, and its
+// regression fixtures were written with it); absent or <= 0: off.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -216,6 +219,8 @@ int main(int argc, char **argv) {
     deck.get("MOMOPT_GPU_WARM_START", warm);
     deck.get("MOMOPT_GPU_LINE_SEARCH", ls);
     deck.get("MOMOPT_GPU_DEVICE", device);
+    double tol_gamma = 0.0;
+    deck.get("MOMOPT_TOL_GAMMA", tol_gamma);
 
     Grid g;
     g.nx = ncx;
@@ -256,7 +261,8 @@ int main(int argc, char **argv) {
     }
     const double max_dt = cfl * (2.0 / (2.0 + theta)) * (g.dx * g.dy) / (g.dx + g.dy);   // :72
     std::printf("--- Init done ---\n");
-    mnc_options opt{max_iter, tol, cond_h_max, prec, warm, ls, delta_ppn, otype, max_bfgs_iter, cond_h_max_bfgs};
+    mnc_options opt{max_iter, tol, cond_h_max, prec, warm, ls, delta_ppn, otype, max_bfgs_iter, cond_h_max_bfgs,
+                    tol_gamma};
 
     double t = t0, dt = max_dt;                       // main.cpp:204-230
     if (!restarting) write_output(g, ctx, nm, t);

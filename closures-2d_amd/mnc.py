@@ -35,7 +35,7 @@ class Options(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("tol_rel", C.c_double), ("cond_h_max", C.c_double),
                 ("precision", C.c_int), ("warm_start", C.c_int), ("line_search", C.c_int),
                 ("delta_ppn", C.c_double), ("opt_type", C.c_int), ("max_bfgs_iter", C.c_int),
-                ("cond_h_max_bfgs", C.c_double)]
+                ("cond_h_max_bfgs", C.c_double), ("tol_gamma", C.c_double)]
 
 
 class Outs(C.Structure):
@@ -127,9 +127,10 @@ def _p(a):
 
 
 def options(max_iter=100, tol=1e-8, cond_h_max=1e10, precision=PREC_FP64, warm_start=WARM_PREVIOUS,
-            line_search=LS_REFERENCE, delta_ppn=0.0, opt_type=0, max_bfgs_iter=5, cond_h_max_bfgs=20.0):
+            line_search=LS_REFERENCE, delta_ppn=0.0, opt_type=0, max_bfgs_iter=5, cond_h_max_bfgs=20.0,
+            tol_gamma=0.0):
     return Options(max_iter, tol, cond_h_max, precision, warm_start, line_search, delta_ppn, opt_type, max_bfgs_iter,
-                   cond_h_max_bfgs)
+                   cond_h_max_bfgs, tol_gamma)
 
 
 class Context:

@@ -86,6 +86,15 @@ typedef struct {
                                   struct is zero-initialised past delta_ppn)    */
     int max_bfgs_iter;         /* MOMOPT_MAX_BFGS_ITER (MNC_OPT_BFGS only, >= 1) */
     double cond_h_max_bfgs;    /* MOMOPT_COND_H_MAX_BFGS (MNC_OPT_BFGS only, > 1) */
+    double tol_gamma;          /* optScaled's supplementary stop test (opt.cpp:52-76,
+                                  288-293; tolGamma, momopt_update.cpp:226): a cell whose
+                                  gradient passes tol_rel stops only when
+                                  gamma = exp(2 |d|_1 m_inf) (Mn; 1 + 2 |d|_1 m_inf for
+                                  PPn) <= tol_gamma, d the Newton direction and
+                                  m_inf = sqrt((2 (floor(sqrt(n)) + 1) + 1) / (2 pi)).
+                                  <= 0 (zero-initialised): off, as the reference's current
+                                  source; its regression fixtures were written with 1.1.
+                                  MNC_OPT_NEWTON only (MNC_ERR_ARG otherwise)         */
 } mnc_options;
 
 /* Optimisers (opt.h:42-46).  MNC_OPT_CHANGE_BASIS in the grid solves

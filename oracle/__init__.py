@@ -30,7 +30,8 @@ class Options(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("tol_rel", C.c_double),
                 ("cond_h_max", C.c_double), ("use_gaunt", C.c_int), ("line_search", C.c_int),
                 ("moment_type", C.c_int), ("delta_ppn", C.c_double),
-                ("max_bfgs_iter", C.c_int), ("cond_h_max_bfgs", C.c_double)]
+                ("max_bfgs_iter", C.c_int), ("cond_h_max_bfgs", C.c_double),
+                ("tol_gamma", C.c_double)]
 
 
 class Outs(C.Structure):
@@ -48,7 +49,7 @@ class Deck(C.Structure):
                 ("cfl_factor", C.c_double), ("tol", C.c_double),
                 ("cond_h_max", C.c_double), ("max_iter", C.c_int),
                 ("use_gaunt", C.c_int), ("theta", C.c_double), ("line_search", C.c_int),
-                ("moment_type", C.c_int), ("delta_ppn", C.c_double)]
+                ("moment_type", C.c_int), ("delta_ppn", C.c_double), ("tol_gamma", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -184,23 +185,24 @@ class Tables:
         return f, g, h
 
     def opt(self, u, alpha, max_iter=100, tol=1e-8, cond_h_max=1e10, use_gaunt=None, line_search=0,
-            moment_type=MOMENT_MN, delta_ppn=0.0):
-        """One opt() call (opt.cpp:82-148); returns (status, u_out, alpha_out, outs)."""
+            moment_type=MOMENT_MN, delta_ppn=0.0, tol_gamma=0.0):
+        """One opt() call (opt.cpp:82-148); returns (status, u_out, alpha_out, outs).
+        tol_gamma > 0 adds the supplementary stop test (opt.cpp:52-76,288-293)."""
         u = np.array(u, dtype=np.float64)
         alpha = np.array(alpha, dtype=np.float64)
         o = Options(max_iter, tol, cond_h_max, int(self.use_gaunt if use_gaunt is None else use_gaunt),
-                    line_search, moment_type, delta_ppn)
+                    line_search, moment_type, delta_ppn, 0, 0.0, tol_gamma)
         outs = Outs()
         st = lib().orc_opt(self.h, _p(u), _p(alpha), C.byref(o), C.byref(outs))
         return st, u, alpha, outs
 
     def opt_batch(self, u, alpha, max_iter=100, tol=1e-8, cond_h_max=1e10, use_gaunt=None,
-                  nthreads=0, line_search=0, moment_type=MOMENT_MN, delta_ppn=0.0):
+                  nthreads=0, line_search=0, moment_type=MOMENT_MN, delta_ppn=0.0, tol_gamma=0.0):
         u = np.array(u, dtype=np.float64, order="C")
         alpha = np.array(alpha, dtype=np.float64, order="C")
         n = u.shape[0]
         o = Options(max_iter, tol, cond_h_max, int(self.use_gaunt if use_gaunt is None else use_gaunt),
-                    line_search, moment_type, delta_ppn)
+                    line_search, moment_type, delta_ppn, 0, 0.0, tol_gamma)
         outs = (Outs * n)()
         st = lib().orc_opt_batch(self.h, n, _p(u), _p(alpha), C.byref(o), outs, nthreads)
         it = np.array([o_.iter for o_ in outs], dtype=np.int32)
@@ -357,7 +359,7 @@ DECK_DEFAULTS = dict(num_cells_x=50, num_cells_y=50, a_x=-1.5, b_x=1.5, a_y=-1.5
                      gaussian_sigma=0.4, floor_value=1e-4, sigma=1.0, init_cond=0,
                      moment_order=3, quad_order=30, cfl_factor=0.9, tol=1e-4,
                      cond_h_max=1e10, max_iter=100, use_gaunt=1, theta=2.0, line_search=0,
-                     moment_type=MOMENT_MN, delta_ppn=1e-10)
+                     moment_type=MOMENT_MN, delta_ppn=1e-10, tol_gamma=0.0)
 
 
 def deck_from_dict(d):

@@ -697,6 +697,21 @@ static void isotropic_alpha(int n, double *alpha, const orc_options *o)   /* opt
     for (int i = 1; i < n; i++) alpha[i] = 0.0;
 }
 
+/* The supplementary stop test's error bound (opt.cpp:52-76):
+ * gamma = exp(2 |d|_1 m_inf) for Mn, 1 + 2 |d|_1 m_inf for PPn, with
+ * m_inf = sqrt((2 P + 1) / (2 pi)) and P = floor(sqrt(n)) + 1. */
+static double modcfl(int n, const double *da, int moment_type)
+{
+    double da_1 = 0;
+    for (int i = 0; i < n; i++)
+        da_1 = da_1 + fabs(da[i]);
+    int pnOrder = (int)floor(sqrt((double)n)) + 1;
+    double m_inf = sqrt((2 * pnOrder + 1) / (2 * M_PI));
+    if (moment_type == ORC_MOMENT_PPN)
+        return 1.0 + 2.0 * da_1 * m_inf;
+    return exp(2.0 * da_1 * m_inf);
+}
+
 static double linesearch(const orc_tables *t, double *alpha, const double *d, double f,
                          const double *g, const double *u, double *alpha_old, const orc_options *o,
                          double fscale)
@@ -798,7 +813,10 @@ static int opt_scaled(const orc_tables *t, double *u, double *alpha, const orc_o
         trsv_lower_t(nm, chol, d);
         for (int i = 0; i < nm; i++) d[i] = -d[i];
         err = nrm2(nm, g);
-        if (err <= rtol) {
+        /* the supplementary criterion gamma <= tolGamma (opt.cpp:288-293) */
+        int gamma_ok = 1;
+        if (o->tol_gamma > 0.0) gamma_ok = modcfl(nm, d, o->moment_type) <= o->tol_gamma;
+        if (err <= rtol && gamma_ok) {
             if (o->moment_type == ORC_MOMENT_MN) {             /* :296-303 */
                 double u0 = u[0] + g[0];
                 for (int i = 0; i < nm; i++) {
@@ -1468,6 +1486,9 @@ int orc_solver_solve_all(orc_solver *s)
     o.line_search = s->deck.line_search;
     o.moment_type = s->deck.moment_type;
     o.delta_ppn = s->deck.delta_ppn;
+    o.max_bfgs_iter = 0;
+    o.cond_h_max_bfgs = 0.0;
+    o.tol_gamma = s->deck.tol_gamma;
     int ncell = s->NX * s->NY, nm = s->nm;
     int status = ORC_OK;
     int maxit = s->stats.max_iter;

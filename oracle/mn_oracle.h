@@ -37,6 +37,9 @@ typedef struct {
     double delta_ppn;  /* PPn regularization delta (opt.h:21) */
     int max_bfgs_iter;       /* optbfgs only: inner iterations per basis (opt.h:15) */
     double cond_h_max_bfgs;  /* optbfgs only: bound on cond(H) cond(H^-1) (opt.h:20) */
+    double tol_gamma;  /* opt only: the supplementary stop test gamma = modcfl(d) <= tol_gamma
+                          (opt.cpp:52-76,288-293, commented out in the reference's current
+                          source; momopt_update.cpp:226 sets 1.1); <= 0 switches it off */
 } orc_options;
 
 enum { ORC_MOMENT_MN = 0, ORC_MOMENT_PPN = 1 };
@@ -167,6 +170,7 @@ typedef struct {
     int line_search;
     int moment_type;    /* MOMOPT_MOMENT_TYPE (momopt_init.cpp:67) */
     double delta_ppn;   /* MOMOPT_DELTA_PPN (momopt_init.cpp:66) */
+    double tol_gamma;   /* orc_options.tol_gamma (momopt_update.cpp:226); <= 0: off */
 } orc_deck;
 
 /* Mirrors OPTIMIZATION_STATS (momopt_solver.h:33-41). */

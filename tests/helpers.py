@@ -57,10 +57,13 @@ def mass(dens, dx, dy):
     return float(dens.sum() * dx * dy)    # util/formats.py:88-93
 
 
+# The supplementary stop test's bound the fixtures were written with
+# (momopt_update.cpp:226, opt.cpp:288-293).
+TOL_GAMMA = 1.1
 # The regression decks of site_scons/testing.py:120-146 with util/decks.py defaults.
 FIXTURE_BASE = dict(num_cells_x=36, num_cells_y=74, a_x=-1.55, b_x=1.45, a_y=-1.62, b_y=1.38,
                     floor_value=1e-4, sigma=1.111, moment_order=3, quad_order=30, cfl_factor=0.9,
-                    tol=1e-4, cond_h_max=1e10, max_iter=100, use_gaunt=1, theta=2.0)
+                    tol=1e-4, cond_h_max=1e10, max_iter=100, use_gaunt=1, theta=2.0, tol_gamma=TOL_GAMMA)
 FIXTURE_IC = {"delta": (0, 0.0), "gaussian": (0, 1.7), "lattice": (1, 1.7), "smooth": (2, 1.7)}
 FIXTURE_T_FINAL = 1.03
 
@@ -90,7 +93,10 @@ def deck_text(d, t_final, extra=None, out_dt=1000.1, opt_type=0):
              "MOMOPT_COND_H_MAX %.17f" % d["cond_h_max"], "MOMOPT_COND_H_MAX_BFGS 20",
              "MOMOPT_MAX_ITER %d" % d["max_iter"], "MOMOPT_MAX_BFGS_ITER 5",
              "MOMOPT_USE_GAUNT %d" % d["use_gaunt"], "MOMOPT_THETA %.17f" % d["theta"],
-             "MOMOPT_DELTA_PPN %.17f" % 1e-10, "MOMOPT_MOMENT_TYPE 0", "MOMOPT_OPTIMIZATION_TYPE %d" % opt_type]
+             "MOMOPT_DELTA_PPN %.17f" % d.get("delta_ppn", 1e-10), "MOMOPT_MOMENT_TYPE %d" % d.get("moment_type", 0),
+             "MOMOPT_OPTIMIZATION_TYPE %d" % opt_type]
+    if d.get("tol_gamma", 0.0) > 0.0 and opt_type == 0:
+        lines.append("MOMOPT_TOL_GAMMA %.17f" % d["tol_gamma"])
     for k, v in (extra or {}).items():
         lines.append("%s %s" % (k, v))
     return "\n".join(lines) + "\n"
@@ -109,8 +115,11 @@ def gpu_grid_from_deck(mnc, gi, d, device=0):
 
 
 def mn_options(mnc, d, **over):
-    return mnc.options(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"],
-                       line_search=d.get("line_search", 0), delta_ppn=d.get("delta_ppn", 0.0), **over)
+    kw = dict(max_iter=d["max_iter"], tol=d["tol"], cond_h_max=d["cond_h_max"],
+              line_search=d.get("line_search", 0), delta_ppn=d.get("delta_ppn", 0.0),
+              tol_gamma=d.get("tol_gamma", 0.0))
+    kw.update(over)
+    return mnc.options(**kw)
 
 
 def config2_batch(w, p, xi, eta, nm, n, rng, near_boundary_frac=0.2, kappa=400.0, scale_k=3.0):

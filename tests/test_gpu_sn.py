@@ -114,3 +114,29 @@ def test_closures_converge_towards_sn_truth():
     print("density rel-L2 against S30: M1 %.3e, M3 %.3e" % (e1, e3))
     assert e3 < 0.25 * e1
     assert e3 < 5e-2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ic", ["gaussian", "smooth", "delta", "lattice"])
+def test_sn_fixture_regression(ic):
+    """mnc_sn_run on the reference's kinetic regression deck (36 x 74, Q = 12,
+    CFL 0.9, 84 steps including the final partial one, to t = 1.03) against
+    the reference's own .sn fixture: density with the file's weights at the
+    reference's 1e-10, and every ordinate."""
+    from test_oracle_sn_fixtures import FIXTURE_IC, SN_Q, read_sn, sn_density, sn_fixture_case, sn_fixture_path
+    from helpers import rel_l2
+    mnc = load_pkg("mnc")
+    print("case sn fixture ic=%s" % ic, flush=True)
+    g, t, f, steps = sn_fixture_case(ic)
+    full = [dt for dt in steps if dt == steps[0]]
+    assert len(full) == len(steps) - 1
+    f, _ = mnc.sn_run(f, g.sigma_t, g.sigma_s, SN_Q, g.dx, g.dy, g.init_x, g.init_y, FIXTURE_IC[ic][0], steps[0],
+                      len(full))
+    f, _ = mnc.sn_run(f, g.sigma_t, g.sigma_s, SN_Q, g.dx, g.dy, g.init_x, g.init_y, FIXTURE_IC[ic][0], steps[-1], 1)
+    h, w, ref = read_sn(sn_fixture_path(ic))
+    fin = f[2:-2, 2:-2]
+    err = rel_l2(sn_density(fin, w), sn_density(ref, w))
+    per = np.abs(fin - ref).max() / np.abs(ref).max()
+    print("GPU Sn %s: density rel L2 %.3e, per-ordinate max rel %.3e" % (ic, err, per))
+    assert err < 1e-10
+    assert per < 1e-12

@@ -1,7 +1,8 @@
 """The oracle's Sn discrete-ordinates step (kinetic/kinetic_update.cpp:41-235,
 kinetic_boundaries.cpp:190-223), the independent truth for the moment
-closures.  The reference holds no Sn fixture the build can use, so the
-restatement is pinned by what the mathematics fixes:
+closures.  Beside the reference's four kinetic fixtures
+(tests/test_oracle_sn_fixtures.py), the restatement is pinned by what the
+mathematics fixes:
 
 - a uniform isotropic state with sigma_t = sigma_s on the periodic grid is
   steady (the ordinates' weights sum to 4 pi, the fluxes vanish);
