@@ -46,6 +46,7 @@ struct mnc_ctx {
     double *d_gz = nullptr, *d_gg = nullptr, *d_gh = nullptr;
     size_t state_cap = 0;
     int state_nm = 0;
+    void *d_noalloc = nullptr;       // the library's fixed scratch array on this device (not owned)
     // tuning
     int rounds = 6;
     int tail_blocks = 0;             // 0: 2 per CU
@@ -275,8 +276,11 @@ int mlp_predict_dev(mnc_ctx *ctx, int n, const double *us, double *alpha) {
 // learned_gate_log10 at this value switches the network off
 constexpr int kGateOff = 5;
 
+// na: the no-allocation entry point's state regions (the library's fixed
+// scratch array) in place of the context's buffers; then nothing is allocated
+// and no timing events are recorded.
 int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_options *o, mnc_outs *outs,
-               const double *alpha_guess, bool check) {
+               const double *alpha_guess, bool check, const mnc::NoAllocState *na = nullptr) {
     if (ncells <= 0) return MNC_OK;
     if (o->opt_type != MNC_OPT_NEWTON)
         return ctx_fail(ctx, MNC_ERR_ARG, "batch solves run the Newton optimiser; the change-of-basis batch is "
@@ -288,12 +292,13 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     if (ctx->has_g64 && (mixed || o->warm_start == MNC_WARM_LEARNED))
         return ctx_fail(ctx, MNC_ERR_ARG, "moment order 8 runs the FP64 form only");
     const bool learned = o->warm_start == MNC_WARM_LEARNED;
-    int st = ensure_state(ctx, size_t(ncells));
+    if (na && learned) return ctx_fail(ctx, MNC_ERR_ARG, "the learned warm start allocates the network's buffers");
+    int st = na ? MNC_OK : ensure_state(ctx, size_t(ncells));
     if (st) return st;
-    const bool timed = true;
-    double *st_d = ctx->d_std;
-    int *st_i = ctx->d_sti, *lists = ctx->d_lists, *ctl = ctx->d_ctl;
-    const size_t cap = ctx->state_cap;
+    const bool timed = na == nullptr;
+    double *st_d = na ? na->st_d : ctx->d_std;
+    int *st_i = na ? na->st_i : ctx->d_sti, *lists = na ? na->lists : ctx->d_lists, *ctl = na ? na->ctl : ctx->d_ctl;
+    const size_t cap = na ? na->cap : ctx->state_cap;
     if (learned && ctx->learned_gate_log10 < kGateOff) {   // buffers of the network's guesses
         if ((st = mlp_check(ctx, true))) return st;
         if (size_t(ncells) > ctx->guess_cap) {
@@ -574,6 +579,12 @@ int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return MNC_ERR_HIP; }
+    // the no-allocation entry point's scratch array: its address on this
+    // device, resolved once per device (the device is current)
+    if (mnc::noalloc_scratch_address(ctx->device, &ctx->d_noalloc) != hipSuccess) {
+        mnc_destroy(ctx);
+        return MNC_ERR_HIP;
+    }
     const auto &t = ctx->tab;
     auto up = [&](auto **dst, const auto &vec) -> int {
         using T = typename std::remove_reference<decltype(vec)>::type::value_type;
@@ -669,6 +680,23 @@ int mnc_opt_batch(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_
     if (st) return st;
     MNC_HIP_CHECK(hipSetDevice(ctx->device));
     return run_newton(ctx, ncells, u, alpha, opt, outs, nullptr, true);
+}
+
+int mnc_opt_batch_noalloc(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_options *opt,
+                          mnc_outs *outs) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (ncells < 0 || (ncells > 0 && (!u || !alpha))) return ctx_fail(ctx, MNC_ERR_ARG, "bad batch arguments");
+    int st = check_opts(ctx, opt);
+    if (st) return st;
+    if (ncells > MNC_NOALLOC_MAX_CELLS)
+        return ctx_fail(ctx, MNC_ERR_ARG, "mnc_opt_batch_noalloc takes at most MNC_NOALLOC_MAX_CELLS (16384) cells");
+    if (ctx->has_g64 || ctx->tab.nm > 15)
+        return ctx_fail(ctx, MNC_ERR_ARG, "mnc_opt_batch_noalloc serves moment orders up to 4");
+    mnc::NoAllocState na;
+    if (!ctx->d_noalloc || !mnc::noalloc_layout(ctx->tab.nm, ctx->tab.np2, ctx->d_noalloc, na))
+        return ctx_fail(ctx, MNC_ERR_ARG, "this order's Newton state does not fit the no-allocation scratch");
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    return run_newton(ctx, ncells, u, alpha, opt, outs, nullptr, true, &na);
 }
 
 int mnc_opt_batch_host(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_options *opt, mnc_outs *outs) {

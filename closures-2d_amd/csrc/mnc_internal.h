@@ -148,6 +148,23 @@ bool launch_newton_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, hipSt
 // Returns false for an order without an instance.
 bool launch_newton_g64_init(const NewtonArgs &a, int nm, int np2, hipStream_t s);
 bool launch_newton_g64_rounds(const NewtonArgs &a, int nm, int np2, int &lin, int rounds, int blocks, hipStream_t s);
+// Scratch of the no-allocation entry point (mnc_opt_batch_noalloc): a
+// file-scope device array in mnc_newton.hip with the Newton state, lists and
+// control words of up to kNoAllocCells cells of moment order <= 4.
+constexpr int kNoAllocCells = MNC_NOALLOC_MAX_CELLS;
+struct NoAllocState {
+    double *st_d = nullptr;
+    int *st_i = nullptr, *lists = nullptr, *ctl = nullptr;
+    size_t cap = 0;
+};
+size_t noalloc_scratch_bytes();
+// The array's address on `device`, which must be current: resolved with
+// hipGetSymbolAddress on the first call for that device and kept per device
+// (each GPU holds its own copy of the array at its own address).
+hipError_t noalloc_scratch_address(int device, void **ptr);
+// The regions of the array at device address base for order (nm, np2); false
+// when that order's state does not fit.
+bool noalloc_layout(int nm, int np2, void *base, NoAllocState &out);
 constexpr int kMixMaxTiles = 16;   // quadrature tiles of 16 points held in LDS (nq <= 256)
 constexpr int kMixParts = 3;       // bf16 parts per table value in the mixed sweep's Hessian table
 int mix_tiles(int nm);             // 16-column tiles of the order-2N moments the mixed kernel expects

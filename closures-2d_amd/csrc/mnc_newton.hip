@@ -18,6 +18,7 @@
 // or LDS: unblocked column Cholesky, Higham's 1-norm estimator (ACM TOMS 14(4),
 // 1988), forward/back substitution.
 #include <cfloat>
+#include <mutex>
 #include <type_traits>
 
 #include "mnc_internal.h"
@@ -2397,6 +2398,62 @@ bool launch_newton_g64_rounds(const NewtonArgs &a, int nm, int np2, int &lin, in
         lin ^= 1;
     }
     return true;
+}
+
+// ---- the no-allocation entry point's scratch ------------------------------------
+//
+// One file-scope device array per GPU holds, for up to kNoAllocCells cells,
+// what mnc_opt_batch keeps in the context's lazily grown buffers: the per-cell
+// Newton state (Store, column stride kNoAllocCells), the kNumLists active lists
+// and the control words.  Sized for the largest order it serves (M4, both
+// Hessian forms have the same store).  Host code reaches it only through the
+// address hipGetSymbolAddress returns on each device; the array's name on the
+// host is not a device address.
+namespace {
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t noalloc_bytes(int dfields, int ifields) {
+    return align256(size_t(dfields) * kNoAllocCells * sizeof(double)) +
+           align256(size_t(ifields) * kNoAllocCells * sizeof(int)) +
+           align256(size_t(kNumLists) * kNoAllocCells * sizeof(int)) + align256(kCtlWords * sizeof(int));
+}
+static_assert(Store<15, 15>::dfields == Store<15, 45>::dfields, "M4 stores");
+constexpr size_t kNoAllocBytes = noalloc_bytes(Store<15, 45>::dfields, Store<15, 45>::ifields);
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+size_t noalloc_scratch_bytes() { return kNoAllocBytes; }
+
+bool noalloc_layout(int nm, int np2, void *base, NoAllocState &out) {
+    int df = 0, ifl = 0;
+    newton_store_fields(nm, np2, &df, &ifl);
+    if (noalloc_bytes(df, ifl) > kNoAllocBytes) return false;
+    unsigned char *p = static_cast<unsigned char *>(base);
+    out.st_d = reinterpret_cast<double *>(p);
+    p += align256(size_t(df) * kNoAllocCells * sizeof(double));
+    out.st_i = reinterpret_cast<int *>(p);
+    p += align256(size_t(ifl) * kNoAllocCells * sizeof(int));
+    out.lists = reinterpret_cast<int *>(p);
+    p += align256(size_t(kNumLists) * kNoAllocCells * sizeof(int));
+    out.ctl = reinterpret_cast<int *>(p);
+    out.cap = kNoAllocCells;
+    return true;
+}
+
+hipError_t noalloc_scratch_address(int device, void **ptr) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void *addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = p;
+    }
+    *ptr = addr[device];
+    return hipSuccess;
 }
 
 bool launch_newton_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, hipStream_t s) {

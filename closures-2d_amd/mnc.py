@@ -18,6 +18,7 @@ OK, ERR_ARG, ERR_U0, ERR_MAXREG, ERR_HIP, ERR_STATE = 0, 1, 2, 3, 4, 5
 PREC_FP64, PREC_MIXED = 0, 1
 LS_REFERENCE, LS_ROBUST = 0, 1
 WARM_PREVIOUS, WARM_ISOTROPIC, WARM_LEARNED = 0, 1, 2
+NOALLOC_MAX_CELLS = 16384   # MNC_NOALLOC_MAX_CELLS
 OPT_NEWTON, OPT_CHANGE_BASIS, OPT_BFGS = 0, 1, 2
 MOMENT_MN, MOMENT_PPN = 0, 1
 KCLASS_NEWTON, KCLASS_FLUX, KCLASS_OTHER, KCLASS_MLP = 0, 1, 2, 3
@@ -86,6 +87,7 @@ def lib():
         L.mnc_get_tables.argtypes = [vp, _dp, _dp, _dp, _dp]
         L.mnc_get_gaunt.argtypes = [vp, _ip, _ip, _dp]
         L.mnc_opt_batch.argtypes = [vp, C.c_int, vp, vp, C.POINTER(Options), vp]
+        L.mnc_opt_batch_noalloc.argtypes = [vp, C.c_int, vp, vp, C.POINTER(Options), vp]
         L.mnc_opt_batch_host.argtypes = [vp, C.c_int, _dp, _dp, C.POINTER(Options), C.POINTER(Outs)]
         L.mnc_optcb_batch_host.argtypes = [vp, C.c_int, _dp, _dp, _dp, _dp, C.POINTER(Options), C.POINTER(Outs)]
         L.mnc_batch_set.argtypes = [vp, C.c_int, _dp, _dp]

@@ -174,6 +174,22 @@ int mnc_opt_batch(mnc_ctx *ctx, int ncells, double *u, double *alpha,
  * reference's c_moments / c_alpha host buffers. */
 int mnc_opt_batch_host(mnc_ctx *ctx, int ncells, double *u, double *alpha,
                        const mnc_options *opt, mnc_outs *outs);
+/* No counterpart in the reference: mnc_opt_batch for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same device arrays, options, results and
+ * errors as mnc_opt_batch, but the per-cell Newton state, the active lists and
+ * the control words live in one fixed device array of the library (about
+ * 27 MB, one copy per GPU, held until the process ends) instead of the
+ * context's buffers, and the call makes no hipMalloc or hipFree.  Limits:
+ * ncells <= MNC_NOALLOC_MAX_CELLS, moment order <= 4, warm start previous or
+ * isotropic (the learned start allocates the network's buffers), no per-launch
+ * timing; otherwise MNC_ERR_ARG.  Every call on one GPU, from any context,
+ * shares that array and the call returns after its kernels finish: calls that
+ * overlap on one GPU (from several host threads) race, so callers serialise
+ * them.  Statistics accumulate in the context as for mnc_opt_batch. */
+#define MNC_NOALLOC_MAX_CELLS 16384
+int mnc_opt_batch_noalloc(mnc_ctx *ctx, int ncells, double *u, double *alpha,
+                          const mnc_options *opt, mnc_outs *outs);
 /* The change-of-basis optimiser on a batch, HOST arrays (copies in and out):
  * replaces the per-cell optcb() loop that solveOptimization runs for
  * MOMOPT_OPTIMIZATION_TYPE 1 (momopt_update.cpp:240-243 -> optcb.cpp:90-415).
