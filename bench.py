@@ -168,14 +168,16 @@ FORCE_TUNE = {}
 TUNE_STEPS = 10           # steps per autotuning trial
 
 
-def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=None, autotune=True, tune_step=None):
+def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=None, autotune=True, tune_step=None,
+             full_run=True):
     """W warm-up steps, then (untimed) on to step `tune_step` (default: the
     middle of the run) and the library's autotuner on that state, then (from
     t = 0 again) K steps timed by wall clock with the tuned launch shape
     (hipGraph replay if the tuner picked it; per-launch timing off), then the
     same K steps (or the first kernel_steps of them) again from t = 0, eagerly,
     with HIP-event timing of every launch for the per-kernel-class split and
-    the roofline."""
+    the roofline; then (full_run) the whole run to t = 3.2 from t = 0 again,
+    wall-timed with the tuned settings."""
     ctx, g = make_grid(mnc, gi, d)
     if mlp_params is not None:
         ctx.mlp_set_params(mlp_params)
@@ -212,21 +214,47 @@ def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=None,
     ctx.sync()
     kst = ctx.stats()
     kn, kms = ctx.timing_get(mnc.KCLASS_NEWTON)
+    tn, tms = ctx.timing_get(mnc.KCLASS_TAIL)
     fn, fms = ctx.timing_get(mnc.KCLASS_FLUX)
     on, oms = ctx.timing_get(mnc.KCLASS_OTHER)
     mn, mms = ctx.timing_get(mnc.KCLASS_MLP)
     ctx.timing_enable(False)
+    full = None
+    if full_run:
+        # the whole run to t = 3.2 from t = 0 with the same tuned settings
+        ctx.grid_set(g.moments, g.alpha)
+        ctx.reset_stats()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for dt in dts:
+            ctx.update(dt, opt)
+        ctx.sync()
+        fwall = time.perf_counter() - t0
+        fst = ctx.stats()
+        fcl = 2 * (g.nx + 4) * (g.ny + 4) * len(dts)
+        assert fst.num_dual_solves == fcl, (fst.num_dual_solves, fcl)
+        full = {"steps": len(dts), "t_final": T_FINAL, "wall_s": fwall, "closures": fcl,
+                "closures_per_s": fcl / fwall, "s_per_timestep": fwall / len(dts),
+                "newton_iters_per_closure": fst.iter_sum / fst.num_dual_solves,
+                "trials_per_closure": fst.trial_sum / fst.num_dual_solves, "max_iter": fst.max_iter,
+                "reg_hist": list(fst.hist_reg), "warm_pick_mlp_iso_prev": list(fst.warm_pick),
+                "tail_cells_per_stage": fst.tail_cells / (2.0 * len(dts)),
+                "tail_iters_per_tail_cell": fst.tail_iter_sum / max(fst.tail_cells, 1)}
+        progress("full run: %d steps in %.2f s" % (len(dts), fwall))
     ctx.close()
     closures = 2 * (g.nx + 4) * (g.ny + 4) * steps
     assert st.num_dual_solves == closures, (st.num_dual_solves, closures)
-    return dict(wall=wall, stats=st, kstats=kst, kernel_steps=ks, newton=(kn, kms), flux=(fn, fms), other=(on, oms),
-                mlp=(mn, mms), closures=closures, moments=m, dts=dts[:steps], tuned=tuned, tune_report=report)
+    return dict(wall=wall, stats=st, kstats=kst, kernel_steps=ks, newton=(kn, kms), tail=(tn, tms), flux=(fn, fms),
+                other=(on, oms), mlp=(mn, mms), closures=closures, moments=m, dts=dts[:steps], tuned=tuned,
+                tune_report=report, full_run=full)
 
 
 def form_summary(r, f_iter, f_trial):
     st, kst = r["stats"], r["kstats"]
-    kn, kms = r["newton"]
+    kms = r["newton"][1] + r["tail"][1]
+    stages = 2 * r["kernel_steps"]
     return {"closures_per_s": r["closures"] / r["wall"], "s_per_timestep": r["wall"] / len(r["dts"]),
+            "timed_steps": len(r["dts"]), "full_run": r["full_run"],
             "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
             "trials_per_closure": st.trial_sum / st.num_dual_solves, "max_iter": st.max_iter,
             "reg_hist": list(st.hist_reg), "warm_pick_mlp_iso_prev": list(st.warm_pick),
@@ -234,7 +262,11 @@ def form_summary(r, f_iter, f_trial):
             "tuned": r["tuned"],
             "autotune_trials": r["tune_report"],
             "kernel_timed_steps": r["kernel_steps"],
-            "kernel_ms": {"newton": kms, "flux": r["flux"][1], "other": r["other"][1], "mlp": r["mlp"][1]},
+            "kernel_ms": {"newton_rounds": r["newton"][1], "tail": r["tail"][1], "flux": r["flux"][1],
+                          "other": r["other"][1], "mlp": r["mlp"][1]},
+            "tail_cells_per_stage": kst.tail_cells / max(stages, 1),
+            "tail_iters_per_tail_cell": kst.tail_iter_sum / max(kst.tail_cells, 1),
+            "tail_trials_per_tail_cell": kst.tail_trial_sum / max(kst.tail_cells, 1),
             "newton_tflops_fp64_equiv": (kst.iter_sum * f_iter + kst.trial_sum * f_trial) / (kms * 1e-3) / 1e12
             if kms > 0 else 0.0}
 
@@ -257,8 +289,9 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
     progress("isolated batch generated (%.1f s)" % gen_s)
     tab = oracle.Tables(4, 16, False)
     m = oracle_sample
-    sto, uo, ao, ito, ro, _ = tab.opt_batch(u[:m], a0[:m], max_iter=100, tol=1e-8, cond_h_max=1e10,
-                                            nthreads=len(os.sched_getaffinity(0)))
+    if m > 0:
+        sto, uo, ao, ito, ro, _ = tab.opt_batch(u[:m], a0[:m], max_iter=100, tol=1e-8, cond_h_max=1e10,
+                                                nthreads=len(os.sched_getaffinity(0)))
     # learned form: network trained on separate exact (u, alpha*) pairs of the same distribution
     ut, at, lt = wl.isolated_batch(w, p, xi, eta, 15, 2_000_000, seed + 1, near_boundary_frac=0.0)
     ust, ast_ = mnc.scaled_pairs(ut, at)
@@ -279,14 +312,25 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
             if rep_ > 0 and (best is None or dt < best):
                 best = dt
         st = ctx.stats()
-        ug = ctx.batch_get()[0][:m]
+        ug, _, itg, rg = ctx.batch_get()
         progress("isolated %s: %.3f s" % (name, best))
-        err = np.linalg.norm(ug - uo, axis=1) / np.linalg.norm(uo, axis=1)
         res[name] = {"closures_per_s": n / best, "s": best,
                      "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
                      "trials_per_closure": st.trial_sum / st.num_dual_solves,
                      "reg_hist": list(st.hist_reg), "warm_pick_mlp_iso_prev": list(st.warm_pick),
-                     "oracle_max_rel_u_err": float(err.max()), "oracle_median_rel_u_err": float(np.median(err))}
+                     "tail_cells": st.tail_cells, "tail_iters_per_tail_cell": st.tail_iter_sum / max(st.tail_cells, 1)}
+        if m > 0:
+            # per cell against the oracle: cells at the same regularization level
+            # solved the same problem to the tolerance; a different level is a
+            # discrete decision of the reference's algorithm taken the other way
+            err = np.linalg.norm(ug[:m] - uo, axis=1) / np.linalg.norm(uo, axis=1)
+            same_r = rg[:m] == ro
+            res[name].update({
+                "oracle_same_r_frac": float(same_r.mean()),
+                "oracle_same_path_frac": float((same_r & (itg[:m] == ito)).mean()),
+                "oracle_same_r_max_rel_u_err": float(err[same_r].max()),
+                "oracle_median_rel_u_err": float(np.median(err)),
+                "oracle_max_rel_u_err": float(err.max())})
     ctx.close()
     return {"workload": "isolated M4 batch, %d vectors (20%% near-boundary lobes), Q=16, isotropic start, "
                         "TOL 1e-8, direct Hessian, reference line search" % n,
@@ -428,6 +472,20 @@ def bench_m8(mnc, gi, steps, n=512, sample=1000):
     wall = time.perf_counter() - t0
     st = ctx.stats()
     m, a = ctx.grid_get()
+    # the same steps again from the start with HIP events around the Newton
+    # launches (the FP64 matrix-core rounds), for the roofline
+    ctx.grid_set(g.moments, g.alpha)
+    ctx.reset_stats()
+    ctx.timing_enable(True)
+    for _ in range(steps):
+        ctx.update(g.max_dt, opt)
+    ctx.sync()
+    kst = ctx.stats()
+    kn, kms = ctx.timing_get(mnc.KCLASS_NEWTON)
+    ctx.timing_enable(False)
+    f_iter, f_trial = flops_per_iter(45, 32 * 32)
+    m8_flops = kst.iter_sum * f_iter + kst.trial_sum * f_trial
+    m8_achieved = m8_flops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
     # closures of `sample` cells of the final state, GPU vs oracle, from the stored alpha
     rng = np.random.default_rng(5)
     cells = rng.choice((n + 4) * (n + 4), sample, replace=False)
@@ -447,7 +505,15 @@ def bench_m8(mnc, gi, steps, n=512, sample=1000):
             "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
             "trials_per_closure": st.trial_sum / st.num_dual_solves, "reg_hist": list(st.hist_reg),
             "oracle_sample": sample, "oracle_max_rel_u_err": float(err.max()),
-            "oracle_same_iterations": float(np.mean(itg == ito))}
+            "oracle_same_iterations": float(np.mean(itg == ito)),
+            "roofline": {"bound": "mfma", "achieved": m8_achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": m8_achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                         "kernel": "Newton solve per stage (sweep_g64_kernel + logic_g64_kernel rounds), HIP events "
+                                   "over the %d steps run again; flops: iterations x %.3f Mflop + trials x %.1f kflop "
+                                   "(the reference's direct-Hessian FP64 count at n = 45, nq = 1024)"
+                                   % (steps, f_iter / 1e6, f_trial / 1e3),
+                         "algorithmic_flops_per_launch": m8_flops / max(2 * steps, 1),
+                         "avg_launch_ms": kms / max(2 * steps, 1)}}
 
 
 def bench_ppn(mnc, gi, d, steps, warmup, cpu_seconds):
@@ -461,15 +527,15 @@ def bench_ppn(mnc, gi, d, steps, warmup, cpu_seconds):
     f_iter, f_trial = flops_per_iter_ppn(n, dp["quad_order"] ** 2)
     opt = mnc.options(max_iter=dp["max_iter"], tol=dp["tol"], cond_h_max=dp["cond_h_max"],
                       line_search=mnc.LS_ROBUST, delta_ppn=dp["delta_ppn"])
-    r = run_form(mnc, gi, dp, opt, steps, warmup)
+    r = run_form(mnc, gi, dp, opt, steps, warmup, full_run=False)
     st, kst = r["stats"], r["kstats"]
-    kn, kms = r["newton"]
+    kms = r["newton"][1] + r["tail"][1]
     achieved = (kst.iter_sum * f_iter + kst.trial_sum * f_trial) / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
     out = {"closures_per_s": r["closures"] / r["wall"], "s_per_timestep": r["wall"] / len(r["dts"]),
            "steps": len(r["dts"]), "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
            "trials_per_closure": st.trial_sum / st.num_dual_solves, "max_iter": st.max_iter,
            "reg_hist": list(st.hist_reg), "tuned": r["tuned"],
-           "kernel_ms": {"newton": kms, "flux": r["flux"][1]},
+           "kernel_ms": {"newton_rounds": r["newton"][1], "tail": r["tail"][1], "flux": r["flux"][1]},
            "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": achieved / FP64_PEAK_TFLOPS,
                         "flops_per_iteration": f_iter, "flops_per_trial": f_trial}}
@@ -481,13 +547,15 @@ def bench_ppn(mnc, gi, d, steps, warmup, cpu_seconds):
     return out
 
 
-def cpu_baseline(d, seconds):
+def cpu_baseline(d, seconds, nthreads=None):
     """The CPU oracle (FP64 C, OpenMP schedule(dynamic,1) over cells, as the
     reference's solveOptimization) on the first steps of the same run, for
-    about `seconds` of CPU time."""
+    about `seconds` of wall clock (at least one step), on nthreads threads
+    (default: all the cores this process may run on)."""
     sys.path.insert(0, ROOT)
     import oracle
-    nthreads = len(os.sched_getaffinity(0))
+    if nthreads is None:
+        nthreads = len(os.sched_getaffinity(0))
     s = oracle.Solver(d, nthreads=nthreads)
     import importlib
     gi = _load("grid_init")
@@ -564,6 +632,9 @@ def main():
     ap.add_argument("--ppn-steps", type=int, default=569, help="steps of the PPn lattice form (0 skips it)")
     ap.add_argument("--ppn-only", action="store_true", help="only the PPn lattice form (for profiling)")
     ap.add_argument("--sn-only", action="store_true", help="only the Sn solver leg")
+    ap.add_argument("--isolated-only", action="store_true",
+                    help="only the isolated closure batch (the three forms, for profiling)")
+    ap.add_argument("--m8-only", action="store_true", help="only the M8 512^2 leg (for profiling)")
     ap.add_argument("--optcb-only", action="store_true",
                     help="only the change-of-basis optimiser's isolated batch (for profiling)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -595,6 +666,18 @@ def main():
         print(json.dumps({"metric": "converged optcb closures/s (isolated M4 batch, TOL 1e-8)",
                           "value": r["closures_per_s"], "unit": "closures/s", "n_gpus": 1,
                           "higher_is_better": True, "config": {"optcb": r, "optbfgs": rb}}))
+        return
+    if args.isolated_only:
+        r = bench_isolated(mnc, args.isolated, 12345, 0 if args.no_cpu else 20000, args.train_steps)
+        print(json.dumps({"metric": "converged closures/s (isolated M4 batch, TOL 1e-8)",
+                          "value": r["forms"]["learned+mixed"]["closures_per_s"], "unit": "closures/s",
+                          "n_gpus": 1, "higher_is_better": True, "config": {"isolated": r}}))
+        return
+    if args.m8_only:
+        r = bench_m8(mnc, gi, args.m8_steps)
+        print(json.dumps({"metric": "converged M8 closures/s (512^2 periodic, Q=32, TOL 1e-8)",
+                          "value": r["closures_per_s"], "unit": "closures/s", "n_gpus": 1,
+                          "higher_is_better": True, "config": {"m8": r}}))
         return
     if args.sn_only:
         r = bench_sn(mnc)
@@ -636,17 +719,29 @@ def main():
     plain = forms.get("plain-fp64", forms[form])
     best = forms[form]
     value = best["closures"] / best["wall"]
-    kn, kms = best["newton"]
     kst = best["kstats"]
     stages = 2 * best["kernel_steps"]                 # two closure solves per Heun step
+    # the dominant kernel: the persistent tail (FP64, one wave per cell), its
+    # Newton iterations and Armijo trials counted on the device
+    tn, tms = best["tail"]
+    tail_flops = kst.tail_iter_sum * f_iter + kst.tail_trial_sum * f_trial
+    tail_achieved = tail_flops / (tms * 1e-3) / 1e12 if tms > 0 else 0.0
+    # the whole Newton stage (prologue, lockstep rounds and tail)
+    sms = best["newton"][1] + tms
     alg_flops = kst.iter_sum * f_iter + kst.trial_sum * f_trial
-    achieved = alg_flops / (kms * 1e-3) / 1e12 if kms > 0 else 0.0
+    achieved = alg_flops / (sms * 1e-3) / 1e12 if sms > 0 else 0.0
     # drift of each form's final state from the plain form's (density rel-L2)
     dens = {k: v["moments"][2:-2, 2:-2, 0] for k, v in forms.items()}
     ref = dens.get("plain-fp64", dens[form])
     drift = {k: float(np.sqrt(((dens[k] - ref) ** 2).sum() / (ref ** 2).sum())) for k in forms}
+    t_end = float(sum(best["dts"]))
+    net = ""
+    if form == "learned+mixed":
+        net = " (network %s)" % ("off: the autotuner's gate 5" if best["tuned"].get("learned_gate_log10") == 5
+                                 else "on")
     out = {
-        "metric": "converged Mn closures/s (M4 lattice 280^2, t=0..3.2, TOL 1e-8), learned+mixed vs plain",
+        "metric": "converged Mn closures/s, M4 lattice 280^2, steps 1-%d of 569 (t = 0 to %.4f), TOL 1e-8, "
+                  "%s%s vs plain FP64" % (args.steps, t_end, form, net),
         "value": value,
         "unit": "closures/s",
         "n_gpus": 1,
@@ -659,23 +754,40 @@ def main():
         "dtype": "f64" if form == "plain-fp64" else "f64+bf16",
         "data": "synthetic (the reference's lattice initial condition, generated in-process)",
         "config": {"workload": "M4 lattice 280x280, Q=16 (nq=256), 7x7 checkerboard on [-3.5,3.5]^2, "
-                               "floor 1e-10, TOL 1e-8, direct Hessian, robust line search, Heun dt=%.6g" % plain["dts"][0],
+                               "floor 1e-10, TOL 1e-8, direct Hessian, robust line search, Heun dt=%.6g; timed "
+                               "window: steps 1-%d of the 569 to t = 3.2 (t = 0 to %.4f); config.forms[*].full_run "
+                               "times all 569 steps" % (plain["dts"][0], args.steps, t_end),
                    "form": form, "closures_per_step": 2 * 284 * 284,
                    "s_per_timestep": best["wall"] / args.steps,
                    "plain_fp64_closures_per_s": plain["closures"] / plain["wall"],
                    "speedup_vs_plain_fp64": value / (plain["closures"] / plain["wall"]),
+                   "full_run_speedup_vs_plain_fp64":
+                       (best["full_run"]["closures_per_s"] / plain["full_run"]["closures_per_s"])
+                       if best["full_run"] and plain["full_run"] else None,
                    "forms": {k: form_summary(v, f_iter, f_trial) for k, v in forms.items()},
                    "final_density_rel_l2_vs_plain": drift,
-                   "mlp": mlp_info},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": stage_traffic(form),
-                     "kernel": "Newton solve per stage (init + lockstep rounds + tail), %s form, HIP events "
-                               "over the %d timed steps run again eagerly; flops are the reference algorithm's FP64 count "
-                               "(iterations x 73.0 kflop + trials x 8.2 kflop); traffic: HBM bytes per stage from "
-                               "calibrated FETCH_SIZE + WRITE_SIZE (profiles/newton_stage_traffic.md)"
-                               % (form, best["kernel_steps"]),
-                     "algorithmic_flops_per_launch": alg_flops / max(stages, 1),
-                     "avg_launch_ms": kms / max(stages, 1)},
+                   "mlp": mlp_info,
+                   "roofline_newton_stage": {
+                       "bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": achieved / FP64_PEAK_TFLOPS, "traffic": stage_traffic(form),
+                       "kernel": "Newton solve per stage (init + lockstep rounds + tail), %s form; flops are the "
+                                 "reference algorithm's FP64 count (iterations x %.1f kflop + trials x %.1f kflop); "
+                                 "traffic: HBM bytes per stage from calibrated FETCH_SIZE + WRITE_SIZE "
+                                 "(profiles/newton_stage_traffic.md)" % (form, f_iter / 1e3, f_trial / 1e3),
+                       "algorithmic_flops_per_launch": alg_flops / max(stages, 1),
+                       "avg_launch_ms": sms / max(stages, 1)}},
+        "roofline": {"bound": "mfma", "achieved": tail_achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": tail_achieved / FP64_PEAK_TFLOPS, "traffic": stage_traffic("tail-" + form),
+                     "kernel": "tail_coop_kernel (the persistent FP64 tail, one wave per cell still iterating "
+                               "after the lockstep rounds), %s form, HIP events on the context's stream over the %d "
+                               "timed steps run again eagerly; flops: the Newton iterations (x %.1f kflop) and "
+                               "Armijo trials (x %.1f kflop) done in the tail, counted on the device: %.1f cells, "
+                               "%.0f iterations and %.0f trials per launch"
+                               % (form, best["kernel_steps"], f_iter / 1e3, f_trial / 1e3,
+                                  kst.tail_cells / max(stages, 1), kst.tail_iter_sum / max(stages, 1),
+                                  kst.tail_trial_sum / max(stages, 1)),
+                     "algorithmic_flops_per_launch": tail_flops / max(tn, 1),
+                     "avg_launch_ms": tms / max(tn, 1)},
     }
     if args.isolated > 0 and not args.plain_only and not args.only:
         out["config"]["isolated"] = bench_isolated(mnc, args.isolated, 12345, 20000, args.train_steps)
@@ -695,10 +807,14 @@ def main():
     if not args.no_cpu:
         progress("CPU baseline")
         cb = cpu_baseline(d, args.cpu_seconds)
+        c1 = cpu_baseline(d, min(args.cpu_seconds, 8.0), nthreads=1)
         out["cpu_baseline"] = {"value": cb["value"], "unit": "closures/s", "cores": cb["cores"], "kind": "port",
                                "sample": "oracle (FP64 C restatement, OpenMP dynamic,1) on the first %d steps "
                                          "of the same lattice run from t=0 (%.1f s, %.2f Newton iterations per "
-                                         "closure)" % (cb["steps"], cb["wall"], cb["iters"])}
+                                         "closure)" % (cb["steps"], cb["wall"], cb["iters"]),
+                               "one_core": {"value": c1["value"], "unit": "closures/s", "cores": 1,
+                                            "sample": "the same oracle on one thread, first %d step(s) (%.1f s)"
+                                                      % (c1["steps"], c1["wall"])}}
     print(json.dumps(out))
 
 

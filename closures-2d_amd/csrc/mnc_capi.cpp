@@ -97,8 +97,8 @@ struct mnc_ctx {
     };
     StepGraph graphs[4];
     int graph_next = 0;
-    long long t_launch[MNC_NUM_KCLASS] = {0, 0, 0, 0};
-    double t_ms[MNC_NUM_KCLASS] = {0, 0, 0, 0};
+    long long t_launch[MNC_NUM_KCLASS] = {};
+    double t_ms[MNC_NUM_KCLASS] = {};
     std::string last_error;
 };
 
@@ -369,14 +369,31 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
         }
         return MNC_OK;
     };
+    // The prologue and lockstep rounds, then the persistent tail, each in its
+    // own timing class (the same launches as one plan).
+    auto launch_timed = [&](const mnc::NewtonArgs &aa, const mnc::LaunchPlan &p, bool reset) -> int {
+        mnc::LaunchPlan body = p, tail = p;
+        body.tail = false;
+        tail.init = false;
+        tail.r0 = tail.r1 = p.r1;
+        int s2;
+        {
+            Timed tm(ctx, MNC_KCLASS_NEWTON, timed);
+            if (reset) mnc::launch_ctl_reset(ctl, ctx->stream);
+            if ((s2 = launch(aa, body))) return s2;
+        }
+        if (p.tail) {
+            Timed tm(ctx, MNC_KCLASS_TAIL, timed);
+            if ((s2 = launch(aa, tail))) return s2;
+        }
+        return MNC_OK;
+    };
     if (learned && ctx->learned_gate_log10 >= kGateOff) {
         // network off: every cell starts from the alpha passed in (isotropicAlpha
         // if it is not finite), as the gate would decide at any threshold above
         // the gradient norms, without the network's or judge_kernel's launches
         if (t.nm != 15) return ctx_fail(ctx, MNC_ERR_ARG, "the learned warm start is for moment order 4");
-        Timed tm(ctx, MNC_KCLASS_NEWTON, timed);
-        mnc::launch_ctl_reset(ctl, ctx->stream);
-        if ((st = launch(a, lp))) return st;
+        if ((st = launch_timed(a, lp, true))) return st;
     } else if (learned) {
         // Round 1 evaluates every cell at its start and parks those failing the
         // gate on the network's list; the network runs on those rows only (the
@@ -404,12 +421,12 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
             Timed tm(ctx, MNC_KCLASS_NEWTON);
             if (!mnc::launch_newton_judge(a, t.nm, t.np2, gaunt, ctx->stream))
                 return ctx_fail(ctx, MNC_ERR_ARG, "no learned warm start for this order");
-            if ((st = launch(a, rest))) return st;
         }
-    } else {
+        if ((st = launch_timed(a, rest, false))) return st;
+    } else if (ctx->has_g64) {
         Timed tm(ctx, MNC_KCLASS_NEWTON, timed);
         mnc::launch_ctl_reset(ctl, ctx->stream);
-        if (ctx->has_g64) {
+        {
             // lockstep rounds in batches until the active list is empty (the
             // count is read back between batches, so this path is not captured)
             if (!mnc::launch_newton_g64_init(a, t.nm, t.np2, ctx->stream))
@@ -424,9 +441,9 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
                 MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
                 if (left == 0) break;
             }
-        } else if ((st = launch(a, lp))) {
-            return st;
         }
+    } else if ((st = launch_timed(a, lp, true))) {
+        return st;
     }
     MNC_HIP_CHECK(hipGetLastError());
     if (!check) return MNC_OK;
@@ -453,16 +470,25 @@ int cb_state_init(mnc_ctx *ctx) {
 // optcb over every cell of the grid (solveOptimization with
 // OPT_TYPE_CHANGE_BASIS, momopt_update.cpp:240-243): u = the moments, alphaM =
 // the grid's alpha, the basis state kept on the device between solves.
+// Allocates the change-of-basis state for nc cells and initialises it as
+// momopt_init.cpp:144-158 (first use, or a larger grid; never inside a capture).
+int cb_state_alloc(mnc_ctx *ctx, int nc) {
+    if (size_t(nc) <= ctx->cb_cells) return MNC_OK;
+    const int nm = ctx->tab.nm, nq = ctx->tab.nq;
+    int st;
+    ctx->cb_cells = size_t(nc);
+    ctx->cb_blocks = std::min(nc, 16 * ctx->num_cu);
+    if ((st = dalloc(ctx, &ctx->d_cbB, size_t(nc) * nm * nm)) || (st = dalloc(ctx, &ctx->d_cbaP, size_t(nc) * nm)) ||
+        (st = dalloc(ctx, &ctx->d_cbws, size_t(ctx->cb_blocks) * nq * nm)) || (st = cb_state_init(ctx)))
+        return st;
+    return MNC_OK;
+}
+
 int run_optcb_grid(mnc_ctx *ctx, int nc, const mnc_options *o, bool check) {
     const int nm = ctx->tab.nm, nq = ctx->tab.nq;
     int st;
-    if (size_t(nc) > ctx->cb_cells) {   // first use: allocate and initialise (never inside a capture)
-        ctx->cb_cells = size_t(nc);
-        ctx->cb_blocks = std::min(nc, 16 * ctx->num_cu);
-        if ((st = dalloc(ctx, &ctx->d_cbB, size_t(nc) * nm * nm)) || (st = dalloc(ctx, &ctx->d_cbaP, size_t(nc) * nm)) ||
-            (st = dalloc(ctx, &ctx->d_cbws, size_t(ctx->cb_blocks) * nq * nm)) || (st = cb_state_init(ctx)))
-            return st;
-    }
+    if ((st = cb_state_alloc(ctx, nc))) return st;
+    (void)nq;
     mnc::CbArgs a{};
     a.ncells = nc;
     a.nm = nm;
@@ -878,6 +904,38 @@ int mnc_grid_create(mnc_ctx *ctx, const mnc_grid_desc *grid, const double *momen
     MNC_HIP_CHECK(hipMemset(ctx->d_flux, 0, n * sizeof(double)));
     MNC_HIP_CHECK(hipMemset(ctx->d_old, 0, n * sizeof(double)));
     ctx->has_grid = true;
+    if (ctx->d_cbB) return cb_state_init(ctx);   // a new grid starts the change-of-basis state afresh
+    return MNC_OK;
+}
+
+int mnc_grid_get_basis(mnc_ctx *ctx, double *P2M, double *alphaP) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (!ctx->has_grid) return ctx_fail(ctx, MNC_ERR_STATE, "no grid");
+    const int nm = ctx->tab.nm;
+    const size_t nc = size_t(ctx->grid.num_cells_x + 4) * (ctx->grid.num_cells_y + 4);
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    int st;
+    if ((st = cb_state_alloc(ctx, int(nc)))) return st;   // before any change-of-basis solve: the initial state
+    if (P2M)
+        MNC_HIP_CHECK(hipMemcpyAsync(P2M, ctx->d_cbB, nc * nm * nm * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (alphaP)
+        MNC_HIP_CHECK(hipMemcpyAsync(alphaP, ctx->d_cbaP, nc * nm * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return MNC_OK;
+}
+
+int mnc_grid_set_basis(mnc_ctx *ctx, const double *P2M, const double *alphaP) {
+    if (!ctx) return MNC_ERR_ARG;
+    if (!ctx->has_grid) return ctx_fail(ctx, MNC_ERR_STATE, "no grid");
+    if (!P2M || !alphaP) return ctx_fail(ctx, MNC_ERR_ARG, "mnc_grid_set_basis needs both P2M and alphaP");
+    const int nm = ctx->tab.nm;
+    const size_t nc = size_t(ctx->grid.num_cells_x + 4) * (ctx->grid.num_cells_y + 4);
+    MNC_HIP_CHECK(hipSetDevice(ctx->device));
+    int st;
+    if ((st = cb_state_alloc(ctx, int(nc)))) return st;
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_cbB, P2M, nc * nm * nm * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    MNC_HIP_CHECK(hipMemcpyAsync(ctx->d_cbaP, alphaP, nc * nm * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    MNC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     return MNC_OK;
 }
 
@@ -1046,6 +1104,9 @@ int mnc_get_stats(mnc_ctx *ctx, mnc_stats *st) {
     st->trial_sum = h[8];
     for (int r = 0; r < 3; ++r) st->warm_pick[r] = h[9 + r];
     st->mixed_deferred = h[12];
+    st->tail_cells = h[13];
+    st->tail_iter_sum = h[14];
+    st->tail_trial_sum = h[15];
     return MNC_OK;
 }
 

@@ -11,6 +11,17 @@
 namespace mnc {
 namespace {
 
+// The left edge x0 + k h of a cell for the lattice source test |x| < 0.5
+// (momopt_update.cpp:328-334), rounded as a product and a sum, as the
+// reference's x86-64 build without FMA evaluates it: on grids whose edges
+// fall on +-0.5 exactly (280 cells on [-3.5, 3.5]: h = 0.025) a contracted
+// fma lands inside and adds a row and a column of source cells.
+// (__dadd_rn / __dmul_rn alone do not stop the contraction: the pragma does.)
+__device__ __forceinline__ double edge_coord(double x0, int k, double h) {
+#pragma clang fp contract(off)
+    return x0 + double(k) * h;
+}
+
 constexpr int kBlock = 256;
 
 // momopt_update.cpp:77-90 (SIGN/MIN/MAX of utils.h:28-30)
@@ -42,8 +53,8 @@ __device__ __forceinline__ void euler_epilogue(const GridArgs &g, int i, int j, 
 #pragma unroll
     for (int k = 1; k < NM; ++k) v[k] = u[k] * (1.0 - dt * st) - dt * F[k];
     if (g.init_cond == 1) {
-        const double x_i = g.init_x + (i - 2) * g.dx;
-        const double y_j = g.init_y + (j - 2) * g.dy;
+        const double x_i = edge_coord(g.init_x, i - 2, g.dx);
+        const double y_j = edge_coord(g.init_y, j - 2, g.dy);
         if (fabs(x_i) < 0.5 && fabs(y_j) < 0.5) v[0] = v[0] + dt * 2.0 * sqrt(M_PI);
     }
     if (stage == 2) {
@@ -152,8 +163,8 @@ __global__ __launch_bounds__(kBlock) void flux_euler_kernel(GridArgs g, int q0, 
 #pragma unroll
     for (int k = 1; k < NM; ++k) v[k] = u[k] * (1.0 - dt * st) - dt * F[k];
     if (g.init_cond == 1) {                                   // lattice source (:328-334)
-        const double x_i = g.init_x + (i - 2) * dx;
-        const double y_j = g.init_y + (j - 2) * dy;
+        const double x_i = edge_coord(g.init_x, i - 2, dx);
+        const double y_j = edge_coord(g.init_y, j - 2, dy);
         if (fabs(x_i) < 0.5 && fabs(y_j) < 0.5) v[0] = v[0] + dt * 2.0 * sqrt(M_PI);
     }
     if (stage == 2) {
@@ -252,8 +263,8 @@ __global__ __launch_bounds__(TX * TY) void flux_fused_kernel(GridArgs g, int sta
 #pragma unroll
     for (int k = 1; k < NM; ++k) v[k] = u[k] * (1.0 - dt * st) - dt * F[k];
     if (g.init_cond == 1) {                                   // lattice source (:328-334)
-        const double x_i = g.init_x + (i - 2) * dx;
-        const double y_j = g.init_y + (j - 2) * dy;
+        const double x_i = edge_coord(g.init_x, i - 2, dx);
+        const double y_j = edge_coord(g.init_y, j - 2, dy);
         if (fabs(x_i) < 0.5 && fabs(y_j) < 0.5) v[0] = v[0] + dt * 2.0 * sqrt(M_PI);
     }
     if (stage == 2) {

@@ -47,7 +47,9 @@ struct NewtonArgs {
     int warm;             // MNC_WARM_*
     long long *stats;     // [0] max iter, [1] iter sum, [2] solves, [3..7] hist, [8] trials,
                           // [9..11] warm starts taken: learned, isotropic, previous,
-                          // [12] mixed-precision steps redone in FP64
+                          // [12] mixed-precision steps redone in FP64, [13] cells the
+                          // persistent tail took, [14], [15] Newton iterations and
+                          // Armijo trials done in the tail
     int ls_mode;          // MNC_LS_*
     const double *alpha_guess; // [ncells][nm] scaled-problem guess (learned warm start) or nullptr
     double gate;          // learned warm start: the network is consulted when the best of isotropic
@@ -83,7 +85,7 @@ struct NewtonArgs {
 constexpr int kCtlWords = 8;
 constexpr int kNumLists = 5;
 constexpr int kMlpListIndex = 4, kMlpListWord = 6;
-constexpr int kStatWords = 13;
+constexpr int kStatWords = 16;   // ... [13] tail cells, [14] tail iterations, [15] tail trials
 
 // Result layout of v_mfma_f64_16x16x4f64 on gfx950 (measured with a one-wave
 // probe): result element r of lane l is row 4r + l/16, column l%16.  (The

@@ -416,10 +416,12 @@ struct Store {
 struct LaneStats {
     int max_iter = 0, err_bits = 0;
     long long sum = 0, n = 0, trials = 0, hist[5] = {0, 0, 0, 0, 0}, warm[3] = {0, 0, 0}, deferred = 0;
+    // persistent tail only: cells taken, Newton iterations and Armijo trials done in it
+    long long tcells = 0, titer = 0, ttrials = 0;
 };
 
 __device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
-    __shared__ long long red[kBlock / 64][14];
+    __shared__ long long red[kBlock / 64][17];
     for (int off = 32; off > 0; off >>= 1) {
         s.max_iter = max(s.max_iter, __shfl_xor(s.max_iter, off));
         s.err_bits |= __shfl_xor(s.err_bits, off);
@@ -431,6 +433,9 @@ __device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
 #pragma unroll
         for (int r = 0; r < 3; ++r) s.warm[r] += __shfl_xor(s.warm[r], off);
         s.deferred += __shfl_xor(s.deferred, off);
+        s.tcells += __shfl_xor(s.tcells, off);
+        s.titer += __shfl_xor(s.titer, off);
+        s.ttrials += __shfl_xor(s.ttrials, off);
     }
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) {
@@ -438,16 +443,19 @@ __device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
         for (int r = 0; r < 5; ++r) red[w][5 + r] = s.hist[r];
         for (int r = 0; r < 3; ++r) red[w][10 + r] = s.warm[r];
         red[w][13] = s.deferred;
+        red[w][14] = s.tcells;
+        red[w][15] = s.titer;
+        red[w][16] = s.ttrials;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        long long v[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        long long v[17] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int ww = 0; ww < kBlock / 64; ++ww) {
             v[0] = max(v[0], red[ww][0]);
             v[1] |= red[ww][1];
-            for (int k = 2; k < 14; ++k) v[k] += red[ww][k];
+            for (int k = 2; k < 17; ++k) v[k] += red[ww][k];
         }
-        if (v[3] || v[4] || v[1] || v[10] || v[11] || v[12] || v[13]) {
+        if (v[3] || v[4] || v[1] || v[10] || v[11] || v[12] || v[13] || v[14]) {
             atomicMax((unsigned long long *)&A.stats[0], (unsigned long long)v[0]);
             atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)v[2]);
             atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)v[3]);
@@ -455,6 +463,11 @@ __device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
             atomicAdd((unsigned long long *)&A.stats[8], (unsigned long long)v[4]);
             for (int r = 0; r < 3; ++r) atomicAdd((unsigned long long *)&A.stats[9 + r], (unsigned long long)v[10 + r]);
             atomicAdd((unsigned long long *)&A.stats[12], (unsigned long long)v[13]);
+            if (v[14]) {
+                atomicAdd((unsigned long long *)&A.stats[13], (unsigned long long)v[14]);
+                atomicAdd((unsigned long long *)&A.stats[14], (unsigned long long)v[15]);
+                atomicAdd((unsigned long long *)&A.stats[15], (unsigned long long)v[16]);
+            }
             if (v[1]) atomicOr(&A.ctl[3], int(v[1]));
         }
     }
@@ -1080,6 +1093,10 @@ __global__ __launch_bounds__(kBlock) void tail_kernel(NewtonArgs A, int lin, int
                 const int s = base + __popcll(m & lane_mask_lt());
                 c = tail_cell(A, lin, count, lin2, dcount, s);
                 if (c < 0) c = -2;
+                if (c >= 0) {
+                    ++st.tcells;
+                    st.titer -= S.i(Store<NM, NP2>::I_TOTAL, c);
+                }
             }
         }
         if (__ballot(c >= 0) == 0) break;
@@ -1088,6 +1105,8 @@ __global__ __launch_bounds__(kBlock) void tail_kernel(NewtonArgs A, int lin, int
             if (cell_logic<NM, NP2, GAUNT>(A, S, c, st) == LOGIC_DONE) c = -1;
         }
     }
+    st.titer += st.sum;           // the converged cells' final totals
+    st.ttrials = st.trials;
     flush_stats(A, st);
 }
 
@@ -1111,6 +1130,10 @@ __global__ __launch_bounds__(kBlock) void tail_wave_kernel(NewtonArgs A, int lin
         s = __shfl(s, 0);
         const int c = tail_cell(A, lin, count, lin2, dcount, s);
         if (c < 0) break;
+        if (lane == 0) {
+            ++st.tcells;
+            st.titer -= S.i(St::I_TOTAL, c);
+        }
         for (;;) {
             double al[NM];
 #pragma unroll
@@ -1142,6 +1165,8 @@ __global__ __launch_bounds__(kBlock) void tail_wave_kernel(NewtonArgs A, int lin
             if (res == LOGIC_DONE) break;
         }
     }
+    st.titer += st.sum;
+    st.ttrials = st.trials;
     flush_stats(A, st);
 }
 
@@ -1387,6 +1412,10 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
         int mode = S.i(St::I_MODE, c), iter = S.i(St::I_ITER, c), total = S.i(St::I_TOTAL, c);
         int reg = S.i(St::I_REG, c), maxit = S.i(St::I_MAXIT, c);
         double t = S.d(St::F_T, c), rtol = S.d(St::F_RTOL, c);
+        if (lane == 0) {
+            ++st.tcells;
+            st.titer -= total;
+        }
         const double u0 = S.d(St::F_U0, c), ascale = S.d(St::F_ASCALE, c);
         double fcur_s = S.d(St::F_FCUR, c), gd_s = S.d(St::F_GD, c), fscale_s = S.d(St::F_FSCALE, c);
         double maxf_s = S.d(St::F_MAXF, c);
@@ -1811,7 +1840,10 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
         }
     }
     // statistics: only lane 0 of each wave counted
-    if (lane == 0 && (st.n || st.trials || st.err_bits)) {
+    if (lane == 0 && (st.n || st.trials || st.err_bits || st.tcells)) {
+        atomicAdd((unsigned long long *)&A.stats[13], (unsigned long long)st.tcells);
+        atomicAdd((unsigned long long *)&A.stats[14], (unsigned long long)(st.titer + st.sum));
+        atomicAdd((unsigned long long *)&A.stats[15], (unsigned long long)st.trials);
         atomicMax((unsigned long long *)&A.stats[0], (unsigned long long)st.max_iter);
         atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)st.sum);
         atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)st.n);

@@ -18,6 +18,13 @@
 
 namespace {
 
+// x0 + k h rounded as a product and a sum: the contraction into one fma is
+// switched off here, as the reference's x86-64 build (no FMA) rounds it.
+__device__ __forceinline__ double edge_coord(double x0, int k, double h) {
+#pragma clang fp contract(off)
+    return x0 + double(k) * h;
+}
+
 constexpr int kSnThreads = 256;
 
 struct SnArgs {
@@ -91,7 +98,9 @@ __global__ __launch_bounds__(kSnThreads) void sn_stage_kernel(SnArgs a, const do
         for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
         double integral = part * a.sigma_s[c] / (4 * M_PI);
         const double st = a.sigma_t[c];
-        const double x_i = a.init_x + i * a.dx, y_j = a.init_y + j * a.dy;
+        // rounded as a product and a sum (no fma), as the reference's build
+        // evaluates the source test |x| < 0.5 (kinetic_update.cpp:173-175)
+        const double x_i = edge_coord(a.init_x, i, a.dx), y_j = edge_coord(a.init_y, j, a.dy);
         const bool src = a.init_cond == 1 && fabs(x_i) < 0.5 && fabs(y_j) < 0.5;
         for (int q = lane; q < nq; q += 64) {
             auto F = [&](int di, int dj) { return in[(size_t(i + di) * NY + (j + dj)) * nq + q]; };

@@ -80,7 +80,7 @@ void initialize_grid(Grid &g, int init_cond, double gaussian_sigma, double floor
     }
 }
 
-void write_output(const Grid &g, mnc_ctx *ctx, int nm, double time) {
+void write_output(const Grid &g, mnc_ctx *ctx, int nm, double time, int opt_type) {
     char name[256];
     std::snprintf(name, sizeof name, "out_%.3f_%d.pn", time, 0);
     std::vector<double> mom(size_t(g.NX) * g.NY * nm);
@@ -132,6 +132,17 @@ void write_output(const Grid &g, mnc_ctx *ctx, int nm, double time) {
     std::fwrite(hdr, sizeof(int64_t), 3, f);
     std::fwrite(&time, sizeof(double), 1, f);
     std::fwrite(alpha.data(), sizeof(double), alpha.size(), f);
+    if (opt_type != MNC_OPT_NEWTON) {
+        // the change-of-basis optimisers continue from each cell's basis and
+        // multipliers in that basis (c_P2M, c_alphaP): appended after alpha
+        std::vector<double> B(size_t(g.NX) * g.NY * nm * nm), aP(size_t(g.NX) * g.NY * nm);
+        if (mnc_grid_get_basis(ctx, B.data(), aP.data()) != MNC_OK) {
+            std::printf("mnc_grid_get_basis: %s\n", mnc_last_error(ctx));
+            std::exit(1);
+        }
+        std::fwrite(B.data(), sizeof(double), B.size(), f);
+        std::fwrite(aP.data(), sizeof(double), aP.size(), f);
+    }
     std::fclose(f);
 }
 
@@ -139,7 +150,8 @@ void write_output(const Grid &g, mnc_ctx *ctx, int nm, double time) {
 // <prefix>.alpha (time and the multipliers of every cell).  Returns false with
 // a message if the files do not match the deck's grid.
 bool read_restart(const std::string &prefix, const Grid &g, int nm, std::vector<double> &mom,
-                  std::vector<double> &alpha, double &time) {
+                  std::vector<double> &alpha, double &time, int opt_type, std::vector<double> &basis,
+                  std::vector<double> &alpha_p) {
     FILE *f = std::fopen((prefix + ".pn").c_str(), "rb");
     if (!f) { std::printf("cannot read %s.pn\n", prefix.c_str()); return false; }
     int64_t nx = 0, ny = 0, n = 0;
@@ -158,6 +170,12 @@ bool read_restart(const std::string &prefix, const Grid &g, int nm, std::vector<
     ok = std::fread(hdr, sizeof(int64_t), 3, f) == 3 && std::fread(&time, sizeof(double), 1, f) == 1;
     ok = ok && hdr[0] == g.nx && hdr[1] == g.ny && hdr[2] == nm;
     ok = ok && std::fread(alpha.data(), sizeof(double), alpha.size(), f) == alpha.size();
+    if (opt_type != MNC_OPT_NEWTON) {     // a change-of-basis run's file carries c_P2M and c_alphaP too
+        basis.assign(size_t(g.NX) * g.NY * nm * nm, 0.0);
+        alpha_p.assign(size_t(g.NX) * g.NY * nm, 0.0);
+        ok = ok && std::fread(basis.data(), sizeof(double), basis.size(), f) == basis.size() &&
+             std::fread(alpha_p.data(), sizeof(double), alpha_p.size(), f) == alpha_p.size();
+    }
     std::fclose(f);
     if (!ok) { std::printf("%s.alpha does not match the deck's grid\n", prefix.c_str()); return false; }
     return true;
@@ -251,12 +269,18 @@ int main(int argc, char **argv) {
     std::string restart;
     double t0 = 0.0;
     const bool restarting = deck.get("MOMOPT_GPU_RESTART", restart);
-    if (restarting && !read_restart(restart, g, nm, mom, alpha, t0)) return 1;
+    std::vector<double> basis, alpha_p;
+    if (restarting && !read_restart(restart, g, nm, mom, alpha, t0, otype, basis, alpha_p)) return 1;
     mnc_grid_desc gd{g.nx, g.ny, g.dx, g.dy, g.init_x, g.init_y, init_cond, theta, delta_ppn};
     st = mnc_grid_create(ctx, &gd, mom.data(), alpha.data(), g.sigma_t.data(), g.sigma_s.data());
     if (st != MNC_OK) { std::printf("mnc_grid_create: %s\n", mnc_last_error(ctx)); return 1; }
     if (restarting && (st = mnc_ghost_fill(ctx)) != MNC_OK) {        // the ghost moments of the .pn's interior
         std::printf("mnc_ghost_fill: %s\n", mnc_last_error(ctx));
+        return 1;
+    }
+    if (restarting && otype != MNC_OPT_NEWTON &&
+        (st = mnc_grid_set_basis(ctx, basis.data(), alpha_p.data())) != MNC_OK) {
+        std::printf("mnc_grid_set_basis: %s\n", mnc_last_error(ctx));
         return 1;
     }
     const double max_dt = cfl * (2.0 / (2.0 + theta)) * (g.dx * g.dy) / (g.dx + g.dy);   // :72
@@ -265,7 +289,7 @@ int main(int argc, char **argv) {
                     tol_gamma};
 
     double t = t0, dt = max_dt;                       // main.cpp:204-230
-    if (!restarting) write_output(g, ctx, nm, t);
+    if (!restarting) write_output(g, ctx, nm, t, otype);
     for (double t_out = std::fmin(t + out_dt, t_final); t_out <= t_final; t_out = std::fmin(t_out + out_dt, t_final)) {
         dt = max_dt;
         while (t < t_out) {
@@ -281,7 +305,7 @@ int main(int argc, char **argv) {
             }
             if ((t * 100 - std::floor(t * 100)) / 100.0 < dt) std::printf("dt = %f   t = %f\n", dt, t);
         }
-        write_output(g, ctx, nm, t);
+        write_output(g, ctx, nm, t, otype);
         if (t_out > t_final - 1e-12 * t_final) break;
     }
     mnc_destroy(ctx);

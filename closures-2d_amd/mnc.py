@@ -21,7 +21,7 @@ WARM_PREVIOUS, WARM_ISOTROPIC, WARM_LEARNED = 0, 1, 2
 NOALLOC_MAX_CELLS = 16384   # MNC_NOALLOC_MAX_CELLS
 OPT_NEWTON, OPT_CHANGE_BASIS, OPT_BFGS = 0, 1, 2
 MOMENT_MN, MOMENT_PPN = 0, 1
-KCLASS_NEWTON, KCLASS_FLUX, KCLASS_OTHER, KCLASS_MLP = 0, 1, 2, 3
+KCLASS_NEWTON, KCLASS_FLUX, KCLASS_OTHER, KCLASS_MLP, KCLASS_TAIL = 0, 1, 2, 3, 4
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -53,7 +53,8 @@ class Stats(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("num_dual_solves", C.c_longlong),
                 ("iter_sum", C.c_longlong), ("hist_reg", C.c_longlong * 5),
                 ("trial_sum", C.c_longlong), ("warm_pick", C.c_longlong * 3),
-                ("mixed_deferred", C.c_longlong)]
+                ("mixed_deferred", C.c_longlong), ("tail_cells", C.c_longlong),
+                ("tail_iter_sum", C.c_longlong), ("tail_trial_sum", C.c_longlong)]
 
 
 class MncError(RuntimeError):
@@ -96,6 +97,8 @@ def lib():
         L.mnc_grid_create.argtypes = [vp, C.POINTER(GridDesc), _dp, _dp, _dp, _dp]
         L.mnc_grid_get.argtypes = [vp, _dp, _dp]
         L.mnc_grid_set.argtypes = [vp, _dp, _dp]
+        L.mnc_grid_get_basis.argtypes = [vp, _dp, _dp]
+        L.mnc_grid_set_basis.argtypes = [vp, _dp, _dp]
         L.mnc_ghost_fill.argtypes = [vp]
         L.mnc_solve_grid.argtypes = [vp, C.POINTER(Options)]
         L.mnc_flux.argtypes = [vp, _dp]
@@ -265,6 +268,21 @@ class Context:
         m = np.ascontiguousarray(moments, dtype=np.float64).ravel()
         a = np.ascontiguousarray(alpha, dtype=np.float64).ravel()
         self._check(lib().mnc_grid_set(self.h, _p(m), _p(a)), "mnc_grid_set")
+
+    def grid_get_basis(self):
+        """The change-of-basis state (P2M [cells][nm*nm] column-major, alphaP
+        [cells][nm]) of every cell including ghosts (mnc_grid_get_basis)."""
+        nx, ny = self.grid
+        nc = (nx + 4) * (ny + 4)
+        B = np.zeros((nc, self.nm * self.nm))
+        aP = np.zeros((nc, self.nm))
+        self._check(lib().mnc_grid_get_basis(self.h, _p(B), _p(aP)), "mnc_grid_get_basis")
+        return B, aP
+
+    def grid_set_basis(self, P2M, alphaP):
+        B = np.ascontiguousarray(P2M, dtype=np.float64).ravel()
+        aP = np.ascontiguousarray(alphaP, dtype=np.float64).ravel()
+        self._check(lib().mnc_grid_set_basis(self.h, _p(B), _p(aP)), "mnc_grid_set_basis")
 
     def ghost_fill(self):
         self._check(lib().mnc_ghost_fill(self.h), "mnc_ghost_fill")

@@ -143,6 +143,10 @@ typedef struct {
     long long warm_pick[3];      /* MNC_WARM_LEARNED starts taken from: [0] the
                                     MLP, [1] isotropicAlpha, [2] the alpha passed in */
     long long mixed_deferred;    /* MNC_PREC_MIXED Newton steps redone in FP64       */
+    long long tail_cells;        /* cells the persistent tail took (those still
+                                    iterating after the lockstep rounds)            */
+    long long tail_iter_sum;     /* Newton iterations done in the tail              */
+    long long tail_trial_sum;    /* Armijo trials done in the tail                  */
 } mnc_stats;
 
 /* Creates a context: builds the quadrature, the real spherical-harmonic table
@@ -238,6 +242,18 @@ int mnc_grid_create(mnc_ctx *ctx, const mnc_grid_desc *grid, const double *momen
 int mnc_grid_get(mnc_ctx *ctx, double *moments, double *alpha);
 int mnc_grid_set(mnc_ctx *ctx, const double *moments, const double *alpha);
 
+/* The change-of-basis optimisers' per-cell state of the grid, the
+ * reference's c_P2M ([cells][n*n], each basis lower triangular and
+ * column-major) and c_alphaP ([cells][n]) (momopt_init.cpp:115-158), ghost
+ * cells included, read or written for restart files: with
+ * MOMOPT_OPTIMIZATION_TYPE 1 or 2 the next solve starts from this state, so a
+ * restart that continues bit for bit needs it beside the moments and alpha.
+ * Before the grid's first change-of-basis solve it reads the initial state
+ * (ones in each basis's first column, isotropic alphaP).  mnc_grid_create and
+ * mnc_grid_set reset it to that initial state. */
+int mnc_grid_get_basis(mnc_ctx *ctx, double *P2M, double *alphaP);
+int mnc_grid_set_basis(mnc_ctx *ctx, const double *P2M, const double *alphaP);
+
 /* Serial communicateBoundaries: periodic ghost copy of moments only
  * (comm.cpp:148-155 -> momopt_boundaries.cpp:190-223); no-op unless
  * init_cond == 2. */
@@ -319,7 +335,9 @@ int mnc_sync(mnc_ctx *ctx);
 #define MNC_KCLASS_FLUX 1
 #define MNC_KCLASS_OTHER 2
 #define MNC_KCLASS_MLP 3
-#define MNC_NUM_KCLASS 4
+#define MNC_KCLASS_TAIL 4       /* the persistent tail of the Newton solve (timed apart
+                                   from MNC_KCLASS_NEWTON, its prologue and rounds) */
+#define MNC_NUM_KCLASS 5
 int mnc_timing_enable(mnc_ctx *ctx, int on);
 int mnc_timing_get(mnc_ctx *ctx, int kclass, long long *launches, double *ms);
 /* Last error text for the most recent failing call on this context. */

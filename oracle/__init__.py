@@ -31,7 +31,8 @@ class Options(C.Structure):
                 ("cond_h_max", C.c_double), ("use_gaunt", C.c_int), ("line_search", C.c_int),
                 ("moment_type", C.c_int), ("delta_ppn", C.c_double),
                 ("max_bfgs_iter", C.c_int), ("cond_h_max_bfgs", C.c_double),
-                ("tol_gamma", C.c_double)]
+                ("tol_gamma", C.c_double), ("trace", _dp), ("trace_cap", C.c_int),
+                ("trace_len", _ip)]
 
 
 class Outs(C.Structure):
@@ -195,6 +196,23 @@ class Tables:
         outs = Outs()
         st = lib().orc_opt(self.h, _p(u), _p(alpha), C.byref(o), C.byref(outs))
         return st, u, alpha, outs
+
+    def opt_trace(self, u, alpha, max_iter=100, tol=1e-8, cond_h_max=1e10, use_gaunt=None, line_search=0,
+                  moment_type=MOMENT_MN, delta_ppn=0.0, tol_gamma=0.0, cap=4096):
+        """One opt() call with optScaled's decisions recorded: returns (status,
+        u_out, alpha_out, outs, records[k][7]) with each record (total
+        iteration, regularization index, decision, condition estimate, |g|,
+        rtol, gamma); decision 0 none, 1 iteration limit, 2 Cholesky failed,
+        3 line search t = 0, 4 condition estimate above cond_h_max, 5 converged."""
+        u = np.array(u, dtype=np.float64)
+        alpha = np.array(alpha, dtype=np.float64)
+        buf = np.zeros((cap, 7))
+        n = C.c_int(0)
+        o = Options(max_iter, tol, cond_h_max, int(self.use_gaunt if use_gaunt is None else use_gaunt),
+                    line_search, moment_type, delta_ppn, 0, 0.0, tol_gamma, _p(buf), cap, C.pointer(n))
+        outs = Outs()
+        st = lib().orc_opt(self.h, _p(u), _p(alpha), C.byref(o), C.byref(outs))
+        return st, u, alpha, outs, buf[:n.value].copy()
 
     def opt_batch(self, u, alpha, max_iter=100, tol=1e-8, cond_h_max=1e10, use_gaunt=None,
                   nthreads=0, line_search=0, moment_type=MOMENT_MN, delta_ppn=0.0, tol_gamma=0.0):

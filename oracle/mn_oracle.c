@@ -788,7 +788,14 @@ static int opt_scaled(const orc_tables *t, double *u, double *alpha, const orc_o
             double rc = orc_rcond(nm, chol, norm);
             condest = 1.0 / rc;
         }
+        int why = 0;
+        if (o->trace) why = iter >= maxIter ? 1 : error != 0 ? 2 : tt == 0 ? 3 : condest > condHMax ? 4 : 0;
         if (iter >= maxIter || error != 0 || tt == 0 || condest > condHMax) {
+            if (o->trace && *o->trace_len < o->trace_cap) {
+                double *rec = o->trace + (size_t)(*o->trace_len)++ * ORC_TRACE_WORDS;
+                rec[0] = totalIter; rec[1] = regIndex; rec[2] = why; rec[3] = condest;
+                rec[4] = nrm2(nm, g); rec[5] = rtol; rec[6] = 0.0;
+            }
             if (regIndex < ORC_NUM_REG - 1) {
                 regIndex++;
                 r = ORC_REGULARIZATIONS[regIndex];
@@ -815,7 +822,16 @@ static int opt_scaled(const orc_tables *t, double *u, double *alpha, const orc_o
         err = nrm2(nm, g);
         /* the supplementary criterion gamma <= tolGamma (opt.cpp:288-293) */
         int gamma_ok = 1;
-        if (o->tol_gamma > 0.0) gamma_ok = modcfl(nm, d, o->moment_type) <= o->tol_gamma;
+        double gamma = 0.0;
+        if (o->tol_gamma > 0.0) {
+            gamma = modcfl(nm, d, o->moment_type);
+            gamma_ok = gamma <= o->tol_gamma;
+        }
+        if (o->trace && *o->trace_len < o->trace_cap) {
+            double *rec = o->trace + (size_t)(*o->trace_len)++ * ORC_TRACE_WORDS;
+            rec[0] = totalIter; rec[1] = regIndex; rec[2] = (err <= rtol && gamma_ok) ? 5 : 0; rec[3] = condest;
+            rec[4] = err; rec[5] = rtol; rec[6] = gamma;
+        }
         if (err <= rtol && gamma_ok) {
             if (o->moment_type == ORC_MOMENT_MN) {             /* :296-303 */
                 double u0 = u[0] + g[0];
@@ -1489,6 +1505,9 @@ int orc_solver_solve_all(orc_solver *s)
     o.max_bfgs_iter = 0;
     o.cond_h_max_bfgs = 0.0;
     o.tol_gamma = s->deck.tol_gamma;
+    o.trace = NULL;
+    o.trace_cap = 0;
+    o.trace_len = NULL;
     int ncell = s->NX * s->NY, nm = s->nm;
     int status = ORC_OK;
     int maxit = s->stats.max_iter;
@@ -1528,6 +1547,18 @@ int orc_solver_solve_all(orc_solver *s)
     s->stats.num_dual_solves += ncell;
     for (int r = 0; r < ORC_NUM_REG; r++) s->stats.hist_reg[r] += hist[r];
     return status;
+}
+
+/* The coordinate x0 + k h of a cell edge for the lattice source test
+ * |x| < 0.5 (momopt_update.cpp:328-334, kinetic_update.cpp:173-175), rounded
+ * as a product and a sum as the reference's build (no FMA) does, also in the
+ * FMA-contracted sensitivity build: where edges fall on +-0.5 exactly (280
+ * cells on [-3.5, 3.5]) a contraction moves a row and a column of cells into
+ * the source, which is a different problem, not a rounding difference. */
+__attribute__((optimize("fp-contract=off"), noinline))
+static double edge_coord(double x0, int k, double h)
+{
+    return x0 + k * h;
 }
 
 static double sign1(double x) { return (x >= 0.0) ? 1.0 : -1.0; }       /* utils.h:30 */
@@ -1634,8 +1665,8 @@ static void euler_stage(orc_solver *s, double dt)
             for (int k = 1; k < nm; k++)
                 u[k] = u[k] * (1.0 - dt * s->sigma_t[c]) - dt * F[k];
             if (s->deck.init_cond == 1) {
-                double x_i = s->init_x + (i - 2) * s->dx;
-                double y_j = s->init_y + (j - 2) * s->dy;
+                double x_i = edge_coord(s->init_x, i - 2, s->dx);
+                double y_j = edge_coord(s->init_y, j - 2, s->dy);
                 if (fabs(x_i) < 0.5 && fabs(y_j) < 0.5)
                     u[0] = u[0] + dt * 2.0 * sqrt(M_PI);
             }
@@ -1757,7 +1788,7 @@ static void sn_stage(int nx, int ny, int nq, const double *w, const double *xi, 
                 double v = f[c * nq + q];
                 v = v - dt * sigma_t[c] * v + dt * integral;
                 if (init_cond == 1) {
-                    const double x_i = init_x + i * dx, y_j = init_y + j * dy;
+                    const double x_i = edge_coord(init_x, i, dx), y_j = edge_coord(init_y, j, dy);
                     if (fabs(x_i) < 0.5 && fabs(y_j) < 0.5) v = v + dt;
                 }
                 f[c * nq + q] = v - dt * flux[c * nq + q];

@@ -40,7 +40,17 @@ typedef struct {
     double tol_gamma;  /* opt only: the supplementary stop test gamma = modcfl(d) <= tol_gamma
                           (opt.cpp:52-76,288-293, commented out in the reference's current
                           source; momopt_update.cpp:226 sets 1.1); <= 0 switches it off */
+    /* opt only, for tests: when trace is set, every iteration of optScaled
+     * appends one record of ORC_TRACE_WORDS doubles (total iteration,
+     * regularization index, decision, condition estimate, |g|, rtol, gamma)
+     * while *trace_len < trace_cap.  Decision: 0 none, 1 the iteration limit,
+     * 2 the Cholesky failed, 3 the line search returned t = 0, 4 the
+     * condition estimate exceeded cond_h_max, 5 converged. */
+    double *trace;
+    int trace_cap;
+    int *trace_len;
 } orc_options;
+#define ORC_TRACE_WORDS 7
 
 enum { ORC_MOMENT_MN = 0, ORC_MOMENT_PPN = 1 };
 

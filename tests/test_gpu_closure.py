@@ -17,18 +17,20 @@ from helpers import config2_batch, isotropic_start, load_pkg, oracle
 CASES = [(2, 12, False), (2, 12, True), (3, 30, True), (3, 30, False), (4, 16, False), (4, 16, True)]
 
 
-def _run(order, Q, gaunt, n, tol, seed, nb_frac=0.2):
+def _run(order, Q, gaunt, n, tol, seed, nb_frac=0.2, line_search=0):
     mnc = load_pkg("mnc")
     o = oracle()
-    print("case order=%d Q=%d gaunt=%d n=%d tol=%g seed=%d" % (order, Q, gaunt, n, tol, seed), flush=True)
+    print("case order=%d Q=%d gaunt=%d n=%d tol=%g seed=%d ls=%d" % (order, Q, gaunt, n, tol, seed, line_search),
+          flush=True)
     ctx = mnc.Context(order, Q, use_gaunt=gaunt)
     tab = o.Tables(order, Q, gaunt)
     rng = np.random.default_rng(seed)
     u, astar, lobe = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, n, rng, near_boundary_frac=nb_frac)
     a0 = isotropic_start(u)
-    opt = mnc.options(max_iter=100, tol=tol, cond_h_max=1e10)
+    opt = mnc.options(max_iter=100, tol=tol, cond_h_max=1e10, line_search=line_search)
     st, ug, ag, itg, rg, ngg = ctx.opt_batch_host(u, a0, opt)
-    sto, uo, ao, ito, ro, ngo = tab.opt_batch(u, a0, max_iter=100, tol=tol, cond_h_max=1e10, nthreads=8)
+    sto, uo, ao, ito, ro, ngo = tab.opt_batch(u, a0, max_iter=100, tol=tol, cond_h_max=1e10, nthreads=8,
+                                              line_search=line_search)
     ctx.close()
     return dict(u=u, astar=astar, lobe=lobe, ug=ug, ag=ag, itg=itg, rg=rg, uo=uo, ao=ao, ito=ito, ro=ro,
                 st=st, sto=sto, tab=tab)
@@ -49,12 +51,15 @@ def test_opt_batch_matches_oracle(order, Q, gaunt):
     smooth = ~R["lobe"]
     same = (R["itg"] == R["ito"]) & (R["rg"] == R["ro"])
     print("identical (iter, r): smooth %.4f, lobes %.4f" % (same[smooth].mean(), same[~smooth].mean()))
-    # two CPU builds of the reference (glibc exp vs exp in long double) agree on
-    # 99.5-99.96% of these smooth cells; FMA contraction and the GPU's exp add
-    # more path divergence at tol 1e-8 (96.7% at M3/Q30), all within the
-    # tolerance checks below
-    assert same[smooth].mean() >= 0.95
-    assert same.mean() >= 0.93
+    # Two CPU builds of the oracle (with and without FMA contraction) follow
+    # the same path on 97-98% of these cells and end 0.2-0.8% at another
+    # regularization level: at TOL 1e-8 the reference's Armijo test compares
+    # f values whose difference has reached f's rounding
+    # (tests/test_oracle_sensitivity.py::test_regularization_decisions_are_rounding_level).
+    # The GPU (FMA, its own exp) measured 96.6-98.0% identical paths and
+    # 98.7-99.4% identical levels.
+    assert same[smooth].mean() >= 0.96
+    assert same.mean() >= 0.955
     sm = same & smooth
     ua = np.abs(R["ug"][sm] - R["uo"][sm]).max(axis=1) / np.abs(R["uo"][sm]).max(axis=1)
     aa = np.abs(R["ag"][sm] - R["ao"][sm]).max(axis=1) / (np.abs(R["ao"][sm]).max(axis=1) + 1.0)
@@ -64,12 +69,47 @@ def test_opt_batch_matches_oracle(order, Q, gaunt):
     assert np.quantile(ua, 0.99) <= 1e-9
     # all cells with the same regularization level solved the same problem to tol
     rs = R["rg"] == R["ro"]
-    assert rs.mean() >= 0.97
+    assert rs.mean() >= 0.98
     un = np.linalg.norm(R["ug"][rs] - R["uo"][rs], axis=1) / np.linalg.norm(R["uo"][rs], axis=1)
     print("same-r cells: max ||u_gpu - u_oracle|| / ||u|| = %.3e (tol %.0e)" % (un.max(), tol))
     assert un.max() <= 4 * tol
+    # cells at another level: one step of the ladder apart, each the solution
+    # of its own blended problem (u[1:] (1 - r), opt.cpp:247-257) to tol, so
+    # the two answers differ by about the blend, i.e. by at most 2 max(r) ||u||
+    dr = ~rs
+    ladder = np.array([0.0, 1e-3, 1e-2, 1e-1, 0.5])
+    ig, io = np.searchsorted(ladder, R["rg"][dr]), np.searchsorted(ladder, R["ro"][dr])
+    assert np.all(np.abs(ig - io) == 1)
+    ud = np.linalg.norm(R["ug"][dr] - R["uo"][dr], axis=1) / np.linalg.norm(R["uo"][dr], axis=1)
+    rmax = np.maximum(R["rg"][dr], R["ro"][dr])
+    print("other-level cells: %d, ||u_gpu - u_oracle|| / ||u|| / max(r): max %.3f"
+          % (dr.sum(), (ud / rmax).max() if dr.any() else 0.0))
+    assert np.all(ud <= 2.0 * rmax + 4 * tol)
+    blend = R["u"][dr].copy()
+    blend[:, 1:] *= (1.0 - R["rg"][dr])[:, None]
+    bd = np.linalg.norm(R["ug"][dr] - blend, axis=1) / np.linalg.norm(blend, axis=1)
+    assert np.all(bd <= 4 * tol)
     # u[0] is preserved exactly (opt.cpp:296-303,107-109)
     assert np.array_equal(R["ug"][:, 0], R["u"][:, 0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order,Q", [(2, 12), (3, 30), (4, 16)])
+def test_opt_batch_robust_line_search_matches_oracle(order, Q):
+    """With the robust line search, where two CPU builds of the oracle agree
+    on the regularization level of every cell, the GPU agrees with the oracle
+    on at least 99.8% of the levels and 99% of the paths (iteration counts),
+    and every cell is within 4 tol of the oracle's answer except the few at
+    another level, which are one ladder step apart."""
+    R = _run(order, Q, False, n=3000, tol=1e-8, seed=1234 + order, line_search=1)
+    assert R["st"] == R["sto"] == 0
+    rs = R["rg"] == R["ro"]
+    same = rs & (R["itg"] == R["ito"])
+    print("robust: same level %.4f, same path %.4f" % (rs.mean(), same.mean()))
+    assert rs.mean() >= 0.998
+    assert same.mean() >= 0.99
+    un = np.linalg.norm(R["ug"][rs] - R["uo"][rs], axis=1) / np.linalg.norm(R["uo"][rs], axis=1)
+    assert un.max() <= 4e-8
 
 
 @pytest.mark.gpu

@@ -83,3 +83,49 @@ def test_m8_sine_grid_steps_match_oracle():
     assert list(st.hist_reg)[1:] == [0, 0, 0, 0]
     assert e <= 1e-9
     assert emax <= 1e-7
+
+
+@pytest.mark.gpu
+def test_m8_512_steps_conserve_mass_and_match_oracle_sample():
+    """The M8 workload at full size (512 x 512 periodic cells, Q = 32, TOL
+    1e-8, robust line search), 3 Heun steps on the GPU: the total mass of the
+    periodic grid with sigma_t = sigma_s is conserved to rounding (a size-
+    independent property of the scheme), and the closures of 400 sampled
+    cells of the final state, solved afresh from the stored alpha (moments
+    perturbed by 1e-3 so that they need Newton steps), agree with the
+    oracle's per cell.  A full 512^2 oracle step would take about a minute
+    per stage on the host, so the grid state itself is checked against the
+    oracle on the 20 x 20 grid above."""
+    import os
+    mnc, gi, wl = load_pkg("mnc"), load_pkg("grid_init"), load_pkg("workloads")
+    print("case m8 512^2", flush=True)
+    n = 512
+    g = gi.initial_state(n, n, -1.0, 1.0, -1.0, 1.0, 2, 0.0, 1e-4, 1.0, 45, 0.9, 2.0)
+    wl.sine_state(g, 45)
+    ctx = mnc.Context(8, 32, use_gaunt=False)
+    ctx.grid_create(g.nx, g.ny, g.dx, g.dy, g.init_x, g.init_y, 2, 2.0, g.moments, g.alpha, g.sigma_t, g.sigma_s)
+    opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, line_search=mnc.LS_ROBUST)
+    m0 = density(g.moments[2:-2, 2:-2]).sum()
+    for _ in range(3):
+        ctx.update(g.max_dt, opt)
+    m, a = ctx.grid_get()
+    st = ctx.stats()
+    dm = abs(density(m[2:-2, 2:-2]).sum() - m0) / m0
+    print("M8 512^2: mass drift %.2e over 3 steps, %.3f iterations per closure" % (dm, st.iter_sum / st.num_dual_solves))
+    assert dm < 1e-12
+    assert st.num_dual_solves == 3 * 2 * (n + 4) ** 2 and st.hist_reg[1:] == [0, 0, 0, 0]
+    rng = np.random.default_rng(5)
+    cells = rng.choice((n + 4) * (n + 4), 400, replace=False)
+    u = m.reshape(-1, 45)[cells] * (1.0 + 1e-3 * rng.standard_normal((400, 45)))
+    u[:, 0] = m.reshape(-1, 45)[cells, 0]
+    a0 = a.reshape(-1, 45)[cells]
+    stg, ug, ag, itg, rg, _ = ctx.opt_batch_host(u, a0, opt)
+    ctx.close()
+    tab = oracle().Tables(8, 32, False)
+    sto, uo, ao, ito, ro, _ = tab.opt_batch(u, a0, max_iter=100, tol=1e-8, cond_h_max=1e10, line_search=1,
+                                            nthreads=min(16, len(os.sched_getaffinity(0))))
+    assert stg == 0 and sto == 0
+    err = np.linalg.norm(ug - uo, axis=1) / np.linalg.norm(uo, axis=1)
+    print("M8 512^2 sample: same iterations %.3f, max ||du||/||u|| %.2e" % ((itg == ito).mean(), err.max()))
+    assert (rg == ro).all() and (itg == ito).mean() >= 0.97
+    assert err.max() <= 4e-8
