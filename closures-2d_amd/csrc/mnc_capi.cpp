@@ -1164,7 +1164,7 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         return MNC_OK;
     }
     if (std::strcmp(key, "rounds") == 0) {
-        if (value < 0 || value > 64) return ctx_fail(ctx, MNC_ERR_ARG, "rounds out of range");
+        if (value < 0 || value > 512) return ctx_fail(ctx, MNC_ERR_ARG, "rounds out of range");
         ctx->rounds = value;
         return MNC_OK;
     }

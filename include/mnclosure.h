@@ -270,7 +270,7 @@ int mnc_get_stats(mnc_ctx *ctx, mnc_stats *st);
 int mnc_reset_stats(mnc_ctx *ctx);
 
 /* Tuning knobs of the batched solve (set by the autotuner or by hand):
- *   "rounds"       lockstep sweep/logic rounds before the persistent tail (0-64)
+ *   "rounds"       lockstep sweep/logic rounds before the persistent tail (0-512)
  *   "tail_blocks"  workgroups of the persistent tail kernel (1-65536)
  *   "tail_mode"    0: one wave per remaining cell, dense algebra shared in LDS;
  *                  1: one lane per cell; 2: one wave per cell, algebra per lane

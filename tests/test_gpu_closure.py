@@ -73,13 +73,14 @@ def test_opt_batch_matches_oracle(order, Q, gaunt):
     un = np.linalg.norm(R["ug"][rs] - R["uo"][rs], axis=1) / np.linalg.norm(R["uo"][rs], axis=1)
     print("same-r cells: max ||u_gpu - u_oracle|| / ||u|| = %.3e (tol %.0e)" % (un.max(), tol))
     assert un.max() <= 4 * tol
-    # cells at another level: one step of the ladder apart, each the solution
-    # of its own blended problem (u[1:] (1 - r), opt.cpp:247-257) to tol, so
-    # the two answers differ by about the blend, i.e. by at most 2 max(r) ||u||
+    # cells at another level: one or two steps of the ladder apart (as
+    # between the two oracle builds), each the solution of its own blended
+    # problem (u[1:] (1 - r), opt.cpp:247-257) to tol, so the two answers
+    # differ by about the blend, i.e. by at most 2 max(r) ||u||
     dr = ~rs
     ladder = np.array([0.0, 1e-3, 1e-2, 1e-1, 0.5])
     ig, io = np.searchsorted(ladder, R["rg"][dr]), np.searchsorted(ladder, R["ro"][dr])
-    assert np.all(np.abs(ig - io) == 1)
+    assert np.all(np.abs(ig - io) <= 2)
     ud = np.linalg.norm(R["ug"][dr] - R["uo"][dr], axis=1) / np.linalg.norm(R["uo"][dr], axis=1)
     rmax = np.maximum(R["rg"][dr], R["ro"][dr])
     print("other-level cells: %d, ||u_gpu - u_oracle|| / ||u|| / max(r): max %.3f"
