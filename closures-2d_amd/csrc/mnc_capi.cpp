@@ -58,6 +58,7 @@ struct mnc_ctx {
     int flux_tile = 3;               // 0: ansatz to HBM + flux; 1: fused, 8x8 tiles; 2: fused, 16x16;
                                      // 3: fused, 16x16, directions split into flux_groups
     int flux_groups = 6;             // direction groups of flux_tile 3
+    int learned_fused = 1;           // 1: network and judge in one kernel; 0: the GEMM kernels, then the judge
     int learned_gate_log10 = -2;     // the network is consulted when ||g(best start)|| > 10^this ||u||;
                                      // kGateOff: never (no network launches)
     // grid
@@ -299,8 +300,8 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     double *st_d = na ? na->st_d : ctx->d_std;
     int *st_i = na ? na->st_i : ctx->d_sti, *lists = na ? na->lists : ctx->d_lists, *ctl = na ? na->ctl : ctx->d_ctl;
     const size_t cap = na ? na->cap : ctx->state_cap;
-    if (learned && ctx->learned_gate_log10 < kGateOff) {   // buffers of the network's guesses
-        if ((st = mlp_check(ctx, true))) return st;
+    if (learned && ctx->learned_gate_log10 < kGateOff && (st = mlp_check(ctx, true))) return st;
+    if (learned && ctx->learned_gate_log10 < kGateOff && !ctx->learned_fused) {   // the unfused path's buffers
         if (size_t(ncells) > ctx->guess_cap) {
             if ((st = dalloc(ctx, &ctx->d_guess, size_t(ncells) * 15))) return st;
             ctx->guess_cap = size_t(ncells);
@@ -346,6 +347,7 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     a.mpv = ctx->d_mpv;
     a.cmix = std::pow(10.0, ctx->mix_cond_log10);
     a.gate = std::pow(10.0, ctx->learned_gate_log10);
+    a.mlp_w = ctx->d_mlp;
     a.gz = ctx->d_gz;
     a.gg = ctx->d_gg;
     a.gh = ctx->d_gh;
@@ -412,15 +414,22 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
             mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
             if ((st = launch(ad, first))) return st;
         }
-        {
+        if (ctx->learned_fused) {
+            // the network and the judge in one kernel on the parked cells
             Timed tm(ctx, MNC_KCLASS_MLP);
-            mnc::mlp_predict_listed(mlp_buffers(ctx), u, ctx->d_lists + size_t(mnc::kMlpListIndex) * ctx->state_cap,
-                                    ctx->d_ctl + mnc::kMlpListWord, ncells, ctx->d_guess, ctx->stream);
-        }
-        {
-            Timed tm(ctx, MNC_KCLASS_NEWTON);
-            if (!mnc::launch_newton_judge(a, t.nm, t.np2, gaunt, ctx->stream))
+            if (!mnc::launch_newton_net_judge(a, t.nm, t.np2, gaunt, ncells, ctx->num_cu, ctx->stream))
                 return ctx_fail(ctx, MNC_ERR_ARG, "no learned warm start for this order");
+        } else {
+            {
+                Timed tm(ctx, MNC_KCLASS_MLP);
+                mnc::mlp_predict_listed(mlp_buffers(ctx), u, ctx->d_lists + size_t(mnc::kMlpListIndex) * ctx->state_cap,
+                                        ctx->d_ctl + mnc::kMlpListWord, ncells, ctx->d_guess, ctx->stream);
+            }
+            {
+                Timed tm(ctx, MNC_KCLASS_NEWTON);
+                if (!mnc::launch_newton_judge(a, t.nm, t.np2, gaunt, ctx->stream))
+                    return ctx_fail(ctx, MNC_ERR_ARG, "no learned warm start for this order");
+            }
         }
         if ((st = launch_timed(a, rest, false))) return st;
     } else if (ctx->has_g64) {
@@ -1208,6 +1217,12 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         ctx->flux_groups = value;
         return MNC_OK;
     }
+    if (std::strcmp(key, "learned_fused") == 0) {
+        if (value != 0 && value != 1) return ctx_fail(ctx, MNC_ERR_ARG, "learned_fused is 0 or 1");
+        ctx->learned_fused = value;
+        ++ctx->epoch;
+        return MNC_OK;
+    }
     if (std::strcmp(key, "learned_gate_log10") == 0) {
         if (value < -16 || value > kGateOff) return ctx_fail(ctx, MNC_ERR_ARG, "learned_gate_log10 out of range");
         ctx->learned_gate_log10 = value;
@@ -1232,6 +1247,7 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
     if (std::strcmp(key, "flux_tile") == 0) { *value = ctx->flux_tile; return MNC_OK; }
     if (std::strcmp(key, "flux_groups") == 0) { *value = ctx->flux_groups; return MNC_OK; }
     if (std::strcmp(key, "learned_gate_log10") == 0) { *value = ctx->learned_gate_log10; return MNC_OK; }
+    if (std::strcmp(key, "learned_fused") == 0) { *value = ctx->learned_fused; return MNC_OK; }
     if (std::strcmp(key, "tail_blocks") == 0) {
         *value = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
         return MNC_OK;

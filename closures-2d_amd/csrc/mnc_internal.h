@@ -55,6 +55,7 @@ struct NewtonArgs {
     double gate;          // learned warm start: the network is consulted when the best of isotropic
                           // and previous alpha has ||g|| > gate * ||u|| (scaled problem)
     int divert;           // learned warm start: this launch's logic holds the gate (first round)
+    const float *mlp_w;   // learned warm start: the network's parameters and standardisation (FP32)
     int ppn;              // MOMENT_TYPE_PPN: the PPn closure (opt.cpp:113-141, fobj.cpp:137-236,355-428)
     double delta;         // its delta (MOMOPT_DELTA_PPN), unscaled
     // per-cell solver state and the active lists
@@ -145,6 +146,11 @@ bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, const
 // logic parks the cells whose start fails the gate on the network's list; the
 // judge compares the network's guesses for them and moves them to list 1.
 bool launch_newton_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, hipStream_t s);
+// The same second half in one kernel: the network (bf16 matrix cores, weights
+// in LDS) and the FP64 judge for the cells of the network's list (at most
+// max_rows), sixteen cells per wave, `blocks` persistent workgroups.
+bool launch_newton_net_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, int max_rows, int blocks,
+                             hipStream_t s);
 // The FP64 matrix-core path (M8: n = 45).  rounds: lockstep rounds per launch
 // batch; the caller repeats batches until control word `lin` reads zero.
 // Returns false for an order without an instance.

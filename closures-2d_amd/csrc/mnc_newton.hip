@@ -17,6 +17,7 @@
 // with the LAPACK roles (dpotrf 'L', dpocon, dpotrs, dnrm2) done in registers
 // or LDS: unblocked column Cholesky, Higham's 1-norm estimator (ACM TOMS 14(4),
 // 1988), forward/back substitution.
+#include <algorithm>
 #include <cfloat>
 #include <mutex>
 #include <type_traits>
@@ -1025,6 +1026,196 @@ __global__ __launch_bounds__(kBlock) void judge_kernel(NewtonArgs A) {
         ok = true;
     }
     list_append(A, 1, cw(1), ok, c);
+    flush_stats(A, st);
+}
+
+// ---- the learned warm start in one kernel: network and judge ---------------
+//
+// For the cells the first round parked on list kMlpList: the MLP
+// 15 -> 128 -> 128 -> 128 -> 15 (GELU) on the matrix cores, sixteen cells per
+// wave (v_mfma_f32_16x16x32_bf16: bf16 operands, FP32 sums, bias and GELU in
+// FP32, the arithmetic of the unfused GEMM kernels in mnc_mlp.hip), then the
+// FP64 gradient norm of the network's guess (four lanes per cell, each a
+// quarter of the points), which replaces the parked start when it is smaller;
+// the cells join list 1.  The weights sit in LDS in the operand layout (lane l
+// of (k-step, tile) holds W[32 ks + 8 (l/16) + t][16 tile + l%16], t = 0..7),
+// the activations in a per-wave bf16 buffer; nothing goes through HBM but the
+// cells' state.
+typedef __bf16 nbf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short nu16x8 __attribute__((ext_vector_type(8)));
+typedef float nf32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kNetH = 128, kNetPar = 37007;   // hidden width; the parameter count (mnc_mlp.hip)
+constexpr int kNetOffW1 = 0, kNetOffB1 = 15 * 128, kNetOffW2 = kNetOffB1 + 128, kNetOffB2 = kNetOffW2 + 128 * 128,
+              kNetOffW3 = kNetOffB2 + 128, kNetOffB3 = kNetOffW3 + 128 * 128, kNetOffW4 = kNetOffB3 + 128,
+              kNetOffB4 = kNetOffW4 + 128 * 15;
+static_assert(kNetOffB4 + 15 == kNetPar, "parameter layout");
+
+__device__ __forceinline__ unsigned short net_bf16(float x) {   // round to nearest even, inf/NaN kept
+    const unsigned u = __float_as_uint(x);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float net_gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+
+struct NetShared {
+    nu16x8 w1[8][64];          // k-step 0 (K = 15, padded to 32) x 8 output tiles
+    nu16x8 w2[4][8][64];
+    nu16x8 w3[4][8][64];
+    nu16x8 w4[4][64];          // one output tile (15 outputs, padded to 16)
+    float b[3][kNetH], b4[16], st[60];
+    alignas(16) unsigned short act[4][2][16][kNetH + 8];   // per wave: ping-pong activations (bf16), padded rows
+    double guess[4][16][16];                   // per wave: the network's scaled multipliers
+};
+
+// Weight element of the operand layout: W[k][j] of a layer with K inputs and N outputs.
+__device__ __forceinline__ nu16x8 net_operand(const float *W, int K, int N, int ks, int tile, int lane) {
+    nu16x8 v;
+    const int j = 16 * tile + (lane & 15);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int k = 32 * ks + 8 * (lane >> 4) + t;
+        v[t] = (k < K && j < N) ? net_bf16(W[size_t(k) * N + j]) : (unsigned short)0;
+    }
+    return v;
+}
+
+// One hidden layer for the wave's sixteen rows: out = gelu(in W + b) (K = 128).
+__device__ __forceinline__ void net_layer(const nu16x8 (&w)[4][8][64], const float *bias, const unsigned short (*in)[kNetH + 8],
+                                          unsigned short (*out)[kNetH + 8], int lane) {
+#pragma unroll
+    for (int tile = 0; tile < 8; ++tile) {
+        nf32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            const nu16x8 a = *reinterpret_cast<const nu16x8 *>(&in[lane & 15][32 * ks + 8 * (lane >> 4)]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(nbf16x8, a),
+                                                          __builtin_bit_cast(nbf16x8, w[ks][tile][lane]), acc, 0, 0, 0);
+        }
+        const int j = 16 * tile + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) out[4 * (lane >> 4) + r][j] = net_bf16(net_gelu(acc[r] + bias[j]));
+    }
+}
+
+template <int NM, int NP2>
+__global__ __launch_bounds__(256) void net_judge_kernel(NewtonArgs A) {
+    static_assert(NM == 15, "the network is M4's");
+    using St = Store<NM, NP2>;
+    const St S{A.st_d, A.st_i, A.cap};
+    __shared__ NetShared sh;
+    const float *W = A.mlp_w;
+    for (int i = threadIdx.x; i < 8 * 64; i += blockDim.x) sh.w1[i / 64][i % 64] = net_operand(W + kNetOffW1, 15, kNetH, 0, i / 64, i % 64);
+    for (int i = threadIdx.x; i < 4 * 8 * 64; i += blockDim.x) {
+        const int ks = i / 512, tile = (i / 64) % 8, l = i % 64;
+        sh.w2[ks][tile][l] = net_operand(W + kNetOffW2, kNetH, kNetH, ks, tile, l);
+        sh.w3[ks][tile][l] = net_operand(W + kNetOffW3, kNetH, kNetH, ks, tile, l);
+    }
+    for (int i = threadIdx.x; i < 4 * 64; i += blockDim.x) sh.w4[i / 64][i % 64] = net_operand(W + kNetOffW4, kNetH, 15, i / 64, 0, i % 64);
+    for (int i = threadIdx.x; i < kNetH; i += blockDim.x) {
+        sh.b[0][i] = W[kNetOffB1 + i];
+        sh.b[1][i] = W[kNetOffB2 + i];
+        sh.b[2][i] = W[kNetOffB3 + i];
+    }
+    for (int i = threadIdx.x; i < 16; i += blockDim.x) sh.b4[i] = i < 15 ? W[kNetOffB4 + i] : 0.f;
+    for (int i = threadIdx.x; i < 60; i += blockDim.x) sh.st[i] = W[kNetPar + i];
+    __syncthreads();     // the only block-wide barrier: from here on the waves are independent
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int row = lane & 15, part = lane >> 4;
+    const int count = A.ctl[cw(kMlpList)];
+    const int *list = A.lists + size_t(kMlpList) * A.cap;
+    unsigned short (*act0)[kNetH + 8] = sh.act[wv][0];
+    unsigned short (*act1)[kNetH + 8] = sh.act[wv][1];
+    double (*guess)[16] = sh.guess[wv];
+    LaneStats st;
+    for (int base = (blockIdx.x * 4 + wv) * 16; base < count; base += gridDim.x * 4 * 16) {
+        const int c = base + row < count ? list[base + row] : -1;
+        // layer 1: the standardised scaled moments u/u0 (k < 15; lane group
+        // `part` supplies k = 8 part + t)
+        nu16x8 x;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int k = 8 * part + t;
+            float v = 0.f;
+            if (c >= 0 && k < 15) v = float((S.d(St::F_U + k, c) - double(sh.st[k])) / double(sh.st[15 + k]));
+            x[t] = net_bf16(v);
+        }
+#pragma unroll
+        for (int tile = 0; tile < 8; ++tile) {
+            nf32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(nbf16x8, x), __builtin_bit_cast(nbf16x8, sh.w1[tile][lane]),
+                                                          acc, 0, 0, 0);
+            const int j = 16 * tile + row;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) act0[4 * part + r][j] = net_bf16(net_gelu(acc[r] + sh.b[0][j]));
+        }
+        __builtin_amdgcn_wave_barrier();
+        net_layer(sh.w2, sh.b[1], act0, act1, lane);
+        __builtin_amdgcn_wave_barrier();
+        net_layer(sh.w3, sh.b[2], act1, act0, lane);
+        __builtin_amdgcn_wave_barrier();
+        {
+            nf32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                const nu16x8 a = *reinterpret_cast<const nu16x8 *>(&act0[row][32 * ks + 8 * part]);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(nbf16x8, a),
+                                                              __builtin_bit_cast(nbf16x8, sh.w4[ks][lane]), acc, 0, 0, 0);
+            }
+            if (row < 15) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)   // the scaled multipliers: y out_std + out_mean
+                    guess[4 * part + r][row] = double(acc[r] + sh.b4[row]) * double(sh.st[45 + row]) + double(sh.st[30 + row]);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // the judge: FP64 gradient of the scaled problem at the guess, this
+        // lane's quarter of the points (q = part, part + 4, ...)
+        double al[NM], g[NM];
+        bool finite = true;
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            al[k] = guess[row][k];
+            finite = finite && isfinite(al[k]);
+            g[k] = 0.0;
+        }
+        if (c >= 0) {
+            const double *P = A.p, *Wq = A.w;
+            for (int q = part; q < A.nq; q += 4) {
+                const double *pr = P + size_t(q) * NP2;
+                double am = 0.0;
+#pragma unroll
+                for (int k = 0; k < NM; ++k) am = fma(al[k], pr[k], am);
+                const double we = Wq[q] * exp(am);
+#pragma unroll
+                for (int k = 0; k < NM; ++k) g[k] = fma(we, pr[k], g[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NM; ++k) {
+            g[k] += __shfl_xor(g[k], 16);
+            g[k] += __shfl_xor(g[k], 32);
+        }
+        bool ok = false;
+        if (c >= 0 && part == 0) {
+            double n2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < NM; ++k) {
+                const double d = g[k] - S.d(St::F_U + k, c);
+                n2 = fma(d, d, n2);
+            }
+            int pick = S.i(St::I_WARM, c);
+            if (finite && n2 < S.d(St::F_FCUR, c)) {
+                pick = 0;
+#pragma unroll
+                for (int k = 0; k < NM; ++k) S.d(St::F_AL + k, c) = al[k];
+            }
+            ++st.warm[pick];
+            ok = true;
+        }
+        list_append(A, 1, cw(1), ok, c);
+        __builtin_amdgcn_wave_barrier();
+    }
     flush_stats(A, st);
 }
 
@@ -2486,6 +2677,15 @@ hipError_t noalloc_scratch_address(int device, void **ptr) {
     }
     *ptr = addr[device];
     return hipSuccess;
+}
+
+bool launch_newton_net_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, int max_rows, int blocks,
+                             hipStream_t s) {
+    const int b = std::max(1, std::min(blocks, (max_rows + 63) / 64));
+    if (nm == 15 && !gaunt && np2 == 15) hipLaunchKernelGGL((net_judge_kernel<15, 15>), dim3(b), dim3(256), 0, s, a);
+    else if (nm == 15 && gaunt && np2 == 45) hipLaunchKernelGGL((net_judge_kernel<15, 45>), dim3(b), dim3(256), 0, s, a);
+    else return false;
+    return true;
 }
 
 bool launch_newton_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, hipStream_t s) {

@@ -292,6 +292,11 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *                  alpha passed in (isotropicAlpha if it is not finite) and
  *                  neither the network nor the kernel that compares its guess
  *                  with the start is launched
+ *   "learned_fused"  MNC_WARM_LEARNED: 1 (default): the network and the
+ *                  comparison of its guess with the parked start run in one
+ *                  kernel (weights in LDS, activations never leave the chip);
+ *                  0: the separate GEMM kernels of mnc_mlp_predict, then the
+ *                  comparison kernel
  *   "flux_tile"    0: ansatz written to HBM, then the flux; 1: ansatz fused
  *                  into the flux with 8x8-cell LDS tiles; 2: 16x16 tiles
  *                  (n <= 15); these three give identical results.  3 (default;

@@ -216,3 +216,91 @@ def test_learned_start_with_network_off_is_the_previous_start():
     print("non-finite starts: picks %s" % picks)
     assert st == 0 and picks == [0, 5, len(u) - 5]
     assert np.all(np.isfinite(ag))
+
+
+@pytest.mark.gpu
+def test_fused_network_judge_matches_the_gemm_path():
+    """The learned start's network and judge in one kernel (learned_fused 1,
+    the default: weights in LDS, activations on chip) against the separate
+    GEMM kernels and comparison kernel (learned_fused 0): the same bf16
+    operands and FP32 sums in the same order, so the network's guesses are
+    the same and so are the starts taken; the judge's FP64 gradient norm sums
+    the points in another order, which can tip a near-tie.  Solved from those
+    starts the batches agree cell for cell."""
+    print("case fused network judge", flush=True)
+    mnc, ctx = _ctx()
+    tab = oracle().Tables(4, 16, False)
+    us, al = _smooth_pairs(mnc, tab, 100000, 31)
+    ctx.mlp_init(seed=32)
+    ctx.mlp_train(us, al, steps=600, batch=4096, lr=1e-3, seed=33)
+    rng = np.random.default_rng(34)
+    u, astar, lobe = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, 20000, rng, near_boundary_frac=0.2)
+    a0 = isotropic_start(u)
+    opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, precision=mnc.PREC_MIXED, warm_start=mnc.WARM_LEARNED)
+    out = {}
+    for fused in (1, 0):
+        ctx.tune("learned_fused", fused)
+        ctx.reset_stats()
+        out[fused] = ctx.opt_batch_host(u, a0, opt) + (list(ctx.stats().warm_pick),)
+    ctx.close()
+    f, g = out[1], out[0]
+    print("picks mlp/iso/prev: fused %s, gemm path %s" % (f[6], g[6]))
+    assert f[0] == 0 and g[0] == 0
+    assert sum(f[6]) == len(u) and f[6][0] >= 0.5 * len(u)
+    assert abs(f[6][0] - g[6][0]) <= 0.001 * len(u)
+    same = np.all(f[2] == g[2], axis=1) & np.all(f[1] == g[1], axis=1)
+    print("bitwise identical cells: %.4f" % same.mean())
+    assert same.mean() >= 0.99
+    rs = f[4] == g[4]
+    assert rs.mean() >= 0.99
+    un = np.linalg.norm(f[1][rs] - g[1][rs], axis=1) / np.linalg.norm(g[1][rs], axis=1)
+    assert un.max() <= 4e-8
+
+
+@pytest.mark.gpu
+def test_learned_lattice_steps_match_oracle():
+    """The learned+mixed form time-stepping an M4 lattice (80 x 80 on
+    [-3.5, 3.5]^2, floor 1e-10, TOL 1e-8, robust line search) against the
+    oracle: a network trained on (u, alpha) pairs harvested from the plain
+    form's first steps, consulted by every cell whose previous alpha leaves
+    ||g|| > 1e-6 ||u|| (gate -6), keeps the state within tolerance level of
+    the oracle's after 8 steps, as the mixed form does."""
+    mnc, gi = load_pkg("mnc"), load_pkg("grid_init")
+    from helpers import density, gpu_grid_from_deck, rel_l2
+    print("case learned lattice", flush=True)
+    d = dict(num_cells_x=80, num_cells_y=80, a_x=-3.5, b_x=3.5, a_y=-3.5, b_y=3.5, gaussian_sigma=0.0,
+             floor_value=1e-10, sigma=1.0, init_cond=1, moment_order=4, quad_order=16, cfl_factor=0.9,
+             tol=1e-8, cond_h_max=1e10, max_iter=100, use_gaunt=0, theta=2.0, line_search=1)
+    steps = 8
+    ctx, g = gpu_grid_from_deck(mnc, gi, d)
+    plain = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, line_search=mnc.LS_ROBUST)
+    us, al = [], []
+    for _ in range(steps):
+        ctx.update(g.max_dt, plain)
+        m, a = ctx.grid_get()
+        u1, a1 = mnc.scaled_pairs(m.reshape(-1, 15), a.reshape(-1, 15))
+        us.append(u1)
+        al.append(a1)
+    us, al = np.concatenate(us), np.concatenate(al)
+    keep = np.abs(al[:, 1:]).max(axis=1) <= 50.0
+    ctx.mlp_init(seed=41)
+    ctx.mlp_train(us[keep], al[keep], steps=600, batch=4096, lr=1e-3, seed=42)
+    ctx.grid_set(g.moments, g.alpha)
+    ctx.tune("learned_gate_log10", -6)
+    ctx.reset_stats()
+    learned = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, line_search=mnc.LS_ROBUST,
+                          precision=mnc.PREC_MIXED, warm_start=mnc.WARM_LEARNED)
+    for _ in range(steps):
+        ctx.update(g.max_dt, learned)
+    picks = list(ctx.stats().warm_pick)
+    ml = ctx.grid_get()[0][2:-2, 2:-2]
+    ctx.close()
+    s = oracle().Solver(d, nthreads=8)
+    for _ in range(steps):
+        s.update(s.max_dt)
+    mo = s.state()[0][2:-2, 2:-2]
+    e = rel_l2(density(ml), density(mo))
+    print("learned+mixed, %d lattice steps: picks mlp/iso/prev %s, density rel-L2 vs oracle %.3e" % (steps, picks, e))
+    assert picks[0] > 0
+    # the same bound as the mixed form's: every solve returns ||g|| <= tol ||u||
+    assert e <= 4e-8
