@@ -165,7 +165,7 @@ T0 = time.perf_counter()
 
 
 FORCE_TUNE = {}
-ISOLATED_ROUNDS = (12, 48, 96, 192)   # lockstep rounds tried on the isolated batch
+ISOLATED_ROUNDS = (6, 12, 48, 96, 192)   # lockstep rounds tried on the isolated batch
 TUNE_STEPS = 10           # steps per autotuning trial
 
 
@@ -277,7 +277,10 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
     lobes, isotropic start, TOL 1e-8, reference line search): converged
     closures/s of each form with u and alpha resident in HBM (mnc_batch_set
     before the clock starts), and each form's error against the FP64 CPU oracle
-    on the first `oracle_sample` vectors."""
+    on the first `oracle_sample` vectors.  Two variants beside it: the same
+    batch with the robust line search (MNC_LS_ROBUST, the lattice decks'), and
+    a smooth batch (no lobes) with the robust line search, where the network
+    can learn every input."""
     sys.path.insert(0, ROOT)
     import oracle
     wl = _load("workloads")
@@ -290,73 +293,91 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
     progress("isolated batch generated (%.1f s)" % gen_s)
     tab = oracle.Tables(4, 16, False)
     m = oracle_sample
-    if m > 0:
-        sto, uo, ao, ito, ro, _ = tab.opt_batch(u[:m], a0[:m], max_iter=100, tol=1e-8, cond_h_max=1e10,
-                                                nthreads=len(os.sched_getaffinity(0)))
     # learned form: network trained on separate exact (u, alpha*) pairs of the same distribution
     ut, at, lt = wl.isolated_batch(w, p, xi, eta, 15, 2_000_000, seed + 1, near_boundary_frac=0.0)
     ust, ast_ = mnc.scaled_pairs(ut, at)
     ctx.mlp_init(seed)
     loss = ctx.mlp_train(ust, ast_, steps=train_steps, batch=4096, lr=1e-3, seed=seed)
     progress("isolated-batch network trained")
-    res = {}
-    for name, kw in (("plain-fp64", {}), ("mixed", {"precision": mnc.PREC_MIXED}),
-                     ("learned+mixed", {"precision": mnc.PREC_MIXED, "warm_start": mnc.WARM_LEARNED})):
-        opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, **kw)
 
-        def timed_solve():
-            ctx.batch_set(u, a0)                   # inputs resident in HBM before the clock starts
-            ctx.reset_stats()
-            ctx.sync()
-            t1 = time.perf_counter()
-            ctx.batch_solve(opt)
-            return time.perf_counter() - t1
-
-        # the lockstep rounds before the persistent tail: a cold-started batch
-        # needs ~20 sweeps per cell, so more rounds move work from the tail
-        # (one wave per cell) to the lane-parallel rounds; keep the fastest
-        # (and, mixed precision, whether steps redone in FP64 wait for the
-        # tail or join FP64 lockstep rounds)
-        timed_solve()                              # first run allocates
-        tries = {}
-        for defer in ((0, 1) if "precision" in kw else (0,)):
-            for rounds in ISOLATED_ROUNDS:
-                ctx.tune("rounds", rounds)
-                if "precision" in kw:
-                    ctx.tune("mix_defer", defer)
-                tries[(rounds, defer)] = min(timed_solve(), timed_solve())
-        best_rounds, best_defer = min(tries, key=tries.get)
-        ctx.tune("rounds", best_rounds)
-        if "precision" in kw:
-            ctx.tune("mix_defer", best_defer)
-        best = min(timed_solve() for _ in range(3))
-        st = ctx.stats()
-        ug, _, itg, rg = ctx.batch_get()
-        progress("isolated %s: %.3f s" % (name, best))
-        res[name] = {"closures_per_s": n / best, "s": best, "rounds": best_rounds, "mix_defer": best_defer,
-                     "tried_s": {"rounds=%d defer=%d" % k: v for k, v in tries.items()},
-                     "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
-                     "trials_per_closure": st.trial_sum / st.num_dual_solves,
-                     "reg_hist": list(st.hist_reg), "warm_pick_mlp_iso_prev": list(st.warm_pick),
-                     "tail_cells": st.tail_cells, "tail_iters_per_tail_cell": st.tail_iter_sum / max(st.tail_cells, 1)}
+    def solve_forms(u, a0, ls):
         if m > 0:
-            # per cell against the oracle: cells at the same regularization level
-            # solved the same problem to the tolerance; a different level is a
-            # discrete decision of the reference's algorithm taken the other way
-            err = np.linalg.norm(ug[:m] - uo, axis=1) / np.linalg.norm(uo, axis=1)
-            same_r = rg[:m] == ro
-            res[name].update({
-                "oracle_same_r_frac": float(same_r.mean()),
-                "oracle_same_path_frac": float((same_r & (itg[:m] == ito)).mean()),
-                "oracle_same_r_max_rel_u_err": float(err[same_r].max()),
-                "oracle_median_rel_u_err": float(np.median(err)),
-                "oracle_max_rel_u_err": float(err.max())})
+            sto, uo, ao, ito, ro, _ = tab.opt_batch(u[:m], a0[:m], max_iter=100, tol=1e-8, cond_h_max=1e10,
+                                                    nthreads=len(os.sched_getaffinity(0)), line_search=ls)
+        res = {}
+        for name, kw in (("plain-fp64", {}), ("mixed", {"precision": mnc.PREC_MIXED}),
+                         ("learned+mixed", {"precision": mnc.PREC_MIXED, "warm_start": mnc.WARM_LEARNED})):
+            opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, line_search=ls, **kw)
+
+            def timed_solve():
+                ctx.batch_set(u, a0)                   # inputs resident in HBM before the clock starts
+                ctx.reset_stats()
+                ctx.sync()
+                t1 = time.perf_counter()
+                ctx.batch_solve(opt)
+                return time.perf_counter() - t1
+
+            # the lockstep rounds before the persistent tail: a cold-started batch
+            # needs ~20 sweeps per cell, so more rounds move work from the tail
+            # (one wave per cell) to the lane-parallel rounds; keep the fastest
+            # (and, mixed precision, whether steps redone in FP64 wait for the
+            # tail or join FP64 lockstep rounds)
+            timed_solve()                              # first run allocates
+            tries = {}
+            for defer in ((0, 1) if "precision" in kw else (0,)):
+                for rounds in ISOLATED_ROUNDS:
+                    ctx.tune("rounds", rounds)
+                    if "precision" in kw:
+                        ctx.tune("mix_defer", defer)
+                    tries[(rounds, defer)] = min(timed_solve(), timed_solve())
+            best_rounds, best_defer = min(tries, key=tries.get)
+            ctx.tune("rounds", best_rounds)
+            if "precision" in kw:
+                ctx.tune("mix_defer", best_defer)
+            best = min(timed_solve() for _ in range(3))
+            st = ctx.stats()
+            ug, _, itg, rg = ctx.batch_get()
+            progress("isolated %s (line search %d): %.3f s" % (name, ls, best))
+            res[name] = {"closures_per_s": n / best, "s": best, "rounds": best_rounds, "mix_defer": best_defer,
+                         "tried_s": {"rounds=%d defer=%d" % k: v for k, v in tries.items()},
+                         "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
+                         "trials_per_closure": st.trial_sum / st.num_dual_solves,
+                         "reg_hist": list(st.hist_reg), "warm_pick_mlp_iso_prev": list(st.warm_pick),
+                         "warm_fallback": st.warm_fallback, "tail_cells": st.tail_cells,
+                         "tail_iters_per_tail_cell": st.tail_iter_sum / max(st.tail_cells, 1)}
+            if m > 0:
+                # per cell against the oracle: cells at the same regularization level
+                # solved the same problem to the tolerance; a different level is a
+                # discrete decision of the reference's algorithm taken the other way
+                err = np.linalg.norm(ug[:m] - uo, axis=1) / np.linalg.norm(uo, axis=1)
+                same_r = rg[:m] == ro
+                res[name].update({
+                    "oracle_same_r_frac": float(same_r.mean()),
+                    "oracle_same_path_frac": float((same_r & (itg[:m] == ito)).mean()),
+                    "oracle_same_r_max_rel_u_err": float(err[same_r].max()),
+                    "oracle_median_rel_u_err": float(np.median(err)),
+                    "oracle_max_rel_u_err": float(err.max())})
+        return res
+
+    res = solve_forms(u, a0, mnc.LS_REFERENCE)
+    robust = solve_forms(u, a0, mnc.LS_ROBUST)
+    us_, _, _ = wl.isolated_batch(w, p, xi, eta, 15, n, seed, near_boundary_frac=0.0)
+    smooth = solve_forms(us_, wl.isotropic_start(us_), mnc.LS_ROBUST)
     ctx.close()
+
+    def gain(r):
+        return r["learned+mixed"]["closures_per_s"] / r["mixed"]["closures_per_s"]
+
     return {"workload": "isolated M4 batch, %d vectors (20%% near-boundary lobes), Q=16, isotropic start, "
                         "TOL 1e-8, direct Hessian, reference line search" % n,
             "generate_s": gen_s, "oracle_sample": m, "mlp_loss_first_last": [float(loss[:10].mean()),
                                                                            float(loss[-50:].mean())],
-            "forms": res}
+            "forms": res, "learned_over_mixed": gain(res),
+            "robust_ls": {"workload": "the same batch, robust line search", "forms": robust,
+                          "learned_over_mixed": gain(robust)},
+            "smooth_robust_ls": {"workload": "%d smooth vectors (no lobes, the network's training "
+                                             "distribution, fresh draws), robust line search" % n,
+                                 "forms": smooth, "learned_over_mixed": gain(smooth)}}
 
 
 def bench_optcb(mnc, n, seed, oracle_sample, bfgs=False):

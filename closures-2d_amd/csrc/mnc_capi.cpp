@@ -59,6 +59,8 @@ struct mnc_ctx {
                                      // 3: fused, 16x16, directions split into flux_groups
     int flux_groups = 6;             // direction groups of flux_tile 3
     int learned_fused = 1;           // 1: network and judge in one kernel; 0: the GEMM kernels, then the judge
+    int learned_fallback_iters = 10;   // a network-started cell failing at level 0 within this many
+                                     // iterations restarts from isotropicAlpha at level 0
     int learned_gate_log10 = -2;     // the network is consulted when ||g(best start)|| > 10^this ||u||;
                                      // kGateOff: never (no network launches)
     // grid
@@ -301,7 +303,9 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     int *st_i = na ? na->st_i : ctx->d_sti, *lists = na ? na->lists : ctx->d_lists, *ctl = na ? na->ctl : ctx->d_ctl;
     const size_t cap = na ? na->cap : ctx->state_cap;
     if (learned && ctx->learned_gate_log10 < kGateOff && (st = mlp_check(ctx, true))) return st;
-    if (learned && ctx->learned_gate_log10 < kGateOff && !ctx->learned_fused) {   // the unfused path's buffers
+    // the one-kernel network and judge holds the quadrature in LDS
+    const bool net_fused = ctx->learned_fused && ctx->tab.nq <= mnc::kNetMaxQ;
+    if (learned && ctx->learned_gate_log10 < kGateOff && !net_fused) {   // the unfused path's buffers
         if (size_t(ncells) > ctx->guess_cap) {
             if ((st = dalloc(ctx, &ctx->d_guess, size_t(ncells) * 15))) return st;
             ctx->guess_cap = size_t(ncells);
@@ -347,6 +351,7 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
     a.mpv = ctx->d_mpv;
     a.cmix = std::pow(10.0, ctx->mix_cond_log10);
     a.gate = std::pow(10.0, ctx->learned_gate_log10);
+    a.fallback_iters = ctx->learned_fallback_iters;
     a.mlp_w = ctx->d_mlp;
     a.gz = ctx->d_gz;
     a.gg = ctx->d_gg;
@@ -414,7 +419,7 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
             mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
             if ((st = launch(ad, first))) return st;
         }
-        if (ctx->learned_fused) {
+        if (net_fused) {
             // the network and the judge in one kernel on the parked cells
             Timed tm(ctx, MNC_KCLASS_MLP);
             if (!mnc::launch_newton_net_judge(a, t.nm, t.np2, gaunt, ncells, ctx->num_cu, ctx->stream))
@@ -1116,6 +1121,7 @@ int mnc_get_stats(mnc_ctx *ctx, mnc_stats *st) {
     st->tail_cells = h[13];
     st->tail_iter_sum = h[14];
     st->tail_trial_sum = h[15];
+    st->warm_fallback = h[16];
     return MNC_OK;
 }
 
@@ -1223,6 +1229,12 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         ++ctx->epoch;
         return MNC_OK;
     }
+    if (std::strcmp(key, "learned_fallback_iters") == 0) {
+        if (value < 0 || value > 100000) return ctx_fail(ctx, MNC_ERR_ARG, "learned_fallback_iters out of range");
+        ctx->learned_fallback_iters = value;
+        ++ctx->epoch;
+        return MNC_OK;
+    }
     if (std::strcmp(key, "learned_gate_log10") == 0) {
         if (value < -16 || value > kGateOff) return ctx_fail(ctx, MNC_ERR_ARG, "learned_gate_log10 out of range");
         ctx->learned_gate_log10 = value;
@@ -1247,6 +1259,7 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
     if (std::strcmp(key, "flux_tile") == 0) { *value = ctx->flux_tile; return MNC_OK; }
     if (std::strcmp(key, "flux_groups") == 0) { *value = ctx->flux_groups; return MNC_OK; }
     if (std::strcmp(key, "learned_gate_log10") == 0) { *value = ctx->learned_gate_log10; return MNC_OK; }
+    if (std::strcmp(key, "learned_fallback_iters") == 0) { *value = ctx->learned_fallback_iters; return MNC_OK; }
     if (std::strcmp(key, "learned_fused") == 0) { *value = ctx->learned_fused; return MNC_OK; }
     if (std::strcmp(key, "tail_blocks") == 0) {
         *value = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;

@@ -56,6 +56,9 @@ struct NewtonArgs {
                           // and previous alpha has ||g|| > gate * ||u|| (scaled problem)
     int divert;           // learned warm start: this launch's logic holds the gate (first round)
     const float *mlp_w;   // learned warm start: the network's parameters and standardisation (FP32)
+    int fallback_iters;   // learned warm start: a start from the network's guess that meets a
+                          // regularization criterion at level 0 within this many iterations
+                          // restarts from isotropicAlpha at level 0
     int ppn;              // MOMENT_TYPE_PPN: the PPn closure (opt.cpp:113-141, fobj.cpp:137-236,355-428)
     double delta;         // its delta (MOMOPT_DELTA_PPN), unscaled
     // per-cell solver state and the active lists
@@ -86,7 +89,8 @@ struct NewtonArgs {
 constexpr int kCtlWords = 8;
 constexpr int kNumLists = 5;
 constexpr int kMlpListIndex = 4, kMlpListWord = 6;
-constexpr int kStatWords = 16;   // ... [13] tail cells, [14] tail iterations, [15] tail trials
+constexpr int kStatWords = 17;   // ... [13] tail cells, [14] tail iterations, [15] tail trials,
+                                 // [16] learned starts that fell back to isotropicAlpha
 
 // Result layout of v_mfma_f64_16x16x4f64 on gfx950 (measured with a one-wave
 // probe): result element r of lane l is row 4r + l/16, column l%16.  (The
@@ -149,6 +153,8 @@ bool launch_newton_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, hipSt
 // The same second half in one kernel: the network (bf16 matrix cores, weights
 // in LDS) and the FP64 judge for the cells of the network's list (at most
 // max_rows), sixteen cells per wave, `blocks` persistent workgroups.
+// The network and judge kernel holds the quadrature in LDS: up to this many points.
+constexpr int kNetMaxQ = 512;
 bool launch_newton_net_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, int max_rows, int blocks,
                              hipStream_t s);
 // The FP64 matrix-core path (M8: n = 45).  rounds: lockstep rounds per launch

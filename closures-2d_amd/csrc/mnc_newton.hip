@@ -414,15 +414,30 @@ struct Store {
 };
 
 // Per-lane statistics, reduced per block and flushed with one atomic each.
+// One lane's counts within one launch (32-bit: a lane counts far fewer than
+// 2^31 events in a launch; the block sums are 64-bit).
 struct LaneStats {
     int max_iter = 0, err_bits = 0;
-    long long sum = 0, n = 0, trials = 0, hist[5] = {0, 0, 0, 0, 0}, warm[3] = {0, 0, 0}, deferred = 0;
+    int sum = 0, n = 0, trials = 0, hist[5] = {0, 0, 0, 0, 0}, warm[3] = {0, 0, 0}, deferred = 0;
     // persistent tail only: cells taken, Newton iterations and Armijo trials done in it
-    long long tcells = 0, titer = 0, ttrials = 0;
+    int tcells = 0, titer = 0, ttrials = 0;
+    int fallback = 0;   // learned starts abandoned for isotropicAlpha (net_fallback)
 };
 
+// Counting into the arrays with a run-time index would place the whole
+// LaneStats in scratch memory; these select with compares instead.
+__device__ __forceinline__ void count_warm(LaneStats &s, int k) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) s.warm[i] += (k == i);
+}
+__device__ __forceinline__ void count_reg(LaneStats &s, int r) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) s.hist[i] += (r == i);
+}
+
+template <int WAVES = kBlock / 64>
 __device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
-    __shared__ long long red[kBlock / 64][17];
+    __shared__ long long red[WAVES][18];
     for (int off = 32; off > 0; off >>= 1) {
         s.max_iter = max(s.max_iter, __shfl_xor(s.max_iter, off));
         s.err_bits |= __shfl_xor(s.err_bits, off);
@@ -437,6 +452,7 @@ __device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
         s.tcells += __shfl_xor(s.tcells, off);
         s.titer += __shfl_xor(s.titer, off);
         s.ttrials += __shfl_xor(s.ttrials, off);
+        s.fallback += __shfl_xor(s.fallback, off);
     }
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) {
@@ -447,16 +463,17 @@ __device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
         red[w][14] = s.tcells;
         red[w][15] = s.titer;
         red[w][16] = s.ttrials;
+        red[w][17] = s.fallback;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        long long v[17] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int ww = 0; ww < kBlock / 64; ++ww) {
+        long long v[18] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int ww = 0; ww < WAVES; ++ww) {
             v[0] = max(v[0], red[ww][0]);
             v[1] |= red[ww][1];
-            for (int k = 2; k < 17; ++k) v[k] += red[ww][k];
+            for (int k = 2; k < 18; ++k) v[k] += red[ww][k];
         }
-        if (v[3] || v[4] || v[1] || v[10] || v[11] || v[12] || v[13] || v[14]) {
+        if (v[3] || v[4] || v[1] || v[10] || v[11] || v[12] || v[13] || v[14] || v[17]) {
             atomicMax((unsigned long long *)&A.stats[0], (unsigned long long)v[0]);
             atomicAdd((unsigned long long *)&A.stats[1], (unsigned long long)v[2]);
             atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)v[3]);
@@ -469,6 +486,7 @@ __device__ void flush_stats(const NewtonArgs &A, LaneStats &s) {
                 atomicAdd((unsigned long long *)&A.stats[14], (unsigned long long)v[15]);
                 atomicAdd((unsigned long long *)&A.stats[15], (unsigned long long)v[16]);
             }
+            if (v[17]) atomicAdd((unsigned long long *)&A.stats[16], (unsigned long long)v[17]);
             if (v[1]) atomicOr(&A.ctl[3], int(v[1]));
         }
     }
@@ -596,8 +614,24 @@ __device__ int cell_init(const NewtonArgs &A, const Store<NM, NP2> &S, int c, La
     S.i(St::I_MAXIT, c) = A.max_iter;
     S.i(St::I_MODE, c) = ST_NEWTON;
     S.i(St::I_WARM, c) = warm;
-    if (A.warm == MNC_WARM_LEARNED && !A.divert) ++st.warm[warm];
+    if (A.warm == MNC_WARM_LEARNED && !A.divert) count_warm(st, warm);
     return INIT_READY;
+}
+
+// The learned warm start's fallback: a cell started from the network's guess
+// (I_WARM 0) that meets a regularization criterion at level 0 -- the iteration
+// limit, a failed factorisation, an exhausted line search, a condition estimate
+// above cond_h_max -- within A.fallback_iters Newton iterations restarts once
+// from isotropicAlpha at level 0 instead of climbing the ladder: the guess, not
+// the problem, may be what failed, and the start from isotropicAlpha is the
+// reference's own (momopt_update.cpp:236-238 with an isotropic previous alpha).
+// A late failure is the problem's (cells beyond what the quadrature resolves
+// fail at level 0 from any start) and climbs the ladder.  I_WARM becomes
+// kWarmFellBack, so a second failure climbs the ladder as the reference does.
+// Only Mn, where the network exists.
+constexpr int kWarmFellBack = 3;
+__device__ __forceinline__ bool net_fallback(const NewtonArgs &A, int reg, int warm, int iter) {
+    return reg == 0 && warm == 0 && iter <= A.fallback_iters;
 }
 
 // One step of optScaled's loop for cell c after a sweep at its evaluation
@@ -772,7 +806,7 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
                 diverted = true;
                 to_mlp = true;
             } else {
-                ++st.warm[warm];
+                count_warm(st, warm);
             }
         }
         if (!diverted) {
@@ -865,6 +899,17 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
     } else if (defer) {
         mode = ST_NEWTON;
         result = LOGIC_DEFER;
+    } else if (do_reg && !PPN && net_fallback(A, reg, S.i(St::I_WARM, c), iter)) {
+        // the network's guess failed: plain Newton from isotropicAlpha, still at
+        // level 0 (u and the tolerance are those of level 0)
+        S.i(St::I_WARM, c) = kWarmFellBack;
+        ++st.fallback;
+        al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
+#pragma unroll
+        for (int k = 1; k < NM; ++k) al[k] = 0.0;
+        iter = 0;
+        t = 1.0;
+        mode = ST_NEWTON;
     } else if (do_reg) {                                           // opt.cpp:246-273
         if (reg < 4) {
             ++reg;
@@ -918,7 +963,7 @@ __device__ int cell_logic(const NewtonArgs &A, const Store<NM, NP2> &S, int c, L
         st.max_iter = total > st.max_iter ? total : st.max_iter;
         st.sum += total;
         ++st.n;
-        ++st.hist[reg];
+        count_reg(st, reg);
         result = LOGIC_DONE;
     }
     if (result != LOGIC_DONE) {
@@ -1019,10 +1064,11 @@ __global__ __launch_bounds__(kBlock) void judge_kernel(NewtonArgs A) {
         int pick = S.i(St::I_WARM, c);
         if (finite && grad_norm2<NM, NP2>(A, cand, us) < S.d(St::F_FCUR, c)) {
             pick = 0;
+            S.i(St::I_WARM, c) = 0;
 #pragma unroll
             for (int k = 0; k < NM; ++k) S.d(St::F_AL + k, c) = cand[k];
         }
-        ++st.warm[pick];
+        count_warm(st, pick);
         ok = true;
     }
     list_append(A, 1, cw(1), ok, c);
@@ -1034,13 +1080,24 @@ __global__ __launch_bounds__(kBlock) void judge_kernel(NewtonArgs A) {
 // For the cells the first round parked on list kMlpList: the MLP
 // 15 -> 128 -> 128 -> 128 -> 15 (GELU) on the matrix cores, sixteen cells per
 // wave (v_mfma_f32_16x16x32_bf16: bf16 operands, FP32 sums, bias and GELU in
-// FP32, the arithmetic of the unfused GEMM kernels in mnc_mlp.hip), then the
-// FP64 gradient norm of the network's guess (four lanes per cell, each a
-// quarter of the points), which replaces the parked start when it is smaller;
-// the cells join list 1.  The weights sit in LDS in the operand layout (lane l
-// of (k-step, tile) holds W[32 ks + 8 (l/16) + t][16 tile + l%16], t = 0..7),
-// the activations in a per-wave bf16 buffer; nothing goes through HBM but the
-// cells' state.
+// FP32, the arithmetic of the unfused GEMM kernels in mnc_mlp.hip, so the
+// guesses are the same), then the judge: the gradient norm of the scaled
+// problem at the guess, also on the matrix cores (v_mfma_f32_16x16x4_f32), which
+// replaces the parked start when it is smaller; the cells join list 1.
+//
+// The judge per wave, sixteen cells c and 16-point tiles of the quadrature:
+//   a[q][c] = sum_k P[q][k] alpha[c][k]               (MFMA, FP32)
+//   S[k][c] += sum_q P[q][k] w_q exp(a[q][c] - m_c)   (MFMA, FP32)
+// with m_c the running maximum of a over the points seen (the sums rescaled by
+// exp(m_old - m_new) when it grows, so no exponential overflows), and
+// g_k = exp(m_c) S[k][c] - u_k, ||g||^2 in FP64.  FP32 resolves ||g|| to about
+// 1e-6 ||u||: enough to rank two starts (the judge only picks the start; the
+// Newton solve that follows is the usual one).
+//
+// LDS: the weights in the bf16 operand layout (lane l of (k-step, tile) holds
+// W[32 ks + 8 (l/16) + t][16 tile + l%16], t = 0..7), the quadrature as FP32
+// (P[q][0..15] with row stride 17, w_q), and per wave one activation buffer
+// (bf16), which the guesses (FP64) reuse after the last layer.
 typedef __bf16 nbf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short nu16x8 __attribute__((ext_vector_type(8)));
 typedef float nf32x4 __attribute__((ext_vector_type(4)));
@@ -1050,6 +1107,9 @@ constexpr int kNetOffW1 = 0, kNetOffB1 = 15 * 128, kNetOffW2 = kNetOffB1 + 128, 
               kNetOffW3 = kNetOffB2 + 128, kNetOffB3 = kNetOffW3 + 128 * 128, kNetOffW4 = kNetOffB3 + 128,
               kNetOffB4 = kNetOffW4 + 128 * 15;
 static_assert(kNetOffB4 + 15 == kNetPar, "parameter layout");
+constexpr int kNetWaves = 8;                  // waves per block (one block per CU: the weights fill LDS)
+constexpr int kNetActStride = kNetH + 8;      // bf16 activation row (16-byte aligned rows, banks offset)
+constexpr int kNetPStride = 17;               // FP32 quadrature row
 
 __device__ __forceinline__ unsigned short net_bf16(float x) {   // round to nearest even, inf/NaN kept
     const unsigned u = __float_as_uint(x);
@@ -1064,9 +1124,15 @@ struct NetShared {
     nu16x8 w3[4][8][64];
     nu16x8 w4[4][64];          // one output tile (15 outputs, padded to 16)
     float b[3][kNetH], b4[16], st[60];
-    alignas(16) unsigned short act[4][2][16][kNetH + 8];   // per wave: ping-pong activations (bf16), padded rows
-    double guess[4][16][16];                   // per wave: the network's scaled multipliers
+    float pq[kNetMaxQ][kNetPStride];   // P[q][k] (k < 15; column 15 zero) in FP32
+    float wq[kNetMaxQ];
+    // per wave: the activations of its sixteen rows (bf16), then their guesses (FP64)
+    union alignas(16) Act {
+        unsigned short a[16][kNetActStride];
+        double guess[16][16];
+    } act[kNetWaves];
 };
+static_assert(sizeof(NetShared) <= 160 * 1024, "LDS");
 
 // Weight element of the operand layout: W[k][j] of a layer with K inputs and N outputs.
 __device__ __forceinline__ nu16x8 net_operand(const float *W, int K, int N, int ks, int tile, int lane) {
@@ -1080,26 +1146,33 @@ __device__ __forceinline__ nu16x8 net_operand(const float *W, int K, int N, int 
     return v;
 }
 
-// One hidden layer for the wave's sixteen rows: out = gelu(in W + b) (K = 128).
-__device__ __forceinline__ void net_layer(const nu16x8 (&w)[4][8][64], const float *bias, const unsigned short (*in)[kNetH + 8],
-                                          unsigned short (*out)[kNetH + 8], int lane) {
+// One hidden layer for the wave's sixteen rows, in place: act = gelu(act W + b)
+// (K = 128).  Every operand is read before the first result is written.
+__device__ __forceinline__ void net_layer(const nu16x8 (&w)[4][8][64], const float *bias,
+                                          unsigned short (*act)[kNetActStride], int lane) {
+    nu16x8 a[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) a[ks] = *reinterpret_cast<const nu16x8 *>(&act[lane & 15][32 * ks + 8 * (lane >> 4)]);
+    nf32x4 acc[8];
 #pragma unroll
     for (int tile = 0; tile < 8; ++tile) {
-        nf32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        acc[tile] = nf32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {
-            const nu16x8 a = *reinterpret_cast<const nu16x8 *>(&in[lane & 15][32 * ks + 8 * (lane >> 4)]);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(nbf16x8, a),
-                                                          __builtin_bit_cast(nbf16x8, w[ks][tile][lane]), acc, 0, 0, 0);
-        }
+        for (int ks = 0; ks < 4; ++ks)
+            acc[tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(nbf16x8, a[ks]),
+                                                                __builtin_bit_cast(nbf16x8, w[ks][tile][lane]), acc[tile], 0, 0, 0);
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int tile = 0; tile < 8; ++tile) {
         const int j = 16 * tile + (lane & 15);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) out[4 * (lane >> 4) + r][j] = net_bf16(net_gelu(acc[r] + bias[j]));
+        for (int r = 0; r < 4; ++r) act[4 * (lane >> 4) + r][j] = net_bf16(net_gelu(acc[tile][r] + bias[j]));
     }
 }
 
 template <int NM, int NP2>
-__global__ __launch_bounds__(256) void net_judge_kernel(NewtonArgs A) {
+__global__ __launch_bounds__(64 * kNetWaves) void net_judge_kernel(NewtonArgs A) {
     static_assert(NM == 15, "the network is M4's");
     using St = Store<NM, NP2>;
     const St S{A.st_d, A.st_i, A.cap};
@@ -1119,16 +1192,22 @@ __global__ __launch_bounds__(256) void net_judge_kernel(NewtonArgs A) {
     }
     for (int i = threadIdx.x; i < 16; i += blockDim.x) sh.b4[i] = i < 15 ? W[kNetOffB4 + i] : 0.f;
     for (int i = threadIdx.x; i < 60; i += blockDim.x) sh.st[i] = W[kNetPar + i];
+    const int nq = A.nq;                     // the host launches this kernel only for nq <= kNetMaxQ
+    const int nt = (nq + 15) / 16;
+    for (int i = threadIdx.x; i < 16 * nt * 16; i += blockDim.x) {
+        const int q = i / 16, k = i % 16;
+        sh.pq[q][k] = (q < nq && k < NM) ? float(A.p[size_t(q) * NP2 + k]) : 0.f;
+        if (k == 0) sh.wq[q] = q < nq ? float(A.w[q]) : 0.f;
+    }
     __syncthreads();     // the only block-wide barrier: from here on the waves are independent
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int row = lane & 15, part = lane >> 4;
     const int count = A.ctl[cw(kMlpList)];
     const int *list = A.lists + size_t(kMlpList) * A.cap;
-    unsigned short (*act0)[kNetH + 8] = sh.act[wv][0];
-    unsigned short (*act1)[kNetH + 8] = sh.act[wv][1];
-    double (*guess)[16] = sh.guess[wv];
+    unsigned short (*act)[kNetActStride] = sh.act[wv].a;
+    double (*guess)[16] = sh.act[wv].guess;
     LaneStats st;
-    for (int base = (blockIdx.x * 4 + wv) * 16; base < count; base += gridDim.x * 4 * 16) {
+    for (int base = (blockIdx.x * kNetWaves + wv) * 16; base < count; base += gridDim.x * kNetWaves * 16) {
         const int c = base + row < count ? list[base + row] : -1;
         // layer 1: the standardised scaled moments u/u0 (k < 15; lane group
         // `part` supplies k = 8 part + t)
@@ -1147,76 +1226,111 @@ __global__ __launch_bounds__(256) void net_judge_kernel(NewtonArgs A) {
                                                           acc, 0, 0, 0);
             const int j = 16 * tile + row;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) act0[4 * part + r][j] = net_bf16(net_gelu(acc[r] + sh.b[0][j]));
+            for (int r = 0; r < 4; ++r) act[4 * part + r][j] = net_bf16(net_gelu(acc[r] + sh.b[0][j]));
         }
         __builtin_amdgcn_wave_barrier();
-        net_layer(sh.w2, sh.b[1], act0, act1, lane);
+        net_layer(sh.w2, sh.b[1], act, lane);
         __builtin_amdgcn_wave_barrier();
-        net_layer(sh.w3, sh.b[2], act1, act0, lane);
+        net_layer(sh.w3, sh.b[2], act, lane);
         __builtin_amdgcn_wave_barrier();
         {
             nf32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int ks = 0; ks < 4; ++ks) {
-                const nu16x8 a = *reinterpret_cast<const nu16x8 *>(&act0[row][32 * ks + 8 * part]);
+                const nu16x8 a = *reinterpret_cast<const nu16x8 *>(&act[row][32 * ks + 8 * part]);
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(nbf16x8, a),
                                                               __builtin_bit_cast(nbf16x8, sh.w4[ks][lane]), acc, 0, 0, 0);
             }
+            // the guesses overwrite the activations just read: keep the
+            // compiler from moving a store above those loads (the hardware
+            // keeps one wave's LDS accesses in order)
+            __asm__ volatile("" ::: "memory");
+            __builtin_amdgcn_wave_barrier();
             if (row < 15) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r)   // the scaled multipliers: y out_std + out_mean
                     guess[4 * part + r][row] = double(acc[r] + sh.b4[row]) * double(sh.st[45 + row]) + double(sh.st[30 + row]);
             }
         }
+        __asm__ volatile("" ::: "memory");
         __builtin_amdgcn_wave_barrier();
-        // the judge: FP64 gradient of the scaled problem at the guess, this
-        // lane's quarter of the points (q = part, part + 4, ...)
-        double al[NM], g[NM];
+        // ---- the judge (this lane: cell `row`, moments part + 4 ks of its
+        // operand, moments 4 part + r of its sums) ----
+        float alf[4];
         bool finite = true;
 #pragma unroll
-        for (int k = 0; k < NM; ++k) {
-            al[k] = guess[row][k];
-            finite = finite && isfinite(al[k]);
-            g[k] = 0.0;
+        for (int ks = 0; ks < 4; ++ks) {
+            const int k = part + 4 * ks;
+            const double v = k < NM ? guess[row][k] : 0.0;
+            finite = finite && isfinite(v);
+            alf[ks] = float(v);
         }
-        if (c >= 0) {
-            const double *P = A.p, *Wq = A.w;
-            for (int q = part; q < A.nq; q += 4) {
-                const double *pr = P + size_t(q) * NP2;
-                double am = 0.0;
+        finite = finite && __shfl_xor(int(finite), 16) && __shfl_xor(int(finite), 32);
+        if (!finite || c < 0) {
 #pragma unroll
-                for (int k = 0; k < NM; ++k) am = fma(al[k], pr[k], am);
-                const double we = Wq[q] * exp(am);
+            for (int ks = 0; ks < 4; ++ks) alf[ks] = 0.f;
+        }
+        nf32x4 sum = {0.f, 0.f, 0.f, 0.f};
+        float m = -FLT_MAX;
+        for (int t = 0; t < nt; ++t) {
+            nf32x4 a = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int k = 0; k < NM; ++k) g[k] = fma(we, pr[k], g[k]);
+            for (int ks = 0; ks < 4; ++ks)       // a[16t + 4 part + r][row]
+                a = __builtin_amdgcn_mfma_f32_16x16x4f32(sh.pq[16 * t + row][part + 4 * ks], alf[ks], a, 0, 0, 0);
+            const int q0 = 16 * t + 4 * part;
+            float tm = -FLT_MAX;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) tm = q0 + r < nq ? fmaxf(tm, a[r]) : tm;
+            tm = fmaxf(tm, __shfl_xor(tm, 16));
+            tm = fmaxf(tm, __shfl_xor(tm, 32));
+            const float mn = fmaxf(m, tm);
+            const float sc = __expf(m - mn);
+            m = mn;
+            float e[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                sum[r] *= sc;
+                e[r] = sh.wq[q0 + r] * __expf(a[r] - mn);   // w = 0 past nq
+            }
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks)       // S[4 part + r][row] += P[q][.] w e over q = 16t + 4 (l/16) + ks
+                sum = __builtin_amdgcn_mfma_f32_16x16x4f32(sh.pq[16 * t + 4 * part + ks][row], e[ks], sum, 0, 0, 0);
+        }
+        const double em = exp(double(m));
+        double n2 = 0.0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int k = 4 * part + r;
+            if (c >= 0 && k < NM) {
+                const double gk = em * double(sum[r]) - S.d(St::F_U + k, c);
+                n2 = fma(gk, gk, n2);
             }
         }
+        n2 += __shfl_xor(n2, 16);
+        n2 += __shfl_xor(n2, 32);
+        const bool take = c >= 0 && finite && n2 < S.d(St::F_FCUR, c);
+        if (take) {
 #pragma unroll
-        for (int k = 0; k < NM; ++k) {
-            g[k] += __shfl_xor(g[k], 16);
-            g[k] += __shfl_xor(g[k], 32);
+            for (int r = 0; r < 4; ++r) {
+                const int k = 4 * part + r;
+                if (k < NM) S.d(St::F_AL + k, c) = guess[row][k];
+            }
         }
         bool ok = false;
         if (c >= 0 && part == 0) {
-            double n2 = 0.0;
-#pragma unroll
-            for (int k = 0; k < NM; ++k) {
-                const double d = g[k] - S.d(St::F_U + k, c);
-                n2 = fma(d, d, n2);
-            }
             int pick = S.i(St::I_WARM, c);
-            if (finite && n2 < S.d(St::F_FCUR, c)) {
+            if (take) {
                 pick = 0;
-#pragma unroll
-                for (int k = 0; k < NM; ++k) S.d(St::F_AL + k, c) = al[k];
+                S.i(St::I_WARM, c) = 0;
             }
-            ++st.warm[pick];
+            count_warm(st, pick);
             ok = true;
         }
         list_append(A, 1, cw(1), ok, c);
+        __asm__ volatile("" ::: "memory");
         __builtin_amdgcn_wave_barrier();
     }
-    flush_stats(A, st);
+    flush_stats<kNetWaves>(A, st);
 }
 
 // One sweep for every cell on list `lin`; also clears the count of the list
@@ -1601,7 +1715,7 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
 #pragma unroll
         for (int k = 0; k < NM; ++k) al[k] = S.d(St::F_AL + k, c);
         int mode = S.i(St::I_MODE, c), iter = S.i(St::I_ITER, c), total = S.i(St::I_TOTAL, c);
-        int reg = S.i(St::I_REG, c), maxit = S.i(St::I_MAXIT, c);
+        int reg = S.i(St::I_REG, c), maxit = S.i(St::I_MAXIT, c), warm = S.i(St::I_WARM, c);
         double t = S.d(St::F_T, c), rtol = S.d(St::F_RTOL, c);
         if (lane == 0) {
             ++st.tcells;
@@ -1969,7 +2083,19 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
             }
             __builtin_amdgcn_wave_barrier();
             bool done = false;
-            if (do_reg) {
+            if (do_reg && !PPN && net_fallback(A, reg, warm, iter)) {   // the network's guess failed
+                warm = kWarmFellBack;
+                if (lane == 0) {
+                    S.i(St::I_WARM, c) = kWarmFellBack;
+                    ++st.fallback;
+                }
+                al[0] = 2.0 * kSqrtPi * log(0.5 / kSqrtPi);
+#pragma unroll
+                for (int k = 1; k < NM; ++k) al[k] = 0.0;
+                iter = 0;
+                t = 1.0;
+                mode = ST_NEWTON;
+            } else if (do_reg) {
                 if (reg < 4) {
                     ++reg;
                     const double r = kReg[reg];
@@ -2022,7 +2148,7 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
                     st.max_iter = total > st.max_iter ? total : st.max_iter;
                     st.sum += total;
                     ++st.n;
-                    ++st.hist[reg];
+                    count_reg(st, reg);
                 }
                 done = true;
             }
@@ -2040,6 +2166,7 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
         atomicAdd((unsigned long long *)&A.stats[2], (unsigned long long)st.n);
         for (int r = 0; r < 5; ++r) atomicAdd((unsigned long long *)&A.stats[3 + r], (unsigned long long)st.hist[r]);
         atomicAdd((unsigned long long *)&A.stats[8], (unsigned long long)st.trials);
+        if (st.fallback) atomicAdd((unsigned long long *)&A.stats[16], (unsigned long long)st.fallback);
         if (st.err_bits) atomicOr(&A.ctl[3], st.err_bits);
     }
 }
@@ -2457,7 +2584,7 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
                 st.max_iter = total > st.max_iter ? total : st.max_iter;
                 st.sum += total;
                 ++st.n;
-                ++st.hist[reg];
+                count_reg(st, reg);
             }
             done = true;
         }
@@ -2681,9 +2808,11 @@ hipError_t noalloc_scratch_address(int device, void **ptr) {
 
 bool launch_newton_net_judge(const NewtonArgs &a, int nm, int np2, bool gaunt, int max_rows, int blocks,
                              hipStream_t s) {
-    const int b = std::max(1, std::min(blocks, (max_rows + 63) / 64));
-    if (nm == 15 && !gaunt && np2 == 15) hipLaunchKernelGGL((net_judge_kernel<15, 15>), dim3(b), dim3(256), 0, s, a);
-    else if (nm == 15 && gaunt && np2 == 45) hipLaunchKernelGGL((net_judge_kernel<15, 45>), dim3(b), dim3(256), 0, s, a);
+    if (a.nq > kNetMaxQ) return false;
+    const int b = std::max(1, std::min(blocks, (max_rows + 16 * kNetWaves - 1) / (16 * kNetWaves)));
+    const dim3 blk(64 * kNetWaves);
+    if (nm == 15 && !gaunt && np2 == 15) hipLaunchKernelGGL((net_judge_kernel<15, 15>), dim3(b), blk, 0, s, a);
+    else if (nm == 15 && gaunt && np2 == 45) hipLaunchKernelGGL((net_judge_kernel<15, 45>), dim3(b), blk, 0, s, a);
     else return false;
     return true;
 }

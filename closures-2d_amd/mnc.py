@@ -54,7 +54,8 @@ class Stats(C.Structure):
                 ("iter_sum", C.c_longlong), ("hist_reg", C.c_longlong * 5),
                 ("trial_sum", C.c_longlong), ("warm_pick", C.c_longlong * 3),
                 ("mixed_deferred", C.c_longlong), ("tail_cells", C.c_longlong),
-                ("tail_iter_sum", C.c_longlong), ("tail_trial_sum", C.c_longlong)]
+                ("tail_iter_sum", C.c_longlong), ("tail_trial_sum", C.c_longlong),
+                ("warm_fallback", C.c_longlong)]
 
 
 class MncError(RuntimeError):

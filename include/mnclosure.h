@@ -51,7 +51,10 @@ extern "C" {
 #define MNC_WARM_PREVIOUS 0    /* start from the alpha passed in (the reference) */
 #define MNC_WARM_ISOTROPIC 1   /* start from isotropicAlpha (opt.cpp:27-35)      */
 #define MNC_WARM_LEARNED 2     /* MLP prediction, judged against the candidates
-                                  above by the FP64 gradient norm               */
+                                  above by the FP64 gradient norm; a cell started
+                                  from the prediction that would regularize at
+                                  level 0 restarts once from isotropicAlpha at
+                                  level 0 (mnc_stats.warm_fallback)             */
 
 #define MNC_LS_REFERENCE 0     /* Armijo on f exactly as linesearch.cpp:21-99     */
 #define MNC_LS_ROBUST 1        /* the same, but a full Newton step whose predicted
@@ -147,6 +150,10 @@ typedef struct {
                                     iterating after the lockstep rounds)            */
     long long tail_iter_sum;     /* Newton iterations done in the tail              */
     long long tail_trial_sum;    /* Armijo trials done in the tail                  */
+    long long warm_fallback;     /* MNC_WARM_LEARNED cells started from the MLP's
+                                    guess that met a regularization criterion at
+                                    level 0 and restarted from isotropicAlpha at
+                                    level 0 (once per cell)                          */
 } mnc_stats;
 
 /* Creates a context: builds the quadrature, the real spherical-harmonic table
@@ -294,9 +301,16 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *                  with the start is launched
  *   "learned_fused"  MNC_WARM_LEARNED: 1 (default): the network and the
  *                  comparison of its guess with the parked start run in one
- *                  kernel (weights in LDS, activations never leave the chip);
- *                  0: the separate GEMM kernels of mnc_mlp_predict, then the
- *                  comparison kernel
+ *                  kernel (weights and quadrature in LDS, activations never
+ *                  leave the chip, the guess's gradient norm from FP32
+ *                  matrix-core sums; quadratures up to 512 points, larger ones
+ *                  take the 0 path); 0: the separate GEMM kernels of
+ *                  mnc_mlp_predict, then the FP64 comparison kernel
+ *   "learned_fallback_iters"  MNC_WARM_LEARNED: a cell started from the
+ *                  network's guess that meets a regularization criterion at
+ *                  level 0 within this many Newton iterations restarts once
+ *                  from isotropicAlpha at level 0 (0..100000, default 10;
+ *                  0 never)
  *   "flux_tile"    0: ansatz written to HBM, then the flux; 1: ansatz fused
  *                  into the flux with 8x8-cell LDS tiles; 2: 16x16 tiles
  *                  (n <= 15); these three give identical results.  3 (default;

@@ -221,12 +221,13 @@ def test_learned_start_with_network_off_is_the_previous_start():
 @pytest.mark.gpu
 def test_fused_network_judge_matches_the_gemm_path():
     """The learned start's network and judge in one kernel (learned_fused 1,
-    the default: weights in LDS, activations on chip) against the separate
-    GEMM kernels and comparison kernel (learned_fused 0): the same bf16
+    the default: weights and quadrature in LDS, activations on chip, the
+    judge's gradient on the matrix cores in FP32) against the separate GEMM
+    kernels and the FP64 comparison kernel (learned_fused 0): the same bf16
     operands and FP32 sums in the same order, so the network's guesses are
-    the same and so are the starts taken; the judge's FP64 gradient norm sums
-    the points in another order, which can tip a near-tie.  Solved from those
-    starts the batches agree cell for cell."""
+    the same; the two judges' gradient norms differ at rounding level (FP32
+    against FP64), which can tip a near-tie between the guess and the parked
+    start.  Solved from those starts the batches agree cell for cell."""
     print("case fused network judge", flush=True)
     mnc, ctx = _ctx()
     tab = oracle().Tables(4, 16, False)
@@ -304,3 +305,39 @@ def test_learned_lattice_steps_match_oracle():
     assert picks[0] > 0
     # the same bound as the mixed form's: every solve returns ||g|| <= tol ||u||
     assert e <= 4e-8
+
+
+@pytest.mark.gpu
+def test_learned_fallback_restores_the_isotropic_answer():
+    """With the reference line search at TOL 1e-8 a start close to the solution
+    can stall just above the tolerance (the Armijo test cannot resolve the
+    decrease of f, t = 0) and the reference algorithm then regularizes: about
+    1% of smooth cells do so from the network's guess though they converge
+    unregularized from isotropicAlpha.  The fallback (learned_fallback_iters,
+    default 10) restarts such a cell once from isotropicAlpha at level 0:
+    the learned form then regularizes no more cells than the isotropic start
+    does, and with the fallback off (0) it regularizes more."""
+    print("case learned fallback", flush=True)
+    mnc, ctx = _ctx()
+    tab = oracle().Tables(4, 16, False)
+    us, al = _smooth_pairs(mnc, tab, 200000, 51)
+    ctx.mlp_init(seed=52)
+    ctx.mlp_train(us, al, steps=1500, batch=4096, lr=1e-3, seed=53)
+    rng = np.random.default_rng(54)
+    u, astar, lobe = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, 100000, rng, near_boundary_frac=0.0)
+    a0 = isotropic_start(u)
+    res = {}
+    for name, warm, fb in (("iso", mnc.WARM_ISOTROPIC, 10), ("learned", mnc.WARM_LEARNED, 10),
+                           ("no-fallback", mnc.WARM_LEARNED, 0)):
+        ctx.tune("learned_fallback_iters", fb)
+        ctx.reset_stats()
+        opt = mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, precision=mnc.PREC_MIXED, warm_start=warm)
+        st, uo, ao, it, r, ng = ctx.opt_batch_host(u, a0, opt)
+        assert st == 0
+        res[name] = (r, ctx.stats().warm_fallback)
+    ctx.close()
+    reg = {k: int((v[0] > 0).sum()) for k, v in res.items()}
+    print("regularized cells: %s; fallbacks with the default %d" % (reg, res["learned"][1]))
+    assert res["learned"][1] > 0 and res["no-fallback"][1] == 0 and res["iso"][1] == 0
+    assert reg["learned"] <= reg["iso"]
+    assert reg["no-fallback"] > reg["learned"]
