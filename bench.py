@@ -778,8 +778,13 @@ def main():
     t_end = float(sum(best["dts"]))
     net = ""
     if form == "learned+mixed":
-        net = " (network %s)" % ("off: the autotuner's gate 5" if best["tuned"].get("learned_gate_log10") == 5
-                                 else "on")
+        gate = best["tuned"].get("learned_gate_log10")
+        picks = list(best["stats"].warm_pick)
+        if gate == 5:
+            net = " (network off: the autotuner's gate 5)"
+        else:
+            net = " (network gate 10^%d: %.2f%% of the closures start from its guess)" % (
+                gate, 100.0 * picks[0] / max(sum(picks), 1))
     out = {
         "metric": "converged Mn closures/s, M4 lattice 280^2, steps 1-%d of 569 (t = 0 to %.4f), TOL 1e-8, "
                   "%s%s vs plain FP64" % (args.steps, t_end, form, net),

@@ -229,28 +229,6 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    return v;
-}
-
-// True when the condition test `1/rcond > limit` (opt.cpp:231-244) is false
-// whatever the estimate: Higham's estimate never exceeds ||A^-1||_1 <=
-// sqrt(n) / lambda_min, and Gershgorin gives lambda_min >= gersh =
-// min_i (2 a_ii - sum_j |a_ij|).  With a factor 2 of margin for rounding, the
-// estimate (the condition number's main cost after the factorisation) is
-// skipped when anorm sqrt(n) / gersh cannot reach the limit; the decision is
-// the reference's either way.
-__device__ __forceinline__ bool cond_proven_below(double anorm, double gersh, int n, double limit) {
-    return gersh > 0.0 && 2.0 * anorm * sqrt(double(n)) <= limit * gersh;
-}
-
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
-    return v;
-}
 
 enum : int { ST_NEWTON = 0, ST_TRIAL = 1 };
 constexpr int kHalvings = 8;   // line-search halvings evaluated per wave sweep in the tail
@@ -1484,156 +1462,6 @@ __global__ __launch_bounds__(kBlock) void tail_wave_kernel(NewtonArgs A, int lin
 // order of every sum is the serial one (the column-oriented back substitution
 // subtracts x_{n-1}, x_{n-2}, ... in the order of the dot-product form), so a
 // cell takes the same path here as in the lockstep kernels up to FMA rounding.
-// ---- wave-parallel dense algebra with the rows in registers ----------------
-//
-// Lane i (< NM) holds row i of the symmetric matrix (row[k] = A[i][k]); after
-// reg_cholesky, row[k] = L[i][k] for k <= i, and col[k] = L[k][i] (k > i) is
-// filled from it for the back substitution.  Entries of other lanes are read
-// with v_readlane (wave-uniform lane index), so each dependent step of the
-// factorisation and of the triangular solves is a register operation rather
-// than an LDS round trip.  The arithmetic and its order are those of
-// cholesky<N>, apply_inv<N> and rcond<N>.
-__device__ __forceinline__ double readlane_d(double v, int l) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffu), l);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-template <int NM>
-__device__ __forceinline__ bool reg_cholesky(double (&row)[NM], int lane) {
-    bool ok = true;
-#pragma unroll
-    for (int j = 0; j < NM; ++j) {
-        double dot = 0.0;
-#pragma unroll
-        for (int k = 0; k < j; ++k) dot = fma(row[k], row[k], dot);
-        double ajj = readlane_d(row[j] - dot, j);
-        ok = ok && (ajj > 0.0);
-        ajj = sqrt(ajj > 0.0 ? ajj : 1.0);
-        double y = row[j];
-#pragma unroll
-        for (int k = 0; k < j; ++k) y = fma(-readlane_d(row[k], j), row[k], y);
-        const double r = 1.0 / ajj;
-        if (lane == j) row[j] = ajj;
-        else if (lane > j && lane < NM) row[j] = y * r;
-    }
-    return ok;
-}
-
-// x <- (L L^T)^{-1} x, x_i on lane i, with the reciprocals of the diagonal
-// (wave-uniform rd[j]) in place of divisions: the tail's Newton direction and
-// the condition estimate's repeated solves.
-template <int NM>
-__device__ __forceinline__ void reg_apply_inv_rcp(const double (&row)[NM], const double (&col)[NM],
-                                                  const double (&rd)[NM], double &x, int lane) {
-#pragma unroll
-    for (int j = 0; j < NM; ++j) {
-        const double xj = readlane_d(x, j) * rd[j];
-        if (lane == j) x = xj;
-        else if (lane > j && lane < NM) x = fma(-xj, row[j], x);
-    }
-#pragma unroll
-    for (int j = NM - 1; j >= 0; --j) {
-        const double xj = readlane_d(x, j) * rd[j];
-        if (lane == j) x = xj;
-        else if (lane < j) x = fma(-col[j], xj, x);
-    }
-}
-
-// Reciprocal 1-norm condition estimate (dpocon's role; Higham 1988), x_i on lane i.
-template <int NM>
-__device__ __forceinline__ double reg_rcond(const double (&row)[NM], const double (&col)[NM], double anorm, int lane) {
-    if (anorm == 0.0) return 0.0;
-    unsigned isgn = 0;
-    double est = 0.0;
-    int jump = 1, j = 0, iter = 0;
-    double x = lane < NM ? 1.0 / NM : 0.0;
-    double rd[NM];
-#pragma unroll
-    for (int k = 0; k < NM; ++k) rd[k] = 1.0 / readlane_d(row[k], k);
-#pragma unroll 1
-    for (int guard = 0; guard < 16; ++guard) {
-        reg_apply_inv_rcp<NM>(row, col, rd, x, lane);
-        double s = 0.0, m = -1.0;
-        int im = 0;
-#pragma unroll
-        for (int i = 0; i < NM; ++i) {                 // dasum / idamax in the serial order
-            const double v = fabs(readlane_d(x, i));
-            s += v;
-            if (v > m) { m = v; im = i; }
-        }
-        const unsigned sg = (unsigned)(__ballot(lane < NM && !(x >= 0.0)) & ((NM >= 32) ? 0xffffffffull : ((1ull << NM) - 1)));
-        const double xj_old = readlane_d(x, j);
-        double nx = 0.0;
-        bool done = false;
-        if (jump == 1) {
-            if (NM == 1) { est = fabs(readlane_d(x, 0)); done = true; }
-            else { est = s; isgn = sg; nx = (lane < NM && (sg >> lane & 1u)) ? -1.0 : 1.0; jump = 2; }
-        } else if (jump == 2) {
-            j = im; iter = 2; nx = (lane == j) ? 1.0 : 0.0; jump = 3;
-        } else if (jump == 3) {
-            const double estold = est;
-            est = s;
-            if (sg == isgn || est <= estold) {
-                nx = ((lane & 1) ? -1.0 : 1.0) * (1.0 + double(lane) / double(NM - 1));
-                jump = 5;
-            } else {
-                nx = (lane < NM && (sg >> lane & 1u)) ? -1.0 : 1.0;
-                isgn = sg;
-                jump = 4;
-            }
-        } else if (jump == 4) {
-            j = im;
-            if (xj_old != m && iter < 5) {
-                ++iter; nx = (lane == j) ? 1.0 : 0.0; jump = 3;
-            } else {
-                nx = ((lane & 1) ? -1.0 : 1.0) * (1.0 + double(lane) / double(NM - 1));
-                jump = 5;
-            }
-        } else {
-            const double temp = 2.0 * (s / double(3 * NM));
-            if (temp > est) est = temp;
-            done = true;
-        }
-        if (done) break;
-        x = lane < NM ? nx : 0.0;
-    }
-    if (!isfinite(est) || est == 0.0) return 0.0;
-    return (1.0 / est) / anorm;
-}
-
-// ||A^{-1}||_1 for A = L L^T from the factor (rows in registers as after
-// reg_cholesky: lane i holds L[i][k], k <= i; rd[k] = 1 / L[k][k]).  Lane j
-// forms column j of Z = L^{-1} by forward substitution (every column at once,
-// the products L[i][k] z_k issued as soon as z_k is final), then column j of
-// A^{-1} = Z^T Z as dot products of Z's columns, and the wave takes the largest
-// absolute column sum: about 250 register reads and 240 FMAs, against the tens
-// of dependent triangular-solve steps of the estimate.  Higham's estimate never
-// exceeds ||A^{-1}||_1, so where anorm times this (with a factor 2 of margin for
-// rounding) is within the limit the estimate cannot fail the condition test.
-template <int NM>
-__device__ __forceinline__ double reg_inv_norm1(const double (&row)[NM], const double (&rd)[NM], int lane) {
-    double z[NM];
-#pragma unroll
-    for (int i = 0; i < NM; ++i) z[i] = (i == lane) ? 1.0 : 0.0;
-#pragma unroll
-    for (int k = 0; k < NM; ++k) {
-        z[k] = z[k] * rd[k];
-#pragma unroll
-        for (int i = k + 1; i < NM; ++i) z[i] = fma(-readlane_d(row[k], i), z[k], z[i]);
-    }
-    double c = 0.0;
-#pragma unroll
-    for (int i = 0; i < NM; ++i) {
-        double d = 0.0;
-#pragma unroll
-        for (int k = i; k < NM; ++k) d = fma(readlane_d(z[k], i), z[k], d);
-        c += fabs(d);
-    }
-    return wave_max(lane < NM ? c : 0.0);
-}
-
 // Per-wave state of the cooperative tail.  The direct Hessian's TRI sums are
 // accumulated in NPASS passes over the quadrature (a flat split of the packed
 // triangle: KH sums, then the rest), so that a pass's partial sums fit the

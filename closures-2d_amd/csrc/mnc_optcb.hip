@@ -4,7 +4,7 @@
 // One 64-lane block per cell.  The cell's basis P2M (n x n lower triangular,
 // column-major as the reference's BLAS calls see it), its multipliers alphaP
 // in that basis, and the quadrature table seen through the basis, pP[q][k] =
-// (P2M^{-1} p_q)_k, live in HBM (pP in a workspace of nq*n doubles per cell).
+// (P2M^{-1} p_q)_k, live in HBM (pP in a workspace of nq*n doubles per block).
 // The quadrature work -- the table's triangular solves, f, g and the Hessian
 // sums, the line search's f -- is spread over the lanes (lane l takes the
 // points q = l, l + 64, ...; wave sums by butterfly).  The n x n algebra
@@ -31,6 +31,7 @@ struct CbShared {
     double gOld[N], sD[N], yD[N], tmp[N];
     double u[N], uP[N], aP[N], aP0[N], gP[N], dP[N], gM[N], aold[N], aM[N];
     double x[N];
+    double e[64];                      // a 64-point chunk's exponentials (cb_fobj_full)
     double f, fscale, rtol, fnew, t, err;
     int action, done, converged;
     int p2min_id;                      // the restart basis: 1 the identity, 0 the cell's input basis
@@ -101,6 +102,72 @@ __device__ void cb_trmm_rln(const double *a, double *b) {
         }
     }
 }
+// The same four triangular kernels shared by the wave, lane i owning x_i (the
+// vector in LDS, read at entry and written at exit); each element sees
+// Netlib's operations in Netlib's order, only the independent ones of a step
+// run side by side.  Row-major (rm) factors are the Hessian's L; column-major
+// (cm) ones the basis P2M.
+// dtrsv('L','N','N'), row-major: column j divides x_j (if nonzero), then x_i -= x_j l_ij
+template <int N>
+__device__ void cb_par_trsv_ln_rm(const double *l, double *x, int lane) {
+    double xi = lane < N ? x[lane] : 0.0;
+    for (int j = 0; j < N; j++) {
+        double xj = readlane_d(xi, j);
+        if (xj != 0.0) {
+            xj = xj / l[j * N + j];
+            if (lane == j) xi = xj;
+            else if (lane > j && lane < N) xi = xi - xj * l[lane * N + j];
+        }
+    }
+    __syncthreads();
+    if (lane < N) x[lane] = xi;
+    __syncthreads();
+}
+// dtrmv('L','T','N'), row-major: x_j <- x_j l_jj + sum_{i > j} l_ij x_i (old x)
+template <int N>
+__device__ void cb_par_trmv_lt_rm(const double *l, double *x, int lane) {
+    const double xo = lane < N ? x[lane] : 0.0;
+    double temp = lane < N ? xo * l[lane * N + lane] : 0.0;
+    for (int i = 1; i < N; i++) {
+        const double xi = readlane_d(xo, i);
+        if (lane < i) temp = temp + l[i * N + lane] * xi;
+    }
+    __syncthreads();
+    if (lane < N) x[lane] = temp;
+    __syncthreads();
+}
+// dtrmv('L','N','N'), column-major: for j = n-1..0 with x_j != 0, x_i += x_j a_ij
+// (i > j) and x_j *= a_jj; x_j is still the input when column j is applied
+template <int N>
+__device__ void cb_par_trmv_ln_cm(const double *a, double *x, int lane) {
+    const double xo = lane < N ? x[lane] : 0.0;
+    double xi = xo;
+    for (int j = N - 1; j >= 0; j--) {
+        const double xj = readlane_d(xo, j);
+        if (xj != 0.0) {
+            if (lane == j) xi = xi * a[j + j * N];
+            else if (lane > j && lane < N) xi = xi + xj * a[lane + j * N];
+        }
+    }
+    __syncthreads();
+    if (lane < N) x[lane] = xi;
+    __syncthreads();
+}
+// dtrsv('L','T','N'), column-major: for i = n-1..0, x_i = temp_i / a_ii, then
+// temp_j -= a_ij x_i for j < i (the dot products' order)
+template <int N>
+__device__ void cb_par_trsv_lt_cm(const double *a, double *x, int lane) {
+    double tj = lane < N ? x[lane] : 0.0;
+    for (int i = N - 1; i >= 0; i--) {
+        const double xi = readlane_d(tj, i) / a[i + i * N];
+        if (lane == i) tj = xi;
+        else if (lane < i) tj = tj - a[i + lane * N] * xi;
+    }
+    __syncthreads();
+    if (lane < N) x[lane] = tj;
+    __syncthreads();
+}
+
 template <int N>
 __device__ void cb_identity(double *a) {
     for (int i = 0; i < N * N; i++) a[i] = 0.0;
@@ -147,34 +214,43 @@ typedef double cb_f64x4 __attribute__((ext_vector_type(4)));
 
 // f, g and H in the current basis (fobj.cpp:37-129,255-287 on pP).  H is an
 // FP64 matrix-core sum, 16 x 16 over 4 points per instruction: lane l supplies
-// point q0 + l/16, component l%16 (A = w e p_c, B = p_c), its exponent the
-// sequential dot over that point's row.  Result element r of lane l is row
-// 4r + l/16, column l%16; the lower triangle is stored and mirrored into L.
-// Lane 0 forms g and f from the first column.
+// point q0 + 4s + l/16, component l%16 (A = w p_c e, B = p_c).  Each point's
+// exponential (the sequential dot over its row) is formed once, by lane
+// q - q0 of its 64-point chunk, and read from LDS by the sixteen lanes that
+// use it.  Result element r of lane l is row 4r + l/16, column l%16; the lower
+// triangle is stored and mirrored into L.  Lane 0 forms g and f from the first
+// column.
 template <int N>
 __device__ void cb_fobj_full(const CbArgs &A, const double *pP, CbShared<N> &sh, int lane) {
     static_assert(N <= 16, "one 16 x 16 tile");
     cb_f64x4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
     const int c = lane & 15, k = lane >> 4;
-    for (int q0 = 0; q0 < A.nq; q0 += 8) {
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            const int q = q0 + 4 * h + k;
-            double av = 0.0, bv = 0.0;
+    for (int q0 = 0; q0 < A.nq; q0 += 64) {
+        {
+            const int q = q0 + lane;
+            double e = 0.0;
             if (q < A.nq) {
                 const double *row = pP + size_t(q) * N;
                 double temp = 0.0;
 #pragma unroll
                 for (int i = 0; i < N; i++) temp = temp + sh.aP[i] * row[i];
-                const double e = exp(temp);
-                if (c < N) {
-                    bv = row[c];
-                    av = A.w[q] * bv * e;
-                }
+                e = exp(temp);
             }
-            if (h == 0) acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc0, 0, 0, 0);
+            sh.e[lane] = e;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int st = 0; st < 16; st++) {
+            const int q = q0 + 4 * st + k;
+            double av = 0.0, bv = 0.0;
+            if (q < A.nq && c < N) {
+                bv = pP[size_t(q) * N + c];
+                av = A.w[q] * bv * sh.e[4 * st + k];
+            }
+            if ((st & 1) == 0) acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc0, 0, 0, 0);
             else acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc1, 0, 0, 0);
         }
+        __syncthreads();
     }
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -330,6 +406,9 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
     __shared__ CbShared<N> sh;
     const int lane = threadIdx.x;
     for (int c = blockIdx.x; c < A.ncells; c += gridDim.x) {
+        // the table seen through the basis: in an HBM workspace per block (an
+        // LDS copy, 61 KB at n = 15 and 512 points, would leave room for two
+        // cells per CU instead of four, and measured slower)
         double *pP = A.ws + size_t(blockIdx.x) * A.nq * N;
         double *ug = A.u + size_t(c) * N;
         double *Bg = A.P2M + size_t(c) * N * N;
@@ -365,15 +444,37 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
             __syncthreads();
             // :247-259: the factor, the 1-norm and the condition estimate, the
             // wave sharing each step's independent updates
-            double norm = 0.0;                              // utils_norm1: row sums in order, their max
-            if (lane < N)
-                for (int j = 0; j < N; j++) norm += fabs(sh.L[lane * N + j]);
+            // the factor with the rows in registers (lane i: row i), then back
+            // to LDS for the basis update and the table solve
+            double row[N];
+            double norm = 0.0, dg = 0.0;                    // utils_norm1: row sums in order, their max
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) norm = fmax(norm, __shfl_xor(norm, o));
+            for (int j = 0; j < N; j++) {
+                row[j] = lane < N ? sh.L[lane * N + j] : 0.0;
+                norm += fabs(row[j]);
+                dg = (j == lane) ? row[j] : dg;
+            }
+            const double gersh = wave_min(lane < N ? 2.0 * dg - norm : DBL_MAX);
+            norm = wave_max(lane < N ? norm : 0.0);
+            const bool chol_ok = reg_cholesky<N>(row, lane);
             __syncthreads();
-            const bool chol_ok = coop_cholesky<N>(reinterpret_cast<double (*)[N]>(sh.L), lane);
+            if (lane < N)
+                for (int j = 0; j < N; j++) sh.L[lane * N + j] = row[j];
+            __syncthreads();
             double condest = 0.0;
-            if (chol_ok) condest = 1.0 / coop_rcond<N>(reinterpret_cast<double (*)[N]>(sh.L), sh.x, norm, lane);
+            // the estimate only where the bounds cannot decide the test (as in
+            // the Newton tail)
+            if (chol_ok && !cond_proven_below(norm, gersh, N, A.cond_h_max)) {
+                double rd[N];
+#pragma unroll
+                for (int k = 0; k < N; k++) rd[k] = 1.0 / readlane_d(row[k], k);
+                if (!(2.0 * norm * reg_inv_norm1<N>(row, rd, lane) <= A.cond_h_max)) {
+                    double col[N];
+#pragma unroll
+                    for (int k = 0; k < N; k++) col[k] = (lane < N && k > lane) ? sh.L[k * N + lane] : 0.0;
+                    condest = 1.0 / reg_rcond<N>(row, col, norm, lane);
+                }
+            }
             if (lane == 0) {
                 if (total == 1 && (!chol_ok || condest > A.cond_h_max)) sh.action = CB_RESET;
                 else if (iter >= maxit || !chol_ok || t == 0.0 || condest > A.cond_h_max) sh.action = CB_REG;
@@ -455,11 +556,9 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
             if (lane < N)
                 for (int j = 0; j < N; j++) sh.P2M[lane + j * N] = brow[j];
             __syncthreads();
-            if (lane == 0) {
-                cb_trsv_ln<N, true>(sh.L, sh.gP);
-                cb_trmv_lt<N, true>(sh.L, sh.aP);
-                cb_trsv_ln<N, true>(sh.L, sh.uP);
-            }
+            cb_par_trsv_ln_rm<N>(sh.L, sh.gP, lane);
+            cb_par_trmv_lt_rm<N>(sh.L, sh.aP, lane);
+            cb_par_trsv_ln_rm<N>(sh.L, sh.uP, lane);
             cb_table_solve<N>(A, pP, sh.L, lane);
             if (A.bfgs)                                     // optbfgs.cpp:371-373: H = H^-1 = I
                 for (int k = lane; k < N * N; k += 64) sh.hB[k] = sh.hI[k] = (k % N == k / N) ? 1.0 : 0.0;
@@ -478,15 +577,18 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
                     for (int i = 0; i < N; i++) sh.dP[i] = -sh.gP[i];
                 }
                 __syncthreads();
+                if (lane < N) sh.gM[lane] = sh.gP[lane];       // :352-354
+                __syncthreads();
+                cb_par_trmv_ln_cm<N>(sh.P2M, sh.gM, lane);
                 if (lane == 0) {
-                    for (int i = 0; i < N; i++) sh.gM[i] = sh.gP[i];    // :352-354
-                    cb_trmv_ln<N>(sh.P2M, sh.gM);
                     sh.err = cb_nrm2(N, sh.gM);
                     sh.converged = sh.err <= sh.rtol;
-                    if (sh.converged) {                     // :358-375
-                        for (int i = 0; i < N; i++) sh.aM[i] = sh.aP[i];
-                        cb_trsv_lt<N>(sh.P2M, sh.aM);
-                    }
+                }
+                __syncthreads();
+                if (sh.converged) {                         // :358-375
+                    if (lane < N) sh.aM[lane] = sh.aP[lane];
+                    __syncthreads();
+                    cb_par_trsv_lt_cm<N>(sh.P2M, sh.aM, lane);
                 }
                 __syncthreads();
                 if (sh.converged) break;
@@ -559,13 +661,18 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
 
 }  // namespace
 
+template <int N>
+void launch_optcb_n(const CbArgs &a, int blocks, hipStream_t s) {
+    hipLaunchKernelGGL(optcb_kernel<N>, dim3(blocks), dim3(64), 0, s, a);
+}
+
 bool launch_optcb(const CbArgs &a, int blocks, hipStream_t s) {
     switch (a.nm) {
-    case 1: hipLaunchKernelGGL(optcb_kernel<1>, dim3(blocks), dim3(64), 0, s, a); return true;
-    case 3: hipLaunchKernelGGL(optcb_kernel<3>, dim3(blocks), dim3(64), 0, s, a); return true;
-    case 6: hipLaunchKernelGGL(optcb_kernel<6>, dim3(blocks), dim3(64), 0, s, a); return true;
-    case 10: hipLaunchKernelGGL(optcb_kernel<10>, dim3(blocks), dim3(64), 0, s, a); return true;
-    case 15: hipLaunchKernelGGL(optcb_kernel<15>, dim3(blocks), dim3(64), 0, s, a); return true;
+    case 1: launch_optcb_n<1>(a, blocks, s); return true;
+    case 3: launch_optcb_n<3>(a, blocks, s); return true;
+    case 6: launch_optcb_n<6>(a, blocks, s); return true;
+    case 10: launch_optcb_n<10>(a, blocks, s); return true;
+    case 15: launch_optcb_n<15>(a, blocks, s); return true;
     default: return false;
     }
 }
