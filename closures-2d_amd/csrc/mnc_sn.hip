@@ -75,22 +75,32 @@ __global__ void sn_ghost_kernel(double *f, double *f2, int nx, int ny, int nq) {
 
 // out <- Euler stage of in for every interior cell; with old, the Heun average
 // 0.5 (stage + old) (old may alias out: each element is read before it is written
-// by the same thread).  One wave per cell, four consecutive cells per block,
-// XCD-aware: workgroups go to the 8 XCDs round robin, so block b takes group
-// (b % 8) * per + b / 8 and each XCD sweeps one contiguous range of rows; the
-// stencil's x neighbours (a row, ~NY nq 8 bytes, apart) are then still in
-// that XCD's L2 when they are reused.
+// by the same thread).  One wave per cell, four consecutive cells per block.
+// XCD-aware and column-tiled: workgroups go to the 8 XCDs round robin, so
+// block b runs on XCD b % 8, which owns one contiguous range of rows; it
+// sweeps that range in strips of kSnStrip columns, strip by strip, row by row
+// within a strip.  The stencil's x neighbours (a row, NY nq 8 bytes, apart)
+// then span five rows of one strip, kSnStrip nq 8 bytes each (640 KB at
+// 256 ordinates), which stay in the XCD's 4 MB L2 until they are reused.
+constexpr int kSnStrip = 64;
+constexpr int kSnWaves = kSnThreads / 64;
+
+__host__ __device__ __forceinline__ int sn_rows_per_xcd(int nx) { return (nx + 7) / 8; }
+
 __global__ __launch_bounds__(kSnThreads) void sn_stage_kernel(SnArgs a, const double *__restrict__ in, double *out,
                                                               const double *old) {
-    constexpr int kWaves = kSnThreads / 64;
     const int NY = a.ny + 4, nq = a.nq;
-    const int ncell = a.nx * a.ny, ngroup = (ncell + kWaves - 1) / kWaves, per = (ngroup + 7) / 8;
     const int lane = threadIdx.x & 63;
     {
-        const int grp = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-        const int cid = grp * kWaves + (threadIdx.x >> 6);
-        if (grp >= ngroup || cid >= ncell) return;   // uniform over the wave; no block barrier follows
-        const int i = 2 + cid / a.ny, j = 2 + cid % a.ny;
+        const int rows_per = sn_rows_per_xcd(a.nx);
+        const int r0 = (blockIdx.x & 7) * rows_per;
+        const int rows = min(a.nx, r0 + rows_per) - r0;
+        const int L = (blockIdx.x >> 3) * kSnWaves + (threadIdx.x >> 6);   // the XCD's cell, strip order
+        if (rows <= 0 || L >= rows * a.ny) return;   // uniform over the wave; no block barrier follows
+        const int full = rows * kSnStrip;             // cells of a full strip
+        const int sidx = L / full, j0 = sidx * kSnStrip;
+        const int ws = min(kSnStrip, a.ny - j0), rem = L - sidx * full;
+        const int i = 2 + r0 + rem / ws, j = 2 + j0 + rem % ws;
         const size_t c = size_t(i) * NY + j;
         double part = 0.0;
         for (int q = lane; q < nq; q += 64) part += a.w[q] * in[c * nq + q];
@@ -102,29 +112,27 @@ __global__ __launch_bounds__(kSnThreads) void sn_stage_kernel(SnArgs a, const do
         // evaluates the source test |x| < 0.5 (kinetic_update.cpp:173-175)
         const double x_i = edge_coord(a.init_x, i, a.dx), y_j = edge_coord(a.init_y, j, a.dy);
         const bool src = a.init_cond == 1 && fabs(x_i) < 0.5 && fabs(y_j) < 0.5;
+        const size_t sx = size_t(NY) * nq;     // x neighbour stride; y neighbours are nq apart
         for (int q = lane; q < nq; q += 64) {
-            auto F = [&](int di, int dj) { return in[(size_t(i + di) * NY + (j + dj)) * nq + q]; };
-            double nF, sF, eF, wF;
-            if (a.eta[q] > 0) {
-                const double s1 = F(0, -2), s2 = F(0, -1), s3 = F(0, 0), s4 = F(0, 1);
-                nF = s3 + 0.5 * sn_slopefit(s2, s3, s4);
-                sF = s2 + 0.5 * sn_slopefit(s1, s2, s3);
-            } else {
-                const double s1 = F(0, -1), s2 = F(0, 0), s3 = F(0, 1), s4 = F(0, 2);
-                nF = s3 - 0.5 * sn_slopefit(s2, s3, s4);
-                sF = s2 - 0.5 * sn_slopefit(s1, s2, s3);
-            }
-            if (a.xi[q] > 0) {
-                const double s1 = F(-2, 0), s2 = F(-1, 0), s3 = F(0, 0), s4 = F(1, 0);
-                eF = s3 + 0.5 * sn_slopefit(s2, s3, s4);
-                wF = s2 + 0.5 * sn_slopefit(s1, s2, s3);
-            } else {
-                const double s1 = F(-1, 0), s2 = F(0, 0), s3 = F(1, 0), s4 = F(2, 0);
-                eF = s3 - 0.5 * sn_slopefit(s2, s3, s4);
-                wF = s2 - 0.5 * sn_slopefit(s1, s2, s3);
-            }
-            const double flux = (eF - wF) * a.xi[q] / a.dx + (nF - sF) * a.eta[q] / a.dy;
-            double v = F(0, 0);
+            // The upwind side differs between the lanes' ordinates, so both
+            // sides' stencils are selected, not branched to (a branch would
+            // run both slope fits on every wave).  s + h * slope with h = +-0.5
+            // is the reference's s +- 0.5 * slope exactly (0.5 * slope is exact).
+            const double *f = in + c * nq + q;
+            const double c0 = f[0];
+            const double ym2 = f[-2 * nq], ym1 = f[-nq], yp1 = f[nq], yp2 = f[2 * nq];
+            const double xm2 = f[-2 * sx], xm1 = f[-sx], xp1 = f[sx], xp2 = f[2 * sx];
+            const double xiq = a.xi[q], etq = a.eta[q];
+            const bool ep = etq > 0, xp = xiq > 0;
+            const double hy = ep ? 0.5 : -0.5, hx = xp ? 0.5 : -0.5;
+            const double y1 = ep ? ym2 : ym1, y2 = ep ? ym1 : c0, y3 = ep ? c0 : yp1, y4 = ep ? yp1 : yp2;
+            const double x1 = xp ? xm2 : xm1, x2 = xp ? xm1 : c0, x3 = xp ? c0 : xp1, x4 = xp ? xp1 : xp2;
+            const double nF = y3 + hy * sn_slopefit(y2, y3, y4);
+            const double sF = y2 + hy * sn_slopefit(y1, y2, y3);
+            const double eF = x3 + hx * sn_slopefit(x2, x3, x4);
+            const double wF = x2 + hx * sn_slopefit(x1, x2, x3);
+            const double flux = (eF - wF) * xiq / a.dx + (nF - sF) * etq / a.dy;
+            double v = c0;
             v = v - a.dt * st * v + a.dt * integral;
             if (src) v = v + a.dt;
             v = v - a.dt * flux;
@@ -166,7 +174,8 @@ extern "C" int mnc_sn_run(int nx, int ny, int quad_order, double dx, double dy, 
         ok(hipMemcpyAsync(eta, t.eta.data(), nq * sizeof(double), hipMemcpyHostToDevice, s)) &&
         ok(hipEventRecord(e0, s))) {
         SnArgs a{nx, ny, nq, init_cond, dx, dy, dt, init_x, init_y, w, xi, eta, st, ss};
-        const int blocks = 8 * (((nx * ny + 3) / 4 + 7) / 8);   // four cells each, in XCD order
+        // four cells each, one row range per XCD
+        const int blocks = 8 * ((sn_rows_per_xcd(nx) * ny + kSnWaves - 1) / kSnWaves);
         const long long gw = 2LL * (nx + ny) * nq;
         const int gblocks = int(std::min<long long>((gw + 255) / 256, 4096));
         for (int k = 0; k < steps && rc == MNC_OK; ++k) {
