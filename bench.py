@@ -193,6 +193,7 @@ def run_form(mnc, gi, d, opt, steps, warmup, mlp_params=None, kernel_steps=None,
     for k, v in FORCE_TUNE.items():
         ctx.tune(k, v)
     tuned = {k: ctx.get_tune(k) for k in ("graph", "rounds", "tail_mode", "tail_blocks", "mix_blocks", "flux_tile",
+                                          "flux_groups", "flux_overlap",
                                           "mix_cond_log10", "mix_defer", "mix_split", "learned_gate_log10")}
     ctx.grid_set(g.moments, g.alpha)        # back to t = 0
     ctx.reset_stats()

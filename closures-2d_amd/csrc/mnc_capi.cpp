@@ -58,6 +58,7 @@ struct mnc_ctx {
     int flux_tile = 3;               // 0: ansatz to HBM + flux; 1: fused, 8x8 tiles; 2: fused, 16x16;
                                      // 3: fused, 16x16, directions split into flux_groups
     int flux_groups = 6;             // direction groups of flux_tile 3
+    int flux_overlap = 1;            // flux_tile 3: the flux of tiles clear of the Newton tail runs beside it
     int learned_fused = 1;           // 1: network and judge in one kernel; 0: the GEMM kernels, then the judge
     int learned_fallback_iters = 10;   // a network-started cell failing at level 0 within this many
                                      // iterations restarts from isotropicAlpha at level 0
@@ -69,6 +70,14 @@ struct mnc_ctx {
     double *d_mom = nullptr, *d_alpha = nullptr, *d_old = nullptr, *d_st = nullptr, *d_ss = nullptr,
            *d_flux = nullptr, *d_ansatz = nullptr, *d_fpart = nullptr;
     int ansatz_chunk = 0;
+    int *d_tflag = nullptr;          // flux_tile 3 tile flags (flux_overlap)
+    hipStream_t stream2 = nullptr;   // the second stream of flux_overlap (default priority)
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // flux_overlap: run_newton leaves the tail for the step to launch
+    bool defer_tail = false, tail_saved = false, tail_mixed = false;
+    mnc::NewtonArgs tail_a{};
+    mnc::LaunchPlan tail_lp{};
+    int tail_lin = 0, tail_flin = 2;
     // batch staging for host entry points
     double *d_bu = nullptr, *d_ba = nullptr;
     mnc_outs *d_bo = nullptr;
@@ -389,7 +398,14 @@ int run_newton(mnc_ctx *ctx, int ncells, double *u, double *alpha, const mnc_opt
             if (reset) mnc::launch_ctl_reset(ctl, ctx->stream);
             if ((s2 = launch(aa, body))) return s2;
         }
-        if (p.tail) {
+        if (p.tail && ctx->defer_tail) {   // the step launches it (flux_overlap)
+            ctx->tail_saved = true;
+            ctx->tail_mixed = mixed;
+            ctx->tail_a = aa;
+            ctx->tail_lp = tail;
+            ctx->tail_lin = p.r1 & 1;   // the lists the tail reads (launch_all, launch_mixed_all)
+            ctx->tail_flin = mixed ? 2 + (p.defer_rounds ? (p.r1 & 1) : 0) : 2;
+        } else if (p.tail) {
             Timed tm(ctx, MNC_KCLASS_TAIL, timed);
             if ((s2 = launch(aa, tail))) return s2;
         }
@@ -566,6 +582,8 @@ mnc::GridArgs grid_args(mnc_ctx *ctx) {
     g.groups = ctx->flux_groups;
     g.ppn = ctx->ppn ? 1 : 0;
     g.delta = ctx->grid.delta_ppn;
+    g.tile_flag = ctx->d_tflag;
+    g.tile_mode = 0;
     return g;
 }
 
@@ -619,6 +637,12 @@ int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return MNC_ERR_HIP; }
+    if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+        mnc_destroy(ctx);
+        return MNC_ERR_HIP;
+    }
     // the no-allocation entry point's scratch array: its address on this
     // device, resolved once per device (the device is current)
     if (mnc::noalloc_scratch_address(ctx->device, &ctx->d_noalloc) != hipSuccess) {
@@ -681,9 +705,12 @@ void mnc_destroy(mnc_ctx *ctx) {
                     ctx->d_mlp, ctx->d_mlp_scratch, ctx->d_mlp_loss, ctx->d_mlp_io, ctx->d_us, ctx->d_guess,
                     ctx->d_ctl, ctx->d_stats, ctx->d_std, ctx->d_sti, ctx->d_lists, ctx->d_mom, ctx->d_alpha, ctx->d_old,
                     ctx->d_st, ctx->d_ss, ctx->d_flux, ctx->d_ansatz, ctx->d_fpart, ctx->d_bu, ctx->d_ba, ctx->d_bo,
-                    ctx->d_cbB, ctx->d_cbaP, ctx->d_cbws};
+                    ctx->d_cbB, ctx->d_cbaP, ctx->d_cbws, ctx->d_tflag};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -903,7 +930,9 @@ int mnc_grid_create(mnc_ctx *ctx, const mnc_grid_desc *grid, const double *momen
         (st = dalloc(ctx, &ctx->d_old, n)) || (st = dalloc(ctx, &ctx->d_flux, n)) ||
         (st = dalloc(ctx, &ctx->d_st, nc)) || (st = dalloc(ctx, &ctx->d_ss, nc)) ||
         (st = dalloc(ctx, &ctx->d_fpart, size_t(mnc::kMaxFluxGroups) * grid->num_cells_x * grid->num_cells_y *
-                                             ctx->tab.nm)))
+                                             ctx->tab.nm)) ||
+        (st = dalloc(ctx, &ctx->d_tflag, size_t((grid->num_cells_x + mnc::kSplitTile - 1) / mnc::kSplitTile) *
+                                             ((grid->num_cells_y + mnc::kSplitTile - 1) / mnc::kSplitTile))))
         return st;
     // ansatz buffer: all q at once unless that exceeds 8 GiB
     const size_t max_elems = (size_t(8) << 30) / sizeof(double);
@@ -1017,6 +1046,38 @@ int mnc_flux(mnc_ctx *ctx, double *flux_host) {
 
 namespace {
 
+// The tail of a grid solve and the stage's flux (flux_tile 3) overlapped:
+// tiles whose stencil region holds no cell left for the tail take their flux
+// on the second stream while the tail runs; the flagged tiles follow the tail.
+// The two share no cell: the tail writes the alpha and moments of its own
+// cells, a clear tile's flux reads alpha of cells the rounds finished and
+// writes the moments of its own interior cells.
+int overlap_tail_flux(mnc_ctx *ctx, int nc, int stage, double dt) {
+    const size_t cap = ctx->state_cap;
+    const int *lists = ctx->d_lists, *ctl = ctx->d_ctl;
+    const int w0 = mnc::newton_list_word(ctx->tail_lin), w1 = mnc::newton_list_word(ctx->tail_flin);
+    mnc::launch_tile_flags(lists + size_t(ctx->tail_lin) * cap, ctl + w0, lists + size_t(ctx->tail_flin) * cap,
+                           ctl + w1, nc, ctx->grid.num_cells_x, ctx->grid.num_cells_y, ctx->d_tflag, ctx->stream);
+    MNC_HIP_CHECK(hipEventRecord(ctx->ev_fork, ctx->stream));
+    MNC_HIP_CHECK(hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+    const auto &t = ctx->tab;
+    const bool gaunt = t.use_gaunt != 0;
+    const bool ok = ctx->tail_mixed ? mnc::launch_newton_mixed(ctx->tail_a, t.nm, t.np2, gaunt, ctx->tail_lp, ctx->stream)
+                                    : mnc::launch_newton_fp64(ctx->tail_a, t.nm, t.np2, gaunt, ctx->tail_lp, ctx->stream);
+    if (!ok) return ctx_fail(ctx, MNC_ERR_ARG, "no Newton tail instance for this moment order");
+    mnc::GridArgs g = grid_args(ctx);
+    g.tile_mode = 1;
+    if (!mnc::launch_flux_fused(g, 3, stage, dt, ctx->stream2))
+        return ctx_fail(ctx, MNC_ERR_ARG, "no flux kernel instance for this moment order");
+    MNC_HIP_CHECK(hipEventRecord(ctx->ev_join, ctx->stream2));
+    MNC_HIP_CHECK(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+    g.tile_mode = 2;
+    if (!mnc::launch_flux_fused(g, 3, stage, dt, ctx->stream))
+        return ctx_fail(ctx, MNC_ERR_ARG, "no flux kernel instance for this moment order");
+    MNC_HIP_CHECK(hipGetLastError());
+    return MNC_OK;
+}
+
 // One Heun step (MomOptSolver::update, momopt_update.cpp:283-382) enqueued on
 // the context's stream with no host synchronisation and no allocation once the
 // buffers exist, so that it can be captured.
@@ -1030,8 +1091,21 @@ int enqueue_step(mnc_ctx *ctx, double dt, const mnc_options *opt) {
         if ((st = mnc_ghost_fill(ctx))) return st;
         if (opt->opt_type != MNC_OPT_NEWTON) {
             if ((st = run_optcb_grid(ctx, nc, opt, false))) return st;
-        } else if ((st = run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr, false))) {
-            return st;
+        } else {
+            // flux_overlap: the tail is launched here, beside the flux of the
+            // tiles whose stencil region it does not touch (eager timing
+            // measures one stream, so it runs the plain sequence)
+            ctx->defer_tail = ctx->flux_overlap && ctx->flux_tile == 3 && !ctx->timing && !ctx->has_g64 &&
+                              ctx->tab.nm <= 15;
+            ctx->tail_saved = false;
+            st = run_newton(ctx, nc, ctx->d_mom, ctx->d_alpha, opt, nullptr, nullptr, false);
+            ctx->defer_tail = false;
+            if (st) return st;
+            if (ctx->tail_saved) {
+                ctx->tail_saved = false;
+                if ((st = overlap_tail_flux(ctx, nc, stage, dt))) return st;
+                continue;
+            }
         }
         // errors of the first stage are seen at the end of the step (the
         // kernels flag them in device memory; the state is left as computed)
@@ -1223,6 +1297,12 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         ctx->flux_groups = value;
         return MNC_OK;
     }
+    if (std::strcmp(key, "flux_overlap") == 0) {
+        if (value != 0 && value != 1) return ctx_fail(ctx, MNC_ERR_ARG, "flux_overlap is 0 or 1");
+        ctx->flux_overlap = value;
+        ++ctx->epoch;
+        return MNC_OK;
+    }
     if (std::strcmp(key, "learned_fused") == 0) {
         if (value != 0 && value != 1) return ctx_fail(ctx, MNC_ERR_ARG, "learned_fused is 0 or 1");
         ctx->learned_fused = value;
@@ -1261,6 +1341,7 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value) {
     if (std::strcmp(key, "learned_gate_log10") == 0) { *value = ctx->learned_gate_log10; return MNC_OK; }
     if (std::strcmp(key, "learned_fallback_iters") == 0) { *value = ctx->learned_fallback_iters; return MNC_OK; }
     if (std::strcmp(key, "learned_fused") == 0) { *value = ctx->learned_fused; return MNC_OK; }
+    if (std::strcmp(key, "flux_overlap") == 0) { *value = ctx->flux_overlap; return MNC_OK; }
     if (std::strcmp(key, "tail_blocks") == 0) {
         *value = ctx->tail_blocks > 0 ? ctx->tail_blocks : 2 * ctx->num_cu;
         return MNC_OK;
@@ -1326,6 +1407,7 @@ int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, cha
     knobs.push_back({"graph", {0, 1}, &ctx->use_graph});
     knobs.push_back({"flux_tile", {0, 1, 2, 3}, &ctx->flux_tile});
     knobs.push_back({"flux_groups", {2, 3, 4, 6, 8}, &ctx->flux_groups});
+    knobs.push_back({"flux_overlap", {0, 1}, &ctx->flux_overlap});
     knobs.push_back({"rounds", {2, 4, 6, 8, 12}, &ctx->rounds});
     knobs.push_back({"tail_mode", {0, 1, 2}, &ctx->tail_mode});
     knobs.push_back({"tail_blocks", {cu, 2 * cu, 4 * cu}, &ctx->tail_blocks});

@@ -322,6 +322,11 @@ void launch_flux_t(const GridArgs &g, int q0, int nqc, int stage, double dt, boo
 // split into the groups: the result moves at rounding level.
 template <int NM, int TX, int TY, int QC>
 __global__ __launch_bounds__(TX * TY) void flux_split_kernel(GridArgs g) {
+    static_assert(TX == kSplitTile && TY == kSplitTile, "the tile flags' geometry");
+    if (g.tile_mode != 0) {   // this launch's share of the tiles (uniform over the block)
+        const bool flagged = g.tile_flag[blockIdx.x * gridDim.y + blockIdx.y] != 0;
+        if (flagged != (g.tile_mode == 2)) return;
+    }
     constexpr int HX = TX + 4, HY = TY + 4, NH = HX * HY, NT = TX * TY;
     constexpr int HPT = (NH + NT - 1) / NT;                  // halo cells evaluated per thread
     __shared__ double E_s[QC][NH];
@@ -408,6 +413,11 @@ __global__ __launch_bounds__(kBlock) void flux_sum_kernel(GridArgs g, int groups
     const int idx = blockIdx.x * kBlock + threadIdx.x;
     if (idx >= g.nx * g.ny) return;
     const int i = 2 + idx / g.ny, j = 2 + idx % g.ny;
+    if (g.tile_mode != 0) {
+        const int tiles_y = (g.ny + kSplitTile - 1) / kSplitTile;
+        const bool flagged = g.tile_flag[((i - 2) / kSplitTile) * tiles_y + (j - 2) / kSplitTile] != 0;
+        if (flagged != (g.tile_mode == 2)) return;
+    }
     double F[NM];
 #pragma unroll
     for (int k = 0; k < NM; ++k) F[k] = g.fpart[size_t(idx) * NM + k];
@@ -447,7 +457,38 @@ void launch_ansatz_t(const GridArgs &g, int q0, int nqc, hipStream_t s) {
     hipLaunchKernelGGL(ansatz_kernel<NM>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, g, q0, nqc);
 }
 
+__global__ void tile_flag_clear_kernel(int *flags, int n) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) flags[t] = 0;
+}
+
+// One thread per list entry: the tiles whose stencil region [16 t, 16 t + 20)
+// (grid coordinates, ghosts included) holds the entry's cell.
+__global__ void tile_flag_mark_kernel(const int *list0, const int *cnt0, const int *list1, const int *cnt1, int ncells,
+                                      int NY, int tiles_x, int tiles_y, int *flags) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ncells) return;
+    const int n0 = *cnt0, n1 = *cnt1;
+    int c = -1;
+    if (s < n0) c = list0[s];
+    else if (s - n0 < n1) c = list1[s - n0];
+    if (c < 0) return;
+    const int ci = c / NY, cj = c % NY;
+    constexpr int T = kSplitTile, R = kSplitTile + 4;
+    const int tx0 = ci >= R ? (ci - R + T) / T : 0, tx1 = min(tiles_x - 1, ci / T);
+    const int ty0 = cj >= R ? (cj - R + T) / T : 0, ty1 = min(tiles_y - 1, cj / T);
+    for (int tx = tx0; tx <= tx1; ++tx)
+        for (int ty = ty0; ty <= ty1; ++ty) flags[tx * tiles_y + ty] = 1;
+}
+
 }  // namespace
+
+void launch_tile_flags(const int *list0, const int *cnt0, const int *list1, const int *cnt1, int ncells, int nx,
+                       int ny, int *flags, hipStream_t s) {
+    const int tx = (nx + kSplitTile - 1) / kSplitTile, ty = (ny + kSplitTile - 1) / kSplitTile;
+    hipLaunchKernelGGL(tile_flag_clear_kernel, dim3((tx * ty + 255) / 256), dim3(256), 0, s, flags, tx * ty);
+    hipLaunchKernelGGL(tile_flag_mark_kernel, dim3((ncells + 255) / 256), dim3(256), 0, s, list0, cnt0, list1, cnt1,
+                       ncells, ny + 4, tx, ty, flags);
+}
 
 void launch_ghost_fill(const GridArgs &g, hipStream_t s) {
     if (g.init_cond != 2) return;

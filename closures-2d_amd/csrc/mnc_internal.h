@@ -143,6 +143,8 @@ struct LaunchPlan {
     int mix_split = 3;                       // bf16 parts in the mixed sweep's Hessian products (2 or 3)
 };
 bool launch_newton_fp64(const NewtonArgs &a, int nm, int np2, bool gaunt, const LaunchPlan &lp, hipStream_t s);
+// The control word holding list l's count.
+int newton_list_word(int l);
 // The same with the lockstep rounds in mixed precision (MFMA sweeps); the tail
 // is FP64.
 bool launch_newton_mixed(const NewtonArgs &a, int nm, int np2, bool gaunt, const LaunchPlan &lp, hipStream_t s);
@@ -225,8 +227,18 @@ struct GridArgs {
     int groups;           // direction groups of flux_tile 3 (1..kMaxFluxGroups)
     int ppn;              // the PPn ansatz (computeAnsatz, momopt_update.cpp:54-57)
     double delta;         // its delta
+    // flux_tile 3 on part of the grid: 0 every tile; 1 the tiles whose flag is
+    // 0 (no unfinished closure in their stencil region); 2 the flagged ones
+    const int *tile_flag; // [tiles_x][tiles_y] of kSplitTile x kSplitTile cells
+    int tile_mode;
 };
 constexpr int kMaxFluxGroups = 8;
+constexpr int kSplitTile = 16;   // flux_tile 3's cells per tile side
+// flags[tx][ty] = 1 for the flux_tile 3 tiles whose stencil region (the tile
+// and two cells around it) holds a cell on list0 (count *cnt0) or list1 (*cnt1);
+// 0 for the others.  Grid cell c (ghosts included) is (c / NY, c % NY).
+void launch_tile_flags(const int *list0, const int *cnt0, const int *list1, const int *cnt1, int ncells, int nx,
+                       int ny, int *flags, hipStream_t s);
 
 // The change-of-basis optimiser (optcb.cpp:90-415) on a batch: one block per
 // cell at a time, ws holds each block's table seen through its cell's basis.

@@ -2500,6 +2500,7 @@ void launch_mixed_all(const NewtonArgs &a, const LaunchPlan &lp, hipStream_t s) 
 }  // namespace
 
 int newton_block() { return kBlock; }
+int newton_list_word(int l) { return l < 2 ? l : l + 2; }   // cw() on the host
 
 __global__ void ctl_reset_kernel(int *ctl) {
     if (threadIdx.x < kCtlWords) ctl[threadIdx.x] = 0;

@@ -318,6 +318,12 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *                  into "flux_groups" groups summed in order and reciprocals
  *                  of dx, dy (rounding-level differences from 0-2)
  *   "flux_groups"  direction groups of flux_tile 3 (1-8, default 6)
+ *   "flux_overlap" flux_tile 3 in mnc_update: 1 (default): after a stage's
+ *                  lockstep rounds, the flux of the tiles whose stencil region
+ *                  holds no cell left for the Newton tail runs on a second
+ *                  stream (default priority) beside the tail, the other tiles'
+ *                  after it; 0: the tail, then every tile.  The same kernels on
+ *                  the same cells: identical results.  Per-launch timing runs 0.
  *   "graph"        1: mnc_update steps are captured into hipGraphs (one per
  *                  distinct dt and options, captured on the second step with
  *                  that key) and replayed; 0: eager launches (default).  While
@@ -334,7 +340,7 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value);
  * -- (learned warm start) "learned_gate_log10" (5, the network off, among
  * the candidates), (mixed precision)
  * "mix_cond_log10", "mix_split", "mix_defer" -- then the launch shapes
- * "graph", "flux_tile", "flux_groups", "rounds", "tail_mode", "tail_blocks" and (mixed
+ * "graph", "flux_tile", "flux_groups", "flux_overlap", "rounds", "tail_mode", "tail_blocks" and (mixed
  * precision) "mix_blocks", and keeps the fastest.  Every candidate converges
  * to the same tolerance; "mix_cond_log10" trades mixed-precision steps
  * against FP64 ones.  The grid state

@@ -12,9 +12,11 @@ import pytest
 from helpers import fixture_deck, gpu_grid_from_deck, load_pkg, mn_options
 
 
-def _steps(mnc, gi, d, dts, opt, graph, between=None):
+def _steps(mnc, gi, d, dts, opt, graph, between=None, tune=None):
     ctx, g = gpu_grid_from_deck(mnc, gi, d)
     ctx.tune("graph", graph)
+    for key, v in (tune or {}).items():
+        ctx.tune(key, v)
     states = []
     for k, dt in enumerate(dts):
         ctx.update(dt, opt)
@@ -116,3 +118,31 @@ def test_autotune_restores_state_and_keeps_results():
     rel = np.abs(mt - mr).max() / np.abs(mr).max()
     print("tuned vs default after 4 steps: max rel %.3e" % rel)
     assert rel <= 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_flux_overlap_equals_serial(precision):
+    """flux_overlap 1 (the default: the flux of the tiles whose stencil region
+    holds no cell left for the Newton tail runs on a second stream beside the
+    tail, the flagged tiles after it) against 0 (tail, then every tile): the
+    same kernels compute the same cells, so the states agree bitwise, eager
+    and captured into a graph (the fork and join are graph edges)."""
+    mnc, gi = load_pkg("mnc"), load_pkg("grid_init")
+    print("case flux overlap %s" % precision, flush=True)
+    d = dict(num_cells_x=100, num_cells_y=90, a_x=-3.5, b_x=3.5, a_y=-3.5, b_y=3.5, gaussian_sigma=0.0,
+             floor_value=1e-10, sigma=1.0, init_cond=1, moment_order=4, quad_order=16, cfl_factor=0.9,
+             tol=1e-8, cond_h_max=1e10, max_iter=100, use_gaunt=0, theta=2.0, line_search=1)
+    prec = mnc.PREC_MIXED if precision == "mixed" else mnc.PREC_FP64
+    opt = mn_options(mnc, d, precision=prec)
+    g0 = gi.initial_state(100, 90, -3.5, 3.5, -3.5, 3.5, 1, 0.0, 1e-10, 1.0, 15, 0.9, 2.0)
+    dts = [g0.max_dt] * 8
+    serial, st_s = _steps(mnc, gi, d, dts, opt, 0, tune={"flux_overlap": 0, "rounds": 2})
+    for graph in (0, 1):
+        over, st_o = _steps(mnc, gi, d, dts, opt, graph, tune={"flux_overlap": 1, "rounds": 2})
+        for k, ((ms, as_), (mo, ao)) in enumerate(zip(serial, over)):
+            assert np.array_equal(ms, mo), "moments differ after step %d (graph %d)" % (k, graph)
+            assert np.array_equal(as_, ao), "alpha differ after step %d (graph %d)" % (k, graph)
+        assert st_s.iter_sum == st_o.iter_sum and st_s.tail_cells == st_o.tail_cells
+    print("tail cells over 8 steps: %d" % st_s.tail_cells)
+    assert st_s.tail_cells > 0
