@@ -2254,10 +2254,11 @@ __global__ __launch_bounds__(256) void sweep_g64_kernel(NewtonArgs A, int lin) {
     }
 }
 
-template <int NM>
+template <int NM, int NACC>
 struct G64Shared {
     double L[NM][NM];   // H, then its Cholesky factor (lower)
     double x[NM], gvec[NM], al[NM];
+    double mom[NACC];   // the sweep's order-2N moments of the cell
 };
 
 // One step of optScaled's logic per cell after sweep_g64_kernel, one wave per
@@ -2267,7 +2268,8 @@ struct G64Shared {
 template <int NM, int NP2>
 __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
     using St = Store<NM, NP2>;
-    __shared__ G64Shared<NM> sh;
+    constexpr int NK = gp_nm(2 * order_of(NM));    // the order-2N moments (153 at n = 45)
+    __shared__ G64Shared<NM, NK> sh;
     const St S{A.st_d, A.st_i, A.cap};
     const int lane = threadIdx.x;
     const int count = A.ctl[lin];
@@ -2307,13 +2309,17 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
         }
         if (newton) {
             ++iter; ++total;                                        // opt.cpp:212-213
+            // the cell's moments into LDS first: each Gaunt term then reads its
+            // moment from LDS, not through a load that waits on the term's index
+            for (int k = lane; k < NK; k += 64) sh.mom[k] = S.d(St::F_ACC + k, c);
+            __builtin_amdgcn_wave_barrier();
             double rs = 0.0, gi = 0.0, dg = 0.0;
             if (lane < NM) {                                        // row `lane` of H (Gaunt map)
                 for (int jj = 0; jj < NM; ++jj) {
                     const int pr = (lane < jj) ? lane * NM + jj : jj * NM + lane;
                     double hij = 0.0;
                     for (int cc = A.mgptr[pr]; cc < A.mgptr[pr + 1]; ++cc)
-                        hij = fma(A.mgv[cc], S.d(St::F_ACC + A.mgk[cc], c), hij);
+                        hij = fma(A.mgv[cc], sh.mom[A.mgk[cc]], hij);
                     sh.L[lane][jj] = hij;
                     rs += fabs(hij);
                     if (jj == lane) dg = hij;
@@ -2570,7 +2576,7 @@ bool launch_newton_g64_init(const NewtonArgs &a, int nm, int np2, hipStream_t s)
 bool launch_newton_g64_rounds(const NewtonArgs &a, int nm, int np2, int &lin, int rounds, int blocks, hipStream_t s) {
     if (!(nm == 45 && np2 == 45)) return false;
     constexpr int KZ = 12, NGT = 3, NHT = 10;   // 48 moment rows, 46 sum columns, 153 order-16 moments
-    if (a.gnmm > 16 * NHT) return false;
+    if (a.gnmm > 16 * NHT || a.gnmm != gp_nm(2 * order_of(45))) return false;   // logic_g64's LDS moments
     for (int r = 0; r < rounds; ++r) {
         hipLaunchKernelGGL((sweep_g64_kernel<45, 45, KZ, NGT, NHT>), dim3(blocks), dim3(256), 0, s, a, lin);
         hipLaunchKernelGGL((logic_g64_kernel<45, 45>), dim3(4 * blocks), dim3(64), 0, s, a, lin);
