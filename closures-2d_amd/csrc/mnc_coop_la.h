@@ -10,6 +10,14 @@
 namespace mnc {
 namespace {
 
+// v (a double) from lane l (wave-uniform l)
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffu), l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // x <- L^{-1} x then L^{-T} x, lane i owning x_i (i < NM); division as dtrsv.
 template <int NM>
 __device__ __forceinline__ void coop_apply_inv(double (*L)[NM], double *x, int lane) {
@@ -29,12 +37,32 @@ __device__ __forceinline__ void coop_apply_inv(double (*L)[NM], double *x, int l
     }
 }
 
+// coop_apply_inv with x in registers (x_i on lane i): x_j is read with
+// v_readlane instead of a store, a barrier and a load per step; the same
+// operations on each element.
+template <int NM>
+__device__ __forceinline__ void coop_apply_inv_reg(double (*L)[NM], double &x, int lane) {
+    for (int j = 0; j < NM; ++j) {
+        const double xj = readlane_d(x, j) / L[j][j];
+        if (lane == j) x = xj;
+        else if (lane > j && lane < NM) x = fma(-xj, L[lane][j], x);
+    }
+    for (int j = NM - 1; j >= 0; --j) {
+        const double xj = readlane_d(x, j) / L[j][j];
+        if (lane == j) x = xj;
+        else if (lane < j) x = fma(-L[j][lane], xj, x);
+    }
+}
+
 // Cholesky in place on the lower triangle (dpotf2 order); uniform result.
 template <int NM>
 __device__ __forceinline__ bool coop_cholesky(double (*L)[NM], int lane) {
     bool ok = true;
     for (int j = 0; j < NM; ++j) {
+        // the inner loops unrolled so that their LDS reads are issued ahead of
+        // the dependent FMA chains (the same FMAs in the same order)
         double dot = 0.0;
+#pragma unroll 8
         for (int k = 0; k < j; ++k) dot = fma(L[j][k], L[j][k], dot);
         double ajj = L[j][j] - dot;
         ok = ok && (ajj > 0.0);
@@ -42,6 +70,7 @@ __device__ __forceinline__ bool coop_cholesky(double (*L)[NM], int lane) {
         double y = 0.0;
         if (lane > j && lane < NM) {
             y = L[lane][j];
+#pragma unroll 8
             for (int k = 0; k < j; ++k) y = fma(-L[j][k], L[lane][k], y);
         }
         __builtin_amdgcn_wave_barrier();
@@ -148,12 +177,6 @@ __device__ __forceinline__ double wave_min(double v) {
 // factorisation and of the triangular solves is a register operation rather
 // than an LDS round trip.  The arithmetic and its order are those of
 // cholesky<N>, apply_inv<N> and rcond<N>.
-__device__ __forceinline__ double readlane_d(double v, int l) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffu), l);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
 
 template <int NM>
 __device__ __forceinline__ bool reg_cholesky(double (&row)[NM], int lane) {

@@ -2318,6 +2318,8 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
                 for (int jj = 0; jj < NM; ++jj) {
                     const int pr = (lane < jj) ? lane * NM + jj : jj * NM + lane;
                     double hij = 0.0;
+                    // the same sum in the same order; four terms' loads in flight
+#pragma unroll 4
                     for (int cc = A.mgptr[pr]; cc < A.mgptr[pr + 1]; ++cc)
                         hij = fma(A.mgv[cc], sh.mom[A.mgk[cc]], hij);
                     sh.L[lane][jj] = hij;
@@ -2342,9 +2344,10 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
             if (iter >= maxit || !chol_ok || t == 0.0 || condest > A.cond_h_max) {
                 do_reg = true;                                      // opt.cpp:244
             } else {
-                if (lane < NM) sh.x[lane] = sh.gvec[lane];
+                double xr = lane < NM ? sh.gvec[lane] : 0.0;
+                coop_apply_inv_reg<NM>(sh.L, xr, lane);             // dpotrs (:278-282)
+                if (lane < NM) sh.x[lane] = xr;
                 __builtin_amdgcn_wave_barrier();
-                coop_apply_inv<NM>(sh.L, sh.x, lane);               // dpotrs (:278-282)
                 bool stop = err <= S.d(St::F_RTOL, c);
                 if (stop && A.tol_gamma > 0.0) {
                     double da1 = 0.0;
