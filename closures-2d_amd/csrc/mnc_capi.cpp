@@ -670,6 +670,10 @@ int mnc_create(const mnc_setup *setup, mnc_ctx **out) {
         mnc_destroy(ctx);
         return MNC_ERR_HIP;
     }
+    // above n = 15 the fused flux's halo of alpha (20 x 20 cells does not fit,
+    // 12 x 12 leaves two waves per CU) is slower than the ansatz through HBM
+    // (M8 512^2: 18.3 against 8.3 ms per stage, the same results)
+    if (t.nm > 15) ctx->flux_tile = 0;
     if (!ctx->ppn && t.nm == 45 && mnc::build_g64_tables(setup->moment_order, setup->quad_order, ctx->g64) == MNC_OK &&
         ctx->g64.kz == 12 && ctx->g64.ngt == 3 && ctx->g64.nht == 10) {   // the instantiated tile counts
         if (up(&ctx->d_gz, ctx->g64.z) || up(&ctx->d_gg, ctx->g64.g) || up(&ctx->d_gh, ctx->g64.h) ||

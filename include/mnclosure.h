@@ -313,8 +313,9 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *                  0 never)
  *   "flux_tile"    0: ansatz written to HBM, then the flux; 1: ansatz fused
  *                  into the flux with 8x8-cell LDS tiles; 2: 16x16 tiles
- *                  (n <= 15); these three give identical results.  3 (default;
- *                  n <= 15, else as 1): 16x16 tiles with the directions split
+ *                  (n <= 15); these three give identical results.  3 (the
+ *                  default at n <= 15; above it 0 is the default and 3 runs
+ *                  as 1): 16x16 tiles with the directions split
  *                  into "flux_groups" groups summed in order and reciprocals
  *                  of dx, dy (rounding-level differences from 0-2)
  *   "flux_groups"  direction groups of flux_tile 3 (1-8, default 6)
