@@ -535,7 +535,8 @@ int run_optcb_grid(mnc_ctx *ctx, int nc, const mnc_options *o, bool check) {
     a.ls_mode = o->line_search;
     a.tol = o->tol_rel;
     a.cond_h_max = o->cond_h_max;
-    a.err = ctx->d_ctl + 3;
+    a.err = ctx->d_ctl + mnc::kErrWord;
+    a.queue = ctx->d_ctl + mnc::kQueueWord;
     a.stats = ctx->d_stats;
     a.bfgs = o->opt_type == MNC_OPT_BFGS;
     a.max_bfgs_iter = o->max_bfgs_iter;
@@ -543,7 +544,7 @@ int run_optcb_grid(mnc_ctx *ctx, int nc, const mnc_options *o, bool check) {
     {
         Timed tm(ctx, MNC_KCLASS_NEWTON);
         mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
-        if (!mnc::launch_optcb(a, ctx->cb_blocks, ctx->stream))
+        if (mnc::launch_optcb(a, ctx->cb_blocks, ctx->num_cu, ctx->stream) < 0)
             return ctx_fail(ctx, MNC_ERR_ARG, "no change-of-basis kernel instance for this moment order");
     }
     MNC_HIP_CHECK(hipGetLastError());
@@ -840,7 +841,8 @@ int mnc_optcb_batch_host(mnc_ctx *ctx, int ncells, double *u, double *P2M, doubl
     a.ls_mode = opt->line_search;
     a.tol = opt->tol_rel;
     a.cond_h_max = opt->cond_h_max;
-    a.err = ctx->d_ctl + 3;
+    a.err = ctx->d_ctl + mnc::kErrWord;
+    a.queue = ctx->d_ctl + mnc::kQueueWord;
     a.bfgs = opt->opt_type == MNC_OPT_BFGS;
     a.max_bfgs_iter = opt->max_bfgs_iter;
     a.cond_h_max_bfgs = opt->cond_h_max_bfgs;
@@ -852,7 +854,7 @@ int mnc_optcb_batch_host(mnc_ctx *ctx, int ncells, double *u, double *P2M, doubl
         mnc::launch_ctl_reset(ctx->d_ctl, ctx->stream);
         {
             Timed tm(ctx, MNC_KCLASS_NEWTON);
-            if (!mnc::launch_optcb(a, blocks, ctx->stream))
+            if (mnc::launch_optcb(a, blocks, ctx->num_cu, ctx->stream) < 0)
                 return ctx_fail(ctx, MNC_ERR_ARG, "no change-of-basis kernel instance for this moment order");
         }
         MNC_HIP_CHECK(hipGetLastError());

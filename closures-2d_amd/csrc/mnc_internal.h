@@ -83,11 +83,13 @@ struct NewtonArgs {
 };
 
 // Control words of a batched solve: [0], [1] counts of lists 0/1 (the
-// lockstep pair), [2] the tail's queue head, [3] error bits, [4], [5] counts of
+// lockstep pair), [2] the tail's queue head (the change-of-basis kernel's
+// cell queue when that runs instead), [3] error bits, [4], [5] counts of
 // lists 2/3 (cells redone in FP64 by a mixed-precision solve), [6] count of
 // list 4 (cells waiting for the learned warm start's network).
 constexpr int kCtlWords = 8;
 constexpr int kNumLists = 5;
+constexpr int kQueueWord = 2, kErrWord = 3;
 constexpr int kMlpListIndex = 4, kMlpListWord = 6;
 constexpr int kStatWords = 17;   // ... [13] tail cells, [14] tail iterations, [15] tail trials,
                                  // [16] learned starts that fell back to isotropicAlpha
@@ -254,8 +256,12 @@ struct CbArgs {
     long long *stats;                 // [kStatWords] as the Newton kernels, or null
     int bfgs, max_bfgs_iter;          // the BFGS variant (optbfgs.cpp:169-513) and its inner count
     double cond_h_max_bfgs;           // its bound on cond(H) cond(H^-1)
+    int *queue;                       // the next cell to take, zero at launch
 };
-bool launch_optcb(const CbArgs &a, int blocks, hipStream_t s);
+// Launches on min(the resident blocks, max_blocks, ncells) blocks (max_blocks:
+// the workspace's size); returns that count, or -1 for an order with no
+// kernel instance.
+int launch_optcb(const CbArgs &a, int max_blocks, int num_cu, hipStream_t s);
 
 void launch_ghost_fill(const GridArgs &g, hipStream_t s);
 void launch_ansatz(const GridArgs &g, int q0, int nqc, hipStream_t s);

@@ -1,15 +1,30 @@
 // mnc_optcb.hip -- the change-of-basis optimiser (src/momopt/opt/optcb.cpp:90-415,
 // MOMOPT_OPTIMIZATION_TYPE 1) for batches of Mn closures on gfx950.
 //
-// One 64-lane block per cell.  The cell's basis P2M (n x n lower triangular,
+// One 64-lane block per cell at a time, the blocks taking cells from a queue
+// (a counter in device memory) so that the few slow cells (near-boundary
+// lobes, many iterations) do not leave a block's statically assigned share
+// late.  The cell's basis P2M (n x n lower triangular,
 // column-major as the reference's BLAS calls see it), its multipliers alphaP
 // in that basis, and the quadrature table seen through the basis, pP[q][k] =
 // (P2M^{-1} p_q)_k, live in HBM (pP in a workspace of nq*n doubles per block).
 // The quadrature work -- the table's triangular solves, f, g and the Hessian
 // sums, the line search's f -- is spread over the lanes (lane l takes the
 // points q = l, l + 64, ...; wave sums by butterfly).  The n x n algebra
-// (Cholesky, condition estimate, the basis products and solves) runs on lane 0
-// in the reference's order, on LDS copies, between block barriers.
+// (Cholesky, condition estimate, the basis products and solves) runs in the
+// reference's order for each element, the independent operations of a step
+// side by side across the lanes, on LDS copies or registers, between block
+// barriers.
+//
+// The block is one wave and the kernel latency-bound, so waves per SIMD count
+// more than anything inside one wave: the kernel is compiled for three (168
+// VGPRs), the grid sized to the occupancy the runtime reports.  Two things
+// keep the register count there without spills in the iteration: the lane
+// index passes through an empty asm at the top of each iteration (the
+// compiler would otherwise hoist dozens of lane masks and lane-dependent
+// constants out of the loop and keep them live across it), and the table
+// solve re-reads the factor from LDS for each point instead of holding all of
+// it in registers.
 #include "mnc_internal.h"
 #include "mnc_coop_la.h"
 
@@ -196,6 +211,7 @@ template <int N>
 __device__ void cb_table_solve(const CbArgs &A, double *pP, const double *l, int lane) {
     for (int q = lane; q < A.nq; q += 64) {
         double x[N];
+        __asm__ volatile("" ::: "memory");          // the factor's reads stay in the loop
 #pragma unroll
         for (int k = 0; k < N; k++) x[k] = pP[size_t(q) * N + k];
         for (int j = 0; j < N; j++) {
@@ -401,14 +417,22 @@ __device__ void cb_isotropic(double *a) {     // isotropicAlpha, Mn (opt.cpp:27-
     for (int i = 1; i < N; i++) a[i] = 0.0;
 }
 
+constexpr int kCbWavesPerSimd = 3;
+
 template <int N>
-__global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCbWavesPerSimd))) void optcb_kernel(CbArgs A) {
     __shared__ CbShared<N> sh;
-    const int lane = threadIdx.x;
-    for (int c = blockIdx.x; c < A.ncells; c += gridDim.x) {
+    __shared__ int next_cell;
+    int lane = threadIdx.x;
+    for (;;) {
+        if (lane == 0) next_cell = atomicAdd(A.queue, 1);
+        __syncthreads();
+        const int c = next_cell;
+        __syncthreads();
+        if (c >= A.ncells) break;
         // the table seen through the basis: in an HBM workspace per block (an
         // LDS copy, 61 KB at n = 15 and 512 points, would leave room for two
-        // cells per CU instead of four, and measured slower)
+        // cells per CU instead of twelve, and measured slower)
         double *pP = A.ws + size_t(blockIdx.x) * A.nq * N;
         double *ug = A.u + size_t(c) * N;
         double *Bg = A.P2M + size_t(c) * N * N;
@@ -440,6 +464,7 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
         for (;;) {
             ++iter;
             ++total;
+            __asm__ volatile("" : "+v"(lane));              // no lane-derived values live across iterations
             cb_fobj_full<N>(A, pP, sh, lane);               // :234-242
             __syncthreads();
             // :247-259: the factor, the 1-norm and the condition estimate, the
@@ -541,20 +566,28 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
                 continue;
             }
             // CB_STEP: the basis moves by L (:333-343).  P2M <- P2M L with
-            // dtrmm('R','L','N','N')'s order for each element, lane i forming
-            // row i from the old row.
-            double brow[N];
-            if (lane < N) {
-                for (int j = 0; j < N; j++) {
-                    double b = sh.L[j * N + j] * sh.P2M[lane + j * N];
-                    for (int k = j + 1; k < N; k++)
-                        if (sh.L[k * N + j] != 0.0) b = b + sh.L[k * N + j] * sh.P2M[lane + k * N];
-                    brow[j] = b;
+            // dtrmm('R','L','N','N')'s order for each element: element e of
+            // the column-major result (row e % n, column e / n) on lane e % 64,
+            // formed from the old P2M, stored after the barrier.
+            constexpr int kEl = (N * N + 63) / 64;
+            double bel[kEl];
+#pragma unroll
+            for (int r = 0; r < kEl; r++) {
+                const int e = lane + 64 * r, i = e % N, j = e / N;
+                double b = 0.0;
+                if (e < N * N) {
+                    b = sh.L[j * N + j] * sh.P2M[i + j * N];
+                    for (int k = j + 1; k < N; k++) {
+                        const double lkj = sh.L[k * N + j];
+                        if (lkj != 0.0) b = b + lkj * sh.P2M[i + k * N];
+                    }
                 }
+                bel[r] = b;
             }
             __syncthreads();
-            if (lane < N)
-                for (int j = 0; j < N; j++) sh.P2M[lane + j * N] = brow[j];
+#pragma unroll
+            for (int r = 0; r < kEl; r++)
+                if (lane + 64 * r < N * N) sh.P2M[lane + 64 * r] = bel[r];
             __syncthreads();
             cb_par_trsv_ln_rm<N>(sh.L, sh.gP, lane);
             cb_par_trmv_lt_rm<N>(sh.L, sh.aP, lane);
@@ -661,19 +694,36 @@ __global__ __launch_bounds__(64) void optcb_kernel(CbArgs A) {
 
 }  // namespace
 
+// Blocks: the resident count (the runtime's occupancy for this instance, a
+// block per wave slot) times the CUs, capped by max_blocks (the workspace's
+// size) and the cell count; blocks beyond the resident ones would only wait.
 template <int N>
-void launch_optcb_n(const CbArgs &a, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(optcb_kernel<N>, dim3(blocks), dim3(64), 0, s, a);
+int optcb_blocks_n(int max_blocks, int num_cu, int ncells) {
+    static int per_cu = 0;
+    if (per_cu == 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, optcb_kernel<N>, 64, 0) != hipSuccess || nb < 1)
+            nb = 4 * kCbWavesPerSimd;
+        per_cu = nb;
+    }
+    const int b = per_cu * num_cu;
+    return b < max_blocks ? (b < ncells ? b : ncells) : (max_blocks < ncells ? max_blocks : ncells);
+}
+template <int N>
+int launch_optcb_n(const CbArgs &a, int max_blocks, int num_cu, hipStream_t s) {
+    const int blocks = optcb_blocks_n<N>(max_blocks, num_cu, a.ncells);
+    if (blocks > 0) hipLaunchKernelGGL(optcb_kernel<N>, dim3(blocks), dim3(64), 0, s, a);
+    return blocks;
 }
 
-bool launch_optcb(const CbArgs &a, int blocks, hipStream_t s) {
+int launch_optcb(const CbArgs &a, int max_blocks, int num_cu, hipStream_t s) {
     switch (a.nm) {
-    case 1: launch_optcb_n<1>(a, blocks, s); return true;
-    case 3: launch_optcb_n<3>(a, blocks, s); return true;
-    case 6: launch_optcb_n<6>(a, blocks, s); return true;
-    case 10: launch_optcb_n<10>(a, blocks, s); return true;
-    case 15: launch_optcb_n<15>(a, blocks, s); return true;
-    default: return false;
+    case 1: return launch_optcb_n<1>(a, max_blocks, num_cu, s);
+    case 3: return launch_optcb_n<3>(a, max_blocks, num_cu, s);
+    case 6: return launch_optcb_n<6>(a, max_blocks, num_cu, s);
+    case 10: return launch_optcb_n<10>(a, max_blocks, num_cu, s);
+    case 15: return launch_optcb_n<15>(a, max_blocks, num_cu, s);
+    default: return -1;
     }
 }
 
