@@ -168,6 +168,37 @@ __device__ __forceinline__ double wave_min(double v) {
     return v;
 }
 
+// v from the lane a DPP control names within each 16-lane row (CTRL: a quad
+// permutation or a row rotation, so that every lane has a source)
+template <int CTRL>
+__device__ __forceinline__ double dpp_row_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The largest (smallest) value of lanes 0..15, wave-uniform: swaps within the
+// quads (lane ^ 1, lane ^ 2), then rotations of the row by 4 and 8 lanes.
+// For vectors of at most 16 elements on lanes 0..15 with neutral values
+// elsewhere this is wave_max (wave_min) exactly -- max and min do not round
+// and ignore a NaN beside a number in any order -- in four DPP steps instead
+// of six LDS permutes.
+__device__ __forceinline__ double row16_max(double v) {
+    v = fmax(v, dpp_row_d<0xB1>(v));
+    v = fmax(v, dpp_row_d<0x4E>(v));
+    v = fmax(v, dpp_row_d<0x124>(v));
+    v = fmax(v, dpp_row_d<0x128>(v));
+    return readlane_d(v, 0);
+}
+__device__ __forceinline__ double row16_min(double v) {
+    v = fmin(v, dpp_row_d<0xB1>(v));
+    v = fmin(v, dpp_row_d<0x4E>(v));
+    v = fmin(v, dpp_row_d<0x124>(v));
+    v = fmin(v, dpp_row_d<0x128>(v));
+    return readlane_d(v, 0);
+}
+
 // ---- wave-parallel dense algebra with the rows in registers ----------------
 //
 // Lane i (< NM) holds row i of the symmetric matrix (row[k] = A[i][k]); after
@@ -309,7 +340,58 @@ __device__ __forceinline__ double reg_inv_norm1(const double (&row)[NM], const d
         for (int k = i; k < NM; ++k) d = fma(readlane_d(z[k], i), z[k], d);
         c += fabs(d);
     }
-    return wave_max(lane < NM ? c : 0.0);
+    if constexpr (NM <= 16) return row16_max(lane < NM ? c : 0.0);
+    else return wave_max(lane < NM ? c : 0.0);
+}
+
+// reg_inv_norm1 that also solves x <- (L L^T)^{-1} x (x_i on lane i) for the
+// cost of one more lane: lane NM carries x through the forward substitution
+// that forms Z, and the back substitution then runs beside the Z^T Z
+// products.  Lane NM's updates fma(-L[i][k], z_k, z_i) are
+// reg_apply_inv_rcp's fma(-x_k, L[i][k], x_i) (the same exact product), in
+// the same order, and the back substitution is its second loop, so x comes
+// out bit for bit as reg_apply_inv_rcp leaves it.
+template <int NM>
+__device__ __forceinline__ double reg_inv_norm1_solve(const double (&row)[NM], const double (&col)[NM],
+                                                      const double (&rd)[NM], double &x, int lane) {
+    static_assert(NM < 64, "lane NM carries the right-hand side");
+    double z[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        const double xi = readlane_d(x, i);
+        z[i] = lane == NM ? xi : ((i == lane) ? 1.0 : 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < NM; ++k) {
+        z[k] = z[k] * rd[k];
+#pragma unroll
+        for (int i = k + 1; i < NM; ++i) z[i] = fma(-readlane_d(row[k], i), z[k], z[i]);
+    }
+    // y = L^{-1} x from lane NM, y_j on lane j (where reg_apply_inv_rcp's
+    // forward loop leaves it), then its back substitution
+    double y = 0.0;
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        const double v = readlane_d(z[j], NM);
+        y = lane == j ? v : y;
+    }
+#pragma unroll
+    for (int j = NM - 1; j >= 0; --j) {
+        const double xj = readlane_d(y, j) * rd[j];
+        if (lane == j) y = xj;
+        else if (lane < j) y = fma(-col[j], xj, y);
+    }
+    x = lane < NM ? y : x;
+    double c = 0.0;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        double d = 0.0;
+#pragma unroll
+        for (int k = i; k < NM; ++k) d = fma(readlane_d(z[k], i), z[k], d);
+        c += fabs(d);
+    }
+    if constexpr (NM <= 16) return row16_max(lane < NM ? c : 0.0);
+    else return wave_max(lane < NM ? c : 0.0);
 }
 
 }  // namespace

@@ -1842,8 +1842,8 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
                     rs += fabs(row[jj]);
                     dg = (jj == lane) ? row[jj] : dg;
                 }
-                const double anorm = wave_max(lane < NM ? rs : 0.0);
-                const double gersh = wave_min(lane < NM ? 2.0 * dg - rs : DBL_MAX);
+                const double anorm = row16_max(lane < NM ? rs : 0.0);
+                const double gersh = row16_min(lane < NM ? 2.0 * dg - rs : DBL_MAX);
                 err = sqrt(gg);
                 const bool chol_ok = reg_cholesky<NM>(row, lane);
                 // the columns of the factor (lane i: L[k][i], k > i) for the back substitution
@@ -1863,15 +1863,20 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
 #pragma unroll
                 for (int k = 0; k < NM; ++k) rd[k] = 1.0 / readlane_d(row[k], k);
                 double condest = 0.0;
+                // where the bounds cannot decide the test, the exact ||A^-1||_1
+                // bound and the direction come from one pass over the factor
+                double xv = gv;
+                bool solved = false;
                 if (chol_ok && !cond_proven_below(anorm, gersh, NM, A.cond_h_max)) {
-                    if (!(2.0 * anorm * reg_inv_norm1<NM>(row, rd, lane) <= A.cond_h_max))
+                    const double inv1 = reg_inv_norm1_solve<NM>(row, col, rd, xv, lane);
+                    solved = true;
+                    if (!(2.0 * anorm * inv1 <= A.cond_h_max))
                         condest = 1.0 / reg_rcond<NM>(row, col, anorm, lane);
                 }
                 if (iter >= maxit || !chol_ok || t == 0.0 || condest > A.cond_h_max) {
                     do_reg = true;
                 } else {
-                    double xv = gv;
-                    reg_apply_inv_rcp<NM>(row, col, rd, xv, lane);
+                    if (!solved) reg_apply_inv_rcp<NM>(row, col, rd, xv, lane);
                     bool stop = err <= rtol;
                     if (stop && A.tol_gamma > 0.0) {
                         double da1 = 0.0;
@@ -1886,7 +1891,7 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
                         double al_l = 0.0;
 #pragma unroll
                         for (int k = 0; k < NM; ++k) al_l = (k == lane) ? al[k] : al_l;
-                        const double maxf = wave_max(lane < NM ? fabs(di) / (fabs(al_l) + 1e-100) : 0.0);
+                        const double maxf = row16_max(lane < NM ? fabs(di) / (fabs(al_l) + 1e-100) : 0.0);
                         double gd = 0.0;
 #pragma unroll
                         for (int i = 0; i < NM; ++i) gd = fma(readlane_d(gv, i), readlane_d(di, i), gd);

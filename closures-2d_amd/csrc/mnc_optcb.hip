@@ -479,8 +479,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCbWavesPerS
                 norm += fabs(row[j]);
                 dg = (j == lane) ? row[j] : dg;
             }
-            const double gersh = wave_min(lane < N ? 2.0 * dg - norm : DBL_MAX);
-            norm = wave_max(lane < N ? norm : 0.0);
+            const double gersh = row16_min(lane < N ? 2.0 * dg - norm : DBL_MAX);
+            norm = row16_max(lane < N ? norm : 0.0);
             const bool chol_ok = reg_cholesky<N>(row, lane);
             __syncthreads();
             if (lane < N)
