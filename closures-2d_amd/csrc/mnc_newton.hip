@@ -232,7 +232,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 enum : int { ST_NEWTON = 0, ST_TRIAL = 1 };
 constexpr int kHalvings = 8;   // line-search halvings evaluated per wave sweep in the tail
-constexpr int kTailPts = 4;    // quadrature points per lane whose w e the tail keeps between passes
+constexpr int kTailCap = 4;    // quadrature points per lane whose w e the tail keeps between passes
+constexpr int kTailPts = 2;    // points per lane whose chains interleave (2 of the 4 at a time:
+                               // with 4, the rows and the partial sums outgrow the VGPRs and the
+                               // sums move through AGPRs, 4% slower on the M4 lattice)
 constexpr int kTailWaves = 4;  // waves (cells in flight) per block of the cooperative tail
 constexpr int kFirstHalvings = 2;   // candidates in the first batch of halvings after a failed trial
 enum : int { LOGIC_CONTINUE = 0, LOGIC_DONE = 1, LOGIC_DEFER = 2, LOGIC_MLP = 3 };
@@ -1484,12 +1487,12 @@ struct CoopShared {
 };
 
 // The quadrature table (transposed, pT[k][q]) and weights in LDS, shared by the
-// block's waves, when nq <= 64 kTailPts and the rows are the moments (direct
+// block's waves, when nq <= 64 kTailCap and the rows are the moments (direct
 // Hessian): LDS reads instead of L2 round trips in every sweep.
 template <int NM, bool ON>
 struct TailTable {
-    double t[ON ? NM : 1][ON ? 64 * kTailPts : 1];
-    double w[ON ? 64 * kTailPts : 1];
+    double t[ON ? NM : 1][ON ? 64 * kTailCap : 1];
+    double w[ON ? 64 * kTailCap : 1];
 };
 
 // v from lane l ^ 1 and from lane l ^ 2 (DPP quad permutations)
@@ -1518,7 +1521,7 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
     CS &sh = shw[threadIdx.x >> 6];
     const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
     const int lane = threadIdx.x & 63;
-    const bool tabled = !GAUNT && A.nq <= 64 * kTailPts;
+    const bool tabled = !GAUNT && A.nq <= 64 * kTailCap;
     if constexpr (!GAUNT) {
         if (tabled) {
             for (int i = threadIdx.x; i < NM * A.nq; i += blockDim.x) tab.t[i / A.nq][i % A.nq] = A.pT[i];
@@ -1559,7 +1562,7 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
         for (;;) {
             // ---- sweep(s): partial sums per lane, reduced through DPP and LDS ----
             double sf = 0.0;
-            double wes[kTailPts];
+            double wes[kTailCap];
 #pragma unroll
             for (int pass = 0; pass < CS::NPASS; ++pass) {
                 // flat range [K0, K1) of the sums of this pass (GAUNT: the
@@ -1572,12 +1575,19 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
                 // Lane l takes the points q = l, l + 64, ...; the moment values
                 // come from the LDS table or from the transposed table in global
                 // memory (for each k the wave reads 64 consecutive doubles).
-                // With at most kTailPts points per lane, w e of the first pass
-                // is kept for the second.
-                const bool cached = A.nq <= 64 * kTailPts;
-                for (int q0 = 0; q0 < A.nq; q0 += 64 * kTailPts) {
+                // With at most kTailCap points per lane, w e of the first pass
+                // is kept for the second.  The points run kTailPts at a time,
+                // a scheduling barrier between the groups so that the next
+                // group's rows are not loaded while this group's are live
+                // (each lane's sums still take its points in ascending order).
+                const bool cached = A.nq <= 64 * kTailCap;
+                for (int q0 = 0; q0 < A.nq; q0 += 64 * kTailCap) {
 #pragma unroll
-                    for (int qi = 0; qi < kTailPts; ++qi) {
+                    for (int qi = 0; qi < kTailCap; ++qi) {
+                        if (qi > 0 && qi % kTailPts == 0) {
+                            if (q0 + 64 * qi >= A.nq) break;
+                            __builtin_amdgcn_sched_barrier(0);
+                        }
                         // points past nq read point 0 and add exact zeros, so
                         // the kTailPts points' chains can interleave (the
                         // order-2N rows of the Gaunt form are too many registers
