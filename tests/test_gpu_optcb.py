@@ -125,6 +125,38 @@ def test_optbfgs_one_inner_iteration_is_optcb_bitwise():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("bfgs", [False, True])
+def test_optcb_cell_results_do_not_depend_on_the_batch(bfgs):
+    """The kernel's blocks take cells from a queue, as many blocks as fit the
+    GPU at once. A cell's answer must not depend on which block took it, on
+    what that block solved before (its workspace), or on the batch around it.
+    With 5,000 cells, more than the resident blocks, some blocks take several
+    cells. The whole batch, reversed, must equal it bit for bit, and so must
+    a single cell and a slice of seven."""
+    mnc = load_pkg("mnc")
+    o = oracle()
+    tab = o.Tables(4, 16, False)
+    rng = np.random.default_rng(2024)
+    u, astar, lobe = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, 5000, rng, near_boundary_frac=0.2)
+    P2M, aP = _identity_start(o, tab.nm, len(u))
+    opt = (mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10, opt_type=mnc.OPT_BFGS, max_bfgs_iter=5,
+                       cond_h_max_bfgs=20.0) if bfgs else mnc.options(max_iter=100, tol=1e-8, cond_h_max=1e10))
+    ok = (mnc.OK, mnc.ERR_MAXREG)
+    print("case optcb batch independence bfgs=%d n=%d" % (bfgs, len(u)), flush=True)
+    ctx = mnc.Context(4, 16, use_gaunt=False)
+    full = ctx.optcb_batch_host(u, P2M, aP, opt, allow_status=ok)
+    rev = ctx.optcb_batch_host(u[::-1], P2M[::-1], aP[::-1], opt, allow_status=ok)
+    one = ctx.optcb_batch_host(u[4321:4322], P2M[4321:4322], aP[4321:4322], opt, allow_status=ok)
+    seven = ctx.optcb_batch_host(u[1000:1007], P2M[1000:1007], aP[1000:1007], opt, allow_status=ok)
+    ctx.close()
+    assert np.all(full[5] > 0)                       # every cell was solved
+    for k in range(1, 8):                            # u, P2M, alphaP, alphaM, iter, r, norm_g
+        assert np.array_equal(full[k], rev[k][::-1])
+        assert np.array_equal(full[k][4321:4322], one[k])
+        assert np.array_equal(full[k][1000:1007], seven[k])
+
+
+@pytest.mark.gpu
 def test_optcb_robust_line_search_matches_oracle():
     tab, u, g, c = _run(4, 16, 400, 7, 0.1, line_search=1)
     _compare(tab, u, g, c, 0.95)
