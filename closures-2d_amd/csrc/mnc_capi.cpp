@@ -1426,6 +1426,10 @@ int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, cha
         for (auto &kb : knobs) {   // coordinate descent, one pass
             const int keep = *kb.slot;
             int arg = keep;
+            // the network is extra work: a gate that consults it must beat the
+            // gate that switches it off by more than the trials' noise (1%)
+            const bool gate = std::strcmp(kb.key, "learned_gate_log10") == 0;
+            double ms_off = (gate && keep == kGateOff) ? best : 0.0;
             for (int v : kb.vals) {
                 if (v == keep || (v < 1 && std::strstr(kb.key, "_blocks") != nullptr)) continue;
                 *kb.slot = v;
@@ -1434,7 +1438,12 @@ int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, cha
                 if ((st = trial(&ms))) break;
                 snprintf(line, sizeof line, "%s=%d %.4f ms/step\n", kb.key, v, ms);
                 rep += line;
+                if (gate && v == kGateOff) ms_off = ms;
                 if (ms < best) { best = ms; arg = v; }
+            }
+            if (gate && !st && arg != kGateOff && ms_off > 0.0 && ms_off <= 1.01 * best) {
+                arg = kGateOff;
+                best = ms_off;
             }
             *kb.slot = arg;
             ++ctx->epoch;

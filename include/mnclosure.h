@@ -342,7 +342,8 @@ int mnc_get_tune(const mnc_ctx *ctx, const char *key, int *value);
  * the candidates), (mixed precision)
  * "mix_cond_log10", "mix_split", "mix_defer" -- then the launch shapes
  * "graph", "flux_tile", "flux_groups", "flux_overlap", "rounds", "tail_mode", "tail_blocks" and (mixed
- * precision) "mix_blocks", and keeps the fastest.  Every candidate converges
+ * precision) "mix_blocks", and keeps the fastest (a gate that consults the
+ * network only when it beats the network off by more than 1%).  Every candidate converges
  * to the same tolerance; "mix_cond_log10" trades mixed-precision steps
  * against FP64 ones.  The grid state
  * and the statistics are restored after every trial, so the call leaves the
