@@ -34,10 +34,14 @@ struct SnArgs {
     const double *sigma_t, *sigma_s;  // [(nx+4)(ny+4)]
 };
 
-__device__ __forceinline__ double sn_minmod(double x, double y) {       // kinetic_update.cpp:22-25
+// The reference's selects as v_min / v_max (kinetic_update.cpp:22-25):
+// fabs(x) < y s ? fabs(x) : y s is fmin(fabs(x), y s) and m > 0 ? m : 0 is
+// fmax(m, 0) -- equal values, NaN included; only the sign of an exact zero
+// slope may differ, and the slope is added to a nonzero state.
+__device__ __forceinline__ double sn_minmod(double x, double y) {
     const double s = (x >= 0.0) ? 1.0 : -1.0;
-    const double m = fabs(x) < y * s ? fabs(x) : y * s;
-    return s * (m > 0.0 ? m : 0.0);
+    const double m = fmin(fabs(x), y * s);
+    return s * fmax(m, 0.0);
 }
 __device__ __forceinline__ double sn_slopefit(double l, double c, double r) {   // :31-35, theta = 2
     return sn_minmod(2.0 * (r - c), sn_minmod(0.5 * (r - l), 2.0 * (c - l)));
