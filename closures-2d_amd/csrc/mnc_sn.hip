@@ -37,7 +37,7 @@ struct SnArgs {
 // The reference's selects as v_min / v_max (kinetic_update.cpp:22-25):
 // fabs(x) < y s ? fabs(x) : y s is fmin(fabs(x), y s) and m > 0 ? m : 0 is
 // fmax(m, 0) -- equal values, NaN included; only the sign of an exact zero
-// slope may differ, and the slope is added to a nonzero state.
+// slope may differ, which a sum with any nonzero term does not see.
 __device__ __forceinline__ double sn_minmod(double x, double y) {
     const double s = (x >= 0.0) ? 1.0 : -1.0;
     const double m = fmin(fabs(x), y * s);
