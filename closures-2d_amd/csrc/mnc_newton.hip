@@ -231,7 +231,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 enum : int { ST_NEWTON = 0, ST_TRIAL = 1 };
-constexpr int kHalvings = 8;   // line-search halvings evaluated per wave sweep in the tail
+constexpr int kHalvings = 2;   // line-search halvings evaluated per wave sweep in the tail (8 measured
+                               // 2-3% slower: the unused candidates' exponentials and registers)
 constexpr int kTailCap = 4;    // quadrature points per lane whose w e the tail keeps between passes
 constexpr int kTailPts = 2;    // points per lane whose chains interleave (2 of the 4 at a time:
                                // with 4, the rows and the partial sums outgrow the VGPRs and the
