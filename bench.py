@@ -323,6 +323,7 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
             # (one wave per cell) to the lane-parallel rounds; keep the fastest
             # (and, mixed precision, whether steps redone in FP64 wait for the
             # tail or join FP64 lockstep rounds)
+            ctx.tune("tail_mode", 0)                  # the context is shared by the forms
             timed_solve()                              # first run allocates
             tries = {}
             for defer in ((0, 1) if "precision" in kw else (0,)):
@@ -335,12 +336,20 @@ def bench_isolated(mnc, n, seed, oracle_sample, train_steps):
             ctx.tune("rounds", best_rounds)
             if "precision" in kw:
                 ctx.tune("mix_defer", best_defer)
+            # the tail at one or at two waves per SIMD (latency against cells in flight)
+            tail_tries = {0: tries[(best_rounds, best_defer)]}
+            ctx.tune("tail_mode", 3)
+            tail_tries[3] = min(timed_solve(), timed_solve())
+            best_tail = min(tail_tries, key=tail_tries.get)
+            ctx.tune("tail_mode", best_tail)
             best = min(timed_solve() for _ in range(3))
             st = ctx.stats()
             ug, _, itg, rg = ctx.batch_get()
             progress("isolated %s (line search %d): %.3f s" % (name, ls, best))
             res[name] = {"closures_per_s": n / best, "s": best, "rounds": best_rounds, "mix_defer": best_defer,
-                         "tried_s": {"rounds=%d defer=%d" % k: v for k, v in tries.items()},
+                         "tail_mode": best_tail,
+                         "tried_s": dict({"rounds=%d defer=%d" % k: v for k, v in tries.items()},
+                                         **{"tail_mode=%d" % k: v for k, v in tail_tries.items()}),
                          "newton_iters_per_closure": st.iter_sum / st.num_dual_solves,
                          "trials_per_closure": st.trial_sum / st.num_dual_solves,
                          "reg_hist": list(st.hist_reg), "warm_pick_mlp_iso_prev": list(st.warm_pick),

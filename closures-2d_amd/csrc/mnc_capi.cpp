@@ -50,7 +50,8 @@ struct mnc_ctx {
     // tuning
     int rounds = 6;
     int tail_blocks = 0;             // 0: 2 per CU
-    int tail_mode = 0;               // 0: wave per cell, 1: lane per cell, 2: wave sweep + serial logic
+    int tail_mode = 0;               // 0: wave per cell, 1: lane per cell, 2: wave sweep + serial logic,
+                                     // 3: wave per cell at two waves per SIMD
     int mix_cond_log10 = 9;          // mixed steps need cond(H) <= 10^this, else FP64
     int mix_defer = 0;               // 0: redone steps wait for the FP64 tail; 1: FP64 lockstep rounds
     int mix_split = 3;               // bf16 parts of e and of the table in the mixed Hessian products
@@ -1269,7 +1270,7 @@ int mnc_tune(mnc_ctx *ctx, const char *key, int value) {
         return MNC_OK;
     }
     if (std::strcmp(key, "tail_mode") == 0) {
-        if (value < 0 || value > 2) return ctx_fail(ctx, MNC_ERR_ARG, "tail_mode out of range");
+        if (value < 0 || value > 3) return ctx_fail(ctx, MNC_ERR_ARG, "tail_mode out of range");
         ctx->tail_mode = value;
         return MNC_OK;
     }
@@ -1415,7 +1416,7 @@ int mnc_autotune(mnc_ctx *ctx, double dt, const mnc_options *opt, int steps, cha
     knobs.push_back({"flux_groups", {2, 3, 4, 6, 8}, &ctx->flux_groups});
     knobs.push_back({"flux_overlap", {0, 1}, &ctx->flux_overlap});
     knobs.push_back({"rounds", {2, 4, 6, 8, 12}, &ctx->rounds});
-    knobs.push_back({"tail_mode", {0, 1, 2}, &ctx->tail_mode});
+    knobs.push_back({"tail_mode", {0, 1, 2, 3}, &ctx->tail_mode});
     knobs.push_back({"tail_blocks", {cu, 2 * cu, 4 * cu}, &ctx->tail_blocks});
     if (opt->precision == MNC_PREC_MIXED) knobs.push_back({"mix_blocks", {cu / 2, cu, 2 * cu}, &ctx->mix_blocks});
     double best = 0.0;

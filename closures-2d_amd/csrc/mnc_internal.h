@@ -136,7 +136,8 @@ int build_g64_tables(int moment_order, int quad_order, G64Tables &m);
 // lockstep sweep/logic rounds r0..r1-1 with compaction (round r reads list
 // r & 1) and, if `tail`, the persistent tail.  tail_mode 0: one wave per
 // remaining cell (cooperative sweeps); 1: one lane per cell; 2: one wave per
-// cell, algebra per lane.
+// cell, algebra per lane; 3: mode 0 held to two waves per SIMD (direct Mn
+// Hessian; mode 0 elsewhere).
 struct LaunchPlan {
     int r0 = 0, r1 = 0;
     bool init = true, tail = true;

@@ -1512,16 +1512,24 @@ __device__ __forceinline__ double dpp_xor2(double v) {
 }
 
 
-template <int NM, int NP2, bool GAUNT, int RS, bool PPN = false>
-__global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A, int lin, int lin2) {
+// WPE: waves per SIMD the registers are held to.  At 1 the kernel takes up to
+// 512 registers, for the latency of few cells (the lattice's tail: 2.65
+// against 2.88 ms/step at 2); at 2 it holds 256, with a few spill slots,
+// for throughput when many cells enter the tail (the isolated lobe batch:
+// 10.9 against 9.2 M closures/s).  At 2 the lane index passes through an
+// empty asm each pass and the state's base offsets are formed per cell, so
+// that the compiler does not keep lane masks and field addresses live
+// across the loops.
+template <int NM, int NP2, bool GAUNT, int RS, bool PPN = false, int WPE = 1>
+__global__ __launch_bounds__(64 * kTailWaves) __attribute__((amdgpu_waves_per_eu(WPE)))
+void tail_coop_kernel(NewtonArgs A, int lin, int lin2) {
     using St = Store<NM, NP2>;
     using CS = CoopShared<NM, NP2, GAUNT, RS, PPN>;
     constexpr int NG = CS::NG;
     __shared__ CS shw[kTailWaves];
     __shared__ TailTable<NM, !GAUNT> tab;
     CS &sh = shw[threadIdx.x >> 6];
-    const Store<NM, NP2> S{A.st_d, A.st_i, A.cap};
-    const int lane = threadIdx.x & 63;
+    int lane = threadIdx.x & 63;
     const bool tabled = !GAUNT && A.nq <= 64 * kTailCap;
     if constexpr (!GAUNT) {
         if (tabled) {
@@ -1540,6 +1548,9 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
         // wave-uniform: state loads become scalar loads
         const int c = __builtin_amdgcn_readfirstlane(tail_cell(A, lin, count, lin2, dcount, s));
         if (c < 0) break;
+        size_t capc = A.cap;
+        if constexpr (WPE > 1) __asm__ volatile("" : "+s"(capc));
+        const Store<NM, NP2> S{A.st_d, A.st_i, capc};
         // The cell's state comes into registers once: the tail runs it to the
         // end, so nothing is stored back until the outputs.  Vectors are
         // lane-distributed (element k on lane k), scalars wave-uniform.
@@ -1561,6 +1572,7 @@ __global__ __launch_bounds__(64 * kTailWaves) void tail_coop_kernel(NewtonArgs A
         double aold_l = lane < NM ? S.d(St::F_AOLD + lane, c) : 0.0;
         double d_l = lane < NM ? S.d(St::F_D + lane, c) : 0.0;
         for (;;) {
+            if constexpr (WPE > 1) __asm__ volatile("" : "+v"(lane));
             // ---- sweep(s): partial sums per lane, reduced through DPP and LDS ----
             double sf = 0.0;
             double wes[kTailCap];
@@ -2469,7 +2481,14 @@ __global__ __launch_bounds__(64) void logic_g64_kernel(NewtonArgs A, int lin) {
 
 template <int NM, int NP2, bool G, int RS, bool PPN = false>
 void launch_tail(const NewtonArgs &a, int lin, int lin2, int tail_blocks, int tail_mode, hipStream_t s) {
-    if (tail_mode == 0 || PPN)   // PPn: the cooperative tail only
+    if constexpr (!G && !PPN) {   // the cooperative tail at two waves per SIMD (direct Mn Hessian)
+        if (tail_mode == 3) {
+            hipLaunchKernelGGL((tail_coop_kernel<NM, NP2, G, RS, PPN, 2>), dim3(tail_blocks * 4 / kTailWaves),
+                               dim3(64 * kTailWaves), 0, s, a, lin, lin2);
+            return;
+        }
+    }
+    if (tail_mode == 0 || tail_mode == 3 || PPN)   // PPn: the cooperative tail only
         hipLaunchKernelGGL((tail_coop_kernel<NM, NP2, G, RS, PPN>), dim3(tail_blocks * 4 / kTailWaves),
                            dim3(64 * kTailWaves), 0, s, a, lin, lin2);
     else if (tail_mode == 2)

@@ -280,7 +280,10 @@ int mnc_reset_stats(mnc_ctx *ctx);
  *   "rounds"       lockstep sweep/logic rounds before the persistent tail (0-512)
  *   "tail_blocks"  workgroups of the persistent tail kernel (1-65536)
  *   "tail_mode"    0: one wave per remaining cell, dense algebra shared in LDS;
- *                  1: one lane per cell; 2: one wave per cell, algebra per lane
+ *                  1: one lane per cell; 2: one wave per cell, algebra per lane;
+ *                  3: as 0 at two waves per SIMD (more cells in flight, each
+ *                  a little slower: for tails of many cells; direct Mn
+ *                  Hessian, elsewhere 0)
  *   "mix_cond_log10"  MNC_PREC_MIXED: a Newton step uses the mixed-precision
  *                  Hessian only while its condition estimate is <= 10^value
  *                  (0-10, default 9); otherwise the step is redone in FP64

@@ -206,10 +206,12 @@ def test_resident_batch_equals_host_batch():
 @pytest.mark.gpu
 def test_tail_modes_take_the_same_path():
     """rounds = 0 sends every cell through the persistent tail.  The wave tail
-    evaluates line-search halvings eight at a time; the lane tail one at a
+    evaluates line-search halvings two at a time; the lane tail one at a
     time.  Both take the reference's decisions, so the iteration counts,
     regularization levels and trial counts agree, and the answers agree to the
-    rounding of their different summation orders."""
+    rounding of their different summation orders.  The wave tail held to two
+    waves per SIMD (mode 3) runs the same arithmetic: its answers are mode 0's
+    bit for bit."""
     mnc = load_pkg("mnc")
     o = oracle()
     tab = o.Tables(4, 16, False)
@@ -217,14 +219,18 @@ def test_tail_modes_take_the_same_path():
     u, _, _ = config2_batch(tab.w, tab.p, tab.xi, tab.eta, tab.nm, 2000, rng, near_boundary_frac=0.5)
     a0 = isotropic_start(u)
     res = {}
-    for mode in (0, 1):
+    for mode in (0, 1, 3):
+        print("case tail_mode=%d n=%d" % (mode, len(u)), flush=True)
         ctx = mnc.Context(4, 16)
         ctx.tune("rounds", 0)
         ctx.tune("tail_mode", mode)
         ctx.reset_stats()
         st, ug, ag, itg, rg, _ = ctx.opt_batch_host(u, a0, mnc.options(tol=1e-8))
-        res[mode] = (st, ug, itg, rg, ctx.stats())
+        res[mode] = (st, ug, itg, rg, ctx.stats(), ag)
         ctx.close()
+    for k in (0, 1, 2, 3, 5):
+        assert np.array_equal(res[3][k], res[0][k])
+    assert res[3][4].trial_sum == res[0][4].trial_sum and res[3][4].iter_sum == res[0][4].iter_sum
     same = (res[0][2] == res[1][2]) & (res[0][3] == res[1][3])
     print("wave vs lane tail: identical (iter, r) %.4f; trials %d vs %d"
           % (same.mean(), res[0][4].trial_sum, res[1][4].trial_sum))
