@@ -1523,6 +1523,9 @@ __device__ __forceinline__ double dpp_xor2(double v) {
 template <int NM, int NP2, bool GAUNT, int RS, bool PPN = false, int WPE = 1>
 __global__ __launch_bounds__(64 * kTailWaves) __attribute__((amdgpu_waves_per_eu(WPE)))
 void tail_coop_kernel(NewtonArgs A, int lin, int lin2) {
+    // points per lane interleaved: at two waves per SIMD one at a time (fewer
+    // live registers; 4-5% faster there), otherwise kTailPts
+    constexpr int PTS = WPE > 1 ? 1 : kTailPts;
     using St = Store<NM, NP2>;
     using CS = CoopShared<NM, NP2, GAUNT, RS, PPN>;
     constexpr int NG = CS::NG;
@@ -1589,7 +1592,7 @@ void tail_coop_kernel(NewtonArgs A, int lin, int lin2) {
                 // come from the LDS table or from the transposed table in global
                 // memory (for each k the wave reads 64 consecutive doubles).
                 // With at most kTailCap points per lane, w e of the first pass
-                // is kept for the second.  The points run kTailPts at a time,
+                // is kept for the second.  The points run PTS at a time,
                 // a scheduling barrier between the groups so that the next
                 // group's rows are not loaded while this group's are live
                 // (each lane's sums still take its points in ascending order).
@@ -1597,12 +1600,12 @@ void tail_coop_kernel(NewtonArgs A, int lin, int lin2) {
                 for (int q0 = 0; q0 < A.nq; q0 += 64 * kTailCap) {
 #pragma unroll
                     for (int qi = 0; qi < kTailCap; ++qi) {
-                        if (qi > 0 && qi % kTailPts == 0) {
+                        if (qi > 0 && qi % PTS == 0) {
                             if (q0 + 64 * qi >= A.nq) break;
                             __builtin_amdgcn_sched_barrier(0);
                         }
                         // points past nq read point 0 and add exact zeros, so
-                        // the kTailPts points' chains can interleave (the
+                        // the PTS points' chains can interleave (the
                         // order-2N rows of the Gaunt form are too many registers
                         // for that: there the points stay sequential)
                         const int q = q0 + 64 * qi + lane;
@@ -1749,7 +1752,7 @@ void tail_coop_kernel(NewtonArgs A, int lin, int lin2) {
                         // candidates outside, the lane's points inside (their
                         // chains interleave); each fj sums the lane's points in
                         // order, as the trial sweep's f does
-                        for (int q0 = 0; q0 < A.nq; q0 += 64 * kTailPts) {
+                        for (int q0 = 0; q0 < A.nq; q0 += 64 * PTS) {
 #pragma unroll
                             for (int jj = 0; jj < kHalvings; ++jj) {
                                 if (jj >= nval) break;                  // wave-uniform
@@ -1758,7 +1761,7 @@ void tail_coop_kernel(NewtonArgs A, int lin, int lin2) {
 #pragma unroll
                                 for (int k = 0; k < NM; ++k) aj[k] = fma(tj, sh.gvec[k], sh.x[k]);
 #pragma unroll
-                                for (int qi = 0; qi < kTailPts; ++qi) {
+                                for (int qi = 0; qi < PTS; ++qi) {
                                     const int q = q0 + 64 * qi + lane;
                                     if (GAUNT && q >= A.nq) break;      // Gaunt form: sequential points
                                     const bool valid = q < A.nq;
