@@ -1526,6 +1526,7 @@ void tail_coop_kernel(NewtonArgs A, int lin, int lin2) {
     // points per lane interleaved: at two waves per SIMD one at a time (fewer
     // live registers; 4-5% faster there), otherwise kTailPts
     constexpr int PTS = WPE > 1 ? 1 : kTailPts;
+    static_assert(NM <= 16, "the max/min reductions over the row lanes take one 16-lane DPP row");
     using St = Store<NM, NP2>;
     using CS = CoopShared<NM, NP2, GAUNT, RS, PPN>;
     constexpr int NG = CS::NG;
