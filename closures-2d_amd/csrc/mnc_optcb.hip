@@ -606,8 +606,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCbWavesPerS
                     double d = 0.0;
                     if (lane < N) d = cb_gemv_row<N>(sh.hI, -1.0, sh.gP, lane);
                     if (lane < N) sh.dP[lane] = d;
-                } else if (lane == 0) {
-                    for (int i = 0; i < N; i++) sh.dP[i] = -sh.gP[i];
+                } else if (lane < N) {
+                    sh.dP[lane] = -sh.gP[lane];
                 }
                 __syncthreads();
                 if (lane < N) sh.gM[lane] = sh.gP[lane];       // :352-354
@@ -635,14 +635,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCbWavesPerS
                 double gd = 0.0;
                 for (int i = 0; i < N; i++) gd += sh.gP[i] * sh.dP[i];
                 __syncthreads();
-                if (lane == 0)
-                    for (int i = 0; i < N; i++) sh.aold[i] = sh.aP[i];
+                if (lane < N) sh.aold[lane] = sh.aP[lane];
                 __syncthreads();
                 double tt = 1.0;
                 bool accepted = false;
                 while (tt * maxf > DBL_EPSILON / 2) {
-                    if (lane == 0)
-                        for (int i = 0; i < N; i++) sh.aP[i] = sh.aold[i] + tt * sh.dP[i];
+                    if (lane < N) sh.aP[lane] = sh.aold[lane] + tt * sh.dP[lane];
                     __syncthreads();
                     const double fnew = cb_fobj_f<N>(A, pP, sh.aP, sh.uP, lane);
                     bool acc = fnew <= sh.f + 1e-3 * tt * gd;
@@ -653,8 +651,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCbWavesPerS
                     tt = 0.5 * tt;
                 }
                 if (!accepted) {
-                    if (lane == 0)
-                        for (int i = 0; i < N; i++) sh.aP[i] = sh.aold[i];
+                    if (lane < N) sh.aP[lane] = sh.aold[lane];
                     tt = 0.0;
                 }
                 t = tt;
