@@ -445,12 +445,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCbWavesPerS
             }
             continue;
         }
+        for (int k = lane; k < N * N; k += 64) sh.P2M[k] = Bg[k];
+        if (lane < N) {
+            const double uk = ug[lane] / u0, ak = aPg[lane];
+            sh.u[lane] = uk;
+            sh.uP[lane] = uk;
+            sh.aP[lane] = ak;
+            sh.aP0[lane] = ak;
+        }
+        __syncthreads();
         if (lane == 0) {
-            for (int k = 0; k < N; k++) sh.u[k] = ug[k] / u0;
-            for (int k = 0; k < N; k++) sh.aP[k] = sh.aP0[k] = aPg[k];
-            for (int k = 0; k < N * N; k++) sh.P2M[k] = Bg[k];
             sh.p2min_id = 0;
-            for (int k = 0; k < N; k++) sh.uP[k] = sh.u[k];
             cb_trsv_ln<N>(sh.P2M, sh.uP);                   // :198-200
             sh.rtol = cb_nrm2(N, sh.u) * A.tol;
             sh.done = 0;
@@ -668,14 +673,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kCbWavesPerS
             }
             if (sh.converged) break;
         }
-        if (sh.converged) {
-            if (lane == 0) {                                // outputs (:367-369, :111-118)
-                sh.aM[0] = sh.aM[0] + log(u0) * 2.0 * kCbSqrtPi;
-                for (int k = 0; k < N; k++) A.alphaM[size_t(c) * N + k] = sh.aM[k];
-                ug[0] = 1.0 * u0;
-                for (int i = 1; i < N; i++) ug[i] = (sh.u[i] * (1.0 - r)) * u0;
-                for (int k = 0; k < N; k++) aPg[k] = sh.aP[k];
-                for (int k = 0; k < N * N; k++) Bg[k] = sh.P2M[k];
+        if (sh.converged) {                                 // outputs (:367-369, :111-118)
+            for (int k = lane; k < N * N; k += 64) Bg[k] = sh.P2M[k];
+            if (lane < N) {
+                A.alphaM[size_t(c) * N + lane] = lane == 0 ? sh.aM[0] + log(u0) * 2.0 * kCbSqrtPi : sh.aM[lane];
+                ug[lane] = lane == 0 ? 1.0 * u0 : (sh.u[lane] * (1.0 - r)) * u0;
+                aPg[lane] = sh.aP[lane];
+            }
+            if (lane == 0) {
                 if (A.outs) { A.outs[c].iter = total; A.outs[c].r = r; A.outs[c].norm_g = sh.err; }
                 if (A.stats) {                              // c_optStats (momopt_update.cpp:254-273)
                     atomicMax((unsigned long long *)&A.stats[0], (unsigned long long)total);
